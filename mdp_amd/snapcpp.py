@@ -169,16 +169,20 @@ def snapdescriptors_pairs(lib, sna, p, dim=3):
     lib.cpuComputeDbidrj(dblist, zr, zi, dur, dui, sna.idxb, sna.idxu_block, sna.idxz_block, sna.map,
                          p["ai"], p["tj"], sna.twojmax, ib, iu, iz, nelem, sna.bnormflag, sna.chemflag,
                          N, ijnum)
-    K = sna.ntypes * sna.ncoeff
+    # Upstream sizes these tallies for ncoeff entries per type even with quadraticflag=1, where the
+    # quadratic terms are written past the end (snapcpp.jl:652-658 vs cpuSnap.cpp:2281-2288); here
+    # each type block holds the linear and the packed quadratic terms.
+    nperdim = sna.ncoeff + (sna.ncoeff * (sna.ncoeff + 1) // 2 if sna.quadraticflag else 0)
+    K = sna.ntypes * nperdim
     bi = np.zeros(K)
-    lib.cpuSnapTallyBispectrum(bi, blist, p["ilist"], p["atomtype"], N, sna.ncoeff, sna.ncoeff,
+    lib.cpuSnapTallyBispectrum(bi, blist, p["ilist"], p["atomtype"], N, sna.ncoeff, nperdim,
                                sna.ntypes, sna.quadraticflag)
     bd = np.zeros(K * 3 * N)
     lib.cpuSnapTallyBispectrumDeriv(bd, blist, dblist, p["ai"], p["ai"], p["aj"], p["ti"], N, ijnum,
-                                    sna.ncoeff, sna.ncoeff, sna.ntypes, sna.quadraticflag)
+                                    sna.ncoeff, nperdim, sna.ntypes, sna.quadraticflag)
     bv = np.zeros(6 * K)
     lib.cpuSnapTallyBispectrumVirial(bv, blist, dblist, p["rij"], p["ai"], p["ti"], N, ijnum, sna.ncoeff,
-                                     sna.ncoeff, sna.ntypes, sna.quadraticflag)
+                                     nperdim, sna.ntypes, sna.quadraticflag)
     bd = bd.reshape(K, N, 3)
     bv = bv.reshape(K, 6)[:, [0, 1, 2, 5, 4, 3]]
     return bi, bd, bv, dict(blist=blist, dblist=dblist, Ur=Ur, Ui=Ui, zr=zr, zi=zi)

@@ -1,0 +1,205 @@
+"""Properties that pin the oracle independently of any upstream output.
+
+Closed-form table sizes, Clebsch-Gordan values against SymPy and their orthogonality, unitarity of
+the Wigner-U layers, rotation/translation invariance, the isolated-atom limit, finite differences of
+the energy, the Euler identity of the cubic bispectrum energy, and the brute-force neighbour list
+against the host builder.
+"""
+import numpy as np
+import pytest
+
+from conftest import relerr
+from mdp_amd import lattice, neighbors, snapcpp
+
+PARAM = [1, 4, 4.67637, 0.99363, 0.0, 1, 1, 0, 0, 0, 0]
+
+
+def _setup(lib, J, x, box, seed=1, switchflag=1, rmin0=0.0):
+    param = list(PARAM)
+    param[1] = J
+    param[4] = rmin0
+    param[6] = switchflag
+    s = snapcpp.initsna(lib, param, [0.5], [1.0])
+    coeff = lattice.snap_coefficients(s.ncoeff, seed=seed)
+    s = snapcpp.initsna(lib, param, [0.5], [1.0], coeff)
+    B = np.diag(box)
+    p = snapcpp.pairs_from_box(x, np.ones(len(x), np.int32), B[0], B[1], B[2], [1, 1, 1], s)
+    return s, p, coeff
+
+
+@pytest.mark.parametrize("J,iu,iz,ib,icg", [(4, 55, 185, 14, 305), (6, 140, 779, 30, 1321),
+                                            (8, 285, 2386, 55, 4098)])
+def test_index_space_sizes(oracle, J, iu, iz, ib, icg):
+    assert (oracle.idxucount(J), oracle.idxzcount(J), oracle.idxbcount(J), oracle.idxcgcount(J)) == \
+        (iu, iz, ib, icg)
+    # sum over Z entries of na*nb (work of one Z evaluation) at J=8 is 41889
+    if J == 8:
+        s = snapcpp.initsna(oracle, [1, 8, 4.67637, 0.99363, 0, 1, 1, 0, 0, 0, 0], [0.5], [1.0])
+        z = s.idxz.reshape(-1, 10)
+        assert int(np.sum(z[:, 5] * z[:, 8])) == 41889
+
+
+def test_clebsch_gordan_vs_sympy(oracle):
+    sympy = pytest.importorskip("sympy")
+    from sympy.physics.quantum.cg import CG
+    J = 4
+    s = snapcpp.initsna(oracle, [1, J, 4.67637, 0.99363, 0, 1, 1, 0, 0, 0, 0], [0.5], [1.0])
+    k = 0
+    half = sympy.Rational(1, 2)
+    for j1 in range(J + 1):
+        for j2 in range(j1 + 1):
+            for j in range(j1 - j2, min(J, j1 + j2) + 1, 2):
+                for m1 in range(j1 + 1):
+                    for m2 in range(j2 + 1):
+                        aa, bb = 2 * m1 - j1, 2 * m2 - j2
+                        want = float(CG(j1 * half, aa * half, j2 * half, bb * half, j * half,
+                                        (aa + bb) * half).doit())
+                        assert abs(s.cglist[k] - want) < 1e-14, (j1, j2, j, m1, m2)
+                        k += 1
+    assert k == len(s.cglist)
+
+
+def test_clebsch_gordan_orthogonality(oracle):
+    J = 8
+    s = snapcpp.initsna(oracle, [1, J, 4.67637, 0.99363, 0, 1, 1, 0, 0, 0, 0], [0.5], [1.0])
+    jd = J + 1
+    for j1 in range(J + 1):
+        for j2 in range(j1 + 1):
+            for j in range(j1 - j2, min(J, j1 + j2) + 1, 2):
+                cg = s.cglist[s.idxcg_block[j + j2 * jd + j1 * jd * jd]:][:(j1 + 1) * (j2 + 1)]
+                cg = cg.reshape(j1 + 1, j2 + 1)
+                for m in range(j + 1):
+                    tot = sum(cg[m1, m2] ** 2 for m1 in range(j1 + 1) for m2 in range(j2 + 1)
+                              if (2 * m1 - j1 + 2 * m2 - j2 + j) // 2 == m)
+                    assert abs(tot - 1.0) < 1e-12
+
+
+def test_wigner_layers_unitary(oracle):
+    """Each layer U_j of a single pair (no switching, weight 1) is a unitary matrix."""
+    J = 8
+    s = snapcpp.initsna(oracle, [1, J, 4.67637, 0.99363, 0, 1, 0, 0, 0, 0, 0], [0.5], [1.0])
+    rng = np.random.default_rng(3)
+    rij = rng.normal(size=(5, 3))
+    rij = (rij / np.linalg.norm(rij, axis=1)[:, None] * rng.uniform(1.5, 4.5, size=(5, 1))).ravel()
+    P = 5
+    ur, ui = np.zeros(s.idxu_max * P), np.zeros(s.idxu_max * P)
+    dur, dui = np.zeros(3 * s.idxu_max * P), np.zeros(3 * s.idxu_max * P)
+    one = np.ones(P, np.int32)
+    oracle.cpuComputeUij(ur, ui, dur, dui, s.rootpqarray, rij, s.wjelem, s.radelem, 0.0, s.rfac0,
+                         s.rcutfac, s.idxu_block, one, one, J, s.idxu_max, P, 0)
+    U = (ur + 1j * ui).reshape(s.idxu_max, P)
+    for p in range(P):
+        for j in range(J + 1):
+            blk = U[s.idxu_block[j]:s.idxu_block[j] + (j + 1) ** 2, p].reshape(j + 1, j + 1)
+            assert np.allclose(blk @ blk.conj().T, np.eye(j + 1), atol=1e-12)
+
+
+def test_isolated_atom_bispectrum_is_zero(oracle):
+    """No neighbours: Utot = identity per layer, so B(j1,j2,j) = j+1 and B - bzero = 0."""
+    J = 8
+    s = snapcpp.initsna(oracle, [1, J, 4.67637, 0.99363, 0, 1, 1, 0, 0, 0, 0], [0.5], [1.0])
+    N = 3
+    Ur, Ui = np.zeros(N * s.idxu_max), np.zeros(N * s.idxu_max)
+    oracle.cpuAddWself2Ui(Ur, Ui, 1.0, s.idxu_block, np.ones(N, np.int32), s.map, 0, 0, s.idxu_max, 1, J, N)
+    bl = np.zeros(N * s.idxb_max)
+    oracle.cpuSnapComputeBi(bl, Ur, Ui, s.cglist, s.bzero, s.idxb, s.idxcg_block, s.idxu_block,
+                            s.idxz_block, J, s.idxb_max, s.idxu_max, s.idxz_max, 1, 0, 1, 0, N)
+    assert np.max(np.abs(bl)) < 1e-13
+
+
+def _rotation(seed):
+    q = np.random.default_rng(seed).normal(size=4)
+    q /= np.linalg.norm(q)
+    a, b, c, d = q
+    return np.array([[a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                     [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+                     [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
+
+
+@pytest.mark.parametrize("J", [4, 8])
+def test_bispectrum_rotation_invariance(oracle, J):
+    x, box = lattice.crystal("bcc", 2, 3.316, 0.1, 5)
+    s, p, _ = _setup(oracle, J, x, box)
+    R = _rotation(9)
+    q = dict(p)
+    q["rij"] = np.ascontiguousarray((p["rij"].reshape(-1, 3) @ R.T).ravel())
+    a = snapcpp.snapdescriptors_pairs(oracle, s, p)[3]["blist"]
+    b = snapcpp.snapdescriptors_pairs(oracle, s, q)[3]["blist"]
+    assert relerr(b, a) < 1e-12
+
+
+@pytest.mark.parametrize("J", [4, 6])
+def test_forces_are_energy_gradient(oracle, J):
+    x, box = lattice.crystal("bcc", 2, 3.316, 0.1, 6)
+    s, p, _ = _setup(oracle, J, x, box)
+    e, f, v = snapcpp.snappotential_pairs(oracle, s, p)
+    B = np.diag(box)
+    h = 1e-5
+    t = np.ones(len(x), np.int32)
+    for ia, d in ((1, 0), (5, 1), (12, 2)):
+        def etot(dx):
+            xx = x.copy()
+            xx[ia, d] += dx
+            pp = snapcpp.pairs_from_box(xx, t, B[0], B[1], B[2], [1, 1, 1], s)
+            return snapcpp.snappotential_pairs(oracle, s, pp)[0].sum()
+        fd = -(etot(h) - etot(-h)) / (2 * h)
+        assert abs(fd - f[ia, d]) < 1e-7 * max(1.0, abs(f[ia, d]))
+    # translation invariance and the virial definition
+    assert np.max(np.abs(f.sum(axis=0))) < 1e-11 * np.max(np.abs(f))
+
+
+def test_euler_identity_energy_from_adjoint(oracle):
+    """E is cubic-homogeneous in Utot, so 2 sum' (U.Y) = 3 sum_k beta_k (B_k + bzero_k)."""
+    J = 8
+    x, box = lattice.crystal("bcc", 2, 3.316, 0.1, 7)
+    s, p, coeff = _setup(oracle, J, x, box)
+    N, iu = p["n"], s.idxu_max
+    Ur, Ui = np.zeros(N * iu), np.zeros(N * iu)
+    oracle.cpuSnapComputeUi(Ur, Ui, s.rootpqarray, p["rij"], s.wjelem, s.radelem, s.rmin0, s.rfac0,
+                            s.rcutfac, s.map, p["ai"], p["ti"], p["tj"], J, iu, N, p["ijnum"], 1, 0)
+    oracle.cpuAddWself2Ui(Ur, Ui, 1.0, s.idxu_block, p["atomtype"], s.map, 0, 0, iu, 1, J, N)
+    e = np.zeros(N)
+    oracle.cpuSnapComputeEi(e, Ur, Ui, s.cglist, s.bzero, s.coeffelem, p["ilist"], s.map, p["atomtype"],
+                            s.idxb, s.idxcg_block, s.idxu_block, J, s.idxb_max, iu, 1, s.ncoeffall, 0, 1,
+                            0, 0, N)
+    Yr, Yi = np.zeros(N * iu), np.zeros(N * iu)
+    oracle.cpuSnapComputeYi(Yr, Yi, Ur, Ui, s.cglist, s.coeffelem, s.map, p["atomtype"], s.idxz,
+                            s.idxb_block, s.idxu_block, s.idxcg_block, J, s.idxb_max, iu, s.idxz_max, 1,
+                            s.ncoeffall, 0, 0, N)
+    w = np.zeros(iu)
+    for j in range(J + 1):
+        for mb in range(j // 2 + 1):
+            for ma in range(j + 1):
+                k = s.idxu_block[j] + (j + 1) * mb + ma
+                w[k] = 1.0 if 2 * mb < j or ma < mb else (0.5 if ma == mb else 0.0)
+    U = (Ur.reshape(iu, N), Ui.reshape(iu, N))
+    Y = (Yr.reshape(iu, N), Yi.reshape(iu, N))
+    dot = 2.0 * np.sum(w[:, None] * (U[0] * Y[0] + U[1] * Y[1]), axis=0)
+    bz = np.array([s.bzero[s.idxb[3 * k + 2]] for k in range(s.idxb_max)])
+    ecubic = e - coeff[0] + coeff[1:] @ bz
+    assert relerr(dot, 3.0 * ecubic) < 1e-12
+
+
+def test_bruteforce_neighbors_match_host_builder(oracle):
+    import ctypes
+    x, box = lattice.crystal("bcc", 3, 3.316, 0.15, 8)
+    n, rc = len(x), 4.97637
+    off = np.zeros(n + 1, np.int32)
+    maxnb = 64 * n
+    nj = np.zeros(maxnb, np.int32)
+    ns = np.zeros(3 * maxnb, np.int32)
+    f = oracle.cdll.oracle_neighbor_bruteforce
+    f.restype = ctypes.c_long
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C")
+    ip = np.ctypeslib.ndpointer(np.int32, flags="C")
+    f.argtypes = [dp, ctypes.c_int, dp, ctypes.c_double, ip, ip, ip, ctypes.c_long]
+    cnt = f(np.ascontiguousarray(x), n, box, rc, off, nj, ns, maxnb)
+    assert cnt > 0
+    B = np.diag(box)
+    y, alist, nl, nn = neighbors.fullneighborlist(x, B[0], B[1], B[2], [1, 1, 1], rc)
+    _, _, shifts = neighbors.ghost_atoms(x, B[0], B[1], B[2], [1, 1, 1], rc)
+    for i in range(n):
+        a = sorted((int(nj[k]), *map(int, ns[3 * k:3 * k + 3])) for k in range(off[i], off[i + 1]))
+        g = nl[nn[i]:nn[i + 1]]
+        b = sorted((int(alist[k]), *map(int, shifts[k])) for k in g)
+        assert a == b, i
