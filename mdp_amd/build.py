@@ -1,0 +1,91 @@
+"""Builds libmdpsnap.so (HIP for gfx950) in-tree: ``python -m mdp_amd.build [--force] [-j N]``.
+
+Each source in csrc/ compiles to its own object under mdp_amd/_build/ in parallel, then one
+``hipcc -shared`` links the library next to this file.  Without --force an object is rebuilt only
+when its source or any header in csrc/ (or include/mdpsnap.h) is newer.
+"""
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libmdpsnap.so")
+HEADER = os.path.join(ROOT, "include", "mdpsnap.h")
+
+SOURCES = ["k_yi.hip", "k_dedr.hip", "k_bz.hip", "k_ui.hip", "k_misc.hip", "md_engine.hip", "dropin.cpp",
+           "runtime.cpp", "snap_tables.cpp"]
+
+ARCH = os.environ.get("MDP_OFFLOAD_ARCH", "gfx950")
+FLAGS = ["-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-munsafe-fp-atomics",
+         "-I", os.path.join(ROOT, "include")]
+
+
+def hipcc():
+    for cand in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found: the HIP toolchain (ROCm) is required to build libmdpsnap")
+
+
+def _newest_header():
+    hs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))] + [HEADER]
+    return max(os.path.getmtime(h) for h in hs if os.path.exists(h))
+
+
+def _compile(src, force, hdr_time):
+    s = os.path.join(CSRC, src)
+    o = os.path.join(OBJ, src + ".o")
+    if not force and os.path.exists(o) and os.path.getmtime(o) >= max(os.path.getmtime(s), hdr_time):
+        return src, 0.0, ""
+    t0 = time.time()
+    cmd = [hipcc()] + FLAGS + ["-c", s, "-o", o]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{out}")
+    return src, time.time() - t0, out
+
+
+def build(force=False, jobs=None, verbose=True):
+    os.makedirs(OBJ, exist_ok=True)
+    hdr_time = _newest_header()
+    jobs = jobs or min(len(SOURCES), max(1, (os.cpu_count() or 4)))
+    t0 = time.time()
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = [ex.submit(_compile, s, force, hdr_time) for s in SOURCES]
+        for f in cf.as_completed(futs):
+            src, dt, out = f.result()
+            if verbose and dt > 0:
+                print(f"  compiled {src} in {dt:.1f}s", flush=True)
+                if out.strip():
+                    print(out, flush=True)
+    objs = [os.path.join(OBJ, s + ".o") for s in SOURCES]
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB} in {time.time() - t0:.1f}s", flush=True)
+    return LIB
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--force", action="store_true")
+    ap.add_argument("-j", "--jobs", type=int, default=None)
+    a = ap.parse_args()
+    try:
+        build(force=a.force, jobs=a.jobs)
+    except RuntimeError as e:
+        print(e, file=sys.stderr)
+        sys.exit(1)

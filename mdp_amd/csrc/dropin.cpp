@@ -1,0 +1,682 @@
+// The 23 drop-in entry points of src/Julia/Potential/cpuSnap.h on the GPU.
+//
+// Every call: validate the request, stage the host inputs (and, for outputs upstream accumulates
+// into, the current output contents) to device buffers, run the HIP kernels on the library stream,
+// copy the results back and synchronise.  Array extents the upstream signature leaves implicit are
+// derived from the index arrays the upstream code itself dereferences.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "../../include/mdpsnap.h"
+#include "runtime.hpp"
+
+using namespace mdp;
+
+namespace {
+
+// Per-argument device buffers, reused across calls (guarded by the library mutex).
+DevBuf g_slot[40];
+DevTables g_tab;
+
+template <class T>
+T* dev(int slot, size_t n) {
+    return static_cast<T*>(g_slot[slot].ensure(sizeof(T) * std::max<size_t>(n, 1)));
+}
+template <class T>
+T* put(int slot, const T* h, size_t n, hipStream_t st) {
+    T* d = dev<T>(slot, n);
+    if (n) MDP_HIP(hipMemcpyAsync(d, h, sizeof(T) * n, hipMemcpyHostToDevice, st));
+    return d;
+}
+template <class T>
+void get(T* h, const T* d, size_t n, hipStream_t st) {
+    if (n) MDP_HIP(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, st));
+}
+void finish(hipStream_t st) {
+    MDP_HIP(hipGetLastError());
+    MDP_HIP(hipStreamSynchronize(st));
+}
+
+int max_of(const int* a, long n, int floor_v = -1) {
+    int m = floor_v;
+    for (long i = 0; i < n; ++i) m = std::max(m, a[i]);
+    return m;
+}
+
+void require_j(int J) {
+    if (!templated_j(J))
+        throw Error("twojmax=" + std::to_string(J) + " is not supported by this build (supported: 2, 4, 6, 8)");
+}
+
+const SnapIndex& index_for(int J) {
+    static SnapIndex cache[9];
+    static bool built[9] = {false};
+    require_j(J);
+    if (!built[J]) {
+        cache[J].build(J, nullptr);
+        built[J] = true;
+    }
+    return cache[J];
+}
+
+// Stage tables: canonical index lists for J plus the caller's numeric tables.
+const SnapDev& tables(int J, const double* rootpq, const double* cg, const double* radelem, const double* wjelem,
+                      const int* map, int ntypes, hipStream_t st) {
+    g_tab.upload(index_for(J), rootpq, cg, radelem, wjelem, map, ntypes, st);
+    return g_tab.view;
+}
+
+template <class F>
+void guarded(F&& f) {
+    std::lock_guard<std::mutex> lk(library_mutex());
+    clear_error();
+    try {
+        f();
+    } catch (const std::exception& e) {
+        set_error(e.what());
+    }
+}
+
+// CSR of pairs by centre atom (stable): off[inum+1]; perm empty when already sorted.
+void pairs_by_atom(const int* aii, int ijnum, int inum, std::vector<int>& off, std::vector<int>& perm) {
+    off.assign(inum + 1, 0);
+    bool sorted = true;
+    for (int p = 0; p < ijnum; ++p) {
+        if (aii[p] < 0 || aii[p] >= inum) throw Error("pair centre index out of range [0, inum)");
+        off[aii[p] + 1]++;
+        if (p && aii[p] < aii[p - 1]) sorted = false;
+    }
+    std::partial_sum(off.begin(), off.end(), off.begin());
+    perm.clear();
+    if (!sorted) {
+        perm.resize(ijnum);
+        std::vector<int> pos(off.begin(), off.end() - 1);
+        for (int p = 0; p < ijnum; ++p) perm[pos[aii[p]]++] = p;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mdp_snap_last_error(void) { return get_error(); }
+
+int idxcgcount(int twojmax) { return count_cg(twojmax); }
+int idxucount(int twojmax) { return count_u(twojmax); }
+int idxbcount(int twojmax) { return count_b(twojmax); }
+int idxzcount(int twojmax) { return count_z(twojmax); }
+
+void cpuInitSna(double* rootpqarray, double* cglist, double* factorial, int* idx_max, int* idxz, int* idxz_block,
+                int* idxb, int* idxb_block, int* idxu_block, int* idxcg_block, int twojmax) {
+    guarded([&] {
+        SnapIndex ix;
+        ix.build(twojmax, factorial);
+        const int jd = twojmax + 1;
+        idx_max[0] = ix.idxcg_max;
+        idx_max[1] = ix.idxu_max;
+        idx_max[2] = ix.idxb_max;
+        idx_max[3] = ix.idxz_max;
+        std::copy(ix.idxz.begin(), ix.idxz.end(), idxz);
+        std::copy(ix.idxb.begin(), ix.idxb.end(), idxb);
+        std::copy(ix.idxu_block.begin(), ix.idxu_block.end(), idxu_block);
+        for (int k = 0; k < jd * jd * jd; ++k) {
+            idxz_block[k] = ix.idxz_block[k];
+            idxb_block[k] = ix.idxb_block[k];
+            idxcg_block[k] = ix.idxcg_block[k];
+        }
+        for (int p = 1; p <= twojmax; ++p)
+            for (int q = 1; q <= twojmax; ++q) rootpqarray[p * jd + q] = ix.rootpq[p * jd + q];
+        std::copy(ix.cglist.begin(), ix.cglist.end(), cglist);
+    });
+}
+
+void cpuNeighPairList(int* pairnum, int* pairlist, double* x, double* rcutsq, int* ilist, int* neighlist, int* neighnum,
+                      int* atomtype, int* alist, int inum, int dim, int ntypes) {
+    guarded([&] {
+        if (inum <= 0) return;
+        hipStream_t st = library_stream();
+        const int imax = max_of(ilist, inum);
+        const int nn = neighnum[imax + 1];
+        const int npts = std::max(max_of(neighlist, nn), imax) + 1;
+        const int natoms = std::max(max_of(alist, npts), imax) + 1;
+        int* d_il = put(0, ilist, inum, st);
+        int* d_nl = put(1, neighlist, nn, st);
+        int* d_nn = put(2, neighnum, imax + 2, st);
+        int* d_at = put(3, atomtype, natoms, st);
+        int* d_al = put(4, alist, npts, st);
+        double* d_x = put(5, x, static_cast<size_t>(npts) * dim, st);
+        double* d_rc = put(6, rcutsq, static_cast<size_t>(ntypes + 1) * (ntypes + 1), st);
+        int* d_cnt = dev<int>(7, inum);
+        int* d_off = dev<int>(8, inum + 1);
+        int* d_tmp = dev<int>(9, scan_tmp_ints(inum));
+        launch_pair_count(d_cnt, d_x, d_rc, d_il, d_nl, d_nn, d_at, d_al, inum, dim, ntypes, st);
+        exclusive_scan(d_cnt, d_off, inum, d_tmp, st);
+        int total = 0;
+        get(&total, d_off + inum, 1, st);
+        MDP_HIP(hipStreamSynchronize(st));
+        int* d_pl = dev<int>(10, total);
+        launch_pair_fill(d_pl, d_off, d_x, d_rc, d_il, d_nl, d_nn, d_at, d_al, inum, dim, ntypes, st);
+        get(pairnum, d_cnt, inum, st);
+        get(pairlist, d_pl, total, st);
+        finish(st);
+    });
+}
+
+void cpuNeighPairs(double* xij, double* x, int* aii, int* ai, int* aj, int* ti, int* tj, int* pairlist, int* pairnumsum,
+                   int* ilist, int* atomtype, int* alist, int inum, int dim) {
+    guarded([&] {
+        if (inum <= 0) return;
+        hipStream_t st = library_stream();
+        const int total = pairnumsum[inum];
+        const int imax = max_of(ilist, inum);
+        const int npts = std::max(max_of(pairlist, total), imax) + 1;
+        const int natoms = std::max(max_of(alist, npts), imax) + 1;
+        int* d_pl = put(0, pairlist, total, st);
+        int* d_ps = put(1, pairnumsum, inum + 1, st);
+        int* d_il = put(2, ilist, inum, st);
+        int* d_at = put(3, atomtype, natoms, st);
+        int* d_al = put(4, alist, npts, st);
+        double* d_x = put(5, x, static_cast<size_t>(npts) * dim, st);
+        double* d_xij = dev<double>(6, static_cast<size_t>(total) * dim);
+        int* d_aii = dev<int>(7, total);
+        int* d_ai = dev<int>(8, total);
+        int* d_aj = dev<int>(9, total);
+        int* d_ti = dev<int>(10, total);
+        int* d_tj = dev<int>(11, total);
+        launch_pair_arrays(d_xij, d_x, d_aii, d_ai, d_aj, d_ti, d_tj, d_pl, d_ps, d_il, d_at, d_al, inum, dim, st);
+        get(xij, d_xij, static_cast<size_t>(total) * dim, st);
+        get(aii, d_aii, total, st);
+        get(ai, d_ai, total, st);
+        get(aj, d_aj, total, st);
+        get(ti, d_ti, total, st);
+        get(tj, d_tj, total, st);
+        finish(st);
+    });
+}
+
+void cpuSnapComputeUi(double* Utotr, double* Utoti, double* rootpqarray, double* rij, double* wjelem, double* radelem,
+                      double rmin0, double rfac0, double rcutfac, int* map, int* aii, int* ti, int* tj, int twojmax,
+                      int idxu_max, int inum, int ijnum, int switchflag, int chemflag) {
+    guarded([&] {
+        if (inum <= 0) return;
+        hipStream_t st = library_stream();
+        const int ntypes = std::max(max_of(ti, ijnum, 1), max_of(tj, ijnum, 1));
+        const int nelem = chemflag ? max_of(map + 1, ntypes, 0) + 1 : 1;
+        SnapDev s = tables(twojmax, rootpqarray, nullptr, radelem, wjelem, map, ntypes, st);
+        if (idxu_max != s.idxu_max) throw Error("idxu_max does not match twojmax");
+        s.rcutfac = rcutfac;
+        s.rfac0 = rfac0;
+        s.rmin0 = rmin0;
+        s.switchflag = switchflag;
+        std::vector<int> off, perm;
+        pairs_by_atom(aii, ijnum, inum, off, perm);
+        const size_t nu = static_cast<size_t>(inum) * idxu_max * nelem;
+        double* d_ur = put(0, Utotr, nu, st);
+        double* d_ui = put(1, Utoti, nu, st);
+        double* d_rij = put(2, rij, 3 * static_cast<size_t>(ijnum), st);
+        int* d_ti = put(3, ti, ijnum, st);
+        int* d_tj = put(4, tj, ijnum, st);
+        int* d_off = put(5, off.data(), off.size(), st);
+        int* d_perm = perm.empty() ? nullptr : put(6, perm.data(), perm.size(), st);
+        if (!chemflag) {
+            launch_ui(s, UI_ACCUMULATE, d_ur, d_ui, d_rij, d_off, d_perm, d_ti, d_tj, inum, 0.0, -1, st);
+        } else {
+            for (int e = 0; e < nelem; ++e) {
+                const size_t o = static_cast<size_t>(inum) * idxu_max * e;
+                launch_ui(s, UI_ACCUMULATE, d_ur + o, d_ui + o, d_rij, d_off, d_perm, d_ti, d_tj, inum, 0.0, e, st);
+            }
+        }
+        get(Utotr, d_ur, nu, st);
+        get(Utoti, d_ui, nu, st);
+        finish(st);
+    });
+}
+
+void cpuAddWself2Ui(double* Utotr, double* Utoti, double wself, int* idxu_block, int* type, int* map, int wselfall_flag,
+                    int chemflag, int idxu_max, int nelements, int twojmax, int inum) {
+    (void)idxu_block;
+    guarded([&] {
+        if (inum <= 0) return;
+        hipStream_t st = library_stream();
+        const int ntypes = max_of(type, inum, 1);
+        SnapDev s = tables(twojmax, nullptr, nullptr, nullptr, nullptr, map, ntypes, st);
+        if (idxu_max != s.idxu_max) throw Error("idxu_max does not match twojmax");
+        const size_t nu = static_cast<size_t>(inum) * idxu_max * nelements;
+        double* d_ur = put(0, Utotr, nu, st);
+        double* d_ui = put(1, Utoti, nu, st);
+        int* d_t = put(2, type, inum, st);
+        launch_add_wself(s, d_ur, d_ui, wself, d_t, wselfall_flag, chemflag, nelements, inum, st);
+        get(Utotr, d_ur, nu, st);
+        get(Utoti, d_ui, nu, st);
+        finish(st);
+    });
+}
+
+namespace {
+// blist for all element triples (layout ii + N*jjb + N*idxb_max*jelem), before bzero
+void all_triples_b(const SnapDev& s, const double* d_ur, const double* d_ui, double* d_b, int nelem, int inum, int bnorm,
+                   hipStream_t st) {
+    const long nu = static_cast<long>(inum) * s.idxu_max;
+    for (int e1 = 0; e1 < nelem; ++e1)
+        for (int e2 = 0; e2 < nelem; ++e2)
+            for (int e3 = 0; e3 < nelem; ++e3) {
+                const long je = (e1 * nelem + e2) * nelem + e3;
+                launch_bi(s, d_ur, d_ui, nu * e1, nu * e2, nu * e3, d_b + je * inum * s.idxb_max, inum, bnorm, st);
+            }
+}
+}  // namespace
+
+void cpuSnapComputeEi(double* eatom, double* Utotr, double* Utoti, double* cglist, double* bzero, double* coeffelem,
+                      int* ilist, int* map, int* type, int* idxb, int* idxcg_block, int* idxu_block, int twojmax,
+                      int idxb_max, int idxu_max, int nelements, int ncoeffall, int bnorm_flag, int bzero_flag,
+                      int wselfall_flag, int chemflag, int inum) {
+    (void)idxb; (void)idxcg_block; (void)idxu_block;
+    guarded([&] {
+        if (inum <= 0) return;
+        hipStream_t st = library_stream();
+        const int ntypes = max_of(type, inum, 1);
+        SnapDev s = tables(twojmax, nullptr, cglist, nullptr, nullptr, map, ntypes, st);
+        if (idxu_max != s.idxu_max || idxb_max != s.idxb_max) throw Error("table sizes do not match twojmax");
+        const int ne = nelements;
+        const size_t nu = static_cast<size_t>(inum) * idxu_max * ne;
+        const int ncoef_rows = chemflag ? max_of(map + 1, ntypes, 0) + 1 : 1;
+        const int eext = max_of(ilist, inum) + 1;
+        double* d_ur = put(0, Utotr, nu, st);
+        double* d_ui = put(1, Utoti, nu, st);
+        double* d_bz = put(2, bzero, twojmax + 1, st);
+        double* d_c = put(3, coeffelem, static_cast<size_t>(ncoeffall) * ncoef_rows, st);
+        int* d_il = put(4, ilist, inum, st);
+        int* d_t = put(5, type, inum, st);
+        double* d_e = put(6, eatom, eext, st);
+        double* d_b = dev<double>(7, static_cast<size_t>(inum) * idxb_max * ne * ne * ne);
+        all_triples_b(s, d_ur, d_ui, d_b, ne, inum, bnorm_flag, st);
+        if (bzero_flag) launch_bzero(s, d_b, d_bz, d_il, d_t, wselfall_flag, chemflag, ne, inum, 0, st);
+        launch_ei_from_b(s, d_e, d_b, d_c, d_il, d_t, chemflag, ne, ncoeffall, inum, st);
+        get(eatom, d_e, eext, st);
+        finish(st);
+    });
+}
+
+void cpuSnapComputeBi(double* blist, double* Utotr, double* Utoti, double* cglist, double* bzero, int* idxb,
+                      int* idxcg_block, int* idxu_block, int* idxz_block, int twojmax, int idxb_max, int idxu_max,
+                      int idxz_max, int nelements, int bnorm_flag, int bzero_flag, int wselfall_flag, int inum) {
+    (void)idxb; (void)idxcg_block; (void)idxu_block; (void)idxz_block; (void)idxz_max;
+    guarded([&] {
+        if (inum <= 0) return;
+        hipStream_t st = library_stream();
+        SnapDev s = tables(twojmax, nullptr, cglist, nullptr, nullptr, nullptr, 1, st);
+        if (idxu_max != s.idxu_max || idxb_max != s.idxb_max) throw Error("table sizes do not match twojmax");
+        const int ne = nelements;
+        const size_t nu = static_cast<size_t>(inum) * idxu_max * ne;
+        const size_t nb = static_cast<size_t>(inum) * idxb_max * ne * ne * ne;
+        double* d_ur = put(0, Utotr, nu, st);
+        double* d_ui = put(1, Utoti, nu, st);
+        double* d_bz = put(2, bzero, twojmax + 1, st);
+        double* d_b = dev<double>(3, nb);
+        all_triples_b(s, d_ur, d_ui, d_b, ne, inum, bnorm_flag, st);
+        if (bzero_flag) launch_bzero(s, d_b, d_bz, nullptr, nullptr, wselfall_flag, 0, ne, inum, 0, st);
+        get(blist, d_b, nb, st);
+        finish(st);
+    });
+}
+
+void cpuSnapComputeYi(double* ylist_r, double* ylist_i, double* Utotr, double* Utoti, double* cglist, double* beta,
+                      int* map, int* type, int* idxz, int* idxb_block, int* idxu_block, int* idxcg_block, int twojmax,
+                      int idxb_max, int idxu_max, int idxz_max, int nelements, int ncoeffall, int bnorm_flag,
+                      int chemflag, int inum) {
+    (void)idxz; (void)idxb_block; (void)idxu_block; (void)idxcg_block; (void)idxz_max;
+    guarded([&] {
+        if (inum <= 0) return;
+        hipStream_t st = library_stream();
+        const int ntypes = max_of(type, inum, 1);
+        SnapDev s = tables(twojmax, nullptr, cglist, nullptr, nullptr, map, ntypes, st);
+        if (idxu_max != s.idxu_max || idxb_max != s.idxb_max) throw Error("table sizes do not match twojmax");
+        const SnapIndex& ix = index_for(twojmax);
+        const int ne = nelements;
+        const size_t nu = static_cast<size_t>(inum) * idxu_max * ne;
+        double* d_ur = put(0, Utotr, nu, st);
+        double* d_ui = put(1, Utoti, nu, st);
+        double* d_yr = dev<double>(2, nu);
+        double* d_yi = dev<double>(3, nu);
+        MDP_HIP(hipMemsetAsync(d_yr, 0, sizeof(double) * nu, st));
+        MDP_HIP(hipMemsetAsync(d_yi, 0, sizeof(double) * nu, st));
+        if (ne == 1 && !chemflag) {
+            std::vector<double> bt = y_beta_table(ix, beta, 0, 0, 0, 1, 0, ncoeffall, bnorm_flag);
+            double* d_bt = put(4, bt.data(), bt.size(), st);
+            launch_yi(s, d_ur, d_ui, d_bt, d_yr, d_yi, nullptr, inum, true, st);
+            MDP_HIP(hipStreamSynchronize(st));
+        } else {
+            std::vector<int> ae(inum);
+            for (int i = 0; i < inum; ++i) ae[i] = chemflag ? map[type[i]] : 0;
+            int* d_ae = put(5, ae.data(), ae.size(), st);
+            const int nie = chemflag ? max_of(ae.data(), inum, 0) + 1 : 1;
+            const long nu1 = static_cast<long>(inum) * idxu_max;
+            std::vector<std::vector<double>> keep;
+            int slot = 6;
+            for (int ie = 0; ie < nie; ++ie)
+                for (int e1 = 0; e1 < ne; ++e1)
+                    for (int e2 = 0; e2 < ne; ++e2)
+                        for (int e3 = 0; e3 < ne; ++e3) {
+                            keep.push_back(y_beta_table(ix, beta, e1, e2, e3, ne, ie, ncoeffall, bnorm_flag));
+                            if (slot >= 40) throw Error("too many element combinations for the staging slots");
+                            double* d_bt = put(slot++, keep.back().data(), keep.back().size(), st);
+                            launch_yi_acc(s, d_ur, d_ui, nu1 * e1, nu1 * e2, nu1 * e3, d_bt, d_yr, d_yi, d_ae,
+                                          chemflag ? ie : -1, inum, st);
+                        }
+            MDP_HIP(hipStreamSynchronize(st));
+        }
+        get(ylist_r, d_yr, nu, st);
+        get(ylist_i, d_yi, nu, st);
+        finish(st);
+    });
+}
+
+void cpuSnapComputeFi(double* fatom, double* vatom, double* ylist_r, double* ylist_i, double* rootpqarray, double* rij,
+                      double* wjelem, double* radelem, double rmin0, double rfac0, double rcutfac, int* map, int* aii,
+                      int* ai, int* aj, int* ti, int* tj, int twojmax, int idxu_max, int inum, int anum, int ijnum,
+                      int switchflag, int chemflag) {
+    (void)anum;
+    guarded([&] {
+        if (ijnum <= 0) return;
+        hipStream_t st = library_stream();
+        const int ntypes = std::max(max_of(ti, ijnum, 1), max_of(tj, ijnum, 1));
+        const int nelem = chemflag ? max_of(map + 1, ntypes, 0) + 1 : 1;
+        SnapDev s = tables(twojmax, rootpqarray, nullptr, radelem, wjelem, map, ntypes, st);
+        if (idxu_max != s.idxu_max) throw Error("idxu_max does not match twojmax");
+        s.rcutfac = rcutfac;
+        s.rfac0 = rfac0;
+        s.rmin0 = rmin0;
+        s.switchflag = switchflag;
+        const size_t nu = static_cast<size_t>(inum) * idxu_max * nelem;
+        const int fext = std::max(max_of(ai, ijnum), max_of(aj, ijnum)) + 1;
+        const int vext = max_of(ai, ijnum) + 1;
+        std::vector<long> yb;
+        if (chemflag) {
+            yb.resize(ijnum);
+            for (int p = 0; p < ijnum; ++p) yb[p] = static_cast<long>(inum) * idxu_max * map[tj[p]];
+        }
+        double* d_yr = put(0, ylist_r, nu, st);
+        double* d_yi = put(1, ylist_i, nu, st);
+        double* d_rij = put(2, rij, 3 * static_cast<size_t>(ijnum), st);
+        int* d_aii = put(3, aii, ijnum, st);
+        int* d_ai = put(4, ai, ijnum, st);
+        int* d_aj = put(5, aj, ijnum, st);
+        int* d_ti = put(6, ti, ijnum, st);
+        int* d_tj = put(7, tj, ijnum, st);
+        long* d_yb = chemflag ? put(8, yb.data(), yb.size(), st) : nullptr;
+        double* d_de = dev<double>(9, 3 * static_cast<size_t>(ijnum));
+        double* d_f = put(10, fatom, 3 * static_cast<size_t>(fext), st);
+        double* d_v = put(11, vatom, 6 * static_cast<size_t>(vext), st);
+        launch_dedr(s, d_yr, d_yi, d_rij, d_aii, d_ti, d_tj, d_yb, d_de, inum, ijnum, nullptr, st);
+        launch_tally_force_atomic(d_f, d_v, d_de, d_rij, d_ai, d_aj, ijnum, st);
+        get(fatom, d_f, 3 * static_cast<size_t>(fext), st);
+        get(vatom, d_v, 6 * static_cast<size_t>(vext), st);
+        finish(st);
+    });
+}
+
+namespace {
+void uij_common(double* Sr, double* Si, double* dr[3], double* di[3], double* rootpqarray, double* rij, double* wjelem,
+                double* radelem, double rmin0, double rfac0, double rcutfac, int* ti, int* tj, int twojmax, int idxu_max,
+                int ijnum, int switch_flag) {
+    if (ijnum <= 0) return;
+    hipStream_t st = library_stream();
+    const int ntypes = std::max(max_of(ti, ijnum, 1), max_of(tj, ijnum, 1));
+    SnapDev s = tables(twojmax, rootpqarray, nullptr, radelem, wjelem, nullptr, ntypes, st);
+    if (idxu_max != s.idxu_max) throw Error("idxu_max does not match twojmax");
+    s.rcutfac = rcutfac;
+    s.rfac0 = rfac0;
+    s.rmin0 = rmin0;
+    s.switchflag = switch_flag;
+    const size_t n = static_cast<size_t>(idxu_max) * ijnum;
+    double* d_sr = dev<double>(0, n);
+    double* d_si = dev<double>(1, n);
+    double* ddr[3];
+    double* ddi[3];
+    for (int k = 0; k < 3; ++k) {   // upstream leaves some derivative rows untouched: stage them
+        ddr[k] = put(2 + k, dr[k], n, st);
+        ddi[k] = put(5 + k, di[k], n, st);
+    }
+    double* d_rij = put(8, rij, 3 * static_cast<size_t>(ijnum), st);
+    int* d_ti = put(9, ti, ijnum, st);
+    int* d_tj = put(10, tj, ijnum, st);
+    launch_uij(s, d_sr, d_si, ddr, ddi, d_rij, d_ti, d_tj, ijnum, st);
+    get(Sr, d_sr, n, st);
+    get(Si, d_si, n, st);
+    for (int k = 0; k < 3; ++k) {
+        get(dr[k], ddr[k], n, st);
+        get(di[k], ddi[k], n, st);
+    }
+    finish(st);
+}
+}  // namespace
+
+void cpuComputeSij(double* Sr, double* Si, double* Srx, double* Six, double* Sry, double* Siy, double* Srz, double* Siz,
+                   double* rootpqarray, double* rij, double* wjelem, double* radelem, double rmin0, double rfac0,
+                   double rcutfac, int* idxu_block, int* ti, int* tj, int twojmax, int idxu_max, int ijnum,
+                   int switch_flag) {
+    (void)idxu_block;
+    guarded([&] {
+        double* dr[3] = {Srx, Sry, Srz};
+        double* di[3] = {Six, Siy, Siz};
+        uij_common(Sr, Si, dr, di, rootpqarray, rij, wjelem, radelem, rmin0, rfac0, rcutfac, ti, tj, twojmax, idxu_max,
+                   ijnum, switch_flag);
+    });
+}
+
+void cpuComputeUij(double* Sr, double* Si, double* dSr, double* dSi, double* rootpqarray, double* rij, double* wjelem,
+                   double* radelem, double rmin0, double rfac0, double rcutfac, int* idxu_block, int* ti, int* tj,
+                   int twojmax, int idxu_max, int ijnum, int switch_flag) {
+    (void)idxu_block;
+    guarded([&] {
+        const size_t n = static_cast<size_t>(idxu_max) * ijnum;
+        double* dr[3] = {dSr, dSr + n, dSr + 2 * n};
+        double* di[3] = {dSi, dSi + n, dSi + 2 * n};
+        uij_common(Sr, Si, dr, di, rootpqarray, rij, wjelem, radelem, rmin0, rfac0, rcutfac, ti, tj, twojmax, idxu_max,
+                   ijnum, switch_flag);
+    });
+}
+
+void cpuZeroUarraytot2(double* Stotr, double* Stoti, double wself, int* idxu_block, int* type, int* map, int* ai,
+                       int wselfall_flag, int chemflag, int idxu_max, int nelements, int twojmax, int inum) {
+    (void)idxu_block;
+    guarded([&] {
+        if (inum <= 0) return;
+        hipStream_t st = library_stream();
+        int* d_t = nullptr;
+        int* d_ai = nullptr;
+        int ntypes = 1;
+        if (chemflag) {   // upstream reads ai[ii] and type[ai[ii]] only in this case
+            const int text = max_of(ai, inum) + 1;
+            ntypes = max_of(type, text, 1);
+            d_ai = put(2, ai, inum, st);
+            d_t = put(3, type, text, st);
+        }
+        SnapDev s = tables(twojmax, nullptr, nullptr, nullptr, nullptr, map, ntypes, st);
+        if (idxu_max != s.idxu_max) throw Error("idxu_max does not match twojmax");
+        const size_t nu = static_cast<size_t>(inum) * idxu_max * nelements;
+        double* d_r = dev<double>(0, nu);
+        double* d_i = dev<double>(1, nu);
+        launch_zero_utot(s, d_r, d_i, wself, d_t, d_ai, wselfall_flag, chemflag, nelements, inum, st);
+        get(Stotr, d_r, nu, st);
+        get(Stoti, d_i, nu, st);
+        finish(st);
+    });
+}
+
+void cpuAddUarraytot(double* Stotr, double* Stoti, double* Sr, double* Si, int* map, int* ai, int* tj, int idxu_max,
+                     int inum, int ijnum, int chemflag) {
+    guarded([&] {
+        if (inum <= 0 || ijnum <= 0) return;
+        hipStream_t st = library_stream();
+        const int ntypes = chemflag ? max_of(tj, ijnum, 1) : 1;
+        const int nelem = chemflag ? max_of(map + 1, ntypes, 0) + 1 : 1;
+        int J = 0;
+        while (count_u(J) < idxu_max) ++J;
+        if (count_u(J) != idxu_max) throw Error("idxu_max is not a U index-space size");
+        SnapDev s = tables(J, nullptr, nullptr, nullptr, nullptr, chemflag ? map : nullptr, ntypes, st);
+        const size_t nu = static_cast<size_t>(inum) * idxu_max * nelem, np = static_cast<size_t>(ijnum) * idxu_max;
+        double* d_tr = put(0, Stotr, nu, st);
+        double* d_ti = put(1, Stoti, nu, st);
+        double* d_sr = put(2, Sr, np, st);
+        double* d_si = put(3, Si, np, st);
+        int* d_ai = put(4, ai, ijnum, st);
+        int* d_tj = put(5, tj, ijnum, st);
+        launch_add_utot(s, d_tr, d_ti, d_sr, d_si, d_ai, d_tj, chemflag, inum, ijnum, st);
+        get(Stotr, d_tr, nu, st);
+        get(Stoti, d_ti, nu, st);
+        finish(st);
+    });
+}
+
+void cpuComputeZi2(double* zlist_r, double* zlist_i, double* Stotr, double* Stoti, double* cglist, int* idxz,
+                   int* idxu_block, int* idxcg_block, int twojmax, int idxu_max, int idxz_max, int nelements,
+                   int bnorm_flag, int inum) {
+    (void)idxz; (void)idxu_block; (void)idxcg_block;
+    guarded([&] {
+        if (inum <= 0) return;
+        hipStream_t st = library_stream();
+        SnapDev s = tables(twojmax, nullptr, cglist, nullptr, nullptr, nullptr, 1, st);
+        if (idxu_max != s.idxu_max || idxz_max != s.idxz_max) throw Error("table sizes do not match twojmax");
+        const int ne = nelements;
+        const size_t nu = static_cast<size_t>(inum) * idxu_max * ne, nz = static_cast<size_t>(inum) * idxz_max * ne * ne;
+        double* d_ur = put(0, Stotr, nu, st);
+        double* d_ui = put(1, Stoti, nu, st);
+        double* d_zr = dev<double>(2, nz);
+        double* d_zi = dev<double>(3, nz);
+        const long nu1 = static_cast<long>(inum) * idxu_max, nz1 = static_cast<long>(inum) * idxz_max;
+        for (int e1 = 0; e1 < ne; ++e1)
+            for (int e2 = 0; e2 < ne; ++e2) {
+                const long o = nz1 * (e1 * ne + e2);
+                launch_zi(s, d_ur, d_ui, nu1 * e1, nu1 * e2, d_zr + o, d_zi + o, inum, bnorm_flag, st);
+            }
+        get(zlist_r, d_zr, nz, st);
+        get(zlist_i, d_zi, nz, st);
+        finish(st);
+    });
+}
+
+void cpuComputeBi2(double* blist, double* zlist_r, double* zlist_i, double* Stotr, double* Stoti, double* bzero,
+                   int* ilist, int* type, int* map, int* idxb, int* idxu_block, int* idxz_block, int twojmax,
+                   int idxb_max, int idxu_max, int idxz_max, int nelements, int bzero_flag, int wselfall_flag,
+                   int chemflag, int inum) {
+    (void)idxb; (void)idxu_block; (void)idxz_block;
+    guarded([&] {
+        if (inum <= 0) return;
+        hipStream_t st = library_stream();
+        const int text = max_of(ilist, inum) + 1;
+        const int ntypes = max_of(type, text, 1);
+        SnapDev s = tables(twojmax, nullptr, nullptr, nullptr, nullptr, map, ntypes, st);
+        if (idxu_max != s.idxu_max || idxz_max != s.idxz_max || idxb_max != s.idxb_max)
+            throw Error("table sizes do not match twojmax");
+        const int ne = nelements;
+        const size_t nu = static_cast<size_t>(inum) * idxu_max * ne, nz = static_cast<size_t>(inum) * idxz_max * ne * ne;
+        const size_t nb = static_cast<size_t>(inum) * idxb_max * ne * ne * ne;
+        double* d_zr = put(0, zlist_r, nz, st);
+        double* d_zi = put(1, zlist_i, nz, st);
+        double* d_ur = put(2, Stotr, nu, st);
+        double* d_ui = put(3, Stoti, nu, st);
+        double* d_bz = put(4, bzero, twojmax + 1, st);
+        int* d_il = put(5, ilist, inum, st);
+        int* d_t = put(6, type, text, st);
+        double* d_b = dev<double>(7, nb);
+        launch_bi2(s, d_b, d_zr, d_zi, d_ur, d_ui, d_bz, d_il, d_t, bzero_flag, wselfall_flag, chemflag, ne, inum, st);
+        get(blist, d_b, nb, st);
+        finish(st);
+    });
+}
+
+void cpuComputeDbidrj(double* dblist, double* zlist_r, double* zlist_i, double* dulist_r, double* dulist_i, int* idxb,
+                      int* idxu_block, int* idxz_block, int* map, int* ai, int* tj, int twojmax, int idxb_max,
+                      int idxu_max, int idxz_max, int nelements, int bnorm_flag, int chemflag, int inum, int ijnum) {
+    (void)idxb; (void)idxu_block; (void)idxz_block;
+    guarded([&] {
+        if (inum <= 0 || ijnum <= 0) return;
+        hipStream_t st = library_stream();
+        const int ntypes = chemflag ? max_of(tj, ijnum, 1) : 1;
+        SnapDev s = tables(twojmax, nullptr, nullptr, nullptr, nullptr, chemflag ? map : nullptr, ntypes, st);
+        if (idxu_max != s.idxu_max || idxz_max != s.idxz_max || idxb_max != s.idxb_max)
+            throw Error("table sizes do not match twojmax");
+        const int ne = nelements;
+        const size_t nz = static_cast<size_t>(inum) * idxz_max * ne * ne;
+        const size_t ndu = 3 * static_cast<size_t>(ijnum) * idxu_max;
+        const size_t ndb = static_cast<size_t>(ijnum) * idxb_max * 3 * ne * ne * ne;
+        double* d_zr = put(0, zlist_r, nz, st);
+        double* d_zi = put(1, zlist_i, nz, st);
+        double* d_dur = put(2, dulist_r, ndu, st);
+        double* d_dui = put(3, dulist_i, ndu, st);
+        int* d_ai = put(4, ai, ijnum, st);
+        int* d_tj = put(5, tj, ijnum, st);
+        double* d_db = dev<double>(6, ndb);
+        MDP_HIP(hipMemsetAsync(d_db, 0, sizeof(double) * ndb, st));
+        launch_dbidrj(s, d_db, d_zr, d_zi, d_dur, d_dui, d_ai, d_tj, bnorm_flag, chemflag, ne, inum, ijnum, st);
+        get(dblist, d_db, ndb, st);
+        finish(st);
+    });
+}
+
+void cpuSnapTallyBispectrum(double* bi, double* bispectrum, int* ilist, int* type, int inum, int ncoeff, int nperdim,
+                            int ntype, int quadraticflag) {
+    guarded([&] {
+        hipStream_t st = library_stream();
+        const size_t nbi = static_cast<size_t>(nperdim) * ntype;
+        double* d_bi = dev<double>(0, nbi);
+        MDP_HIP(hipMemsetAsync(d_bi, 0, sizeof(double) * std::max<size_t>(nbi, 1), st));
+        if (inum > 0) {
+            const int text = max_of(ilist, inum) + 1;
+            double* d_b = put(1, bispectrum, static_cast<size_t>(inum) * ncoeff, st);
+            int* d_il = put(2, ilist, inum, st);
+            int* d_t = put(3, type, text, st);
+            launch_tally_bi(d_bi, d_b, d_il, d_t, inum, ncoeff, nperdim, quadraticflag, st);
+        }
+        get(bi, d_bi, nbi, st);
+        finish(st);
+    });
+}
+
+void cpuSnapTallyBispectrumDeriv(double* db, double* bispectrum, double* dbdr, int* aii, int* ai, int* aj, int* ti,
+                                 int inum, int ijnum, int ncoeff, int nperdim, int ntype, int quadraticflag) {
+    guarded([&] {
+        hipStream_t st = library_stream();
+        const size_t ndb = static_cast<size_t>(inum) * 3 * nperdim * ntype;
+        double* d_db = dev<double>(0, ndb);
+        MDP_HIP(hipMemsetAsync(d_db, 0, sizeof(double) * std::max<size_t>(ndb, 1), st));
+        if (ijnum > 0) {
+            double* d_b = put(1, bispectrum, static_cast<size_t>(inum) * ncoeff, st);
+            double* d_dbdr = put(2, dbdr, static_cast<size_t>(ijnum) * 3 * ncoeff, st);
+            int* d_aii = put(3, aii, ijnum, st);
+            int* d_ai = put(4, ai, ijnum, st);
+            int* d_aj = put(5, aj, ijnum, st);
+            int* d_ti = put(6, ti, ijnum, st);
+            launch_tally_bd(d_db, d_b, d_dbdr, d_aii, d_ai, d_aj, d_ti, inum, ijnum, ncoeff, nperdim, quadraticflag, st);
+        }
+        get(db, d_db, ndb, st);
+        finish(st);
+    });
+}
+
+void cpuSnapTallyBispectrumVirial(double* bv, double* bispectrum, double* dbdr, double* rij, int* aii, int* ti, int inum,
+                                  int ijnum, int ncoeff, int nperdim, int ntype, int quadraticflag) {
+    guarded([&] {
+        hipStream_t st = library_stream();
+        const size_t nbv = 6 * static_cast<size_t>(nperdim) * ntype;
+        double* d_bv = dev<double>(0, nbv);
+        MDP_HIP(hipMemsetAsync(d_bv, 0, sizeof(double) * std::max<size_t>(nbv, 1), st));
+        if (ijnum > 0) {
+            double* d_b = put(1, bispectrum, static_cast<size_t>(inum) * ncoeff, st);
+            double* d_dbdr = put(2, dbdr, static_cast<size_t>(ijnum) * 3 * ncoeff, st);
+            double* d_rij = put(3, rij, 3 * static_cast<size_t>(ijnum), st);
+            int* d_aii = put(4, aii, ijnum, st);
+            int* d_ti = put(5, ti, ijnum, st);
+            launch_tally_bv(d_bv, d_b, d_dbdr, d_rij, d_aii, d_ti, inum, ijnum, ncoeff, nperdim, quadraticflag, st);
+        }
+        get(bv, d_bv, nbv, st);
+        finish(st);
+    });
+}
+
+}  // extern "C"

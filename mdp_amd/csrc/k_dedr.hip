@@ -1,0 +1,77 @@
+// compute_duidrj + compute_deidrj for gfx950 (cpuSnap.cpp:991-1376 semantics, without the tally).
+//
+// Lane = pair.  Each lane runs the row-chain recursion of U and dU/dx,y,z of its pair entirely in
+// registers (compile-time J) and contracts every finished row with Y of the pair's centre atom on
+// the fly:  dE/dr_ij = 2 sum' (dsfac U u + sfac dU) . Y_i, with the j=0 term dsfac u Y_0 / 2 and the
+// middle column of even layers weighted 1/2.  The result is written per pair; the force/virial
+// tally is a separate deterministic gather.
+#include <hip/hip_runtime.h>
+
+#include "snap_device.hpp"
+#include "snap_launch.hpp"
+
+namespace mdp {
+
+template <int J>
+__global__ __launch_bounds__(64) void k_dedr(SnapDev s, const double* __restrict__ Yr, const double* __restrict__ Yi,
+                                             const double* __restrict__ rij, const int* __restrict__ yatom,
+                                             const int* __restrict__ ti, const int* __restrict__ tj,
+                                             const long* __restrict__ ybase, double* __restrict__ dedr, int N,
+                                             int npairs_host, const int* __restrict__ npairs_dev) {
+    const int p = blockIdx.x * 64 + threadIdx.x;
+    const int np = npairs_dev ? *npairs_dev : npairs_host;
+    if (p >= np) return;
+    const int tjp = tj[p];
+    const double rc = (s.radelem[ti[p]] + s.radelem[tjp]) * s.rcutfac;
+    PairGeom g;
+    pair_geometry<true>(rij[3 * p], rij[3 * p + 1], rij[3 * p + 2], rc, s.rmin0, s.rfac0, s.switchflag, s.wjelem[tjp], g);
+    const size_t n = static_cast<size_t>(N);
+    const size_t yb = static_cast<size_t>(yatom[p]) + (ybase ? static_cast<size_t>(ybase[p]) : 0);
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+    row_chain<J, 3, 0, J / 2>(g, s.rootpq, [&](auto Jc, auto MBc, auto& pr, auto& pi, auto& dpr, auto& dpi) {
+        constexpr int JJ = decltype(Jc)::value, MB = decltype(MBc)::value;
+        if constexpr (JJ == 0) {
+            const double y0 = 0.5 * Yr[yb];
+            d0 = g.dsfac * g.u[0] * y0;
+            d1 = g.dsfac * g.u[1] * y0;
+            d2 = g.dsfac * g.u[2] * y0;
+        } else {
+            constexpr int LAST = (JJ == 2 * MB) ? MB : JJ;
+            const size_t o = static_cast<size_t>(u_block(JJ) + (JJ + 1) * MB) * n + yb;
+#pragma unroll
+            for (int ma = 0; ma <= LAST; ++ma) {
+                double yr = Yr[o + ma * n], yi = Yi[o + ma * n];
+                if (JJ == 2 * MB && ma == MB) { yr *= 0.5; yi *= 0.5; }
+                const double ar = g.dsfac * pr[ma], ai = g.dsfac * pi[ma];
+                d0 += (ar * g.u[0] + g.sfac * dpr[0][ma]) * yr + (ai * g.u[0] + g.sfac * dpi[0][ma]) * yi;
+                d1 += (ar * g.u[1] + g.sfac * dpr[1][ma]) * yr + (ai * g.u[1] + g.sfac * dpi[1][ma]) * yi;
+                d2 += (ar * g.u[2] + g.sfac * dpr[2][ma]) * yr + (ai * g.u[2] + g.sfac * dpi[2][ma]) * yi;
+            }
+        }
+    });
+    dedr[3 * static_cast<size_t>(p)] = 2.0 * d0;
+    dedr[3 * static_cast<size_t>(p) + 1] = 2.0 * d1;
+    dedr[3 * static_cast<size_t>(p) + 2] = 2.0 * d2;
+}
+
+void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const double* rij, const int* yatom,
+                 const int* ti, const int* tj, const long* ybase, double* dedr, int N, int npairs_host,
+                 const int* npairs_dev, hipStream_t st) {
+    if (npairs_host <= 0) return;
+    dim3 grid((npairs_host + 63) / 64);
+#define DE_CASE(JV)                                                                                            \
+    case JV:                                                                                                   \
+        hipLaunchKernelGGL((k_dedr<JV>), grid, dim3(64), 0, st, s, Yr, Yi, rij, yatom, ti, tj, ybase, dedr, N, \
+                           npairs_host, npairs_dev);                                                           \
+        break;
+    switch (s.J) {
+        DE_CASE(2)
+        DE_CASE(4)
+        DE_CASE(6)
+        DE_CASE(8)
+        default: break;
+    }
+#undef DE_CASE
+}
+
+}  // namespace mdp

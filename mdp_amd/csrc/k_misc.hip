@@ -1,0 +1,660 @@
+// Device kernels of the descriptor path (cpuComputeSij/Uij, Zero/AddUarraytot, Bi2, Dbidrj), the
+// tallies, the pair-list builders, AddWself2Ui and a device-wide exclusive scan.  These follow the
+// upstream functions one output element (or one pair) per thread with runtime loops; the
+// performance-critical force path lives in k_ui/k_yi/k_dedr and the MD engine.
+#include <hip/hip_runtime.h>
+
+#include "snap_device.hpp"
+#include "snap_launch.hpp"
+
+namespace mdp {
+
+static inline int nblk(long n, int b) { return static_cast<int>((n + b - 1) / b); }
+
+__device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+// ================================================================================================
+// exclusive scan (three phases)
+// ================================================================================================
+
+constexpr int SCAN_BLOCK = 1024;
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_block(const int* __restrict__ in, int* __restrict__ out,
+                                                           int* __restrict__ bsum, int n) {
+    __shared__ int sh[SCAN_BLOCK];
+    const int i = blockIdx.x * SCAN_BLOCK + threadIdx.x;
+    const int v = (i < n) ? in[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < SCAN_BLOCK; d <<= 1) {
+        const int t = (threadIdx.x >= static_cast<unsigned>(d)) ? sh[threadIdx.x - d] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    if (i < n) out[i] = sh[threadIdx.x] - v;
+    if (threadIdx.x == SCAN_BLOCK - 1) bsum[blockIdx.x] = sh[threadIdx.x];
+}
+
+// exclusive scan of the block sums by one workgroup (chunks of 1024), total to *total
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_sums(int* __restrict__ bsum, int nb, int* __restrict__ total) {
+    __shared__ int sh[SCAN_BLOCK];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += SCAN_BLOCK) {
+        const int i = base + threadIdx.x;
+        const int v = (i < nb) ? bsum[i] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int d = 1; d < SCAN_BLOCK; d <<= 1) {
+            const int t = (threadIdx.x >= static_cast<unsigned>(d)) ? sh[threadIdx.x - d] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nb) bsum[i] = carry + sh[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == SCAN_BLOCK - 1) carry += sh[threadIdx.x];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_add(int* __restrict__ out, const int* __restrict__ bsum, int n) {
+    const int i = blockIdx.x * SCAN_BLOCK + threadIdx.x;
+    if (i < n) out[i] += bsum[blockIdx.x];
+}
+
+size_t scan_tmp_ints(int n) { return static_cast<size_t>((n + SCAN_BLOCK - 1) / SCAN_BLOCK) + 1; }
+
+void exclusive_scan(const int* in, int* out, int n, int* tmp, hipStream_t st) {
+    const int nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    if (nb > 0) hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(SCAN_BLOCK), 0, st, in, out, tmp, n);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, st, tmp, nb, out + n);
+    if (nb > 0) hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(SCAN_BLOCK), 0, st, out, tmp, n);
+}
+
+// ================================================================================================
+// per-pair U and dU (layer-by-layer recursion, cpuSnap.cpp:1412-1833)
+// ================================================================================================
+
+__global__ __launch_bounds__(128) void k_uij(SnapDev s, double* __restrict__ Sr, double* __restrict__ Si,
+                                             double* dr0, double* dr1, double* dr2, double* di0, double* di1,
+                                             double* di2, const double* __restrict__ rij, const int* __restrict__ ti,
+                                             const int* __restrict__ tj, int ijnum) {
+    const int ij = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ij >= ijnum) return;
+    const int J = s.J, jd = J + 1;
+    const double rc = (s.radelem[ti[ij]] + s.radelem[tj[ij]]) * s.rcutfac;
+    PairGeom g;
+    pair_geometry<true>(rij[3 * ij], rij[3 * ij + 1], rij[3 * ij + 2], rc, s.rmin0, s.rfac0, s.switchflag,
+                        s.wjelem[tj[ij]], g);
+    double* dR[3] = {dr0, dr1, dr2};
+    double* dI[3] = {di0, di1, di2};
+    const size_t P = static_cast<size_t>(ijnum);
+#define AT(arr, k) arr[ij + P * static_cast<size_t>(k)]
+    AT(Sr, 0) = 1.0;
+    AT(Si, 0) = 0.0;
+    for (int k = 0; k < 3; ++k) { AT(dR[k], 0) = 0.0; AT(dI[k], 0) = 0.0; }
+    for (int j = 1; j <= J; ++j) {
+        int jju = u_block(j), jjup = u_block(j - 1);
+        for (int mb = 0; 2 * mb <= j; ++mb) {
+            AT(Sr, jju) = 0.0;
+            AT(Si, jju) = 0.0;
+            for (int k = 0; k < 3; ++k) { AT(dR[k], jju) = 0.0; AT(dI[k], jju) = 0.0; }
+            for (int ma = 0; ma < j; ++ma) {
+                double rp = s.rootpq[(j - ma) * jd + (j - mb)];
+                const double ur = AT(Sr, jjup), ui = AT(Si, jjup);
+                double dur[3], dui[3];
+                for (int k = 0; k < 3; ++k) { dur[k] = AT(dR[k], jjup); dui[k] = AT(dI[k], jjup); }
+                AT(Sr, jju) += rp * (g.ar * ur + g.ai * ui);
+                AT(Si, jju) += rp * (g.ar * ui - g.ai * ur);
+                for (int k = 0; k < 3; ++k) {
+                    AT(dR[k], jju) += rp * (g.dar[k] * ur + g.dai[k] * ui + g.ar * dur[k] + g.ai * dui[k]);
+                    AT(dI[k], jju) += rp * (g.dar[k] * ui - g.dai[k] * ur + g.ar * dui[k] - g.ai * dur[k]);
+                }
+                rp = s.rootpq[(ma + 1) * jd + (j - mb)];
+                AT(Sr, jju + 1) = -rp * (g.br * ur + g.bi * ui);
+                AT(Si, jju + 1) = -rp * (g.br * ui - g.bi * ur);
+                for (int k = 0; k < 3; ++k) {
+                    AT(dR[k], jju + 1) = -rp * (g.dbr[k] * ur + g.dbi[k] * ui + g.br * dur[k] + g.bi * dui[k]);
+                    AT(dI[k], jju + 1) = -rp * (g.dbr[k] * ui - g.dbi[k] * ur + g.br * dui[k] - g.bi * dur[k]);
+                }
+                ++jju;
+                ++jjup;
+            }
+            ++jju;
+        }
+        int lo = u_block(j), hi = lo + (j + 1) * (j + 1) - 1;
+        for (int mb = 0; 2 * mb <= j; ++mb) {
+            const bool with_d = (j % 2 == 1 && mb == j / 2);
+            for (int ma = 0; ma <= j; ++ma, ++lo, --hi) {
+                const bool even = ((ma + mb) & 1) == 0;
+                const double vr = AT(Sr, lo), vi = AT(Si, lo);
+                AT(Sr, hi) = even ? vr : -vr;
+                AT(Si, hi) = even ? -vi : vi;
+                if (with_d)
+                    for (int k = 0; k < 3; ++k) {
+                        const double wr = AT(dR[k], lo), wi = AT(dI[k], lo);
+                        AT(dR[k], hi) = even ? wr : -wr;
+                        AT(dI[k], hi) = even ? -wi : wi;
+                    }
+            }
+        }
+    }
+    for (int j = 0; j <= J; ++j) {
+        int jju = u_block(j);
+        for (int mb = 0; 2 * mb <= j; ++mb)
+            for (int ma = 0; ma <= j; ++ma, ++jju) {
+                const double vr = AT(Sr, jju), vi = AT(Si, jju);
+                for (int k = 0; k < 3; ++k) {
+                    AT(dR[k], jju) = g.dsfac * vr * g.u[k] + g.sfac * AT(dR[k], jju);
+                    AT(dI[k], jju) = g.dsfac * vi * g.u[k] + g.sfac * AT(dI[k], jju);
+                }
+            }
+    }
+    for (int k = 0; k < s.idxu_max; ++k) {
+        AT(Sr, k) = g.sfac * AT(Sr, k);
+        AT(Si, k) = g.sfac * AT(Si, k);
+    }
+#undef AT
+}
+
+void launch_uij(const SnapDev& s, double* Sr, double* Si, double* const dSr[3], double* const dSi[3],
+                const double* rij, const int* ti, const int* tj, int ijnum, hipStream_t st) {
+    if (ijnum <= 0) return;
+    hipLaunchKernelGGL(k_uij, dim3(nblk(ijnum, 128)), dim3(128), 0, st, s, Sr, Si, dSr[0], dSr[1], dSr[2], dSi[0],
+                       dSi[1], dSi[2], rij, ti, tj, ijnum);
+}
+
+// ================================================================================================
+// Utot assembly
+// ================================================================================================
+
+__global__ void k_zero_utot(SnapDev s, double* __restrict__ Sr, double* __restrict__ Si, double wself,
+                            const int* __restrict__ type, const int* __restrict__ ai, int wselfall, int chemflag,
+                            int nelem, int N) {
+    const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ii >= N) return;
+    const int ielem = chemflag ? s.map[type[ai[ii]]] : 0;
+    const size_t n = static_cast<size_t>(N);
+    for (int je = 0; je < nelem; ++je) {
+        const size_t base = ii + n * s.idxu_max * je;
+        int jju = 0;
+        for (int j = 0; j <= s.J; ++j)
+            for (int mb = 0; mb <= j; ++mb)
+                for (int ma = 0; ma <= j; ++ma, ++jju) {
+                    Sr[base + n * jju] = (ma == mb && (je == ielem || wselfall)) ? wself : 0.0;
+                    Si[base + n * jju] = 0.0;
+                }
+    }
+}
+
+void launch_zero_utot(const SnapDev& s, double* Sr, double* Si, double wself, const int* type, const int* ai,
+                      int wselfall, int chemflag, int nelem, int N, hipStream_t st) {
+    if (N <= 0) return;
+    hipLaunchKernelGGL(k_zero_utot, dim3(nblk(N, 128)), dim3(128), 0, st, s, Sr, Si, wself, type, ai, wselfall,
+                       chemflag, nelem, N);
+}
+
+__global__ void k_add_utot(SnapDev s, double* __restrict__ Tr, double* __restrict__ Ti, const double* __restrict__ Sr,
+                           const double* __restrict__ Si, const int* __restrict__ ai, const int* __restrict__ tj,
+                           int chemflag, int N, int ijnum) {
+    const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= static_cast<long>(ijnum) * s.idxu_max) return;
+    const int ij = static_cast<int>(idx % ijnum);
+    const long jju = idx / ijnum;
+    const size_t n = static_cast<size_t>(N);
+    const size_t d = ai[ij] + n * jju + (chemflag ? n * s.idxu_max * s.map[tj[ij]] : 0);
+    atomic_add(&Tr[d], Sr[idx]);
+    atomic_add(&Ti[d], Si[idx]);
+}
+
+void launch_add_utot(const SnapDev& s, double* Tr, double* Ti, const double* Sr, const double* Si, const int* ai,
+                     const int* tj, int chemflag, int N, int ijnum, hipStream_t st) {
+    const long tot = static_cast<long>(ijnum) * s.idxu_max;
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(k_add_utot, dim3(nblk(tot, 256)), dim3(256), 0, st, s, Tr, Ti, Sr, Si, ai, tj, chemflag, N, ijnum);
+}
+
+// AddWself2Ui (cpuSnap.cpp:492-542): one thread per (atom, element block)
+__global__ void k_add_wself(SnapDev s, double* __restrict__ Ur, double* __restrict__ Ui, double wself,
+                            const int* __restrict__ type, int wselfall, int chemflag, int nelem, int N) {
+    const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= static_cast<long>(N) * nelem) return;
+    const int ii = static_cast<int>(idx % N), je = static_cast<int>(idx / N);
+    const int ielem = chemflag ? s.map[type[ii]] : 0;
+    const size_t n = static_cast<size_t>(N);
+    const size_t base = ii + n * s.idxu_max * je;
+    for (int j = 0; j <= s.J; ++j) {
+        const int b = u_block(j);
+        if (je == ielem || wselfall)
+            for (int m = 0; m <= j; ++m) Ur[base + n * (b + (j + 1) * m + m)] += wself;
+        int lo = b, hi = b + (j + 1) * (j + 1) - 1;
+        for (int mb = 0; 2 * mb < j; ++mb)
+            for (int ma = 0; ma <= j; ++ma, ++lo, --hi) {
+                const double vr = Ur[base + n * lo], vi = Ui[base + n * lo];
+                const bool even = ((ma + mb) & 1) == 0;
+                Ur[base + n * hi] = even ? vr : -vr;
+                Ui[base + n * hi] = even ? -vi : vi;
+            }
+    }
+}
+
+void launch_add_wself(const SnapDev& s, double* Ur, double* Ui, double wself, const int* type, int wselfall,
+                      int chemflag, int nelem, int N, hipStream_t st) {
+    const long tot = static_cast<long>(N) * nelem;
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(k_add_wself, dim3(nblk(tot, 128)), dim3(128), 0, st, s, Ur, Ui, wself, type, wselfall, chemflag,
+                       nelem, N);
+}
+
+// ================================================================================================
+// B from a precomputed Z list (cpuComputeBi2, cpuSnap.cpp:1971-2072) and the bzero shift
+// ================================================================================================
+
+__global__ void k_bi2(SnapDev s, double* __restrict__ blist, const double* __restrict__ zr,
+                      const double* __restrict__ zi, const double* __restrict__ Sr, const double* __restrict__ Si,
+                      int nelem, int N) {
+    const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long nb = static_cast<long>(N) * s.idxb_max;
+    if (idx >= nb * nelem * nelem * nelem) return;
+    const int ii = static_cast<int>(idx % N);
+    const int jjb = static_cast<int>((idx / N) % s.idxb_max);
+    const int jelem = static_cast<int>(idx / nb);
+    const int e3 = jelem % nelem, e2 = (jelem / nelem) % nelem, e1 = jelem / (nelem * nelem);
+    const int jd = s.J + 1;
+    const int j1 = s.idxb[3 * jjb], j2 = s.idxb[3 * jjb + 1], j = s.idxb[3 * jjb + 2];
+    const size_t n = static_cast<size_t>(N);
+    const size_t zoff = ii + n * s.idxz_max * (e2 + nelem * e1), uoff = ii + n * s.idxu_max * e3;
+    int jjz = s.idxz_block[j + j2 * jd + j1 * jd * jd], jju = u_block(j);
+    double sum = 0.0;
+    for (int mb = 0; 2 * mb < j; ++mb)
+        for (int ma = 0; ma <= j; ++ma, ++jjz, ++jju)
+            sum += Sr[uoff + n * jju] * zr[zoff + n * jjz] + Si[uoff + n * jju] * zi[zoff + n * jjz];
+    if (j % 2 == 0) {
+        const int mb = j / 2;
+        for (int ma = 0; ma < mb; ++ma, ++jjz, ++jju)
+            sum += Sr[uoff + n * jju] * zr[zoff + n * jjz] + Si[uoff + n * jju] * zi[zoff + n * jjz];
+        sum += 0.5 * (Sr[uoff + n * jju] * zr[zoff + n * jjz] + Si[uoff + n * jju] * zi[zoff + n * jjz]);
+    }
+    blist[idx] = 2.0 * sum;
+}
+
+// own == 1: subtract from the atom's own-element triple only (cpuSnap.cpp:2022-2035), else from every
+// triple with elem1 == elem2 == elem3 (cpuSnap.cpp:699-708); wselfall: from every triple.
+__global__ void k_bzero(SnapDev s, double* __restrict__ blist, const double* __restrict__ bzero,
+                        const int* __restrict__ ilist, const int* __restrict__ type, int wselfall, int chemflag,
+                        int nelem, int N, int own) {
+    const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long nb = static_cast<long>(N) * s.idxb_max;
+    if (idx >= nb * nelem * nelem * nelem) return;
+    const int ii = static_cast<int>(idx % N);
+    const int jjb = static_cast<int>((idx / N) % s.idxb_max);
+    const int jelem = static_cast<int>(idx / nb);
+    const int e3 = jelem % nelem, e2 = (jelem / nelem) % nelem, e1 = jelem / (nelem * nelem);
+    bool sub;
+    if (wselfall) {
+        sub = true;
+    } else if (own) {
+        const int ie = chemflag ? s.map[type[ilist[ii]]] : 0;
+        sub = (e1 == ie && e2 == ie && e3 == ie);
+    } else {
+        sub = (e1 == e2 && e1 == e3);
+    }
+    if (sub) blist[idx] -= bzero[s.idxb[3 * jjb + 2]];
+}
+
+void launch_bi2(const SnapDev& s, double* blist, const double* zr, const double* zi, const double* Sr, const double* Si,
+                const double* bzero, const int* ilist, const int* type, int bzero_flag, int wselfall, int chemflag,
+                int nelem, int N, hipStream_t st) {
+    const long tot = static_cast<long>(N) * s.idxb_max * nelem * nelem * nelem;
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(k_bi2, dim3(nblk(tot, 128)), dim3(128), 0, st, s, blist, zr, zi, Sr, Si, nelem, N);
+    if (bzero_flag)
+        hipLaunchKernelGGL(k_bzero, dim3(nblk(tot, 256)), dim3(256), 0, st, s, blist, bzero, ilist, type, wselfall,
+                           chemflag, nelem, N, 1);
+}
+
+void launch_bzero(const SnapDev& s, double* blist, const double* bzero, const int* ilist, const int* type, int wselfall,
+                  int chemflag, int nelem, int N, int own, hipStream_t st) {
+    const long tot = static_cast<long>(N) * s.idxb_max * nelem * nelem * nelem;
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(k_bzero, dim3(nblk(tot, 256)), dim3(256), 0, st, s, blist, bzero, ilist, type, wselfall, chemflag,
+                       nelem, N, own);
+}
+
+// eatom[ilist[ii]] += beta0 + sum_k beta_k B_k, in upstream order (element triple, then jjb)
+__global__ void k_ei_from_b(SnapDev s, double* __restrict__ eatom, const double* __restrict__ blist,
+                            const double* __restrict__ coeff, const int* __restrict__ ilist,
+                            const int* __restrict__ type, int chemflag, int nelem, int ncoeffall, int N) {
+    const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ii >= N) return;
+    const int ie = chemflag ? s.map[type[ii]] : 0;
+    const size_t n = static_cast<size_t>(N);
+    double e = eatom[ilist[ii]];
+    const int nt = nelem * nelem * nelem;
+    for (int je = 0; je < nt; ++je)
+        for (int jjb = 0; jjb < s.idxb_max; ++jjb) {
+            const int ic = jjb + s.idxb_max * je;
+            if (ic == 0) e += coeff[ie * ncoeffall];
+            e += coeff[1 + ic + ie * ncoeffall] * blist[ii + n * jjb + n * s.idxb_max * je];
+        }
+    eatom[ilist[ii]] = e;
+}
+
+void launch_ei_from_b(const SnapDev& s, double* eatom, const double* blist, const double* coeff, const int* ilist,
+                      const int* type, int chemflag, int nelem, int ncoeffall, int N, hipStream_t st) {
+    if (N <= 0) return;
+    hipLaunchKernelGGL(k_ei_from_b, dim3(nblk(N, 128)), dim3(128), 0, st, s, eatom, blist, coeff, ilist, type, chemflag,
+                       nelem, ncoeffall, N);
+}
+
+// ================================================================================================
+// dB/dr (cpuComputeDbidrj, cpuSnap.cpp:2074-2263): one thread per (pair, jjb)
+// ================================================================================================
+
+__device__ __forceinline__ void dz_half(double out[3], const double* __restrict__ zr, const double* __restrict__ zi,
+                                        const double* __restrict__ dur, const double* __restrict__ dui, size_t z0,
+                                        size_t zs, int jjz, int jju, int jl, size_t ij, size_t ps, size_t nu) {
+    out[0] = out[1] = out[2] = 0.0;
+    for (int mb = 0; 2 * mb < jl; ++mb)
+        for (int ma = 0; ma <= jl; ++ma, ++jjz, ++jju) {
+            const double a = zr[z0 + zs * jjz], b = zi[z0 + zs * jjz];
+            for (int k = 0; k < 3; ++k) out[k] += dur[ij + ps * jju + nu * k] * a + dui[ij + ps * jju + nu * k] * b;
+        }
+    if (jl % 2 == 0) {
+        const int mb = jl / 2;
+        for (int ma = 0; ma < mb; ++ma, ++jjz, ++jju) {
+            const double a = zr[z0 + zs * jjz], b = zi[z0 + zs * jjz];
+            for (int k = 0; k < 3; ++k) out[k] += dur[ij + ps * jju + nu * k] * a + dui[ij + ps * jju + nu * k] * b;
+        }
+        const double a = zr[z0 + zs * jjz], b = zi[z0 + zs * jjz];
+        for (int k = 0; k < 3; ++k) out[k] += (dur[ij + ps * jju + nu * k] * a + dui[ij + ps * jju + nu * k] * b) * 0.5;
+    }
+}
+
+__global__ void k_dbidrj(SnapDev s, double* __restrict__ dblist, const double* __restrict__ zr,
+                         const double* __restrict__ zi, const double* __restrict__ dur, const double* __restrict__ dui,
+                         const int* __restrict__ ai, const int* __restrict__ tj, int bnorm, int chemflag, int nelem,
+                         int N, int ijnum) {
+    const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= static_cast<long>(ijnum) * s.idxb_max) return;
+    const int ij = static_cast<int>(idx % ijnum), jjb = static_cast<int>(idx / ijnum);
+    const int jd = s.J + 1;
+    const int e3 = chemflag ? s.map[tj[ij]] : 0;
+    const int i = ai[ij];
+    const int j1 = s.idxb[3 * jjb], j2 = s.idxb[3 * jjb + 1], j = s.idxb[3 * jjb + 2];
+    const size_t n = static_cast<size_t>(N), P = static_cast<size_t>(ijnum);
+    const size_t nz = n * s.idxz_max, nbm = P * s.idxb_max, nu = P * s.idxu_max;
+    const double j1fac = (j + 1) / (j1 + 1.0), j2fac = (j + 1) / (j2 + 1.0);
+    for (int e1 = 0; e1 < nelem; ++e1)
+        for (int e2 = 0; e2 < nelem; ++e2) {
+            const size_t z0 = i + nz * (e1 * nelem + e2);
+            double* dbdr = dblist + nbm * 3 * ((e1 * nelem + e2) * nelem + e3);
+            const size_t o = ij + P * 3 * jjb;
+            double sv[3];
+            dz_half(sv, zr, zi, dur, dui, z0, n, s.idxz_block[j + j2 * jd + j1 * jd * jd], u_block(j), j, ij, P, nu);
+            for (int k = 0; k < 3; ++k) dbdr[o + P * k] += 2.0 * sv[k];
+            dz_half(sv, zr, zi, dur, dui, z0, n, s.idxz_block[j1 + j2 * jd + j * jd * jd], u_block(j1), j1, ij, P, nu);
+            for (int k = 0; k < 3; ++k) dbdr[o + P * k] += bnorm ? 2.0 * sv[k] : 2.0 * sv[k] * j1fac;
+            dz_half(sv, zr, zi, dur, dui, z0, n, s.idxz_block[j2 + j1 * jd + j * jd * jd], u_block(j2), j2, ij, P, nu);
+            for (int k = 0; k < 3; ++k) dbdr[o + P * k] += bnorm ? 2.0 * sv[k] : 2.0 * sv[k] * j2fac;
+        }
+}
+
+void launch_dbidrj(const SnapDev& s, double* dblist, const double* zr, const double* zi, const double* dur,
+                   const double* dui, const int* ai, const int* tj, int bnorm, int chemflag, int nelem, int N, int ijnum,
+                   hipStream_t st) {
+    const long tot = static_cast<long>(ijnum) * s.idxb_max;
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(k_dbidrj, dim3(nblk(tot, 128)), dim3(128), 0, st, s, dblist, zr, zi, dur, dui, ai, tj, bnorm,
+                       chemflag, nelem, N, ijnum);
+}
+
+// ================================================================================================
+// tallies (cpuSnap.cpp:2265-2470); packed quadratic rows start after the ncoeff linear terms
+// ================================================================================================
+
+__device__ __forceinline__ int quad_row(int nc, int ic) { return nc + nc * (nc + 1) / 2 - (nc - ic) * (nc - ic + 1) / 2; }
+
+__global__ void k_tally_bi(double* __restrict__ bi, const double* __restrict__ b, const int* __restrict__ ilist,
+                           const int* __restrict__ type, int N, int ncoeff, int nperdim, int quadratic) {
+    const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= static_cast<long>(N) * ncoeff) return;
+    const int ii = static_cast<int>(idx % N), ic = static_cast<int>(idx / N);
+    const int n = nperdim * (type[ilist[ii]] - 1);
+    const double v = b[idx];
+    atomic_add(&bi[ic + n], v);
+    if (quadratic == 1) {
+        int k = n + quad_row(ncoeff, ic);
+        atomic_add(&bi[k], 0.5 * v * v);
+        // upstream placement: the first cross term shares the slot of the square term
+        for (int jc = ic + 1; jc < ncoeff; ++jc) atomic_add(&bi[k++], v * b[ii + static_cast<long>(N) * jc]);
+    }
+}
+
+void launch_tally_bi(double* bi, const double* b, const int* ilist, const int* type, int N, int ncoeff, int nperdim,
+                     int quadratic, hipStream_t st) {
+    const long tot = static_cast<long>(N) * ncoeff;
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(k_tally_bi, dim3(nblk(tot, 256)), dim3(256), 0, st, bi, b, ilist, type, N, ncoeff, nperdim,
+                       quadratic);
+}
+
+__global__ void k_tally_bd(double* __restrict__ db, const double* __restrict__ b, const double* __restrict__ dbdr,
+                           const int* __restrict__ aii, const int* __restrict__ ai, const int* __restrict__ aj,
+                           const int* __restrict__ ti, int N, int ijnum, int ncoeff, int nperdim, int quadratic) {
+    const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= static_cast<long>(ijnum) * ncoeff) return;
+    const int ij = static_cast<int>(idx % ijnum), ic = static_cast<int>(idx / ijnum);
+    const int ii = aii[ij], i = ai[ij], j = aj[ij];
+    const int n = nperdim * (ti[ij] - 1);
+    const size_t P = static_cast<size_t>(ijnum);
+    const size_t nij = P * 3 * ic;
+    const double bx = dbdr[ij + nij], by = dbdr[ij + P + nij], bz = dbdr[ij + 2 * P + nij];
+    size_t nii = static_cast<size_t>(N) * 3 * (ic + n);
+    atomic_add(&db[3 * i + nii], bx);
+    atomic_add(&db[1 + 3 * i + nii], by);
+    atomic_add(&db[2 + 3 * i + nii], bz);
+    atomic_add(&db[3 * j + nii], -bx);
+    atomic_add(&db[1 + 3 * j + nii], -by);
+    atomic_add(&db[2 + 3 * j + nii], -bz);
+    if (quadratic) {
+        const double bv = b[ii + static_cast<size_t>(N) * ic];
+        int k = quad_row(ncoeff, ic);
+        double tx = bv * bx, ty = bv * by, tz = bv * bz;
+        nii = static_cast<size_t>(N) * 3 * (k + n);
+        atomic_add(&db[3 * i + nii], tx);
+        atomic_add(&db[1 + 3 * i + nii], ty);
+        atomic_add(&db[2 + 3 * i + nii], tz);
+        atomic_add(&db[3 * j + nii], -tx);
+        atomic_add(&db[1 + 3 * j + nii], -ty);
+        atomic_add(&db[2 + 3 * j + nii], -tz);
+        for (int jc = ic + 1; jc < ncoeff; ++jc) {
+            const size_t nj = P * 3 * jc;
+            const double cx = dbdr[ij + nj], cy = dbdr[ij + P + nj], cz = dbdr[ij + 2 * P + nj];
+            const double c = b[ii + static_cast<size_t>(N) * jc];
+            tx = bv * cx + bx * c;
+            ty = bv * cy + by * c;
+            tz = bv * cz + bz * c;
+            ++k;
+            nii = static_cast<size_t>(N) * 3 * (k + n);
+            atomic_add(&db[3 * i + nii], tx);
+            atomic_add(&db[1 + 3 * i + nii], ty);
+            atomic_add(&db[2 + 3 * i + nii], tz);
+            atomic_add(&db[3 * j + nii], -tx);
+            atomic_add(&db[1 + 3 * j + nii], -ty);
+            atomic_add(&db[2 + 3 * j + nii], -tz);
+        }
+    }
+}
+
+void launch_tally_bd(double* db, const double* b, const double* dbdr, const int* aii, const int* ai, const int* aj,
+                     const int* ti, int N, int ijnum, int ncoeff, int nperdim, int quadratic, hipStream_t st) {
+    const long tot = static_cast<long>(ijnum) * ncoeff;
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(k_tally_bd, dim3(nblk(tot, 256)), dim3(256), 0, st, db, b, dbdr, aii, ai, aj, ti, N, ijnum,
+                       ncoeff, nperdim, quadratic);
+}
+
+__device__ __forceinline__ void add_vir(double* v, double dx, double dy, double dz, double bx, double by, double bz) {
+    atomic_add(&v[0], dx * bx);
+    atomic_add(&v[1], dy * by);
+    atomic_add(&v[2], dz * bz);
+    atomic_add(&v[3], dx * by);
+    atomic_add(&v[4], dx * bz);
+    atomic_add(&v[5], dy * bz);
+}
+
+__global__ void k_tally_bv(double* __restrict__ bv, const double* __restrict__ b, const double* __restrict__ dbdr,
+                           const double* __restrict__ rij, const int* __restrict__ aii, const int* __restrict__ ti, int N,
+                           int ijnum, int ncoeff, int nperdim, int quadratic) {
+    const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= static_cast<long>(ijnum) * ncoeff) return;
+    const int ij = static_cast<int>(idx % ijnum), ic = static_cast<int>(idx / ijnum);
+    const int ii = aii[ij];
+    const int n = nperdim * (ti[ij] - 1);
+    const size_t P = static_cast<size_t>(ijnum);
+    const size_t nij = P * 3 * ic;
+    const double dx = -rij[3 * ij], dy = -rij[3 * ij + 1], dz = -rij[3 * ij + 2];
+    const double bx = dbdr[ij + nij], by = dbdr[ij + P + nij], bz = dbdr[ij + 2 * P + nij];
+    add_vir(bv + 6 * (ic + n), dx, dy, dz, bx, by, bz);
+    if (quadratic) {
+        const double bval = b[ii + static_cast<size_t>(N) * ic];
+        int k = quad_row(ncoeff, ic);
+        add_vir(bv + 6 * (k + n), dx, dy, dz, bval * bx, bval * by, bval * bz);
+        for (int jc = ic + 1; jc < ncoeff; ++jc) {
+            const size_t nj = P * 3 * jc;
+            const double c = b[ii + static_cast<size_t>(N) * jc];
+            ++k;
+            add_vir(bv + 6 * (k + n), dx, dy, dz, bval * dbdr[ij + nj] + bx * c, bval * dbdr[ij + P + nj] + by * c,
+                    bval * dbdr[ij + 2 * P + nj] + bz * c);
+        }
+    }
+}
+
+void launch_tally_bv(double* bv, const double* b, const double* dbdr, const double* rij, const int* aii, const int* ti,
+                     int N, int ijnum, int ncoeff, int nperdim, int quadratic, hipStream_t st) {
+    const long tot = static_cast<long>(ijnum) * ncoeff;
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(k_tally_bv, dim3(nblk(tot, 256)), dim3(256), 0, st, bv, b, dbdr, rij, aii, ti, N, ijnum, ncoeff,
+                       nperdim, quadratic);
+}
+
+__global__ void k_tally_force_atomic(double* __restrict__ f, double* __restrict__ v, const double* __restrict__ dedr,
+                                     const double* __restrict__ rij, const int* __restrict__ ai,
+                                     const int* __restrict__ aj, int ijnum) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= ijnum) return;
+    const double dx = dedr[3 * p], dy = dedr[3 * p + 1], dz = dedr[3 * p + 2];
+    const double x = rij[3 * p], y = rij[3 * p + 1], z = rij[3 * p + 2];
+    const int a = ai[p], b = aj[p];
+    atomic_add(&f[3 * a], dx);
+    atomic_add(&f[3 * a + 1], dy);
+    atomic_add(&f[3 * a + 2], dz);
+    atomic_add(&f[3 * b], -dx);
+    atomic_add(&f[3 * b + 1], -dy);
+    atomic_add(&f[3 * b + 2], -dz);
+    atomic_add(&v[6 * a + 0], -x * dx);
+    atomic_add(&v[6 * a + 1], -y * dy);
+    atomic_add(&v[6 * a + 2], -z * dz);
+    atomic_add(&v[6 * a + 3], -x * dy);
+    atomic_add(&v[6 * a + 4], -x * dz);
+    atomic_add(&v[6 * a + 5], -y * dz);
+}
+
+void launch_tally_force_atomic(double* f, double* v, const double* dedr, const double* rij, const int* ai, const int* aj,
+                               int ijnum, hipStream_t st) {
+    if (ijnum <= 0) return;
+    hipLaunchKernelGGL(k_tally_force_atomic, dim3(nblk(ijnum, 256)), dim3(256), 0, st, f, v, dedr, rij, ai, aj, ijnum);
+}
+
+// ================================================================================================
+// pair lists from a CSR neighbour list (cpuSnap.cpp:258-310)
+// ================================================================================================
+
+__device__ __forceinline__ bool in_pair(const double* __restrict__ x, int i, int j, int dim, double rcsq) {
+#pragma clang fp contract(off)
+    const double dx = x[j * dim] - x[i * dim];
+    const double dy = x[j * dim + 1] - x[i * dim + 1];
+    const double dz = x[j * dim + 2] - x[i * dim + 2];
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    return d2 < rcsq && d2 > 1e-20;
+}
+
+__global__ void k_pair_count(int* __restrict__ pairnum, const double* __restrict__ x, const double* __restrict__ rcutsq,
+                             const int* __restrict__ ilist, const int* __restrict__ neighlist,
+                             const int* __restrict__ neighnum, const int* __restrict__ atomtype,
+                             const int* __restrict__ alist, int inum, int dim, int ntypes) {
+    const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ii >= inum) return;
+    const int i = ilist[ii], it = atomtype[i];
+    int c = 0;
+    for (int l = neighnum[i]; l < neighnum[i + 1]; ++l) {
+        const int j = neighlist[l];
+        if (in_pair(x, i, j, dim, rcutsq[atomtype[alist[j]] + it * (ntypes + 1)])) ++c;
+    }
+    pairnum[ii] = c;
+}
+
+__global__ void k_pair_fill(int* __restrict__ pairlist, const int* __restrict__ off, const double* __restrict__ x,
+                            const double* __restrict__ rcutsq, const int* __restrict__ ilist,
+                            const int* __restrict__ neighlist, const int* __restrict__ neighnum,
+                            const int* __restrict__ atomtype, const int* __restrict__ alist, int inum, int dim,
+                            int ntypes) {
+    const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ii >= inum) return;
+    const int i = ilist[ii], it = atomtype[i];
+    int c = off[ii];
+    for (int l = neighnum[i]; l < neighnum[i + 1]; ++l) {
+        const int j = neighlist[l];
+        if (in_pair(x, i, j, dim, rcutsq[atomtype[alist[j]] + it * (ntypes + 1)])) pairlist[c++] = j;
+    }
+}
+
+void launch_pair_count(int* pairnum, const double* x, const double* rcutsq, const int* ilist, const int* neighlist,
+                       const int* neighnum, const int* atomtype, const int* alist, int inum, int dim, int ntypes,
+                       hipStream_t st) {
+    if (inum <= 0) return;
+    hipLaunchKernelGGL(k_pair_count, dim3(nblk(inum, 128)), dim3(128), 0, st, pairnum, x, rcutsq, ilist, neighlist,
+                       neighnum, atomtype, alist, inum, dim, ntypes);
+}
+
+void launch_pair_fill(int* pairlist, const int* off, const double* x, const double* rcutsq, const int* ilist,
+                      const int* neighlist, const int* neighnum, const int* atomtype, const int* alist, int inum, int dim,
+                      int ntypes, hipStream_t st) {
+    if (inum <= 0) return;
+    hipLaunchKernelGGL(k_pair_fill, dim3(nblk(inum, 128)), dim3(128), 0, st, pairlist, off, x, rcutsq, ilist, neighlist,
+                       neighnum, atomtype, alist, inum, dim, ntypes);
+}
+
+__global__ void k_pair_arrays(double* __restrict__ xij, const double* __restrict__ x, int* __restrict__ aii,
+                              int* __restrict__ ai, int* __restrict__ aj, int* __restrict__ ti, int* __restrict__ tj,
+                              const int* __restrict__ pairlist, const int* __restrict__ pairnumsum,
+                              const int* __restrict__ ilist, const int* __restrict__ atomtype,
+                              const int* __restrict__ alist, int inum, int dim) {
+    const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ii >= inum) return;
+    const int i = ilist[ii];
+    for (int k = pairnumsum[ii]; k < pairnumsum[ii + 1]; ++k) {
+        const int j = pairlist[k];
+        aii[k] = ii;
+        ai[k] = i;
+        aj[k] = alist[j];
+        ti[k] = atomtype[i];
+        tj[k] = atomtype[alist[j]];
+        for (int d = 0; d < dim; ++d) xij[static_cast<size_t>(k) * dim + d] = x[j * dim + d] - x[i * dim + d];
+    }
+}
+
+void launch_pair_arrays(double* xij, const double* x, int* aii, int* ai, int* aj, int* ti, int* tj, const int* pairlist,
+                        const int* pairnumsum, const int* ilist, const int* atomtype, const int* alist, int inum, int dim,
+                        hipStream_t st) {
+    if (inum <= 0) return;
+    hipLaunchKernelGGL(k_pair_arrays, dim3(nblk(inum, 128)), dim3(128), 0, st, xij, x, aii, ai, aj, ti, tj, pairlist,
+                       pairnumsum, ilist, atomtype, alist, inum, dim);
+}
+
+}  // namespace mdp

@@ -1,0 +1,121 @@
+// compute_ui for gfx950 (cpuSnap.cpp:312-542 semantics).
+//
+// Lane = atom: a wavefront covers 64 consecutive atoms and each lane walks its own pairs.  The U
+// rows of one "row group" (rows RLO..RHI of the half domain) are accumulated in registers over all
+// pairs, so there are no atomics and the result is independent of scheduling.  Rows below RLO are
+// advanced only up to the seed of the next row.  At twojmax = 8 the 155 half-U elements are split
+// into four groups ({0}, {1}, {2}, {3,4}: 45/42/35/33 elements) run as four launches.  U is stored
+// atom-fastest, so every store of a wave is one contiguous 512-byte segment.
+#include <hip/hip_runtime.h>
+
+#include "snap_device.hpp"
+#include "snap_launch.hpp"
+
+namespace mdp {
+
+bool templated_j(int J) { return J == 2 || J == 4 || J == 6 || J == 8; }
+
+template <int J, int RLO, int RHI, int MODE>
+__global__ __launch_bounds__(64) void k_ui(SnapDev s, double* __restrict__ Ur, double* __restrict__ Ui,
+                                           const double* __restrict__ rij, const int* __restrict__ pair_off,
+                                           const int* __restrict__ perm, const int* __restrict__ ti,
+                                           const int* __restrict__ tj, int N, double wself, int elem_sel) {
+    const int atom = blockIdx.x * 64 + threadIdx.x;
+    if (atom >= N) return;
+    constexpr int NE = rows_elems(J, RLO, RHI);
+    double accr[NE], acci[NE];
+#pragma unroll
+    for (int k = 0; k < NE; ++k) { accr[k] = 0.0; acci[k] = 0.0; }
+
+    const int q0 = pair_off[atom], q1 = pair_off[atom + 1];
+    for (int q = q0; q < q1; ++q) {
+        const int p = perm ? perm[q] : q;
+        const int tjp = tj[p];
+        if (elem_sel >= 0 && s.map[tjp] != elem_sel) continue;
+        const double rc = (s.radelem[ti[p]] + s.radelem[tjp]) * s.rcutfac;
+        PairGeom g;
+        pair_geometry<false>(rij[3 * p], rij[3 * p + 1], rij[3 * p + 2], rc, s.rmin0, s.rfac0, s.switchflag,
+                             s.wjelem[tjp], g);
+        const double sf = g.sfac;
+        row_chain<J, 0, RLO, RHI>(g, s.rootpq, [&](auto Jc, auto MBc, auto& pr, auto& pi, auto&, auto&) {
+            constexpr int JJ = decltype(Jc)::value, MB = decltype(MBc)::value;
+            constexpr int base = rows_slot(J, RLO, JJ, MB, 0);
+#pragma unroll
+            for (int ma = 0; ma <= JJ; ++ma) {
+                accr[base + ma] += sf * pr[ma];
+                acci[base + ma] += sf * pi[ma];
+            }
+        });
+    }
+
+    const size_t n = static_cast<size_t>(N);
+    static_for<RLO, RHI + 1>([&](auto MBc) {
+        constexpr int MB = decltype(MBc)::value;
+        static_for<2 * MB, J + 1>([&](auto Jc) {
+            constexpr int JJ = decltype(Jc)::value;
+            constexpr int base = rows_slot(J, RLO, JJ, MB, 0);
+            constexpr int row = u_block(JJ) + (JJ + 1) * MB;
+#pragma unroll
+            for (int ma = 0; ma <= JJ; ++ma) {
+                const size_t o = static_cast<size_t>(row + ma) * n + atom;
+                if constexpr (MODE == UI_ACCUMULATE) {
+                    Ur[o] += accr[base + ma];
+                    Ui[o] += acci[base + ma];
+                } else {
+                    const double vr = accr[base + ma] + (ma == MB ? wself : 0.0);
+                    const double vi = acci[base + ma];
+                    Ur[o] = vr;
+                    Ui[o] = vi;
+                    if (2 * MB < JJ) {   // mirror into row j-mb: U(j-mb, j-ma) = (-1)^(ma+mb) conj U(mb, ma)
+                        const size_t om =
+                            static_cast<size_t>(u_block(JJ) + (JJ + 1) * (JJ - MB) + (JJ - ma)) * n + atom;
+                        const bool even = ((ma + MB) % 2) == 0;
+                        Ur[om] = even ? vr : -vr;
+                        Ui[om] = even ? -vi : vi;
+                    }
+                }
+            }
+        });
+    });
+}
+
+template <int J, int MODE>
+static void ui_groups(const SnapDev& s, double* Ur, double* Ui, const double* rij, const int* off, const int* perm,
+                      const int* ti, const int* tj, int N, double wself, int esel, hipStream_t st) {
+    dim3 grid((N + 63) / 64), block(64);
+    if constexpr (J == 8) {
+        hipLaunchKernelGGL((k_ui<8, 0, 0, MODE>), grid, block, 0, st, s, Ur, Ui, rij, off, perm, ti, tj, N, wself, esel);
+        hipLaunchKernelGGL((k_ui<8, 1, 1, MODE>), grid, block, 0, st, s, Ur, Ui, rij, off, perm, ti, tj, N, wself, esel);
+        hipLaunchKernelGGL((k_ui<8, 2, 2, MODE>), grid, block, 0, st, s, Ur, Ui, rij, off, perm, ti, tj, N, wself, esel);
+        hipLaunchKernelGGL((k_ui<8, 3, 4, MODE>), grid, block, 0, st, s, Ur, Ui, rij, off, perm, ti, tj, N, wself, esel);
+    } else if constexpr (J == 6) {
+        hipLaunchKernelGGL((k_ui<6, 0, 0, MODE>), grid, block, 0, st, s, Ur, Ui, rij, off, perm, ti, tj, N, wself, esel);
+        hipLaunchKernelGGL((k_ui<6, 1, 1, MODE>), grid, block, 0, st, s, Ur, Ui, rij, off, perm, ti, tj, N, wself, esel);
+        hipLaunchKernelGGL((k_ui<6, 2, 3, MODE>), grid, block, 0, st, s, Ur, Ui, rij, off, perm, ti, tj, N, wself, esel);
+    } else {
+        hipLaunchKernelGGL((k_ui<J, 0, J / 2, MODE>), grid, block, 0, st, s, Ur, Ui, rij, off, perm, ti, tj, N, wself,
+                           esel);
+    }
+}
+
+void launch_ui(const SnapDev& s, UiMode mode, double* Ur, double* Ui, const double* rij, const int* pair_off,
+               const int* perm, const int* ti, const int* tj, int N, double wself, int esel, hipStream_t st) {
+    if (N <= 0) return;
+#define UI_CASE(JV)                                                                                   \
+    case JV:                                                                                          \
+        if (mode == UI_ACCUMULATE)                                                                    \
+            ui_groups<JV, UI_ACCUMULATE>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, esel, st); \
+        else                                                                                          \
+            ui_groups<JV, UI_FULL_WSELF>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, esel, st); \
+        break;
+    switch (s.J) {
+        UI_CASE(2)
+        UI_CASE(4)
+        UI_CASE(6)
+        UI_CASE(8)
+        default: break;
+    }
+#undef UI_CASE
+}
+
+}  // namespace mdp

@@ -1,0 +1,819 @@
+// Device-resident SNAP molecular dynamics (mdp_md_* in mdpsnap.h).
+//
+// Per step (velocity Verlet, NVE, metal units; src/C++/Core/cpuVelocityIntegration.cpp:307-340 and
+// src/C++/Integration/Integration.cpp:17-156 for the upstream loop):
+//   1. v += dt/2 f/m ; x += dt v
+//   2. every `rebuild_every` steps: wrap x into the box, bin atoms into cells, build the full
+//      neighbour list at rcut+skin (a workgroup per cell stages the 27 surrounding cells in LDS)
+//      and its transpose (incoming entries per atom, sorted)
+//   3. pair list: neighbour entries with d^2 < rc(ti,tj)^2 and d^2 > 1e-20 (cpuSnap.cpp:258-286),
+//      compacted in CSR order by centre atom
+//   4. SNAP: Utot (k_ui), Y + energy partials (k_yi), dE/dr per pair (k_dedr)
+//   5. tally per atom without atomics: f_i = sum over own pairs - sum over incoming pairs,
+//      W_i = -sum_j r_ij (x) dE/dr_ij, E_i from the energy partials; then v += dt/2 f/m.
+// Neighbour entries are (owner j, image shift) pairs; positions stay unwrapped between rebuilds and
+// the displacement of a neighbour is always (x_j + shift*L) - x_i, evaluated without contraction.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "../../include/mdpsnap.h"
+#include "runtime.hpp"
+#include "snap_device.hpp"
+
+using namespace mdp;
+
+namespace {
+
+constexpr int NBMAX = 96;      // neighbour slots per atom at rcut+skin
+constexpr int CELLCAP = 48;    // atoms per cell the LDS tile accepts
+constexpr int SHK = 8;         // image shifts are coded in [-SHK, SHK] per axis
+constexpr double FTM2V = 1.0 / 1.0364269e-4;   // metal units, eV/A/amu -> A/ps^2
+constexpr double MVV2E = 1.0364269e-4;
+
+enum Timer { T_VV1, T_REBUILD, T_PAIRS, T_UI, T_YI, T_DEDR, T_TALLY, T_COUNT };
+const char* const kTimerNames[T_COUNT] = {"integrate", "neighbor_build", "pair_list", "compute_ui",
+                                           "compute_yi", "compute_dedr", "tally_integrate"};
+
+__host__ __device__ inline int shift_code(int sx, int sy, int sz) {
+    const int w = 2 * SHK + 1;
+    return (sx + SHK) + w * ((sy + SHK) + w * (sz + SHK));
+}
+__host__ __device__ inline void shift_decode(int c, int& sx, int& sy, int& sz) {
+    const int w = 2 * SHK + 1;
+    sx = c % w - SHK;
+    sy = (c / w) % w - SHK;
+    sz = c / (w * w) - SHK;
+}
+
+// squared distance and image displacement evaluated without FMA contraction, so that neighbour and
+// pair membership are reproducible bit for bit on the host
+__device__ __forceinline__ double dist2_nc(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+    return dx * dx + dy * dy + dz * dz;
+}
+__device__ __forceinline__ double image_nc(double xj, int s, double L, double xi) {
+#pragma clang fp contract(off)
+    return (xj + s * L) - xi;
+}
+
+// ---- kernels -------------------------------------------------------------------------------------
+
+__global__ void k_vv1(double* __restrict__ x, double* __restrict__ v, const double* __restrict__ f,
+                      const int* __restrict__ type, const double* __restrict__ massinv_t, double dtf, double dt, int N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double dtfm = dtf * massinv_t[type[i]];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double vi = v[3 * i + d] + dtfm * f[3 * i + d];
+        v[3 * i + d] = vi;
+        x[3 * i + d] += dt * vi;
+    }
+}
+
+// wrap into [0, L), record the maximum squared displacement since the last build, bin into cells
+__global__ void k_wrap_bin(double* __restrict__ x, double* __restrict__ xref, int* __restrict__ cell_of,
+                           int* __restrict__ cell_cnt, unsigned long long* __restrict__ maxd2_bits, double Lx,
+                           double Ly, double Lz, int ncx, int ncy, int ncz, int N, int check,
+                           int* __restrict__ flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double L[3] = {Lx, Ly, Lz};
+    const int nc[3] = {ncx, ncy, ncz};
+    double d2 = 0.0;
+    int c[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double xi = x[3 * i + d];
+        if (!isfinite(xi)) {   // keep the cell index in range; the host reports the error
+            atomicOr(flags, 8);
+            xi = 0.0;
+        }
+        if (check) {
+            const double dd = xi - xref[3 * i + d];
+            d2 += dd * dd;
+        }
+        xi -= floor(xi / L[d]) * L[d];
+        if (xi >= L[d]) xi -= L[d];
+        if (xi < 0.0) xi = 0.0;
+        x[3 * i + d] = xi;
+        xref[3 * i + d] = xi;
+        c[d] = min(static_cast<int>(xi * nc[d] / L[d]), nc[d] - 1);
+    }
+    if (check) atomicMax(maxd2_bits, static_cast<unsigned long long>(__double_as_longlong(d2)));
+    const int cid = c[0] + ncx * (c[1] + ncy * c[2]);
+    cell_of[i] = cid;
+    atomicAdd(&cell_cnt[cid], 1);
+}
+
+__global__ void k_cell_fill(const int* __restrict__ cell_of, const int* __restrict__ cell_start,
+                            int* __restrict__ cell_fill, int* __restrict__ cell_atoms, int N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int c = cell_of[i];
+    const int pos = atomicAdd(&cell_fill[c], 1);
+    cell_atoms[cell_start[c] + pos] = i;
+}
+
+// ascending atom order inside each cell (deterministic list order)
+__global__ void k_cell_sort(const int* __restrict__ cell_start, int* __restrict__ cell_atoms, int ncell,
+                            int* __restrict__ overflow) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int b = cell_start[c], e = cell_start[c + 1];
+    if (e - b > CELLCAP) atomicOr(overflow, 2);
+    for (int k = b + 1; k < e; ++k) {
+        const int v = cell_atoms[k];
+        int m = k - 1;
+        while (m >= b && cell_atoms[m] > v) {
+            cell_atoms[m + 1] = cell_atoms[m];
+            --m;
+        }
+        cell_atoms[m + 1] = v;
+    }
+}
+
+// One workgroup (64 lanes) per cell: the atoms of the 27 surrounding cells (periodic images applied)
+// are staged in LDS in a fixed order, then each lane scans the tile for its centre atom.
+__global__ __launch_bounds__(64) void k_neighbors_cell(const double* __restrict__ x, const int* __restrict__ cell_start,
+                                                       const int* __restrict__ cell_atoms, int* __restrict__ nb_cnt,
+                                                       int* __restrict__ nb_j, int* __restrict__ nb_s, double Lx,
+                                                       double Ly, double Lz, int ncx, int ncy, int ncz, double rcs2,
+                                                       int* __restrict__ overflow) {
+    __shared__ double tx[27 * CELLCAP], ty[27 * CELLCAP], tz[27 * CELLCAP];
+    __shared__ int tj[27 * CELLCAP], ts[27 * CELLCAP];
+    __shared__ int toff[28];
+    const int c = blockIdx.x;
+    const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < 27; ++k) {
+            const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
+            const int nx = (cx + dx + ncx) % ncx, ny = (cy + dy + ncy) % ncy, nz = (cz + dz + ncz) % ncz;
+            const int nc = nx + ncx * (ny + ncy * nz);
+            toff[k] = acc;
+            acc += min(cell_start[nc + 1] - cell_start[nc], CELLCAP);
+        }
+        toff[27] = acc;
+    }
+    __syncthreads();
+    for (int k = 0; k < 27; ++k) {
+        const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
+        const int ux = cx + dx, uy = cy + dy, uz = cz + dz;
+        const int sx = (ux < 0) ? -1 : (ux >= ncx ? 1 : 0);
+        const int sy = (uy < 0) ? -1 : (uy >= ncy ? 1 : 0);
+        const int sz = (uz < 0) ? -1 : (uz >= ncz ? 1 : 0);
+        const int nc = (ux - sx * ncx) + ncx * ((uy - sy * ncy) + ncy * (uz - sz * ncz));
+        const int b = cell_start[nc], cnt = toff[k + 1] - toff[k];
+        for (int m = threadIdx.x; m < cnt; m += 64) {
+            const int j = cell_atoms[b + m];
+            const int t = toff[k] + m;
+            tx[t] = image_nc(x[3 * j], sx, Lx, 0.0);
+            ty[t] = image_nc(x[3 * j + 1], sy, Ly, 0.0);
+            tz[t] = image_nc(x[3 * j + 2], sz, Lz, 0.0);
+            tj[t] = j;
+            ts[t] = shift_code(sx, sy, sz);
+        }
+    }
+    __syncthreads();
+    const int b0 = cell_start[c], n0 = cell_start[c + 1] - b0;
+    const int ntile = toff[27];
+    const int self_code = shift_code(0, 0, 0);
+    for (int m = threadIdx.x; m < n0; m += 64) {
+        const int i = cell_atoms[b0 + m];
+        const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
+        int cnt = 0;
+        for (int t = 0; t < ntile; ++t) {
+            if (tj[t] == i && ts[t] == self_code) continue;
+            const double d2 = dist2_nc(tx[t] - xi, ty[t] - yi, tz[t] - zi);
+            if (d2 <= rcs2) {
+                if (cnt < NBMAX) {
+                    nb_j[static_cast<size_t>(i) * NBMAX + cnt] = tj[t];
+                    nb_s[static_cast<size_t>(i) * NBMAX + cnt] = ts[t];
+                }
+                ++cnt;
+            }
+        }
+        if (cnt > NBMAX) atomicOr(overflow, 1);
+        nb_cnt[i] = min(cnt, NBMAX);
+    }
+}
+
+// Small boxes (fewer than three cells along an axis): every atom against every periodic image.
+__global__ void k_neighbors_brute(const double* __restrict__ x, int* __restrict__ nb_cnt, int* __restrict__ nb_j,
+                                  int* __restrict__ nb_s, double Lx, double Ly, double Lz, double rcs, double rcs2,
+                                  int N, int* __restrict__ overflow) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int rx = min(static_cast<int>(ceil(rcs / Lx)) + 1, SHK), ry = min(static_cast<int>(ceil(rcs / Ly)) + 1, SHK),
+              rz = min(static_cast<int>(ceil(rcs / Lz)) + 1, SHK);
+    const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
+    int cnt = 0;
+    for (int sz = -rz; sz <= rz; ++sz)
+        for (int sy = -ry; sy <= ry; ++sy)
+            for (int sx = -rx; sx <= rx; ++sx)
+                for (int j = 0; j < N; ++j) {
+                    if (j == i && sx == 0 && sy == 0 && sz == 0) continue;
+                    const double d2 = dist2_nc(image_nc(x[3 * j], sx, Lx, xi), image_nc(x[3 * j + 1], sy, Ly, yi),
+                                               image_nc(x[3 * j + 2], sz, Lz, zi));
+                    if (d2 <= rcs2) {
+                        if (cnt < NBMAX) {
+                            nb_j[static_cast<size_t>(i) * NBMAX + cnt] = j;
+                            nb_s[static_cast<size_t>(i) * NBMAX + cnt] = shift_code(sx, sy, sz);
+                        }
+                        ++cnt;
+                    }
+                }
+    if (cnt > NBMAX) atomicOr(overflow, 1);
+    nb_cnt[i] = min(cnt, NBMAX);
+}
+
+// incoming entries: for every entry e = (i -> j) append e to j's list, then sort each list
+__global__ void k_incoming(const int* __restrict__ nb_cnt, const int* __restrict__ nb_j, int* __restrict__ in_cnt,
+                           int* __restrict__ in_e, int N, int* __restrict__ overflow) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    for (int k = 0; k < nb_cnt[i]; ++k) {
+        const int e = i * NBMAX + k;
+        const int j = nb_j[e];
+        const int slot = atomicAdd(&in_cnt[j], 1);
+        if (slot < NBMAX)
+            in_e[static_cast<size_t>(j) * NBMAX + slot] = e;
+        else
+            atomicOr(overflow, 4);
+    }
+}
+
+__global__ void k_incoming_sort(int* __restrict__ in_cnt, int* __restrict__ in_e, int N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int n = min(in_cnt[i], NBMAX);
+    in_cnt[i] = n;
+    int* a = in_e + static_cast<size_t>(i) * NBMAX;
+    for (int k = 1; k < n; ++k) {
+        const int v = a[k];
+        int m = k - 1;
+        while (m >= 0 && a[m] > v) {
+            a[m + 1] = a[m];
+            --m;
+        }
+        a[m + 1] = v;
+    }
+}
+
+__device__ __forceinline__ void entry_disp(const double* __restrict__ x, int i, int j, int code, double Lx, double Ly,
+                                           double Lz, double& dx, double& dy, double& dz) {
+    int sx, sy, sz;
+    shift_decode(code, sx, sy, sz);
+    dx = image_nc(x[3 * j], sx, Lx, x[3 * i]);
+    dy = image_nc(x[3 * j + 1], sy, Ly, x[3 * i + 1]);
+    dz = image_nc(x[3 * j + 2], sz, Lz, x[3 * i + 2]);
+}
+
+__global__ void k_pair_count_md(const double* __restrict__ x, const int* __restrict__ type,
+                                const int* __restrict__ nb_cnt, const int* __restrict__ nb_j,
+                                const int* __restrict__ nb_s, const double* __restrict__ rc2_tt, int ntypes,
+                                int* __restrict__ pcnt, double Lx, double Ly, double Lz, int N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int ti = type[i];
+    int c = 0;
+    for (int k = 0; k < nb_cnt[i]; ++k) {
+        const int e = i * NBMAX + k, j = nb_j[e];
+        double dx, dy, dz;
+        entry_disp(x, i, j, nb_s[e], Lx, Ly, Lz, dx, dy, dz);
+        const double d2 = dist2_nc(dx, dy, dz);
+        if (d2 < rc2_tt[ti * (ntypes + 1) + type[j]] && d2 > 1e-20) ++c;
+    }
+    pcnt[i] = c;
+}
+
+__global__ void k_pair_fill_md(const double* __restrict__ x, const int* __restrict__ type,
+                               const int* __restrict__ nb_cnt, const int* __restrict__ nb_j,
+                               const int* __restrict__ nb_s, const double* __restrict__ rc2_tt, int ntypes,
+                               const int* __restrict__ poff, int* __restrict__ pi_, int* __restrict__ pj,
+                               int* __restrict__ pti, int* __restrict__ ptj, double* __restrict__ prij,
+                               int* __restrict__ entry_pair, double Lx, double Ly, double Lz, int N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int ti = type[i];
+    int p = poff[i];
+    for (int k = 0; k < nb_cnt[i]; ++k) {
+        const int e = i * NBMAX + k, j = nb_j[e];
+        double dx, dy, dz;
+        entry_disp(x, i, j, nb_s[e], Lx, Ly, Lz, dx, dy, dz);
+        const double d2 = dist2_nc(dx, dy, dz);
+        if (d2 < rc2_tt[ti * (ntypes + 1) + type[j]] && d2 > 1e-20) {
+            pi_[p] = i;
+            pj[p] = j;
+            pti[p] = ti;
+            ptj[p] = type[j];
+            prij[3 * static_cast<size_t>(p)] = dx;
+            prij[3 * static_cast<size_t>(p) + 1] = dy;
+            prij[3 * static_cast<size_t>(p) + 2] = dz;
+            entry_pair[e] = p;
+            ++p;
+        } else {
+            entry_pair[e] = -1;
+        }
+    }
+}
+
+// f, per-atom virial and energy of atom i without atomics; optionally the second velocity half-step
+__global__ void k_tally(const double* __restrict__ dedr, const double* __restrict__ prij, const int* __restrict__ poff,
+                        const int* __restrict__ in_cnt, const int* __restrict__ in_e,
+                        const int* __restrict__ entry_pair, const double* __restrict__ epart, int nrows, double e0,
+                        double* __restrict__ f, double* __restrict__ vir, double* __restrict__ eatom,
+                        double* __restrict__ v, const int* __restrict__ type, const double* __restrict__ massinv_t,
+                        double dtf, int integrate, int N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double fx = 0, fy = 0, fz = 0, w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0, w5 = 0;
+    for (int p = poff[i]; p < poff[i + 1]; ++p) {
+        const double gx = dedr[3 * static_cast<size_t>(p)], gy = dedr[3 * static_cast<size_t>(p) + 1],
+                     gz = dedr[3 * static_cast<size_t>(p) + 2];
+        const double rx = prij[3 * static_cast<size_t>(p)], ry = prij[3 * static_cast<size_t>(p) + 1],
+                     rz = prij[3 * static_cast<size_t>(p) + 2];
+        fx += gx;
+        fy += gy;
+        fz += gz;
+        w0 -= rx * gx;
+        w1 -= ry * gy;
+        w2 -= rz * gz;
+        w3 -= rx * gy;
+        w4 -= rx * gz;
+        w5 -= ry * gz;
+    }
+    const int nin = in_cnt[i];
+    for (int k = 0; k < nin; ++k) {
+        const int p = entry_pair[in_e[static_cast<size_t>(i) * NBMAX + k]];
+        if (p >= 0) {
+            fx -= dedr[3 * static_cast<size_t>(p)];
+            fy -= dedr[3 * static_cast<size_t>(p) + 1];
+            fz -= dedr[3 * static_cast<size_t>(p) + 2];
+        }
+    }
+    double e = 0.0;
+    for (int r = 0; r < nrows; ++r) e += epart[static_cast<size_t>(r) * N + i];
+    f[3 * i] = fx;
+    f[3 * i + 1] = fy;
+    f[3 * i + 2] = fz;
+    vir[6 * i] = w0;
+    vir[6 * i + 1] = w1;
+    vir[6 * i + 2] = w2;
+    vir[6 * i + 3] = w3;
+    vir[6 * i + 4] = w4;
+    vir[6 * i + 5] = w5;
+    eatom[i] = e0 + (2.0 / 3.0) * e;
+    if (integrate) {
+        const double dtfm = dtf * massinv_t[type[i]];
+        v[3 * i] += dtfm * fx;
+        v[3 * i + 1] += dtfm * fy;
+        v[3 * i + 2] += dtfm * fz;
+    }
+}
+
+inline int nb(long n, int b) { return static_cast<int>((n + b - 1) / b); }
+
+}  // namespace
+
+// ---- engine --------------------------------------------------------------------------------------
+
+struct mdp_md {
+    int N = 0, ntypes = 0, J = 0;
+    double L[3] = {0, 0, 0};
+    double dt = 0, dtf = 0, skin = 0, rcmax = 0;
+    int rebuild_every = 10;
+    int ncell[3] = {0, 0, 0};
+    bool use_cells = false;
+    double e0 = 0;
+    int nrows = 0;
+    hipStream_t st = nullptr;
+    DevTables tab;
+    std::vector<double> massinv_h, rc2_h;
+    // device state
+    DevBuf x, v, f, type, massinv, xref, cell_of, cell_cnt, cell_start, cell_fill, cell_atoms, scan_tmp, nb_cnt, nb_j,
+        nb_s, in_cnt, in_e, pcnt, poff, pi, pj, pti, ptj, prij, entry_pair, Ur, Ui, Yr, Yi, epart, dedr, eatom, vir,
+        beta_t, rc2, flags, maxd2;
+    long stats[6] = {0, 0, 0, 0, 0, 0};
+    bool forces_current = false;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct Rec { int t; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+    size_t ev_next = 0;
+    double t_ms[T_COUNT] = {0};
+    long t_n[T_COUNT] = {0};
+
+    hipEvent_t ev() {
+        if (ev_next == ev_pool.size()) {
+            hipEvent_t e;
+            MDP_HIP(hipEventCreate(&e));
+            ev_pool.push_back(e);
+        }
+        return ev_pool[ev_next++];
+    }
+    template <class F>
+    void timed(int t, F&& f) {
+        if (!timing) {
+            f();
+            return;
+        }
+        hipEvent_t a = ev(), b = ev();
+        MDP_HIP(hipEventRecord(a, st));
+        f();
+        MDP_HIP(hipEventRecord(b, st));
+        recs.push_back({t, a, b});
+        if (recs.size() > 4096) flush_timers();
+    }
+    void flush_timers() {
+        if (recs.empty()) return;
+        MDP_HIP(hipStreamSynchronize(st));
+        for (auto& r : recs) {
+            float ms = 0;
+            MDP_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            t_ms[r.t] += ms;
+            t_n[r.t] += 1;
+        }
+        recs.clear();
+        ev_next = 0;
+    }
+    ~mdp_md() {
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            for (auto e : ev_pool) (void)hipEventDestroy(e);
+            (void)hipStreamDestroy(st);
+        }
+    }
+
+    void init(const mdp_md_params* p, int natoms, const double* box, const double* x0, const double* v0,
+              const int* types) {
+        if (!p || natoms <= 0 || !box || !x0 || !types) throw Error("mdp_md_create: invalid arguments");
+        if (!templated_j(p->twojmax)) throw Error("mdp_md_create: twojmax must be 2, 4, 6 or 8");
+        if (p->rebuild_every < 1) throw Error("mdp_md_create: rebuild_every must be >= 1");
+        N = natoms;
+        ntypes = p->ntypes;
+        J = p->twojmax;
+        for (int d = 0; d < 3; ++d) {
+            L[d] = box[d];
+            if (!(L[d] > 0)) throw Error("mdp_md_create: box lengths must be positive");
+        }
+        for (long k = 0; k < 3L * N; ++k)
+            if (!std::isfinite(x0[k]) || (v0 && !std::isfinite(v0[k])))
+                throw Error("mdp_md_create: non-finite position or velocity");
+        for (int i = 0; i < N; ++i)
+            if (types[i] < 1 || types[i] > ntypes) throw Error("mdp_md_create: atom type out of range 1..ntypes");
+        dt = p->dt;
+        dtf = 0.5 * dt * FTM2V;
+        skin = p->skin;
+        rebuild_every = p->rebuild_every;
+        MDP_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+
+        SnapIndex ix;
+        ix.build(J, nullptr);
+        std::vector<double> radelem(ntypes + 1, 0.0), wjelem(ntypes + 1, 0.0);
+        std::vector<int> map(ntypes + 1, 0);
+        rcmax = 0;
+        for (int t = 1; t <= ntypes; ++t) {
+            radelem[t] = p->elemradius[t - 1];
+            wjelem[t] = p->elemweight[t - 1];
+            rcmax = std::max(rcmax, 2.0 * radelem[t] * p->rcutfac);
+        }
+        tab.upload(ix, nullptr, nullptr, radelem.data(), wjelem.data(), map.data(), ntypes, st);
+        tab.view.rcutfac = p->rcutfac;
+        tab.view.rfac0 = p->rfac0;
+        tab.view.rmin0 = p->rmin0;
+        tab.view.switchflag = p->switchflag;
+        // chemflag = 0: every atom uses the first coefficient block (cpuSnap.cpp:573, :924)
+        const int ncoeffall = ix.idxb_max + 1;
+        std::vector<double> bt = y_beta_table(ix, p->coeff, 0, 0, 0, 1, 0, ncoeffall, p->bnormflag);
+        e0 = p->coeff[0];
+        if (p->bzeroflag) {
+            for (int k = 0; k < ix.idxb_max; ++k) {
+                const int j = ix.idxb[3 * k + 2];
+                const double bz = p->bnormflag ? 1.0 : (j + 1.0);
+                e0 -= p->coeff[1 + k] * bz;
+            }
+        }
+        nrows = n_half_rows(J);
+        massinv_h.assign(ntypes + 1, 0.0);
+        for (int t = 1; t <= ntypes; ++t) massinv_h[t] = 1.0 / p->mass[t - 1];
+        rc2_h.assign((ntypes + 1) * (ntypes + 1), 0.0);
+        for (int a = 1; a <= ntypes; ++a)
+            for (int b = 1; b <= ntypes; ++b) {
+                const double rc = (radelem[a] + radelem[b]) * p->rcutfac;
+                rc2_h[a * (ntypes + 1) + b] = rc * rc;
+            }
+        const double rcs = rcmax + skin;
+        for (int d = 0; d < 3; ++d) ncell[d] = std::max(1, static_cast<int>(std::floor(L[d] / rcs)));
+        use_cells = ncell[0] >= 3 && ncell[1] >= 3 && ncell[2] >= 3;
+        if (!use_cells && N > 200000) throw Error("mdp_md_create: box too small for cell lists at this atom count");
+        const size_t n = N, iu = ix.idxu_max, ncells = static_cast<size_t>(ncell[0]) * ncell[1] * ncell[2];
+        auto upload = [&](DevBuf& b, const void* h, size_t bytes) {
+            b.ensure(bytes);
+            MDP_HIP(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, st));
+        };
+        upload(x, x0, sizeof(double) * 3 * n);
+        std::vector<double> vz(3 * n, 0.0);
+        upload(v, v0 ? v0 : vz.data(), sizeof(double) * 3 * n);
+        upload(type, types, sizeof(int) * n);
+        upload(massinv, massinv_h.data(), sizeof(double) * massinv_h.size());
+        upload(beta_t, bt.data(), sizeof(double) * bt.size());
+        upload(rc2, rc2_h.data(), sizeof(double) * rc2_h.size());
+        f.ensure(sizeof(double) * 3 * n);
+        MDP_HIP(hipMemsetAsync(f.p, 0, sizeof(double) * 3 * n, st));
+        xref.ensure(sizeof(double) * 3 * n);
+        cell_of.ensure(sizeof(int) * n);
+        cell_cnt.ensure(sizeof(int) * (ncells + 1));
+        cell_start.ensure(sizeof(int) * (ncells + 1));
+        cell_fill.ensure(sizeof(int) * (ncells + 1));
+        cell_atoms.ensure(sizeof(int) * n);
+        scan_tmp.ensure(sizeof(int) * (scan_tmp_ints(static_cast<int>(std::max(n, ncells))) + 8));
+        nb_cnt.ensure(sizeof(int) * n);
+        nb_j.ensure(sizeof(int) * n * NBMAX);
+        nb_s.ensure(sizeof(int) * n * NBMAX);
+        in_cnt.ensure(sizeof(int) * n);
+        in_e.ensure(sizeof(int) * n * NBMAX);
+        pcnt.ensure(sizeof(int) * n);
+        poff.ensure(sizeof(int) * (n + 1));
+        entry_pair.ensure(sizeof(int) * n * NBMAX);
+        Ur.ensure(sizeof(double) * n * iu);
+        Ui.ensure(sizeof(double) * n * iu);
+        Yr.ensure(sizeof(double) * n * iu);
+        Yi.ensure(sizeof(double) * n * iu);
+        epart.ensure(sizeof(double) * n * nrows);
+        eatom.ensure(sizeof(double) * n);
+        vir.ensure(sizeof(double) * 6 * n);
+        flags.ensure(sizeof(int) * 4);
+        maxd2.ensure(sizeof(unsigned long long));
+        MDP_HIP(hipMemsetAsync(Yr.p, 0, sizeof(double) * n * iu, st));
+        MDP_HIP(hipMemsetAsync(Yi.p, 0, sizeof(double) * n * iu, st));
+        MDP_HIP(hipStreamSynchronize(st));
+    }
+
+    void ensure_pairs(size_t cap) {
+        if (pi.cap >= sizeof(int) * std::max<size_t>(cap, 1)) return;
+        const size_t c = cap + cap / 4 + 64;
+        pi.ensure(sizeof(int) * c);
+        pj.ensure(sizeof(int) * c);
+        pti.ensure(sizeof(int) * c);
+        ptj.ensure(sizeof(int) * c);
+        prij.ensure(sizeof(double) * 3 * c);
+        dedr.ensure(sizeof(double) * 3 * c);
+    }
+
+    void rebuild(bool check_disp) {
+        const double rcs = rcmax + skin;
+        const size_t ncells = static_cast<size_t>(ncell[0]) * ncell[1] * ncell[2];
+        MDP_HIP(hipMemsetAsync(cell_cnt.p, 0, sizeof(int) * (ncells + 1), st));
+        MDP_HIP(hipMemsetAsync(cell_fill.p, 0, sizeof(int) * (ncells + 1), st));
+        MDP_HIP(hipMemsetAsync(flags.p, 0, sizeof(int) * 4, st));
+        MDP_HIP(hipMemsetAsync(maxd2.p, 0, sizeof(unsigned long long), st));
+        MDP_HIP(hipMemsetAsync(in_cnt.p, 0, sizeof(int) * N, st));
+        hipLaunchKernelGGL(k_wrap_bin, dim3(nb(N, 256)), dim3(256), 0, st, x.as<double>(), xref.as<double>(),
+                           cell_of.as<int>(), cell_cnt.as<int>(), maxd2.as<unsigned long long>(), L[0], L[1], L[2],
+                           ncell[0], ncell[1], ncell[2], N, check_disp ? 1 : 0, flags.as<int>());
+        if (use_cells) {
+            exclusive_scan(cell_cnt.as<int>(), cell_start.as<int>(), static_cast<int>(ncells), scan_tmp.as<int>(), st);
+            hipLaunchKernelGGL(k_cell_fill, dim3(nb(N, 256)), dim3(256), 0, st, cell_of.as<int>(), cell_start.as<int>(),
+                               cell_fill.as<int>(), cell_atoms.as<int>(), N);
+            hipLaunchKernelGGL(k_cell_sort, dim3(nb(static_cast<long>(ncells), 256)), dim3(256), 0, st,
+                               cell_start.as<int>(), cell_atoms.as<int>(), static_cast<int>(ncells), flags.as<int>());
+            hipLaunchKernelGGL(k_neighbors_cell, dim3(static_cast<unsigned>(ncells)), dim3(64), 0, st, x.as<double>(),
+                               cell_start.as<int>(), cell_atoms.as<int>(), nb_cnt.as<int>(), nb_j.as<int>(),
+                               nb_s.as<int>(), L[0], L[1], L[2], ncell[0], ncell[1], ncell[2], rcs * rcs,
+                               flags.as<int>());
+        } else {
+            hipLaunchKernelGGL(k_neighbors_brute, dim3(nb(N, 64)), dim3(64), 0, st, x.as<double>(), nb_cnt.as<int>(),
+                               nb_j.as<int>(), nb_s.as<int>(), L[0], L[1], L[2], rcs, rcs * rcs, N, flags.as<int>());
+        }
+        hipLaunchKernelGGL(k_incoming, dim3(nb(N, 256)), dim3(256), 0, st, nb_cnt.as<int>(), nb_j.as<int>(),
+                           in_cnt.as<int>(), in_e.as<int>(), N, flags.as<int>());
+        hipLaunchKernelGGL(k_incoming_sort, dim3(nb(N, 256)), dim3(256), 0, st, in_cnt.as<int>(), in_e.as<int>(), N);
+        // a single small readback per rebuild: overflow flags, displacement and entry count
+        int fl[4];
+        unsigned long long md2 = 0;
+        MDP_HIP(hipMemcpyAsync(fl, flags.p, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
+        MDP_HIP(hipMemcpyAsync(&md2, maxd2.p, sizeof(md2), hipMemcpyDeviceToHost, st));
+        MDP_HIP(hipStreamSynchronize(st));
+        if (fl[0]) {
+            stats[5] += 1;
+            throw Error(std::string("neighbour build failed:") + ((fl[0] & 1) ? " >96 neighbours per atom" : "") +
+                        ((fl[0] & 2) ? " >48 atoms per cell" : "") + ((fl[0] & 4) ? " >96 incoming entries" : "") +
+                        ((fl[0] & 8) ? " non-finite positions" : ""));
+        }
+        double d2;
+        std::memcpy(&d2, &md2, sizeof(d2));
+        stats[2] += 1;
+        if (check_disp && d2 > 0.25 * skin * skin) stats[3] += 1;
+        // entry count bounds the pair count until the next rebuild
+        std::vector<int> cnt(N);
+        MDP_HIP(hipMemcpy(cnt.data(), nb_cnt.p, sizeof(int) * N, hipMemcpyDeviceToHost));
+        long tot = 0;
+        for (int c : cnt) tot += c;
+        stats[1] = tot;
+        ensure_pairs(static_cast<size_t>(tot));
+    }
+
+    void pairs() {
+        hipLaunchKernelGGL(k_pair_count_md, dim3(nb(N, 256)), dim3(256), 0, st, x.as<double>(), type.as<int>(),
+                           nb_cnt.as<int>(), nb_j.as<int>(), nb_s.as<int>(), rc2.as<double>(), ntypes, pcnt.as<int>(),
+                           L[0], L[1], L[2], N);
+        exclusive_scan(pcnt.as<int>(), poff.as<int>(), N, scan_tmp.as<int>(), st);
+        hipLaunchKernelGGL(k_pair_fill_md, dim3(nb(N, 256)), dim3(256), 0, st, x.as<double>(), type.as<int>(),
+                           nb_cnt.as<int>(), nb_j.as<int>(), nb_s.as<int>(), rc2.as<double>(), ntypes, poff.as<int>(),
+                           pi.as<int>(), pj.as<int>(), pti.as<int>(), ptj.as<int>(), prij.as<double>(),
+                           entry_pair.as<int>(), L[0], L[1], L[2], N);
+    }
+
+    void snap_and_tally(bool integrate) {
+        const SnapDev& s = tab.view;
+        timed(T_UI, [&] {
+            launch_ui(s, UI_FULL_WSELF, Ur.as<double>(), Ui.as<double>(), prij.as<double>(), poff.as<int>(), nullptr,
+                      pti.as<int>(), ptj.as<int>(), N, 1.0, -1, st);
+        });
+        timed(T_YI, [&] {
+            launch_yi(s, Ur.as<double>(), Ui.as<double>(), beta_t.as<double>(), Yr.as<double>(), Yi.as<double>(),
+                      epart.as<double>(), N, false, st);
+        });
+        timed(T_DEDR, [&] {
+            launch_dedr(s, Yr.as<double>(), Yi.as<double>(), prij.as<double>(), pi.as<int>(), pti.as<int>(),
+                        ptj.as<int>(), nullptr, dedr.as<double>(), N, static_cast<int>(stats[1]), poff.as<int>() + N,
+                        st);
+        });
+        timed(T_TALLY, [&] {
+            hipLaunchKernelGGL(k_tally, dim3(nb(N, 256)), dim3(256), 0, st, dedr.as<double>(), prij.as<double>(),
+                               poff.as<int>(), in_cnt.as<int>(), in_e.as<int>(), entry_pair.as<int>(),
+                               epart.as<double>(), nrows, e0, f.as<double>(), vir.as<double>(), eatom.as<double>(),
+                               v.as<double>(), type.as<int>(), massinv.as<double>(), dtf, integrate ? 1 : 0, N);
+        });
+    }
+
+    void compute() {
+        timed(T_REBUILD, [&] { rebuild(false); });
+        timed(T_PAIRS, [&] { pairs(); });
+        snap_and_tally(false);
+        MDP_HIP(hipGetLastError());
+        MDP_HIP(hipStreamSynchronize(st));
+        int np = 0;
+        MDP_HIP(hipMemcpy(&np, poff.as<int>() + N, sizeof(int), hipMemcpyDeviceToHost));
+        stats[0] = np;
+        forces_current = true;
+    }
+
+    void run(int nsteps) {
+        if (!forces_current) compute();
+        for (int s = 0; s < nsteps; ++s) {
+            timed(T_VV1, [&] {
+                hipLaunchKernelGGL(k_vv1, dim3(nb(N, 256)), dim3(256), 0, st, x.as<double>(), v.as<double>(),
+                                   f.as<double>(), type.as<int>(), massinv.as<double>(), dtf, dt, N);
+            });
+            stats[4] += 1;
+            if (stats[4] % rebuild_every == 0) timed(T_REBUILD, [&] { rebuild(true); });
+            timed(T_PAIRS, [&] { pairs(); });
+            snap_and_tally(true);
+        }
+        MDP_HIP(hipGetLastError());
+        MDP_HIP(hipStreamSynchronize(st));
+        int np = 0;
+        MDP_HIP(hipMemcpy(&np, poff.as<int>() + N, sizeof(int), hipMemcpyDeviceToHost));
+        stats[0] = np;
+    }
+};
+
+namespace {
+template <class F>
+int api(F&& f) {
+    clear_error();
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int mdp_md_create(mdp_md** md, const mdp_md_params* p, int natoms, const double* box, const double* x, const double* v,
+                  const int* types) {
+    return api([&] {
+        if (!md) throw Error("mdp_md_create: null handle pointer");
+        *md = nullptr;
+        auto h = std::make_unique<mdp_md>();
+        h->init(p, natoms, box, x, v, types);
+        *md = h.release();
+    });
+}
+
+int mdp_md_destroy(mdp_md* md) {
+    return api([&] { delete md; });
+}
+
+int mdp_md_compute(mdp_md* md) {
+    return api([&] { md->compute(); });
+}
+
+int mdp_md_run(mdp_md* md, int nsteps) {
+    return api([&] { md->run(nsteps); });
+}
+
+int mdp_md_get(mdp_md* md, double* x, double* v, double* f, double* eatom, double* vatom) {
+    return api([&] {
+        const size_t n = md->N;
+        MDP_HIP(hipStreamSynchronize(md->st));
+        if (x) MDP_HIP(hipMemcpy(x, md->x.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+        if (v) MDP_HIP(hipMemcpy(v, md->v.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+        if (f) MDP_HIP(hipMemcpy(f, md->f.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+        if (eatom) MDP_HIP(hipMemcpy(eatom, md->eatom.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+        if (vatom) MDP_HIP(hipMemcpy(vatom, md->vir.p, sizeof(double) * 6 * n, hipMemcpyDeviceToHost));
+    });
+}
+
+int mdp_md_neighbors(mdp_md* md, int* off, int* nbj, int* nbshift, long* nentries) {
+    return api([&] {
+        const int N = md->N;
+        MDP_HIP(hipStreamSynchronize(md->st));
+        std::vector<int> cnt(N), js(static_cast<size_t>(N) * NBMAX), ss(static_cast<size_t>(N) * NBMAX);
+        MDP_HIP(hipMemcpy(cnt.data(), md->nb_cnt.p, sizeof(int) * N, hipMemcpyDeviceToHost));
+        long tot = 0;
+        for (int c : cnt) tot += c;
+        if (nentries) *nentries = tot;
+        if (!nbj) return;
+        MDP_HIP(hipMemcpy(js.data(), md->nb_j.p, sizeof(int) * js.size(), hipMemcpyDeviceToHost));
+        MDP_HIP(hipMemcpy(ss.data(), md->nb_s.p, sizeof(int) * ss.size(), hipMemcpyDeviceToHost));
+        long k = 0;
+        for (int i = 0; i < N; ++i) {
+            off[i] = static_cast<int>(k);
+            for (int m = 0; m < cnt[i]; ++m, ++k) {
+                nbj[k] = js[static_cast<size_t>(i) * NBMAX + m];
+                int sx, sy, sz;
+                shift_decode(ss[static_cast<size_t>(i) * NBMAX + m], sx, sy, sz);
+                nbshift[3 * k] = sx;
+                nbshift[3 * k + 1] = sy;
+                nbshift[3 * k + 2] = sz;
+            }
+        }
+        off[N] = static_cast<int>(k);
+    });
+}
+
+int mdp_md_stats(mdp_md* md, long* out6) {
+    return api([&] {
+        for (int k = 0; k < 6; ++k) out6[k] = md->stats[k];
+    });
+}
+
+int mdp_md_set_timing(mdp_md* md, int on) {
+    return api([&] {
+        md->flush_timers();
+        md->timing = on != 0;
+        for (int k = 0; k < T_COUNT; ++k) {
+            md->t_ms[k] = 0;
+            md->t_n[k] = 0;
+        }
+    });
+}
+
+int mdp_md_timers(mdp_md* md, double* ms_total, long* launches, int n) {
+    return api([&] {
+        md->flush_timers();
+        for (int k = 0; k < n && k < T_COUNT; ++k) {
+            ms_total[k] = md->t_ms[k];
+            launches[k] = md->t_n[k];
+        }
+    });
+}
+
+const char* mdp_md_timer_name(int k) { return (k >= 0 && k < T_COUNT) ? kTimerNames[k] : ""; }
+int mdp_md_num_timers(void) { return T_COUNT; }
+
+int mdp_md_energy(mdp_md* md, double* epot, double* ekin) {
+    return api([&] {
+        const size_t n = md->N;
+        std::vector<double> e(n), v(3 * n);
+        std::vector<int> t(n);
+        MDP_HIP(hipStreamSynchronize(md->st));
+        MDP_HIP(hipMemcpy(e.data(), md->eatom.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+        MDP_HIP(hipMemcpy(v.data(), md->v.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+        MDP_HIP(hipMemcpy(t.data(), md->type.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+        double ep = 0, ek = 0;
+        for (size_t i = 0; i < n; ++i) {
+            ep += e[i];
+            const double m = 1.0 / md->massinv_h[t[i]];
+            ek += 0.5 * m * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+        }
+        if (epot) *epot = ep;
+        if (ekin) *ekin = ek * MVV2E;
+    });
+}
+
+}  // extern "C"

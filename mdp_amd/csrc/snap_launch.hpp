@@ -1,0 +1,142 @@
+// Launchers of the SNAP device kernels.  All pointers are device pointers; arrays use the layouts
+// of src/Julia/Potential/cpuSnap.cpp (atom-fastest U/Y/Z/B blocks, AoS r_ij, 1-based types).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace mdp {
+
+// Per-parameter-set constants every kernel needs (device pointers to the staged tables).
+struct SnapDev {
+    int J = 0, idxu_max = 0, idxz_max = 0, idxb_max = 0, idxcg_max = 0;
+    double rcutfac = 0, rfac0 = 0, rmin0 = 0;
+    int switchflag = 1;
+    const double* rootpq = nullptr;   // [(J+2)^2]
+    const double* cg = nullptr;       // [idxcg_max]
+    const double* radelem = nullptr;  // [ntypes+1], 1-based
+    const double* wjelem = nullptr;   // [ntypes+1], 1-based
+    const int* map = nullptr;         // [ntypes+1] type -> element
+    const int* idxz = nullptr;        // [10*idxz_max]
+    const int* idxb = nullptr;        // [3*idxb_max]
+    const int* idxu_block = nullptr;  // [J+1]
+    const int* idxz_block = nullptr;  // [(J+1)^3]
+    const int* idxb_block = nullptr;  // [(J+1)^3]
+    const int* idxcg_block = nullptr; // [(J+1)^3]
+};
+
+bool templated_j(int J);   // J has the compile-time specialised kernels
+
+// ---- force path ---------------------------------------------------------------------------------
+
+enum UiMode { UI_ACCUMULATE = 0, UI_FULL_WSELF = 1 };
+// Utot of every atom from its pairs (pair_off CSR over atoms, pairs optionally permuted by perm).
+//   UI_ACCUMULATE : U[half rows] += sum_p sfac*U_p (cpuSnapComputeUi semantics, j=0 included)
+//   UI_FULL_WSELF : U = wself*I + sum_p sfac*U_p on all (j+1)^2 elements (mirror applied)
+// elem_sel >= 0 keeps only pairs whose neighbour element map[tj] equals elem_sel.
+void launch_ui(const SnapDev& s, UiMode mode, double* Ur, double* Ui, const double* rij, const int* pair_off,
+               const int* perm, const int* ti, const int* tj, int natoms, double wself, int elem_sel,
+               hipStream_t st);
+
+// Y[half rows] of every atom (Y zeroed by the caller where untouched).  beta_t[triple] holds the
+// effective weight of each coupling triple (multiplicity, (j1+1)/(j+1) and 1/(j+1) factors folded
+// in).  full_mid: also the ma > mb entries of the middle row of even layers (upstream output);
+// epart != nullptr: per (row, atom) partial of sum' U.Y for the energy identity.
+void launch_yi(const SnapDev& s, const double* Ur, const double* Ui, const double* beta_t, double* Yr,
+               double* Yi, double* epart, int natoms, bool full_mid, hipStream_t st);
+
+// B of the idxb triples: blist[ii + N*jjb] = 2 sum' U*(j).Z(j1,j2,j)  (before bzero; Z/(j+1) if bnorm).
+// U1/U2/U3 element blocks given by their offsets (elements) into Ur/Ui.
+void launch_bi(const SnapDev& s, const double* Ur, const double* Ui, long off1, long off2, long off3,
+               double* blist, int natoms, int bnorm, hipStream_t st);
+
+// zlist (all idxz entries) for element pair (off1, off2) into z + N*jjz layout, divided by (j+1) if bnorm.
+void launch_zi(const SnapDev& s, const double* Ur, const double* Ui, long off1, long off2, double* zr, double* zi,
+               int natoms, int bnorm, hipStream_t st);
+
+// dE/dr_ij per pair: 2 sum' d(sfac U_ij)/dr . Y(atom of the pair); pairs p in [0, *npairs) (device count)
+// or [0, npairs_host) when npairs_dev == nullptr.  yatom[p] = row of Y used (aii), ybase[p] optional
+// element offset (elements) for chemflag.
+void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const double* rij, const int* yatom,
+                 const int* ti, const int* tj, const long* ybase, double* dedr, int natoms, int npairs_host,
+                 const int* npairs_dev, hipStream_t st);
+
+// ---- generic helpers ----------------------------------------------------------------------------
+
+// exclusive prefix sum of n ints: out[0..n] (out[n] = total); tmp needs scan_tmp_ints(n) ints
+size_t scan_tmp_ints(int n);
+void exclusive_scan(const int* in, int* out, int n, int* tmp, hipStream_t st);
+
+}  // namespace mdp
+
+namespace mdp {
+
+// ---- descriptor path, tallies and pair arrays (generic kernels) ------------------------------
+
+// cpuComputeSij / cpuComputeUij: per-pair U and d(sfac U)/dr; dS[k] are the x/y/z derivative arrays
+// (each [idxu_max*ijnum] at ij + ijnum*jju).
+void launch_uij(const SnapDev& s, double* Sr, double* Si, double* const dSr[3], double* const dSi[3],
+                const double* rij, const int* ti, const int* tj, int ijnum, hipStream_t st);
+
+// Stot = wself on the diagonal of the own-element block (element read through type[ai[ii]]).
+void launch_zero_utot(const SnapDev& s, double* Sr, double* Si, double wself, const int* type, const int* ai,
+                      int wselfall, int chemflag, int nelem, int inum, hipStream_t st);
+
+// Stot[ai + N*jju + N*idxu_max*elem(tj)] += S[ij + ijnum*jju]
+void launch_add_utot(const SnapDev& s, double* Stotr, double* Stoti, const double* Sr, const double* Si,
+                     const int* ai, const int* tj, int chemflag, int inum, int ijnum, hipStream_t st);
+
+// blist = 2 sum' Stot*.Z from a precomputed zlist, all element triples; then -bzero.
+void launch_bi2(const SnapDev& s, double* blist, const double* zr, const double* zi, const double* Sr,
+                const double* Si, const double* bzero, const int* ilist, const int* type, int bzero_flag,
+                int wselfall, int chemflag, int nelem, int inum, hipStream_t st);
+
+// subtract bzero[j] from blist of the triples upstream does (own-element triple, or all)
+void launch_bzero(const SnapDev& s, double* blist, const double* bzero, const int* ilist, const int* type,
+                  int wselfall, int chemflag, int nelem, int inum, int use_ilist, hipStream_t st);
+
+// eatom[ilist[ii]] += beta0 + sum beta_k B_k from a blist holding all element triples (bzero applied)
+void launch_ei_from_b(const SnapDev& s, double* eatom, const double* blist, const double* coeffelem,
+                      const int* ilist, const int* type, int chemflag, int nelem, int ncoeffall, int inum,
+                      hipStream_t st);
+
+// dblist of cpuComputeDbidrj (zeroed by the caller)
+void launch_dbidrj(const SnapDev& s, double* dblist, const double* zr, const double* zi, const double* dur,
+                   const double* dui, const int* ai, const int* tj, int bnorm, int chemflag, int nelem, int inum,
+                   int ijnum, hipStream_t st);
+
+void launch_tally_bi(double* bi, const double* b, const int* ilist, const int* type, int inum, int ncoeff,
+                     int nperdim, int quadratic, hipStream_t st);
+void launch_tally_bd(double* db, const double* b, const double* dbdr, const int* aii, const int* ai,
+                     const int* aj, const int* ti, int inum, int ijnum, int ncoeff, int nperdim, int quadratic,
+                     hipStream_t st);
+void launch_tally_bv(double* bv, const double* b, const double* dbdr, const double* rij, const int* aii,
+                     const int* ti, int inum, int ijnum, int ncoeff, int nperdim, int quadratic, hipStream_t st);
+
+// fatom[3*ai] += dE, fatom[3*aj] -= dE, vatom[6*ai + (xx,yy,zz,xy,xz,yz)] += -r (x) dE  (atomic)
+void launch_tally_force_atomic(double* fatom, double* vatom, const double* dedr, const double* rij, const int* ai,
+                               const int* aj, int ijnum, hipStream_t st);
+
+// cpuNeighPairList: pairnum[ii] = count of neighbours j with d2 < rcutsq(type pair), d2 > 1e-20
+void launch_pair_count(int* pairnum, const double* x, const double* rcutsq, const int* ilist, const int* neighlist,
+                       const int* neighnum, const int* atomtype, const int* alist, int inum, int dim, int ntypes,
+                       hipStream_t st);
+void launch_pair_fill(int* pairlist, const int* pairoff, const double* x, const double* rcutsq, const int* ilist,
+                      const int* neighlist, const int* neighnum, const int* atomtype, const int* alist, int inum,
+                      int dim, int ntypes, hipStream_t st);
+// cpuNeighPairs
+void launch_pair_arrays(double* xij, const double* x, int* aii, int* ai, int* aj, int* ti, int* tj,
+                        const int* pairlist, const int* pairnumsum, const int* ilist, const int* atomtype,
+                        const int* alist, int inum, int dim, hipStream_t st);
+
+// AddWself2Ui on device (wself on the diagonal, mirror of rows 2mb < j)
+void launch_add_wself(const SnapDev& s, double* Ur, double* Ui, double wself, const int* type, int wselfall,
+                      int chemflag, int nelem, int inum, hipStream_t st);
+
+// generic compute_yi accumulate for element blocks: Y[e3] += sum beta_t(e1,e2,e3,ielem) Z(e1,e2) over
+// the atoms whose element (map[type[ii]]) equals isel (all atoms if isel < 0)
+void launch_yi_acc(const SnapDev& s, const double* Ur, const double* Ui, long off1, long off2, long off3,
+                   const double* beta_t, double* Yr, double* Yi, const int* atom_elem, int isel, int N,
+                   hipStream_t st);
+
+}  // namespace mdp

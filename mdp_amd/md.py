@@ -1,0 +1,150 @@
+"""Python handle on the device-resident SNAP MD engine of libmdpsnap (mdp_md_* in include/mdpsnap.h).
+
+The engine plays the role of the upstream velocity-Verlet drivers (src/C++/Integration/
+Integration.cpp:17-156, src/Julia/Integration/velocityverlet.jl:3-113) around the SNAP force call:
+positions, velocities, neighbour lists and all SNAP intermediates stay on the GPU.
+"""
+import ctypes
+
+import numpy as np
+
+from . import library
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("twojmax", ctypes.c_int), ("ntypes", ctypes.c_int),
+                ("rcutfac", ctypes.c_double), ("rfac0", ctypes.c_double), ("rmin0", ctypes.c_double),
+                ("switchflag", ctypes.c_int), ("bzeroflag", ctypes.c_int), ("bnormflag", ctypes.c_int),
+                ("wselfallflag", ctypes.c_int),
+                ("elemradius", ctypes.POINTER(ctypes.c_double)), ("elemweight", ctypes.POINTER(ctypes.c_double)),
+                ("coeff", ctypes.POINTER(ctypes.c_double)), ("mass", ctypes.POINTER(ctypes.c_double)),
+                ("dt", ctypes.c_double), ("skin", ctypes.c_double), ("rebuild_every", ctypes.c_int)]
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+_bound = False
+
+
+def _bind():
+    global _bound
+    c = library().cdll
+    if _bound:
+        return c
+    vp = ctypes.c_void_p
+    c.mdp_md_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(_Params), ctypes.c_int, ctypes.c_void_p,
+                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    for fn in ("mdp_md_destroy", "mdp_md_compute"):
+        getattr(c, fn).argtypes = [vp]
+    c.mdp_md_run.argtypes = [vp, ctypes.c_int]
+    c.mdp_md_get.argtypes = [vp] + [ctypes.c_void_p] * 5
+    c.mdp_md_neighbors.argtypes = [vp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.POINTER(ctypes.c_long)]
+    c.mdp_md_stats.argtypes = [vp, ctypes.c_void_p]
+    c.mdp_md_set_timing.argtypes = [vp, ctypes.c_int]
+    c.mdp_md_timers.argtypes = [vp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    c.mdp_md_timer_name.restype = ctypes.c_char_p
+    c.mdp_md_timer_name.argtypes = [ctypes.c_int]
+    c.mdp_md_num_timers.argtypes = []
+    c.mdp_md_energy.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    _bound = True
+    return c
+
+
+class SnapMD:
+    """One periodic orthogonal system integrated NVE with plain FP64 SNAP on the GPU.
+
+    ``coeff`` holds [beta0, beta_1..beta_ncoeff] (first block used for every atom, as upstream does
+    with chemflag = 0); ``dt`` in ps, positions in Angstrom, velocities in Angstrom/ps, masses in amu.
+    """
+
+    def __init__(self, x, box, types, twojmax, coeff, elemradius, elemweight, mass, rcutfac=4.67637,
+                 rfac0=0.99363, rmin0=0.0, switchflag=1, bzeroflag=1, bnormflag=0, wselfallflag=0, v=None,
+                 dt=0.0005, skin=0.3, rebuild_every=10):
+        self._c = _bind()
+        self.x0 = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        self.n = len(self.x0)
+        self.box = np.ascontiguousarray(box, np.float64).reshape(3)
+        self.types = np.ascontiguousarray(types, np.int32).reshape(self.n)
+        v = np.zeros_like(self.x0) if v is None else np.ascontiguousarray(v, np.float64).reshape(-1, 3)
+        self._keep = [np.ascontiguousarray(elemradius, np.float64).ravel(),
+                      np.ascontiguousarray(elemweight, np.float64).ravel(),
+                      np.ascontiguousarray(coeff, np.float64).ravel(),
+                      np.ascontiguousarray(mass, np.float64).ravel()]
+        ntypes = len(self._keep[0])
+        p = _Params(twojmax, ntypes, rcutfac, rfac0, rmin0, switchflag, bzeroflag, bnormflag, wselfallflag,
+                    _dp(self._keep[0]), _dp(self._keep[1]), _dp(self._keep[2]), _dp(self._keep[3]),
+                    dt, skin, rebuild_every)
+        h = ctypes.c_void_p()
+        rc = self._c.mdp_md_create(ctypes.byref(h), ctypes.byref(p), self.n, self.box.ctypes.data,
+                                   self.x0.ctypes.data, v.ctypes.data, self.types.ctypes.data)
+        self._check(rc, "mdp_md_create")
+        self._h = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what}: {library().cdll.mdp_snap_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._c.mdp_md_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def compute(self):
+        """Rebuild neighbours and evaluate energy/forces/virials at the current positions."""
+        self._check(self._c.mdp_md_compute(self._h), "mdp_md_compute")
+
+    def run(self, nsteps):
+        self._check(self._c.mdp_md_run(self._h, int(nsteps)), "mdp_md_run")
+
+    def state(self):
+        n = self.n
+        x, v, f = (np.zeros((n, 3)) for _ in range(3))
+        e, w = np.zeros(n), np.zeros((n, 6))
+        self._check(self._c.mdp_md_get(self._h, x.ctypes.data, v.ctypes.data, f.ctypes.data, e.ctypes.data,
+                                       w.ctypes.data), "mdp_md_get")
+        return dict(x=x, v=v, f=f, eatom=e, vatom=w)
+
+    def neighbors(self):
+        """(offsets[n+1], j[E], shift[E,3]) of the last neighbour build."""
+        tot = ctypes.c_long()
+        self._check(self._c.mdp_md_neighbors(self._h, None, None, None, ctypes.byref(tot)), "mdp_md_neighbors")
+        off = np.zeros(self.n + 1, np.int32)
+        nj = np.zeros(max(tot.value, 1), np.int32)
+        ns = np.zeros((max(tot.value, 1), 3), np.int32)
+        self._check(self._c.mdp_md_neighbors(self._h, off.ctypes.data, nj.ctypes.data, ns.ctypes.data,
+                                             ctypes.byref(tot)), "mdp_md_neighbors")
+        return off, nj[:tot.value], ns[:tot.value]
+
+    def stats(self):
+        out = np.zeros(6, np.int64)
+        self._check(self._c.mdp_md_stats(self._h, out.ctypes.data), "mdp_md_stats")
+        keys = ("pairs", "neighbor_entries", "rebuilds", "dangerous_rebuilds", "steps", "overflows")
+        return dict(zip(keys, (int(v) for v in out)))
+
+    def set_timing(self, on=True):
+        self._check(self._c.mdp_md_set_timing(self._h, 1 if on else 0), "mdp_md_set_timing")
+
+    def timers(self):
+        k = self._c.mdp_md_num_timers()
+        ms = np.zeros(k)
+        n = np.zeros(k, np.int64)
+        self._check(self._c.mdp_md_timers(self._h, ms.ctypes.data, n.ctypes.data, k), "mdp_md_timers")
+        return {self._c.mdp_md_timer_name(i).decode(): (float(ms[i]), int(n[i])) for i in range(k)}
+
+    def energy(self):
+        ep, ek = ctypes.c_double(), ctypes.c_double()
+        self._check(self._c.mdp_md_energy(self._h, ctypes.byref(ep), ctypes.byref(ek)), "mdp_md_energy")
+        return ep.value, ek.value

@@ -1,0 +1,88 @@
+"""CPU-oracle energies, forces and virials of a sample of atoms of a large periodic system.
+
+The oracle cannot evaluate 128,000 atoms at twojmax=8 in seconds, but the quantities of a sampled
+atom i only depend on a neighbourhood: E_i and the pair virial of i need the pairs of i; the force
+on i needs dE/dr of the pairs (i, j) and (k, i), hence Y of i and of its neighbours k.  This builds
+that local problem (centres = sample + neighbours, minimum-image pairs from a periodic k-d tree,
+box >> rcut), runs the oracle chain on it and returns the sampled rows.
+"""
+import numpy as np
+from scipy.spatial import cKDTree
+
+from mdp_amd import snapcpp
+
+
+def _pairs_for(x, box, centers, rc, tree):
+    rows, rij_l, aj_l = [], [], []
+    lists = tree.query_ball_point(x[centers], r=rc * (1.0 + 1e-9))
+    for a, i in enumerate(centers):
+        js = np.array(sorted(lists[a]), dtype=np.int64)
+        js = js[js != i]
+        d = x[js] - x[i]
+        d -= box * np.round(d / box)
+        d2 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
+        keep = (d2 < rc * rc) & (d2 > 1e-20)
+        rows.append(np.full(int(keep.sum()), a, np.int32))
+        rij_l.append(d[keep])
+        aj_l.append(js[keep])
+    return np.concatenate(rows), np.concatenate(rij_l), np.concatenate(aj_l)
+
+
+def sampled_eval(lib, sna, x, box, sample, rc):
+    """Oracle (eatom, fatom, vatom) of the atoms in ``sample`` (global indices)."""
+    x = np.ascontiguousarray(x, np.float64)
+    box = np.asarray(box, np.float64)
+    tree = cKDTree(np.mod(x, box), boxsize=box)
+    sample = np.asarray(sample, np.int64)
+    # neighbours of the sample (their Y enters the force on the sample)
+    lists = tree.query_ball_point(np.mod(x[sample], box), r=rc * (1.0 + 1e-9))
+    nbr = set()
+    for l in lists:
+        nbr.update(l)
+    centers = list(sample) + sorted(nbr - set(sample.tolist()))
+    centers = np.array(centers, np.int64)
+    local = {int(g): k for k, g in enumerate(centers)}
+    xw = np.mod(x, box)
+    ai, rij, ajg = _pairs_for(xw, box, centers, rc, tree)
+    # keep pairs whose centre is a sampled atom, or whose neighbour is a sampled atom
+    S = len(sample)
+    is_s = np.zeros(len(x), bool)
+    is_s[sample] = True
+    keep = (ai < S) | is_s[ajg]
+    # Utot/Y need every pair of every centre; the force tally only the kept ones
+    n = len(centers)
+    aj_local = np.array([local.get(int(g), -1) for g in ajg], np.int32)
+    p_all = dict(rij=np.ascontiguousarray(rij.ravel()), ai=ai.astype(np.int32), aj=np.maximum(aj_local, 0),
+                 ti=np.ones(len(ai), np.int32), tj=np.ones(len(ai), np.int32),
+                 atomtype=np.ones(n, np.int32), ilist=np.arange(n, dtype=np.int32), n=n, ijnum=len(ai))
+    N, iu = n, sna.idxu_max
+    Ur, Ui = np.zeros(N * iu), np.zeros(N * iu)
+    lib.cpuSnapComputeUi(Ur, Ui, sna.rootpqarray, p_all["rij"], sna.wjelem, sna.radelem, sna.rmin0, sna.rfac0,
+                         sna.rcutfac, sna.map, p_all["ai"], p_all["ti"], p_all["tj"], sna.twojmax, iu, N,
+                         p_all["ijnum"], sna.switchflag, 0)
+    lib.cpuAddWself2Ui(Ur, Ui, sna.wself, sna.idxu_block, p_all["atomtype"], sna.map, sna.wselfallflag, 0, iu, 1,
+                       sna.twojmax, N)
+    e = np.zeros(N)
+    lib.cpuSnapComputeEi(e, Ur, Ui, sna.cglist, sna.bzero, sna.coeffelem, p_all["ilist"], sna.map,
+                         p_all["atomtype"], sna.idxb, sna.idxcg_block, sna.idxu_block, sna.twojmax, sna.idxb_max,
+                         iu, 1, sna.ncoeffall, sna.bnormflag, sna.bzeroflag, sna.wselfallflag, 0, N)
+    Yr, Yi = np.zeros(N * iu), np.zeros(N * iu)
+    lib.cpuSnapComputeYi(Yr, Yi, Ur, Ui, sna.cglist, sna.coeffelem, sna.map, p_all["atomtype"], sna.idxz,
+                         sna.idxb_block, sna.idxu_block, sna.idxcg_block, sna.twojmax, sna.idxb_max, iu,
+                         sna.idxz_max, 1, sna.ncoeffall, sna.bnormflag, 0, N)
+    sel = np.nonzero(keep)[0]
+    rk = np.ascontiguousarray(rij[sel].ravel())
+    aik = ai[sel].astype(np.int32)
+    ajk = np.where(aj_local[sel] >= 0, aj_local[sel], N).astype(np.int32)   # N = sink row for outsiders
+    f = np.zeros(3 * (N + 1))
+    v = np.zeros(6 * (N + 1))
+    one = np.ones(len(sel), np.int32)
+    lib.cpuSnapComputeFi(f, v, Yr, Yi, sna.rootpqarray, rk, sna.wjelem, sna.radelem, sna.rmin0, sna.rfac0,
+                         sna.rcutfac, sna.map, aik, aik, ajk, one, one, sna.twojmax, iu, N, N + 1, len(sel),
+                         sna.switchflag, 0)
+    return e[:S], f.reshape(-1, 3)[:S], v.reshape(-1, 6)[:S]
+
+
+def init(lib, J, coeff, rcutfac=4.67637, rfac0=0.99363, rmin0=0.0):
+    param = [1, J, rcutfac, rfac0, rmin0, 1, 1, 0, 0, 0, 0]
+    return snapcpp.initsna(lib, param, [0.5], [1.0], coeff)

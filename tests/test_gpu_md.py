@@ -1,0 +1,145 @@
+"""GPU MD engine: neighbour lists as canonical sets, first-step energy/force/virial parity with the CPU
+oracle (small systems whole; the 128,000-atom twojmax=8 system on a seeded sample of atoms), and
+size-independent properties of the dynamics (momentum and energy conservation).
+
+Tolerances: neighbour sets bit-exact; energies, forces, virials within 1e-10 relative to the max
+norm of the oracle array.
+"""
+import numpy as np
+import pytest
+
+from conftest import relerr
+from mdp_amd import lattice, snapcpp
+
+pytestmark = pytest.mark.gpu
+
+RCUTFAC, RFAC0 = 4.67637, 0.99363
+MASS = 180.95
+TOL = 1e-10
+
+
+def _engine(x, box, J, coeff, v=None, skin=0.3, every=10, dt=0.0005):
+    from mdp_amd.md import SnapMD
+    return SnapMD(x, box, np.ones(len(x), np.int32), J, coeff, [0.5], [1.0], [MASS], v=v, dt=dt, skin=skin,
+                  rebuild_every=every)
+
+
+def _coeff(oracle, J, seed):
+    s = snapcpp.initsna(oracle, [1, J, RCUTFAC, RFAC0, 0, 1, 1, 0, 0, 0, 0], [0.5], [1.0])
+    return lattice.snap_coefficients(s.ncoeff, seed=seed)
+
+
+def _oracle_bruteforce(oracle, x, box, rc):
+    import ctypes
+    n = len(x)
+    off = np.zeros(n + 1, np.int32)
+    maxnb = 128 * n
+    nj = np.zeros(maxnb, np.int32)
+    ns = np.zeros(3 * maxnb, np.int32)
+    f = oracle.cdll.oracle_neighbor_bruteforce
+    f.restype = ctypes.c_long
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C")
+    ip = np.ctypeslib.ndpointer(np.int32, flags="C")
+    f.argtypes = [dp, ctypes.c_int, dp, ctypes.c_double, ip, ip, ip, ctypes.c_long]
+    cnt = f(np.ascontiguousarray(x), n, np.ascontiguousarray(box), rc, off, nj, ns, maxnb)
+    assert cnt > 0
+    return off, nj, ns.reshape(-1, 3)
+
+
+@pytest.mark.parametrize("ncell,a", [(6, 3.316), (2, 3.316), (5, 3.0)])
+def test_neighbor_sets_bit_exact(oracle, ncell, a):
+    x, box = lattice.crystal("bcc", ncell, a, 0.15, 100 + ncell)
+    md = _engine(x, box, 4, _coeff(oracle, 4, 1))
+    md.compute()
+    off, nj, ns = md.neighbors()
+    xw = md.state()["x"]
+    assert np.array_equal(xw, x)   # already inside the box: wrapping leaves it unchanged
+    o2, j2, s2 = _oracle_bruteforce(oracle, xw, box, RCUTFAC + 0.3)
+    for i in range(len(x)):
+        a_ = sorted(zip(nj[off[i]:off[i + 1]].tolist(), map(tuple, ns[off[i]:off[i + 1]].tolist())))
+        b_ = sorted(zip(j2[o2[i]:o2[i + 1]].tolist(), map(tuple, s2[o2[i]:o2[i + 1]].tolist())))
+        assert a_ == b_, i
+
+
+@pytest.mark.parametrize("J,ncell,seed", [(4, 6, 1), (6, 6, 2), (8, 6, 3), (8, 4, 4)])
+def test_first_step_vs_oracle(oracle, J, ncell, seed):
+    x, box = lattice.crystal("bcc", ncell, 3.316, 0.1, seed)
+    coeff = _coeff(oracle, J, seed)
+    md = _engine(x, box, J, coeff)
+    md.compute()
+    st = md.state()
+    s = snapcpp.initsna(oracle, [1, J, RCUTFAC, RFAC0, 0, 1, 1, 0, 0, 0, 0], [0.5], [1.0], coeff)
+    B = np.diag(box)
+    p = snapcpp.pairs_from_box(x, np.ones(len(x), np.int32), B[0], B[1], B[2], [1, 1, 1], s)
+    e, f, v = snapcpp.snappotential_pairs(oracle, s, p)
+    assert md.stats()["pairs"] == p["ijnum"]
+    assert relerr(st["eatom"], e) < TOL
+    assert relerr(st["f"], f) < TOL
+    assert relerr(st["vatom"], v) < TOL
+
+
+def test_first_step_128k_twojmax8_sampled(oracle):
+    """The headline system: BCC Ta 40^3 cells (128,000 atoms), sigma = 0.1 A, twojmax = 8."""
+    import sampled_oracle
+    x, box = lattice.crystal("bcc", 40, 3.316, 0.1, 2024)
+    coeff = _coeff(oracle, 8, 2024)
+    md = _engine(x, box, 8, coeff)
+    md.compute()
+    st = md.state()
+    sample = np.random.default_rng(5).choice(len(x), size=24, replace=False)
+    s = sampled_oracle.init(oracle, 8, coeff)
+    e, f, v = sampled_oracle.sampled_eval(oracle, s, x, box, sample, RCUTFAC)
+    assert relerr(st["eatom"][sample], e) < TOL
+    assert relerr(st["f"][sample], f) < TOL
+    assert relerr(st["vatom"][sample], v) < TOL
+    # size-independent checks on the whole system
+    assert np.max(np.abs(st["f"].sum(axis=0))) < 1e-9 * np.max(np.abs(st["f"]))
+    assert np.all(np.isfinite(st["f"]))
+
+
+def test_nve_conservation(oracle):
+    x, box = lattice.crystal("bcc", 6, 3.316, 0.05, 9)
+    v = lattice.maxwell_boltzmann(len(x), MASS, 300.0, seed=9)
+    md = _engine(x, box, 6, _coeff(oracle, 6, 9), v=v, dt=0.0005)
+    md.compute()
+    e0 = sum(md.energy())
+    p0 = v.sum(axis=0) * MASS
+    md.run(200)
+    e1 = sum(md.energy())
+    st = md.state()
+    assert abs(e1 - e0) < 2e-4 * len(x) * 0.0259      # drift << kT per atom
+    assert np.max(np.abs(st["v"].sum(axis=0) * MASS - p0)) < 1e-8
+    stats = md.stats()
+    assert stats["rebuilds"] == 1 + 20 and stats["overflows"] == 0
+
+
+def test_short_trajectory_vs_oracle_integration(oracle):
+    """Ten velocity-Verlet steps driven by oracle forces on the host against the GPU engine."""
+    from mdp_amd.lattice import FTM2V_METAL
+    J, dt, steps = 4, 0.0005, 10
+    x, box = lattice.crystal("bcc", 4, 3.316, 0.05, 11)
+    v = lattice.maxwell_boltzmann(len(x), MASS, 300.0, seed=11)
+    coeff = _coeff(oracle, J, 11)
+    md = _engine(x, box, J, coeff, v=v, dt=dt, every=1)
+    md.compute()
+    md.run(steps)
+    g = md.state()
+    s = snapcpp.initsna(oracle, [1, J, RCUTFAC, RFAC0, 0, 1, 1, 0, 0, 0, 0], [0.5], [1.0], coeff)
+    B = np.diag(box)
+    t = np.ones(len(x), np.int32)
+
+    def force(xx):
+        p = snapcpp.pairs_from_box(np.mod(xx, box), t, B[0], B[1], B[2], [1, 1, 1], s)
+        return snapcpp.snappotential_pairs(oracle, s, p)[1]
+
+    dtf = 0.5 * dt * FTM2V_METAL / MASS
+    xh, vh = x.copy(), v.copy()
+    f = force(xh)
+    for _ in range(steps):
+        vh += dtf * f
+        xh += dt * vh
+        f = force(xh)
+        vh += dtf * f
+    d = np.mod(g["x"] - xh + 0.5 * box, box) - 0.5 * box
+    assert np.max(np.abs(d)) < 1e-9
+    assert relerr(g["v"], vh) < 1e-9

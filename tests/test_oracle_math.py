@@ -203,3 +203,16 @@ def test_bruteforce_neighbors_match_host_builder(oracle):
         g = nl[nn[i]:nn[i + 1]]
         b = sorted((int(alist[k]), *map(int, shifts[k])) for k in g)
         assert a == b, i
+
+
+def test_sampled_oracle_matches_full_evaluation(oracle):
+    """The local-neighbourhood evaluation used for large systems equals the full oracle chain."""
+    import sampled_oracle
+    x, box = lattice.crystal("bcc", 4, 3.316, 0.1, 17)
+    s, p, coeff = _setup(oracle, 6, x, box, seed=5)
+    e, f, v = snapcpp.snappotential_pairs(oracle, s, p)
+    sample = np.array([0, 7, 63, 100])
+    es, fs, vs = sampled_oracle.sampled_eval(oracle, s, x, box, sample, 4.67637)
+    assert relerr(es, e[sample]) < 1e-13
+    assert relerr(fs, f[sample]) < 1e-11
+    assert relerr(vs, v[sample]) < 1e-11
