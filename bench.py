@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Plain FP64 SNAP molecular dynamics on one MI355X: atom-steps/s at twojmax=8, 128,000 atoms.
+
+Workload (one "step" = one velocity-Verlet NVE step of the whole system): BCC tantalum,
+40 x 40 x 40 cubic cells (128,000 atoms), a = 3.316 A, seeded Gaussian displacements sigma = 0.1 A,
+twojmax = 8 (55 bispectrum components), rcutfac = 4.67637, rfac0 = 0.99363, rmin0 = 0, switching and
+bzero on, linear coefficients beta_k ~ N(0, 0.05) with beta0 = -11.7 eV, 300 K Maxwell-Boltzmann
+velocities, mass 180.95 amu, dt = 0.5 fs, neighbour skin 0.3 A rebuilt every 10 steps.  Every step
+integrates, amortises the neighbour rebuild and evaluates SNAP energy, forces and virials in FP64.
+
+Prints one JSON line on stdout; diagnostics go to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "atom-steps/s, SNAP 2J=8 @128k atoms; surrogate vs plain at ≤0.05 eV/Å RMSE"
+FP64_PEAK_TFLOPS = 78.6      # MI355X published FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+J, NCELL, A0, SIGMA = 8, 40, 3.316, 0.1
+RCUTFAC, RFAC0, MASS, DT, SKIN, EVERY, TEMP = 4.67637, 0.99363, 180.95, 0.0005, 0.3, 10, 300.0
+SEED = 20241202
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(ncell=NCELL):
+    from mdp_amd import lattice
+    from mdp_amd.flops import snap_flops  # noqa: F401
+    x, box = lattice.crystal("bcc", ncell, A0, SIGMA, SEED)
+    v = lattice.maxwell_boltzmann(len(x), MASS, TEMP, seed=SEED + 1)
+    ncoeff = 55
+    coeff = lattice.snap_coefficients(ncoeff, beta0=-11.7, sigma=0.05, seed=SEED + 2)
+    return x, box, v, coeff
+
+
+def periodic_pairs(x, box, rc):
+    """Minimum-image pair arrays (centre-sorted) of an orthogonal periodic box, box > 2 rc."""
+    from scipy.spatial import cKDTree
+    xw = np.mod(x, box)
+    tree = cKDTree(xw, boxsize=box)
+    ij = tree.query_pairs(rc, output_type="ndarray")
+    ij = np.concatenate([ij, ij[:, ::-1]])
+    d = xw[ij[:, 1]] - xw[ij[:, 0]]
+    d -= box * np.round(d / box)
+    d2 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
+    keep = (d2 < rc * rc) & (d2 > 1e-20)
+    ij, d = ij[keep], d[keep]
+    order = np.lexsort((ij[:, 1], ij[:, 0]))
+    ij, d = ij[order], d[order]
+    n = len(x)
+    return dict(rij=np.ascontiguousarray(d.ravel()), ai=ij[:, 0].astype(np.int32), aj=ij[:, 1].astype(np.int32),
+                ti=np.ones(len(ij), np.int32), tj=np.ones(len(ij), np.int32), atomtype=np.ones(n, np.int32),
+                ilist=np.arange(n, dtype=np.int32), n=n, ijnum=len(ij))
+
+
+def cpu_baseline(ncell):
+    """The upstream SNAP C library (oracle/_ref, compiled from the upstream source) -- or, where the
+    upstream tree was absent at build time, this project's CPU restatement -- timed on one host core
+    for the first force call (Ui -> AddWself2Ui -> Ei -> Yi -> Fi, snapcpp.jl:386-423 order) of an
+    ncell^3-cell cut of the same lattice."""
+    from mdp_amd import _abi, snapcpp
+    ref = os.path.join(ROOT, "oracle", "_ref", "libcpusnap_ref.so")
+    port = os.path.join(ROOT, "oracle", "liboracle.so")
+    kind, path = ("reference", ref) if os.path.exists(ref) else ("port", port)
+    lib = _abi.bind(path)
+    x, box, _, coeff = workload(ncell)
+    param = [1, J, RCUTFAC, RFAC0, 0.0, 1, 1, 0, 0, 0, 0]
+    s = snapcpp.initsna(lib, param, [0.5], [1.0], coeff)
+    p = periodic_pairs(x, box, RCUTFAC)
+    t0 = time.perf_counter()
+    snapcpp.snappotential_pairs(lib, s, p)
+    dt = time.perf_counter() - t0
+    return dict(value=len(x) / dt, unit="atom-steps/s", cores=1, kind=kind,
+                sample=f"first E/F/V force call of {len(x)} atoms ({ncell}^3 BCC cells, same a/sigma/twojmax/"
+                       f"coefficients, {p['ijnum']} pairs), single thread, {dt:.2f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--ncell", type=int, default=NCELL)
+    ap.add_argument("--cpu-ncell", type=int, default=24, help="cells per side of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print("bench.py runs on exactly one GPU: the SNAP MD path is measured per GPU (multi-GPU spatial "
+              "decomposition is not part of this build); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import mdp_amd
+    from mdp_amd.flops import snap_flops
+    from mdp_amd.md import SnapMD
+
+    mdp_amd.library()   # fails loudly without the HIP library
+    x, box, v, coeff = workload(a.ncell)
+    n = len(x)
+    md = SnapMD(x, box, np.ones(n, np.int32), J, coeff, [0.5], [1.0], [MASS], rcutfac=RCUTFAC, rfac0=RFAC0,
+                v=v, dt=DT, skin=SKIN, rebuild_every=EVERY)
+    t0 = time.perf_counter()
+    md.compute()
+    log(f"first force call (incl. neighbour build): {1e3 * (time.perf_counter() - t0):.1f} ms")
+    md.run(a.warmup)
+    e0 = sum(md.energy())
+    md.set_timing(True)
+    t0 = time.perf_counter()
+    md.run(a.steps)                       # returns after the stream is synchronised
+    elapsed = time.perf_counter() - t0
+    timers = md.timers()
+    stats = md.stats()
+    e1 = sum(md.energy())
+    value = n * a.steps / elapsed
+    log(f"{a.steps} steps in {elapsed:.3f} s -> {value:.4e} atom-steps/s; pairs/atom {stats['pairs'] / n:.2f}; "
+        f"rebuilds {stats['rebuilds']} (dangerous {stats['dangerous_rebuilds']}); "
+        f"energy drift {(e1 - e0) / n:.3e} eV/atom")
+    for k, (ms, cnt) in timers.items():
+        log(f"  {k:16s} {ms:9.2f} ms total, {cnt:6d} launches, {ms / max(cnt, 1):8.4f} ms avg")
+
+    fl = snap_flops(J)
+    ppa = stats["pairs"] / n
+    per_launch = {"compute_yi": fl["yi_md"] * n, "compute_dedr": fl["dedr"] * stats["pairs"],
+                  "compute_ui": fl["ui"] * stats["pairs"]}
+    dom = max(per_launch, key=lambda k: timers[k][0])
+    ms_avg = timers[dom][0] / max(timers[dom][1], 1)
+    achieved = per_launch[dom] / (ms_avg * 1e-3) / 1e12
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_bcc40_j8.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("bytes_per_launch")
+    step_flops = n * (ppa * (fl["ui"] + fl["dedr"]) + fl["yi_md"])
+    log(f"dominant kernel {dom}: {ms_avg:.3f} ms/launch, {achieved:.2f} TFLOP/s "
+        f"({100 * achieved / FP64_PEAK_TFLOPS:.1f}% of FP64 peak); whole step "
+        f"{step_flops / (elapsed / a.steps) / 1e12:.2f} TFLOP/s")
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": 1e3 * elapsed / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded BCC lattice, displacements, velocities and SNAP coefficients)",
+        "config": {"workload": f"plain FP64 SNAP NVE MD, BCC Ta {a.ncell}^3 cells ({n} atoms), twojmax=8, "
+                               f"dt=0.5 fs, skin 0.3 A rebuilt every {EVERY} steps",
+                   "atoms": n, "twojmax": J, "pairs_per_atom": round(ppa, 3), "parallelism": "1 GPU"},
+        "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "flops_per_launch": per_launch[dom], "ms_per_launch": ms_avg},
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu_baseline:
+        t0 = time.perf_counter()
+        out["cpu_baseline"] = cpu_baseline(a.cpu_ncell)
+        log(f"cpu baseline: {out['cpu_baseline']['value']:.1f} atom-steps/s ({time.perf_counter() - t0:.1f} s)")
+    md.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

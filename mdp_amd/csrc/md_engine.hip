@@ -430,7 +430,7 @@ struct mdp_md {
         f();
         MDP_HIP(hipEventRecord(b, st));
         recs.push_back({t, a, b});
-        if (recs.size() > 4096) flush_timers();
+        if (recs.size() > 100000) flush_timers();   // no mid-run synchronisation in practice
     }
     void flush_timers() {
         if (recs.empty()) return;

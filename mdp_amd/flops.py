@@ -1,0 +1,66 @@
+"""Algorithmic FP64 operation counts of the SNAP force path (an FMA counts 2, a multiply or add 1),
+enumerated from the loop bounds of the index spaces -- the denominators of the roofline figures.
+
+* ``ui``   per pair : half-U row-chain recursion (ma = 0: 8, interior: 18, ma = j: 8 per row update)
+                      plus the sfac-weighted accumulation (4 per emitted element).
+* ``dedr`` per pair : U recursion as above plus three derivative recursions (48 / 102 / 48 per row
+                      update) plus the contraction with Y (32 per contracted element: 2 for dsfac*U and
+                      10 per Cartesian component).
+* ``yi``   per atom : the upstream compute_yi work (cpuSnap.cpp:891-989): sum over the idxz entries of
+                      10*na*nb (complex product and CG weight) + 4*nb (row weight) + 4 (beta * Z).
+* ``yi_md`` per atom: the same restricted to the Y entries the force needs (middle row of even layers
+                      only up to ma = mb) plus 4 per entry for the energy partial U.Y.
+* ``ei``   per atom : the upstream fused energy (cpuSnap.cpp:544-719): Z over the half domain of each
+                      bispectrum triple (10*na*nb + 4*nb) + 4 for the U.Z product + 2 for beta*B.
+"""
+from functools import lru_cache
+
+
+def _cg_n(m, j1, j2, j):
+    lo = max(0, (2 * m - j - j2 + j1) // 2) if (2 * m - j - j2 + j1) >= 0 else 0
+    return min(j1, (2 * m - j + j2 + j1) // 2) - lo + 1
+
+
+def _triples(J):
+    for j1 in range(J + 1):
+        for j2 in range(j1 + 1):
+            for j in range(j1 - j2, min(J, j1 + j2) + 1, 2):
+                yield j1, j2, j
+
+
+def _row_updates(J, derivs):
+    first, mid, last = (8, 18, 8) if not derivs else (8 + 48, 18 + 102, 8 + 48)
+    tot = 0
+    for mb in range(J // 2 + 1):
+        for j in range(max(1, 2 * mb), J + 1):
+            tot += first + mid * (j - 1) + last
+    return tot
+
+
+@lru_cache(maxsize=None)
+def snap_flops(J):
+    emitted = sum(j + 1 for j in range(J + 1) for mb in range(j // 2 + 1))
+    contracted = sum((mb + 1 if j == 2 * mb else j + 1) for j in range(J + 1) for mb in range(j // 2 + 1))
+    ui = _row_updates(J, False) + 4 * emitted
+    dedr = _row_updates(J, True) + 32 * contracted
+    yi = yi_md = ei = 0
+    for j1, j2, j in _triples(J):
+        for mb in range(j // 2 + 1):
+            nb = _cg_n(mb, j1, j2, j)
+            for ma in range(j + 1):
+                na = _cg_n(ma, j1, j2, j)
+                w = 10 * na * nb + 4 * nb
+                yi += w + 4
+                if 2 * mb < j or ma <= mb:
+                    yi_md += w + 4 + 4
+                    if j >= j1:
+                        ei += w + 4
+        if j >= j1:
+            ei += 2
+    return dict(ui=ui, dedr=dedr, yi=yi, yi_md=yi_md, ei=ei)
+
+
+def step_flops_per_atom(J, pairs_per_atom):
+    """Plain MD step (Ui + Yi with energy + dE/dr) per atom."""
+    f = snap_flops(J)
+    return pairs_per_atom * (f["ui"] + f["dedr"]) + f["yi_md"]
