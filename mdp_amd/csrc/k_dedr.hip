@@ -12,7 +12,7 @@
 
 namespace mdp {
 
-template <int J>
+template <int J, bool HALF>
 __global__ __launch_bounds__(64) void k_dedr(SnapDev s, const double* __restrict__ Yr, const double* __restrict__ Yi,
                                              const double* __restrict__ rij, const int* __restrict__ yatom,
                                              const int* __restrict__ ti, const int* __restrict__ tj,
@@ -37,7 +37,8 @@ __global__ __launch_bounds__(64) void k_dedr(SnapDev s, const double* __restrict
             d2 = g.dsfac * g.u[2] * y0;
         } else {
             constexpr int LAST = (JJ == 2 * MB) ? MB : JJ;
-            const size_t o = static_cast<size_t>(u_block(JJ) + (JJ + 1) * MB) * n + yb;
+            constexpr int row = (HALF ? h_block(JJ) : u_block(JJ)) + (JJ + 1) * MB;
+            const size_t o = static_cast<size_t>(row) * n + yb;
 #pragma unroll
             for (int ma = 0; ma <= LAST; ++ma) {
                 double yr = Yr[o + ma * n], yi = Yi[o + ma * n];
@@ -56,13 +57,17 @@ __global__ __launch_bounds__(64) void k_dedr(SnapDev s, const double* __restrict
 
 void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const double* rij, const int* yatom,
                  const int* ti, const int* tj, const long* ybase, double* dedr, int N, int npairs_host,
-                 const int* npairs_dev, hipStream_t st) {
+                 const int* npairs_dev, hipStream_t st, bool half) {
     if (npairs_host <= 0) return;
     dim3 grid((npairs_host + 63) / 64);
-#define DE_CASE(JV)                                                                                            \
-    case JV:                                                                                                   \
-        hipLaunchKernelGGL((k_dedr<JV>), grid, dim3(64), 0, st, s, Yr, Yi, rij, yatom, ti, tj, ybase, dedr, N, \
-                           npairs_host, npairs_dev);                                                           \
+#define DE_CASE(JV)                                                                                       \
+    case JV:                                                                                              \
+        if (half)                                                                                         \
+            hipLaunchKernelGGL((k_dedr<JV, true>), grid, dim3(64), 0, st, s, Yr, Yi, rij, yatom, ti, tj, ybase, \
+                               dedr, N, npairs_host, npairs_dev);                                         \
+        else                                                                                              \
+            hipLaunchKernelGGL((k_dedr<JV, false>), grid, dim3(64), 0, st, s, Yr, Yi, rij, yatom, ti, tj, ybase, \
+                               dedr, N, npairs_host, npairs_dev);                                         \
         break;
     switch (s.J) {
         DE_CASE(2)

@@ -61,6 +61,10 @@ __global__ __launch_bounds__(64) void k_ui(SnapDev s, double* __restrict__ Ur, d
                 if constexpr (MODE == UI_ACCUMULATE) {
                     Ur[o] += accr[base + ma];
                     Ui[o] += acci[base + ma];
+                } else if constexpr (MODE == UI_HALF_WSELF) {
+                    const size_t oh = static_cast<size_t>(h_block(JJ) + (JJ + 1) * MB + ma) * n + atom;
+                    Ur[oh] = accr[base + ma] + (ma == MB ? wself : 0.0);
+                    Ui[oh] = acci[base + ma];
                 } else {
                     const double vr = accr[base + ma] + (ma == MB ? wself : 0.0);
                     const double vi = acci[base + ma];
@@ -105,6 +109,8 @@ void launch_ui(const SnapDev& s, UiMode mode, double* Ur, double* Ui, const doub
     case JV:                                                                                          \
         if (mode == UI_ACCUMULATE)                                                                    \
             ui_groups<JV, UI_ACCUMULATE>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, esel, st); \
+        else if (mode == UI_HALF_WSELF)                                                               \
+            ui_groups<JV, UI_HALF_WSELF>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, esel, st); \
         else                                                                                          \
             ui_groups<JV, UI_FULL_WSELF>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, esel, st); \
         break;

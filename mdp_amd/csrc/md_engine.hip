@@ -399,7 +399,7 @@ struct mdp_md {
     // device state
     DevBuf x, v, f, type, massinv, xref, cell_of, cell_cnt, cell_start, cell_fill, cell_atoms, scan_tmp, nb_cnt, nb_j,
         nb_s, in_cnt, in_e, pcnt, poff, pi, pj, pti, ptj, prij, entry_pair, Ur, Ui, Yr, Yi, epart, dedr, eatom, vir,
-        beta_t, rc2, flags, maxd2;
+        beta_t, rc2, flags, maxd2, row_order;
     long stats[6] = {0, 0, 0, 0, 0, 0};
     bool forces_current = false;
     // timing
@@ -514,7 +514,9 @@ struct mdp_md {
         for (int d = 0; d < 3; ++d) ncell[d] = std::max(1, static_cast<int>(std::floor(L[d] / rcs)));
         use_cells = ncell[0] >= 3 && ncell[1] >= 3 && ncell[2] >= 3;
         if (!use_cells && N > 200000) throw Error("mdp_md_create: box too small for cell lists at this atom count");
-        const size_t n = N, iu = ix.idxu_max, ncells = static_cast<size_t>(ncell[0]) * ncell[1] * ncell[2];
+        // U and Y of the MD path live in the compact half layout (rows 2mb <= j)
+        const size_t n = N, iu = static_cast<size_t>(ix.half_count()),
+                     ncells = static_cast<size_t>(ncell[0]) * ncell[1] * ncell[2];
         auto upload = [&](DevBuf& b, const void* h, size_t bytes) {
             b.ensure(bytes);
             MDP_HIP(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, st));
@@ -526,6 +528,9 @@ struct mdp_md {
         upload(massinv, massinv_h.data(), sizeof(double) * massinv_h.size());
         upload(beta_t, bt.data(), sizeof(double) * bt.size());
         upload(rc2, rc2_h.data(), sizeof(double) * rc2_h.size());
+        std::vector<int> order(nrows);
+        yi_row_order(J, order.data());
+        upload(row_order, order.data(), sizeof(int) * order.size());
         f.ensure(sizeof(double) * 3 * n);
         MDP_HIP(hipMemsetAsync(f.p, 0, sizeof(double) * 3 * n, st));
         xref.ensure(sizeof(double) * 3 * n);
@@ -635,17 +640,17 @@ struct mdp_md {
     void snap_and_tally(bool integrate) {
         const SnapDev& s = tab.view;
         timed(T_UI, [&] {
-            launch_ui(s, UI_FULL_WSELF, Ur.as<double>(), Ui.as<double>(), prij.as<double>(), poff.as<int>(), nullptr,
+            launch_ui(s, UI_HALF_WSELF, Ur.as<double>(), Ui.as<double>(), prij.as<double>(), poff.as<int>(), nullptr,
                       pti.as<int>(), ptj.as<int>(), N, 1.0, -1, st);
         });
         timed(T_YI, [&] {
-            launch_yi(s, Ur.as<double>(), Ui.as<double>(), beta_t.as<double>(), Yr.as<double>(), Yi.as<double>(),
-                      epart.as<double>(), N, false, st);
+            launch_yi_half(s, Ur.as<double>(), Ui.as<double>(), beta_t.as<double>(), Yr.as<double>(), Yi.as<double>(),
+                           epart.as<double>(), row_order.as<int>(), N, st);
         });
         timed(T_DEDR, [&] {
             launch_dedr(s, Yr.as<double>(), Yi.as<double>(), prij.as<double>(), pi.as<int>(), pti.as<int>(),
                         ptj.as<int>(), nullptr, dedr.as<double>(), N, static_cast<int>(stats[1]), poff.as<int>() + N,
-                        st);
+                        st, true);
         });
         timed(T_TALLY, [&] {
             hipLaunchKernelGGL(k_tally, dim3(nb(N, 256)), dim3(256), 0, st, dedr.as<double>(), prij.as<double>(),

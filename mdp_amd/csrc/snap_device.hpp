@@ -39,6 +39,14 @@ __host__ __device__ constexpr int u_block(int j) {
     return n;
 }
 
+// first element of layer j in the compact half layout (rows 2mb <= j only), and its size
+__host__ __device__ constexpr int h_block(int j) {
+    int n = 0;
+    for (int k = 0; k < j; ++k) n += (k / 2 + 1) * (k + 1);
+    return n;
+}
+__host__ __device__ constexpr int n_half_elems(int J) { return h_block(J + 1); }
+
 // coupling triple (j1 >= j2 -> j) exists
 __host__ __device__ constexpr bool coupled(int j1, int j2, int j) {
     return j2 <= j1 && j >= j1 - j2 && j <= j1 + j2 && ((j1 + j2 + j) % 2 == 0);

@@ -29,10 +29,12 @@ bool templated_j(int J);   // J has the compile-time specialised kernels
 
 // ---- force path ---------------------------------------------------------------------------------
 
-enum UiMode { UI_ACCUMULATE = 0, UI_FULL_WSELF = 1 };
+enum UiMode { UI_ACCUMULATE = 0, UI_FULL_WSELF = 1, UI_HALF_WSELF = 2 };
 // Utot of every atom from its pairs (pair_off CSR over atoms, pairs optionally permuted by perm).
 //   UI_ACCUMULATE : U[half rows] += sum_p sfac*U_p (cpuSnapComputeUi semantics, j=0 included)
 //   UI_FULL_WSELF : U = wself*I + sum_p sfac*U_p on all (j+1)^2 elements (mirror applied)
+//   UI_HALF_WSELF : as UI_FULL_WSELF but only rows 2mb <= j, in the compact half layout
+//                   (element (j,mb,ma) at h_block(j) + (j+1)*mb + ma)
 // elem_sel >= 0 keeps only pairs whose neighbour element map[tj] equals elem_sel.
 void launch_ui(const SnapDev& s, UiMode mode, double* Ur, double* Ui, const double* rij, const int* pair_off,
                const int* perm, const int* ti, const int* tj, int natoms, double wself, int elem_sel,
@@ -44,6 +46,14 @@ void launch_ui(const SnapDev& s, UiMode mode, double* Ur, double* Ui, const doub
 // epart != nullptr: per (row, atom) partial of sum' U.Y for the energy identity.
 void launch_yi(const SnapDev& s, const double* Ur, const double* Ui, const double* beta_t, double* Yr,
                double* Yi, double* epart, int natoms, bool full_mid, hipStream_t st);
+
+// MD path: Y (half layout, entries the force uses) + energy partials from a half-layout U.  The half U
+// of 64 atoms is staged in LDS; the workgroup's waves take output rows from a shared counter in the
+// order row_order (device array of n_half_rows(J) row ids, most expensive first).
+void launch_yi_half(const SnapDev& s, const double* Uhr, const double* Uhi, const double* beta_t, double* Yhr,
+                    double* Yhi, double* epart, const int* row_order, int natoms, hipStream_t st);
+// row ids of the half domain sorted by decreasing compute_yi work
+void yi_row_order(int J, int* out);
 
 // B of the idxb triples: blist[ii + N*jjb] = 2 sum' U*(j).Z(j1,j2,j)  (before bzero; Z/(j+1) if bnorm).
 // U1/U2/U3 element blocks given by their offsets (elements) into Ur/Ui.
@@ -59,7 +69,7 @@ void launch_zi(const SnapDev& s, const double* Ur, const double* Ui, long off1, 
 // element offset (elements) for chemflag.
 void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const double* rij, const int* yatom,
                  const int* ti, const int* tj, const long* ybase, double* dedr, int natoms, int npairs_host,
-                 const int* npairs_dev, hipStream_t st);
+                 const int* npairs_dev, hipStream_t st, bool half_layout = false);
 
 // ---- generic helpers ----------------------------------------------------------------------------
 
