@@ -111,7 +111,8 @@ def main():
                 v=v, dt=DT, skin=SKIN, rebuild_every=EVERY)
     t0 = time.perf_counter()
     md.compute()
-    log(f"first force call (incl. neighbour build): {1e3 * (time.perf_counter() - t0):.1f} ms")
+    log(f"first force call (incl. neighbour build and autotuning): {1e3 * (time.perf_counter() - t0):.1f} ms")
+    log(f"autotuner: {json.dumps(md.tuning())}")
     md.run(a.warmup)
     e0 = sum(md.energy())
     md.set_timing(True)
@@ -151,7 +152,8 @@ def main():
         "data": "synthetic (seeded BCC lattice, displacements, velocities and SNAP coefficients)",
         "config": {"workload": f"plain FP64 SNAP NVE MD, BCC Ta {a.ncell}^3 cells ({n} atoms), twojmax=8, "
                                f"dt=0.5 fs, skin 0.3 A rebuilt every {EVERY} steps",
-                   "atoms": n, "twojmax": J, "pairs_per_atom": round(ppa, 3), "parallelism": "1 GPU"},
+                   "atoms": n, "twojmax": J, "pairs_per_atom": round(ppa, 3), "parallelism": "1 GPU",
+                   "autotuned": md.tuning().get("chosen", {})},
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "flops_per_launch": per_launch[dom], "ms_per_launch": ms_avg},

@@ -186,6 +186,10 @@ int mdp_md_timers(mdp_md *md, double *ms_total, long *launches, int n);
 const char *mdp_md_timer_name(int k);
 int mdp_md_num_timers(void);
 
+/* Launch variants chosen by the engine's autotuner after the first force evaluation, as a JSON object
+ * with the measured ms per launch of every variant (set MDP_AUTOTUNE=0 to keep the defaults). */
+int mdp_md_tuning(mdp_md *md, char *buf, int len);
+
 /* Total potential and kinetic energy (eV) of the current state. */
 int mdp_md_energy(mdp_md *md, double *epot, double *ekin);
 

@@ -55,19 +55,72 @@ __global__ __launch_bounds__(64) void k_dedr(SnapDev s, const double* __restrict
     dedr[3 * static_cast<size_t>(p) + 2] = 2.0 * d2;
 }
 
+
+// Same contraction with lane = (pair, Cartesian component): a wave holds 21 pairs x 3 components
+// (lane 63 idle).  Each lane carries U and one derivative recursion, about half the register state of
+// k_dedr, which lets several waves share a SIMD; U is recomputed by the three lanes of a pair.
+template <int J, bool HALF, int WPE>
+__global__ __launch_bounds__(64, WPE) void k_dedr_split(SnapDev s, const double* __restrict__ Yr,
+                                                   const double* __restrict__ Yi, const double* __restrict__ rij,
+                                                   const int* __restrict__ yatom, const int* __restrict__ ti,
+                                                   const int* __restrict__ tj, const long* __restrict__ ybase,
+                                                   double* __restrict__ dedr, int N, int npairs_host,
+                                                   const int* __restrict__ npairs_dev) {
+    const int lane = threadIdx.x;
+    const int p = blockIdx.x * 21 + lane / 3, comp = lane % 3;
+    const int np = npairs_dev ? *npairs_dev : npairs_host;
+    if (lane >= 63 || p >= np) return;
+    const int tjp = tj[p];
+    const double rc = (s.radelem[ti[p]] + s.radelem[tjp]) * s.rcutfac;
+    PairGeom g;
+    pair_geometry<true>(rij[3 * p], rij[3 * p + 1], rij[3 * p + 2], rc, s.rmin0, s.rfac0, s.switchflag, s.wjelem[tjp], g);
+    // move this lane's component into slot 0
+    const double uk = comp == 0 ? g.u[0] : (comp == 1 ? g.u[1] : g.u[2]);
+    g.dar[0] = comp == 0 ? g.dar[0] : (comp == 1 ? g.dar[1] : g.dar[2]);
+    g.dai[0] = comp == 0 ? g.dai[0] : (comp == 1 ? g.dai[1] : g.dai[2]);
+    g.dbr[0] = comp == 0 ? g.dbr[0] : (comp == 1 ? g.dbr[1] : g.dbr[2]);
+    g.dbi[0] = comp == 0 ? g.dbi[0] : (comp == 1 ? g.dbi[1] : g.dbi[2]);
+    const size_t n = static_cast<size_t>(N);
+    const size_t yb = static_cast<size_t>(yatom[p]) + (ybase ? static_cast<size_t>(ybase[p]) : 0);
+    const double dsu = g.dsfac * uk;
+    double d = 0.0;
+    row_chain<J, 1, 0, J / 2>(g, s.rootpq, [&](auto Jc, auto MBc, auto& pr, auto& pi, auto& dpr, auto& dpi) {
+        constexpr int JJ = decltype(Jc)::value, MB = decltype(MBc)::value;
+        if constexpr (JJ == 0) {
+            d = dsu * (0.5 * Yr[yb]);
+        } else {
+            constexpr int LAST = (JJ == 2 * MB) ? MB : JJ;
+            constexpr int row = (HALF ? h_block(JJ) : u_block(JJ)) + (JJ + 1) * MB;
+            const size_t o = static_cast<size_t>(row) * n + yb;
+#pragma unroll
+            for (int ma = 0; ma <= LAST; ++ma) {
+                double yr = Yr[o + ma * n], yi = Yi[o + ma * n];
+                if (JJ == 2 * MB && ma == MB) { yr *= 0.5; yi *= 0.5; }
+                d += (dsu * pr[ma] + g.sfac * dpr[0][ma]) * yr + (dsu * pi[ma] + g.sfac * dpi[0][ma]) * yi;
+            }
+        }
+    });
+    dedr[3 * static_cast<size_t>(p) + comp] = 2.0 * d;
+}
+
 void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const double* rij, const int* yatom,
                  const int* ti, const int* tj, const long* ybase, double* dedr, int N, int npairs_host,
-                 const int* npairs_dev, hipStream_t st, bool half) {
+                 const int* npairs_dev, hipStream_t st, bool half, int variant) {
     if (npairs_host <= 0) return;
-    dim3 grid((npairs_host + 63) / 64);
-#define DE_CASE(JV)                                                                                       \
-    case JV:                                                                                              \
-        if (half)                                                                                         \
-            hipLaunchKernelGGL((k_dedr<JV, true>), grid, dim3(64), 0, st, s, Yr, Yi, rij, yatom, ti, tj, ybase, \
-                               dedr, N, npairs_host, npairs_dev);                                         \
-        else                                                                                              \
-            hipLaunchKernelGGL((k_dedr<JV, false>), grid, dim3(64), 0, st, s, Yr, Yi, rij, yatom, ti, tj, ybase, \
-                               dedr, N, npairs_host, npairs_dev);                                         \
+    const dim3 g3((npairs_host + 63) / 64), g1((npairs_host + 20) / 21);
+#define DE_ARGS Yr, Yi, rij, yatom, ti, tj, ybase, dedr, N, npairs_host, npairs_dev
+#define DE_CASE(JV)                                                                                            \
+    case JV:                                                                                                   \
+        if (!half)                                                                                             \
+            hipLaunchKernelGGL((k_dedr<JV, false>), g3, dim3(64), 0, st, s, DE_ARGS);                          \
+        else if (variant == 0)                                                                                 \
+            hipLaunchKernelGGL((k_dedr<JV, true>), g3, dim3(64), 0, st, s, DE_ARGS);                           \
+        else if (variant == 2)                                                                                 \
+            hipLaunchKernelGGL((k_dedr_split<JV, true, 2>), g1, dim3(64), 0, st, s, DE_ARGS);                  \
+        else if (variant == 3)                                                                                 \
+            hipLaunchKernelGGL((k_dedr_split<JV, true, 3>), g1, dim3(64), 0, st, s, DE_ARGS);                  \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_dedr_split<JV, true, 4>), g1, dim3(64), 0, st, s, DE_ARGS);                  \
         break;
     switch (s.J) {
         DE_CASE(2)
@@ -77,6 +130,13 @@ void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const dou
         default: break;
     }
 #undef DE_CASE
+#undef DE_ARGS
+}
+
+int dedr_variants(int* ids) {
+    static const int v[] = {0, 2, 3, 4};
+    for (int k = 0; k < 4; ++k) ids[k] = v[k];
+    return 4;
 }
 
 }  // namespace mdp

@@ -1,0 +1,145 @@
+// compute_yi of the MD path on gfx950: half-layout U staged in LDS (kernel templates; the
+// instantiations live in k_yih_*.hip so that they compile in parallel).
+//
+// One workgroup owns 64 consecutive atoms.  It first copies their half-domain U (rows 2mb <= j, 155
+// complex elements per atom at twojmax = 8: 2 x 155 x 64 doubles = 158,720 bytes of the 160 KB LDS)
+// from HBM with coalesced loads, then its W wavefronts take output rows (j, mb) from a shared counter,
+// heaviest first.  Every lane is one atom; the U rows a coupling needs are read from LDS (rows with
+// 2mb > j through the mirror symmetry U(mb,ma) = (-1)^(mb+ma) conj U(j-mb, j-ma)), so each U element
+// crosses HBM once per step instead of once per output row.  Y (only the entries the force uses) and
+// the per-row energy partials sum' U.Y go straight to HBM.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "snap_device.hpp"
+#include "snap_launch.hpp"
+
+namespace mdp {
+
+// one U row of layer J1 (row mb1, any mb1 in 0..J1) of this lane's atom from the LDS half layout
+template <int J1>
+__device__ __forceinline__ void lds_row(int mb1, const double* __restrict__ lr, const double* __restrict__ li,
+                                        int lane, double (&ar)[J1 + 1], double (&ai)[J1 + 1]) {
+    if (2 * mb1 <= J1) {
+        const int b = (h_block(J1) + (J1 + 1) * mb1) * 64 + lane;
+#pragma unroll
+        for (int m = 0; m <= J1; ++m) {
+            ar[m] = lr[b + m * 64];
+            ai[m] = li[b + m * 64];
+        }
+    } else {
+        const int b = (h_block(J1) + (J1 + 1) * (J1 - mb1)) * 64 + lane;
+        const double sm = (mb1 & 1) ? -1.0 : 1.0;
+#pragma unroll
+        for (int m = 0; m <= J1; ++m) {
+            const double s = (m & 1) ? -sm : sm;
+            ar[m] = s * lr[b + (J1 - m) * 64];
+            ai[m] = -s * li[b + (J1 - m) * 64];
+        }
+    }
+}
+
+template <int J, int J1, int J2, int JO>
+__device__ __forceinline__ void z_row_lds(int mb, int malim, const double* __restrict__ lr,
+                                          const double* __restrict__ li, int lane, const double* __restrict__ cg,
+                                          double scale, double (&zr)[JO + 1], double (&zi)[JO + 1]) {
+    constexpr int CGB = cg_block(J, J1, J2, JO);
+    const double* __restrict__ c = cg + CGB;
+    const int mb1min = max(0, (2 * mb - JO - J2 + J1) / 2);
+    const int mb2max = (2 * mb - JO - (2 * mb1min - J1) + J2) / 2;
+    const int nb = min(J1, (2 * mb - JO + J2 + J1) / 2) - mb1min + 1;
+    for (int ib = 0; ib < nb; ++ib) {
+        const int mb1 = mb1min + ib, mb2 = mb2max - ib;
+        const double cb = c[mb1 * (J2 + 1) + mb2] * scale;
+        double ar[J1 + 1], ai[J1 + 1], br[J2 + 1], bi[J2 + 1];
+        lds_row<J1>(mb1, lr, li, lane, ar, ai);
+        lds_row<J2>(mb2, lr, li, lane, br, bi);
+        static_for<0, JO + 1>([&](auto MAc) {
+            constexpr int MA = decltype(MAc)::value;
+            constexpr int LO = cg_lo(MA, J1, J2, JO), HI2 = cg_hi2(MA, J1, J2, JO), NA = cg_n(MA, J1, J2, JO);
+            if (MA <= malim) {
+                double sr = 0.0, si = 0.0;
+                static_for<0, NA>([&](auto IAc) {
+                    constexpr int IA = decltype(IAc)::value;
+                    constexpr int M1 = LO + IA, M2 = HI2 - IA;
+                    const double cga = c[M1 * (J2 + 1) + M2];
+                    sr += cga * (ar[M1] * br[M2] - ai[M1] * bi[M2]);
+                    si += cga * (ar[M1] * bi[M2] + ai[M1] * br[M2]);
+                });
+                zr[MA] += cb * sr;
+                zi[MA] += cb * si;
+            }
+        });
+    }
+}
+
+template <int J, int JO>
+__device__ __forceinline__ void y_row_lds(const SnapDev& s, int row, int mb, int atom, bool live, int N,
+                                          const double* __restrict__ lr, const double* __restrict__ li, int lane,
+                                          const double* __restrict__ beta_t, double* __restrict__ Yr,
+                                          double* __restrict__ Yi, double* __restrict__ epart) {
+    const int malim = (2 * mb == JO) ? mb : JO;
+    double yr[JO + 1], yi[JO + 1];
+#pragma unroll
+    for (int m = 0; m <= JO; ++m) { yr[m] = 0.0; yi[m] = 0.0; }
+    static_for<0, J + 1>([&](auto J1c) {
+        constexpr int J1 = decltype(J1c)::value;
+        static_for<0, J1 + 1>([&](auto J2c) {
+            constexpr int J2 = decltype(J2c)::value;
+            if constexpr (coupled(J1, J2, JO)) {
+                const double beta = beta_t[triple_id(J, J1, J2, JO)];
+                z_row_lds<J, J1, J2, JO>(mb, malim, lr, li, lane, s.cg, beta, yr, yi);
+            }
+        });
+    });
+    if (!live) return;
+    const size_t n = static_cast<size_t>(N);
+    const int hrow = h_block(JO) + (JO + 1) * mb;
+    const size_t o = static_cast<size_t>(hrow) * n + atom;
+    double e = 0.0;
+#pragma unroll
+    for (int ma = 0; ma <= JO; ++ma) {
+        if (ma <= malim) {
+            Yr[o + ma * n] = yr[ma];
+            Yi[o + ma * n] = yi[ma];
+            const double w = (2 * mb < JO || ma < mb) ? 1.0 : 0.5;
+            e += w * (lr[(hrow + ma) * 64 + lane] * yr[ma] + li[(hrow + ma) * 64 + lane] * yi[ma]);
+        }
+    }
+    epart[static_cast<size_t>(row) * n + atom] = e;
+}
+
+template <int J, int W>
+__global__ __launch_bounds__(64 * W) void k_yi_half(SnapDev s, const double* __restrict__ Ur,
+                                                    const double* __restrict__ Ui, const double* __restrict__ beta_t,
+                                                    double* __restrict__ Yr, double* __restrict__ Yi,
+                                                    double* __restrict__ epart, const int* __restrict__ order, int N) {
+    constexpr int HE = n_half_elems(J);
+    constexpr int NR = n_half_rows(J);
+    __shared__ double lr[HE * 64], li[HE * 64];
+    __shared__ int next_row;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int atom = blockIdx.x * 64 + lane;
+    const bool live = atom < N;
+    const size_t n = static_cast<size_t>(N);
+    if (threadIdx.x == 0) next_row = 0;
+    for (int e = wave; e < HE; e += W) {
+        lr[e * 64 + lane] = live ? Ur[static_cast<size_t>(e) * n + atom] : 0.0;
+        li[e * 64 + lane] = live ? Ui[static_cast<size_t>(e) * n + atom] : 0.0;
+    }
+    __syncthreads();
+    for (;;) {
+        int r = 0;
+        if (lane == 0) r = atomicAdd(&next_row, 1);
+        r = __shfl(r, 0);
+        if (r >= NR) break;
+        const int row = order[r];
+        const int jo = half_row_j(row), mb = half_row_mb(row);
+        static_for<0, J + 1>([&](auto JOc) {
+            constexpr int JO = decltype(JOc)::value;
+            if (jo == JO) y_row_lds<J, JO>(s, row, mb, atom, live, N, lr, li, lane, beta_t, Yr, Yi, epart);
+        });
+    }
+}
+
+}  // namespace mdp

@@ -11,12 +11,15 @@
 //   4. SNAP: Utot (k_ui), Y + energy partials (k_yi), dE/dr per pair (k_dedr)
 //   5. tally per atom without atomics: f_i = sum over own pairs - sum over incoming pairs,
 //      W_i = -sum_j r_ij (x) dE/dr_ij, E_i from the energy partials; then v += dt/2 f/m.
-// Neighbour entries are (owner j, image shift) pairs; positions stay unwrapped between rebuilds and
+// Neighbour entries are (owner j, image shift) pairs stored slot-major (slot k of atom i at k*N + i,
+// so that per-atom loops read coalesced across the lanes of a wave); positions stay unwrapped between rebuilds and
 // the displacement of a neighbour is always (x_j + shift*L) - x_i, evaluated without contraction.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <string>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -144,7 +147,7 @@ __global__ __launch_bounds__(64) void k_neighbors_cell(const double* __restrict_
                                                        const int* __restrict__ cell_atoms, int* __restrict__ nb_cnt,
                                                        int* __restrict__ nb_j, int* __restrict__ nb_s, double Lx,
                                                        double Ly, double Lz, int ncx, int ncy, int ncz, double rcs2,
-                                                       int* __restrict__ overflow) {
+                                                       int N, int* __restrict__ overflow) {
     __shared__ double tx[27 * CELLCAP], ty[27 * CELLCAP], tz[27 * CELLCAP];
     __shared__ int tj[27 * CELLCAP], ts[27 * CELLCAP];
     __shared__ int toff[28];
@@ -193,8 +196,8 @@ __global__ __launch_bounds__(64) void k_neighbors_cell(const double* __restrict_
             const double d2 = dist2_nc(tx[t] - xi, ty[t] - yi, tz[t] - zi);
             if (d2 <= rcs2) {
                 if (cnt < NBMAX) {
-                    nb_j[static_cast<size_t>(i) * NBMAX + cnt] = tj[t];
-                    nb_s[static_cast<size_t>(i) * NBMAX + cnt] = ts[t];
+                    nb_j[static_cast<size_t>(cnt) * N + i] = tj[t];
+                    nb_s[static_cast<size_t>(cnt) * N + i] = ts[t];
                 }
                 ++cnt;
             }
@@ -223,8 +226,8 @@ __global__ void k_neighbors_brute(const double* __restrict__ x, int* __restrict_
                                                image_nc(x[3 * j + 2], sz, Lz, zi));
                     if (d2 <= rcs2) {
                         if (cnt < NBMAX) {
-                            nb_j[static_cast<size_t>(i) * NBMAX + cnt] = j;
-                            nb_s[static_cast<size_t>(i) * NBMAX + cnt] = shift_code(sx, sy, sz);
+                            nb_j[static_cast<size_t>(cnt) * N + i] = j;
+                            nb_s[static_cast<size_t>(cnt) * N + i] = shift_code(sx, sy, sz);
                         }
                         ++cnt;
                     }
@@ -239,11 +242,11 @@ __global__ void k_incoming(const int* __restrict__ nb_cnt, const int* __restrict
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     for (int k = 0; k < nb_cnt[i]; ++k) {
-        const int e = i * NBMAX + k;
+        const int e = k * N + i;
         const int j = nb_j[e];
         const int slot = atomicAdd(&in_cnt[j], 1);
         if (slot < NBMAX)
-            in_e[static_cast<size_t>(j) * NBMAX + slot] = e;
+            in_e[static_cast<size_t>(slot) * N + j] = e;
         else
             atomicOr(overflow, 4);
     }
@@ -254,15 +257,16 @@ __global__ void k_incoming_sort(int* __restrict__ in_cnt, int* __restrict__ in_e
     if (i >= N) return;
     const int n = min(in_cnt[i], NBMAX);
     in_cnt[i] = n;
-    int* a = in_e + static_cast<size_t>(i) * NBMAX;
+    int* a = in_e + i;   // slot k at a[k*N]
+    const size_t st = static_cast<size_t>(N);
     for (int k = 1; k < n; ++k) {
-        const int v = a[k];
+        const int v = a[k * st];
         int m = k - 1;
-        while (m >= 0 && a[m] > v) {
-            a[m + 1] = a[m];
+        while (m >= 0 && a[m * st] > v) {
+            a[(m + 1) * st] = a[m * st];
             --m;
         }
-        a[m + 1] = v;
+        a[(m + 1) * st] = v;
     }
 }
 
@@ -284,7 +288,7 @@ __global__ void k_pair_count_md(const double* __restrict__ x, const int* __restr
     const int ti = type[i];
     int c = 0;
     for (int k = 0; k < nb_cnt[i]; ++k) {
-        const int e = i * NBMAX + k, j = nb_j[e];
+        const int e = k * N + i, j = nb_j[e];
         double dx, dy, dz;
         entry_disp(x, i, j, nb_s[e], Lx, Ly, Lz, dx, dy, dz);
         const double d2 = dist2_nc(dx, dy, dz);
@@ -304,7 +308,7 @@ __global__ void k_pair_fill_md(const double* __restrict__ x, const int* __restri
     const int ti = type[i];
     int p = poff[i];
     for (int k = 0; k < nb_cnt[i]; ++k) {
-        const int e = i * NBMAX + k, j = nb_j[e];
+        const int e = k * N + i, j = nb_j[e];
         double dx, dy, dz;
         entry_disp(x, i, j, nb_s[e], Lx, Ly, Lz, dx, dy, dz);
         const double d2 = dist2_nc(dx, dy, dz);
@@ -351,7 +355,7 @@ __global__ void k_tally(const double* __restrict__ dedr, const double* __restric
     }
     const int nin = in_cnt[i];
     for (int k = 0; k < nin; ++k) {
-        const int p = entry_pair[in_e[static_cast<size_t>(i) * NBMAX + k]];
+        const int p = entry_pair[in_e[static_cast<size_t>(k) * N + i]];
         if (p >= 0) {
             fx -= dedr[3 * static_cast<size_t>(p)];
             fy -= dedr[3 * static_cast<size_t>(p) + 1];
@@ -402,6 +406,10 @@ struct mdp_md {
         beta_t, rc2, flags, maxd2, row_order;
     long stats[6] = {0, 0, 0, 0, 0, 0};
     bool forces_current = false;
+    // autotuned launch variants (see autotune())
+    int dedr_variant = 3, yi_waves = 8;
+    bool tuned = false;
+    std::string tuning_json = "{}";
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -592,7 +600,7 @@ struct mdp_md {
                                cell_start.as<int>(), cell_atoms.as<int>(), static_cast<int>(ncells), flags.as<int>());
             hipLaunchKernelGGL(k_neighbors_cell, dim3(static_cast<unsigned>(ncells)), dim3(64), 0, st, x.as<double>(),
                                cell_start.as<int>(), cell_atoms.as<int>(), nb_cnt.as<int>(), nb_j.as<int>(),
-                               nb_s.as<int>(), L[0], L[1], L[2], ncell[0], ncell[1], ncell[2], rcs * rcs,
+                               nb_s.as<int>(), L[0], L[1], L[2], ncell[0], ncell[1], ncell[2], rcs * rcs, N,
                                flags.as<int>());
         } else {
             hipLaunchKernelGGL(k_neighbors_brute, dim3(nb(N, 64)), dim3(64), 0, st, x.as<double>(), nb_cnt.as<int>(),
@@ -645,12 +653,12 @@ struct mdp_md {
         });
         timed(T_YI, [&] {
             launch_yi_half(s, Ur.as<double>(), Ui.as<double>(), beta_t.as<double>(), Yr.as<double>(), Yi.as<double>(),
-                           epart.as<double>(), row_order.as<int>(), N, st);
+                           epart.as<double>(), row_order.as<int>(), N, st, yi_waves);
         });
         timed(T_DEDR, [&] {
             launch_dedr(s, Yr.as<double>(), Yi.as<double>(), prij.as<double>(), pi.as<int>(), pti.as<int>(),
                         ptj.as<int>(), nullptr, dedr.as<double>(), N, static_cast<int>(stats[1]), poff.as<int>() + N,
-                        st, true);
+                        st, true, dedr_variant);
         });
         timed(T_TALLY, [&] {
             hipLaunchKernelGGL(k_tally, dim3(nb(N, 256)), dim3(256), 0, st, dedr.as<double>(), prij.as<double>(),
@@ -660,10 +668,67 @@ struct mdp_md {
         });
     }
 
+    // Time every compiled variant of the two heaviest kernels on the live data (both are pure
+    // functions of their inputs, so re-running them is harmless) and keep the fastest.
+    template <class F>
+    double time_launch(F&& f, int reps) {
+        hipEvent_t a = ev(), b = ev();
+        f();   // warm-up (code object load, caches)
+        MDP_HIP(hipEventRecord(a, st));
+        for (int r = 0; r < reps; ++r) f();
+        MDP_HIP(hipEventRecord(b, st));
+        MDP_HIP(hipEventSynchronize(b));
+        float ms = 0;
+        MDP_HIP(hipEventElapsedTime(&ms, a, b));
+        ev_next -= 2;
+        return ms / reps;
+    }
+    void autotune() {
+        const SnapDev& s = tab.view;
+        const char* env = std::getenv("MDP_AUTOTUNE");
+        if (env && std::string(env) == "0") {
+            tuned = true;
+            return;
+        }
+        std::string js = "{";
+        int ids[8];
+        int nv = dedr_variants(ids);
+        double best = 1e30;
+        js += "\"compute_dedr\": {";
+        for (int k = 0; k < nv; ++k) {
+            const double t = time_launch([&] {
+                launch_dedr(s, Yr.as<double>(), Yi.as<double>(), prij.as<double>(), pi.as<int>(), pti.as<int>(),
+                            ptj.as<int>(), nullptr, dedr.as<double>(), N, static_cast<int>(stats[1]),
+                            poff.as<int>() + N, st, true, ids[k]);
+            }, 3);
+            js += (k ? ", " : "") + std::string("\"") + std::to_string(ids[k]) + "\": " + std::to_string(t);
+            if (t < best) { best = t; dedr_variant = ids[k]; }
+        }
+        js += "}, \"compute_yi\": {";
+        nv = yi_half_variants(J, ids);
+        best = 1e30;
+        for (int k = 0; k < nv; ++k) {
+            const double t = time_launch([&] {
+                launch_yi_half(s, Ur.as<double>(), Ui.as<double>(), beta_t.as<double>(), Yr.as<double>(),
+                               Yi.as<double>(), epart.as<double>(), row_order.as<int>(), N, st, ids[k]);
+            }, 3);
+            js += (k ? ", " : "") + std::string("\"") + std::to_string(ids[k]) + "\": " + std::to_string(t);
+            if (t < best) { best = t; yi_waves = ids[k]; }
+        }
+        js += "}, \"chosen\": {\"compute_dedr\": " + std::to_string(dedr_variant) +
+              ", \"compute_yi_waves\": " + std::to_string(yi_waves) + "}}";
+        tuning_json = js;
+        tuned = true;
+    }
+
     void compute() {
         timed(T_REBUILD, [&] { rebuild(false); });
         timed(T_PAIRS, [&] { pairs(); });
         snap_and_tally(false);
+        if (!tuned) {
+            autotune();
+            snap_and_tally(false);   // final state from the chosen variants
+        }
         MDP_HIP(hipGetLastError());
         MDP_HIP(hipStreamSynchronize(st));
         int np = 0;
@@ -759,9 +824,9 @@ int mdp_md_neighbors(mdp_md* md, int* off, int* nbj, int* nbshift, long* nentrie
         for (int i = 0; i < N; ++i) {
             off[i] = static_cast<int>(k);
             for (int m = 0; m < cnt[i]; ++m, ++k) {
-                nbj[k] = js[static_cast<size_t>(i) * NBMAX + m];
+                nbj[k] = js[static_cast<size_t>(m) * N + i];
                 int sx, sy, sz;
-                shift_decode(ss[static_cast<size_t>(i) * NBMAX + m], sx, sy, sz);
+                shift_decode(ss[static_cast<size_t>(m) * N + i], sx, sy, sz);
                 nbshift[3 * k] = sx;
                 nbshift[3 * k + 1] = sy;
                 nbshift[3 * k + 2] = sz;
@@ -795,6 +860,16 @@ int mdp_md_timers(mdp_md* md, double* ms_total, long* launches, int n) {
             ms_total[k] = md->t_ms[k];
             launches[k] = md->t_n[k];
         }
+    });
+}
+
+int mdp_md_tuning(mdp_md* md, char* buf, int len) {
+    return api([&] {
+        if (!buf || len <= 0) return;
+        const std::string& j = md->tuning_json;
+        const size_t n = std::min(static_cast<size_t>(len - 1), j.size());
+        std::memcpy(buf, j.data(), n);
+        buf[n] = '\0';
     });
 }
 

@@ -51,7 +51,9 @@ void launch_yi(const SnapDev& s, const double* Ur, const double* Ui, const doubl
 // of 64 atoms is staged in LDS; the workgroup's waves take output rows from a shared counter in the
 // order row_order (device array of n_half_rows(J) row ids, most expensive first).
 void launch_yi_half(const SnapDev& s, const double* Uhr, const double* Uhi, const double* beta_t, double* Yhr,
-                    double* Yhi, double* epart, const int* row_order, int natoms, hipStream_t st);
+                    double* Yhi, double* epart, const int* row_order, int natoms, hipStream_t st, int waves = 8);
+// waves-per-workgroup variants available for twojmax J (ids = waves); returns their number
+int yi_half_variants(int J, int* ids);
 // row ids of the half domain sorted by decreasing compute_yi work
 void yi_row_order(int J, int* out);
 
@@ -69,7 +71,10 @@ void launch_zi(const SnapDev& s, const double* Ur, const double* Ui, long off1, 
 // element offset (elements) for chemflag.
 void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const double* rij, const int* yatom,
                  const int* ti, const int* tj, const long* ybase, double* dedr, int natoms, int npairs_host,
-                 const int* npairs_dev, hipStream_t st, bool half_layout = false);
+                 const int* npairs_dev, hipStream_t st, bool half_layout = false, int variant = 0);
+// Variants of the half-layout dE/dr kernel (0: all three components per lane; 2, 3, 4: one component per
+// lane with at least that many waves per SIMD).  Writes the ids, returns their number.
+int dedr_variants(int* ids);
 
 // ---- generic helpers ----------------------------------------------------------------------------
 

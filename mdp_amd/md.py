@@ -53,6 +53,7 @@ def _bind():
     c.mdp_md_timer_name.argtypes = [ctypes.c_int]
     c.mdp_md_num_timers.argtypes = []
     c.mdp_md_energy.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    c.mdp_md_tuning.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
     _bound = True
     return c
 
@@ -143,6 +144,13 @@ class SnapMD:
         n = np.zeros(k, np.int64)
         self._check(self._c.mdp_md_timers(self._h, ms.ctypes.data, n.ctypes.data, k), "mdp_md_timers")
         return {self._c.mdp_md_timer_name(i).decode(): (float(ms[i]), int(n[i])) for i in range(k)}
+
+    def tuning(self):
+        """Launch variants the autotuner chose (and their measured times), as a dict."""
+        import json
+        buf = ctypes.create_string_buffer(4096)
+        self._check(self._c.mdp_md_tuning(self._h, buf, 4096), "mdp_md_tuning")
+        return json.loads(buf.value.decode() or "{}")
 
     def energy(self):
         ep, ek = ctypes.c_double(), ctypes.c_double()
