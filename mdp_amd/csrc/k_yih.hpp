@@ -74,7 +74,7 @@ __device__ __forceinline__ void z_row_lds(int mb, int malim, const double* __res
 }
 
 template <int J, int JO>
-__device__ __forceinline__ void y_row_lds(const SnapDev& s, int row, int mb, int atom, bool live, int N,
+__device__ __forceinline__ void y_row_lds(const double* __restrict__ cg, int row, int mb, int atom, bool live, int N,
                                           const double* __restrict__ lr, const double* __restrict__ li, int lane,
                                           const double* __restrict__ beta_t, double* __restrict__ Yr,
                                           double* __restrict__ Yi, double* __restrict__ epart) {
@@ -88,7 +88,7 @@ __device__ __forceinline__ void y_row_lds(const SnapDev& s, int row, int mb, int
             constexpr int J2 = decltype(J2c)::value;
             if constexpr (coupled(J1, J2, JO)) {
                 const double beta = beta_t[triple_id(J, J1, J2, JO)];
-                z_row_lds<J, J1, J2, JO>(mb, malim, lr, li, lane, s.cg, beta, yr, yi);
+                z_row_lds<J, J1, J2, JO>(mb, malim, lr, li, lane, cg, beta, yr, yi);
             }
         });
     });
@@ -109,8 +109,10 @@ __device__ __forceinline__ void y_row_lds(const SnapDev& s, int row, int mb, int
     epart[static_cast<size_t>(row) * n + atom] = e;
 }
 
+// The coefficient tables (cg, beta_t, order) are separate read-only, non-aliased kernel arguments so
+// that their wave-uniform loads go through the scalar cache into SGPRs.
 template <int J, int W>
-__global__ __launch_bounds__(64 * W) void k_yi_half(SnapDev s, const double* __restrict__ Ur,
+__global__ __launch_bounds__(64 * W) void k_yi_half(const double* __restrict__ cg, const double* __restrict__ Ur,
                                                     const double* __restrict__ Ui, const double* __restrict__ beta_t,
                                                     double* __restrict__ Yr, double* __restrict__ Yi,
                                                     double* __restrict__ epart, const int* __restrict__ order, int N) {
@@ -131,13 +133,14 @@ __global__ __launch_bounds__(64 * W) void k_yi_half(SnapDev s, const double* __r
     for (;;) {
         int r = 0;
         if (lane == 0) r = atomicAdd(&next_row, 1);
-        r = __shfl(r, 0);
+        // scalar (wave-uniform) row id: every coefficient address below becomes a scalar load
+        r = __builtin_amdgcn_readfirstlane(__shfl(r, 0));
         if (r >= NR) break;
-        const int row = order[r];
+        const int row = __builtin_amdgcn_readfirstlane(order[r]);
         const int jo = half_row_j(row), mb = half_row_mb(row);
         static_for<0, J + 1>([&](auto JOc) {
             constexpr int JO = decltype(JOc)::value;
-            if (jo == JO) y_row_lds<J, JO>(s, row, mb, atom, live, N, lr, li, lane, beta_t, Yr, Yi, epart);
+            if (jo == JO) y_row_lds<J, JO>(cg, row, mb, atom, live, N, lr, li, lane, beta_t, Yr, Yi, epart);
         });
     }
 }

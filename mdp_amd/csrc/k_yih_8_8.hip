@@ -5,7 +5,7 @@ namespace mdp {
 
 void launch_yi_half_8_8(const SnapDev& s, const double* Ur, const double* Ui, const double* beta_t, double* Yr,
                           double* Yi, double* epart, const int* order, int N, hipStream_t st) {
-    hipLaunchKernelGGL((k_yi_half<8, 8>), dim3((N + 63) / 64), dim3(64 * 8), 0, st, s, Ur, Ui, beta_t, Yr, Yi,
+    hipLaunchKernelGGL((k_yi_half<8, 8>), dim3((N + 63) / 64), dim3(64 * 8), 0, st, s.cg, Ur, Ui, beta_t, Yr, Yi,
                        epart, order, N);
 }
 
