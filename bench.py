@@ -8,6 +8,20 @@ bzero on, linear coefficients beta_k ~ N(0, 0.05) with beta0 = -11.7 eV, 300 K M
 velocities, mass 180.95 amu, dt = 0.5 fs, neighbour skin 0.3 A rebuilt every 10 steps.  Every step
 integrates, amortises the neighbour rebuild and evaluates SNAP energy, forces and virials in FP64.
 
+With --workload fit it measures the batched fitting-descriptor path instead: per-atom B and per-pair
+dB/dr (twojmax = 8, FP64) of 512 independent periodic configurations (atom counts log-uniform in
+[16, 2000]; BCC a ~ U[3.2, 3.45], FCC a ~ U[4.0, 4.3] and random packings at 0.0554 atoms/A^3 with 2.2 A
+minimum separation, all displaced by sigma = 0.1 A), one step = one replay of the captured batch graph
+(pair build, Utot, Z + B, dB/dr), reported as atoms/s.
+
+With --workload surrogate it measures the learned surrogate: a per-atom 14 -> 64 -> 64 -> 1 SiLU MLP
+(BF16 matrix-core kernels) over twojmax=4 descriptors, trained on the GPU on 4000 250-atom BCC/liquid
+configurations sampled from plain twojmax=8 SNAP MD at 300-3000 K (energies and forces, coefficients of
+the headline workload), then used for NVE MD of 80^3 BCC cells with 0.5% of the sites vacant
+(1,018,880 atoms, 600 K, dt = 1 fs) with a plain FP64 twojmax=8 audit every 50 steps that falls back
+to plain forces when the RMS force difference exceeds 0.05 eV/A.  Reported: surrogate atom-steps/s with
+audits (value), without audits, the plain-GPU rate on the same system, and the force errors.
+
 Prints one JSON line on stdout; diagnostics go to stderr.
 """
 import argparse
@@ -85,8 +99,159 @@ def cpu_baseline(ncell):
                        f"coefficients, {p['ijnum']} pairs), single thread, {dt:.2f} s")
 
 
+def bench_fit(a):
+    """Batched B + dB/dr of the 512-configuration set, timed over graph replays."""
+    import mdp_amd
+    from mdp_amd import lattice
+    from mdp_amd.fit import SnapFit
+    from mdp_amd.flops import snap_flops
+
+    mdp_amd.library()
+    t0 = time.perf_counter()
+    batch = lattice.fitting_batch()
+    cfg = [(x, box) for x, box, _ in batch]
+    natoms = sum(len(c[0]) for c in cfg)
+    log(f"512 configurations, {natoms} atoms, generated in {time.perf_counter() - t0:.1f} s")
+    fit = SnapFit(cfg, J, rcutfac=RCUTFAC, rfac0=RFAC0)
+    t0 = time.perf_counter()
+    fit.compute()
+    log(f"first compute (pair sizing, autotuning, graph capture): {1e3 * (time.perf_counter() - t0):.1f} ms")
+    log(f"autotuner: {json.dumps(fit.tuning())}")
+    _, p_off = fit.layout()
+    npairs = int(p_off[-1])
+    prof = fit.profile()
+    log(f"stages (ms, one uncaptured run): {json.dumps(prof)}")
+    fit.replay(a.warmup)
+    fit.sync()
+    t0 = time.perf_counter()
+    fit.replay(a.steps)
+    fit.sync()
+    elapsed = time.perf_counter() - t0
+    ms = 1e3 * elapsed / a.steps
+    value = natoms / (elapsed / a.steps)
+    fl = snap_flops(J)
+    dom = max(("zb", "dbdr"), key=lambda k: prof[k])
+    work = fl["dbdr"] * npairs if dom == "dbdr" else fl["zb"] * natoms
+    achieved = work / (prof[dom] * 1e-3) / 1e12
+    out_bytes = 8 * 3 * 55 * npairs + 8 * 55 * natoms
+    log(f"{a.steps} replays: {ms:.3f} ms per batch, {value:.4e} atoms/s, {npairs} pairs "
+        f"({npairs / natoms:.2f}/atom); output {out_bytes / 1e9:.2f} GB per batch; {dom}: {achieved:.2f} TFLOP/s")
+    out = {
+        "metric": "atoms/s, batched SNAP B + dB/dr, 2J=8, 512 configurations", "value": value, "unit": "atoms/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded BCC/FCC/random-packing configurations)",
+        "config": {"workload": "batched fitting descriptors: per-atom B and per-pair dB/dr of 512 periodic "
+                               "configurations (16-2000 atoms), twojmax=8, one captured graph per batch",
+                   "atoms": natoms, "pairs": npairs, "twojmax": J, "parallelism": "1 GPU",
+                   "autotuned": fit.tuning().get("chosen", {})},
+        "roofline": {"bound": "mfma", "kernel": "fit_" + dom, "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_launch": work, "ms_per_launch": prof[dom]},
+        "cpu_baseline": None,
+    }
+    fit.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_surrogate(a):
+    """Train the surrogate on plain-SNAP data, then run surrogate MD on 80^3 BCC cells with vacancies."""
+    import mdp_amd
+    from mdp_amd import lattice
+    from mdp_amd import surrogate as S
+    from mdp_amd.md import SnapMD
+
+    mdp_amd.library()
+    t0 = time.perf_counter()
+    data = S.make_training_set(nconfig=a.train_configs, log=log)
+    log(f"plain-SNAP training data: {len(data)} configurations in {time.perf_counter() - t0:.1f} s")
+    ntest = max(1, len(data) // 10)
+    rng = np.random.default_rng(SEED + 7)
+    perm = rng.permutation(len(data))
+    train = [data[i] for i in perm[ntest:]]
+    test = [data[i] for i in perm[:ntest]]
+    t0 = time.perf_counter()
+    ds_tr, ds_te = S.DeviceSet(train, log=log), S.DeviceSet(test)
+    log(f"twojmax=4 descriptors: {ds_tr.N + ds_te.N} atoms, {ds_tr.P + ds_te.P} pairs in {time.perf_counter() - t0:.1f} s")
+    mu, sg, es, esh = S.affine_from(ds_tr)
+    m0 = S.Model.init(mu, sg, es, esh, seed=1)
+    t0 = time.perf_counter()
+    model, hist = S.train(ds_tr, m0, steps=a.train_steps, lr=3e-3, lr_end=1e-4, log=log, every=250)
+    t_train = time.perf_counter() - t0
+    f_te, e_te = S.evaluate(ds_te, model)
+    f_lin, e_lin, _ = S.linear_baseline(ds_tr, ds_te)
+    log(f"training {a.train_steps} steps in {t_train:.1f} s; held-out force RMSE {f_te:.4f} eV/A "
+        f"(linear twojmax=4 SNAP {f_lin:.4f}), energy RMSE {e_te:.4f} eV/atom (linear {e_lin:.4f})")
+
+    # production system: 80^3 BCC cells, 0.5% vacancies, 600 K, dt = 1 fs
+    x, box = lattice.crystal("bcc", a.sur_ncell, A0, SIGMA, SEED + 10)
+    keep = np.ones(len(x), bool)
+    keep[np.random.default_rng(SEED + 11).choice(len(x), int(round(0.005 * len(x))), replace=False)] = False
+    x = np.ascontiguousarray(x[keep])
+    n = len(x)
+    v = lattice.maxwell_boltzmann(n, MASS, 600.0, seed=SEED + 12)
+    coeff = S.plain_coefficients()
+    md = SnapMD(x, box, np.ones(n, np.int32), J, coeff, [0.5], [1.0], [MASS], rcutfac=RCUTFAC, rfac0=RFAC0, v=v,
+                dt=0.001, skin=SKIN, rebuild_every=EVERY)
+    md.compute()
+    md.run(a.warmup)
+    md.set_timing(True)
+    t0 = time.perf_counter()
+    md.run(a.plain_steps)
+    t_plain = (time.perf_counter() - t0) / a.plain_steps
+    md.set_timing(False)
+    md.set_surrogate(model, audit_every=0)
+    md.compute()
+    md.run(a.warmup)
+    t0 = time.perf_counter()
+    md.run(a.noaudit_steps)
+    t_noaudit = (time.perf_counter() - t0) / a.noaudit_steps
+    md.set_surrogate(model, audit_every=50, audit_tol=0.05)
+    md.compute()
+    md.set_timing(True)
+    t0 = time.perf_counter()
+    md.run(a.steps)
+    elapsed = time.perf_counter() - t0
+    timers = md.timers()
+    aud = md.audits()
+    st = md.stats()
+    md.close()
+    value = n * a.steps / elapsed
+    log(f"{n} atoms: plain {1e3 * t_plain:.2f} ms/step, surrogate without audits {1e3 * t_noaudit:.2f} ms/step, "
+        f"with audits {1e3 * elapsed / a.steps:.2f} ms/step; audits {aud}")
+    for k, (ms, cnt) in timers.items():
+        if cnt:
+            log(f"  {k:16s} {ms:9.2f} ms total, {cnt:6d} launches, {ms / max(cnt, 1):8.4f} ms avg")
+    out = {
+        "metric": "atom-steps/s, surrogate MD (twojmax=4 MLP) with plain twojmax=8 audits every 50 steps",
+        "value": value, "unit": "atom-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * elapsed / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 MLP / f64 descriptors, forces and integration",
+        "data": "synthetic (plain-SNAP MD samples for training; seeded BCC lattice with vacancies for MD)",
+        "config": {"workload": f"surrogate NVE MD, BCC Ta {a.sur_ncell}^3 cells with 0.5% vacancies ({n} atoms), "
+                               "600 K, dt=1 fs, audit every 50 steps", "atoms": n, "parallelism": "1 GPU",
+                   "train_configs": len(train), "train_steps": a.train_steps},
+        "surrogate": {"atom_steps_per_s_without_audits": n / t_noaudit, "plain_gpu_atom_steps_per_s": n / t_plain,
+                      "speedup_vs_plain_with_audits": (n * a.steps / elapsed) / (n / t_plain),
+                      "speedup_vs_plain_without_audits": t_plain / t_noaudit,
+                      "heldout_force_rmse": f_te, "heldout_energy_rmse_per_atom": e_te,
+                      "linear_j4_force_rmse": f_lin, "linear_j4_energy_rmse_per_atom": e_lin,
+                      "audit_force_rmse_mean": aud["mean_rmse"], "audit_force_rmse_max": aud["max_rmse"],
+                      "audits": aud["audits"], "fallbacks": aud["fallbacks"], "target_force_rmse": 0.05,
+                      "pairs_per_atom": st["pairs"] / n},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--workload", choices=("md", "fit", "surrogate"), default="md")
+    ap.add_argument("--train-configs", type=int, default=4000)
+    ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--sur-ncell", type=int, default=80)
+    ap.add_argument("--plain-steps", type=int, default=20)
+    ap.add_argument("--noaudit-steps", type=int, default=500)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
@@ -99,6 +264,13 @@ def main():
         print("bench.py runs on exactly one GPU: the SNAP MD path is measured per GPU (multi-GPU spatial "
               "decomposition is not part of this build); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if a.workload == "fit":
+        bench_fit(a)
+        return
+    if a.workload == "surrogate":
+        bench_surrogate(a)
+        return
 
     import mdp_amd
     from mdp_amd.flops import snap_flops

@@ -14,6 +14,9 @@
  * plays the role of the upstream C++ velocity-Verlet driver, src/C++/Integration/Integration.cpp,
  * around the same SNAP kernels): neighbour lists, SNAP forces and NVE integration stay on the GPU.
  *
+ * Part 3 computes fitting descriptors (B and dB/dr) of many configurations in one batch.
+ * Part 4 holds the kernels of the learned surrogate (per-atom MLP over bispectrum descriptors).
+ *
  * Errors: the drop-in functions return void like upstream; a failure leaves a message readable
  * with mdp_snap_last_error() (empty string when the last call succeeded).  Part 2 returns 0 on
  * success, nonzero on failure (message likewise).
@@ -192,6 +195,110 @@ int mdp_md_tuning(mdp_md *md, char *buf, int len);
 
 /* Total potential and kinetic energy (eV) of the current state. */
 int mdp_md_energy(mdp_md *md, double *epot, double *ekin);
+
+/* Learned surrogate (no upstream counterpart): from the next force evaluation on, forces and
+ * energies come from the per-atom MLP of part 4 (params: its FP32 parameters; aff: mu[16] |
+ * 1/sigma[16] | e_scale | e_shift over descriptors B - bzero_j of twojmax 2 or 4, nin = their count),
+ * through the descriptor chain rule.  Every audit_every steps (0 = never) plain SNAP is evaluated on
+ * the same state; if the RMS force difference exceeds audit_tol (eV/A) that step uses the plain
+ * forces.  mdp_md_clear_surrogate returns to plain SNAP. */
+int mdp_md_set_surrogate(mdp_md *md, const float *params, const double *aff, int nin, int twojmax,
+                         int audit_every, double audit_tol);
+int mdp_md_clear_surrogate(mdp_md *md);
+/* [0] audits, [1] fallbacks to plain forces, [2] last, [3] max, [4] mean RMS force difference (eV/A) */
+int mdp_md_audits(mdp_md *md, double *out5);
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Part 3: batched fitting descriptors                                                             */
+/* ---------------------------------------------------------------------------------------------- */
+
+/* Per-atom bispectrum B and per-pair dB/dr of many independent periodic configurations at once.
+ * Upstream computes them one configuration at a time (cpuComputeUij -> cpuZeroUarraytot2 ->
+ * cpuAddUarraytot -> cpuComputeZi2 -> cpuComputeBi2 -> cpuComputeDbidrj, cpuSnap.cpp:1620-2263,
+ * driven by snapcpp.jl:539-665 per configuration).  Here the pair build, Utot, Z + B and dB/dr of
+ * the whole batch are one captured hipGraph; dB/dr is a persistent kernel over all atoms.
+ * Single chemical element block (chemflag = 0); wself = 1. */
+typedef struct mdp_fit mdp_fit;   /* opaque */
+
+typedef struct {
+    int twojmax;                /* 2, 4, 6 or 8 */
+    int ntypes;                 /* atom types 1..ntypes */
+    double rcutfac, rfac0, rmin0;
+    int switchflag, bzeroflag, bnormflag;
+    const double *elemradius;   /* [ntypes] */
+    const double *elemweight;   /* [ntypes] */
+} mdp_fit_params;
+
+/* nconfig configurations; configuration c has natoms[c] atoms (concatenated in x[3*sum] AoS and
+ * types[sum], 1-based) in the orthogonal periodic box boxes[3c..3c+2].  Pairs of atom i are all
+ * periodic images x_j + s*L (j of the same configuration, the atom itself at s = 0 excluded) with
+ * |x_j + s*L - x_i|^2 <= ((r_ti + r_tj) rcutfac)^2, ordered by j then (sz, sy, sx). */
+int mdp_fit_create(mdp_fit **h, const mdp_fit_params *p, int nconfig, const int *natoms, const double *boxes,
+                   const double *x, const int *types);
+int mdp_fit_destroy(mdp_fit *h);
+/* Replace all positions (same configurations and atom counts). */
+int mdp_fit_set_positions(mdp_fit *h, const double *x);
+/* Pairs, Utot, Z, B and dB/dr of every configuration (the first call also autotunes the dB/dr
+ * launch and captures the graph); synchronous. */
+int mdp_fit_compute(mdp_fit *h);
+/* Replay the captured graph nreps times without waiting (timing loops); mdp_fit_sync waits. */
+int mdp_fit_replay(mdp_fit *h, int nreps);
+int mdp_fit_sync(mdp_fit *h);
+/* atom_off[nconfig+1], pair_off[nconfig+1]: first atom / pair of each configuration (totals last). */
+int mdp_fit_layout(mdp_fit *h, long *atom_off, long *pair_off);
+/* Outputs of the last compute, configuration after configuration, each in the upstream layout of
+ * that configuration alone: blist block of configuration c at atom_off[c]*idxb_max, element
+ * [ii + N_c*jjb] (cpuComputeBi2); dblist block at pair_off[c]*3*idxb_max, element
+ * [ij + P_c*k + 3*P_c*jjb] (cpuComputeDbidrj).  Either pointer may be NULL. */
+int mdp_fit_get(mdp_fit *h, double *blist, double *dblist);
+/* Pair arrays of the last compute: ai/aj local 0-based atom indices within each configuration (aj is
+ * the owner of the image), rij[3P] AoS.  Any pointer may be NULL. */
+int mdp_fit_pairs(mdp_fit *h, int *ai, int *aj, double *rij);
+/* Autotuner choices and timings (JSON). */
+int mdp_fit_tuning(mdp_fit *h, char *buf, int len);
+/* ms of one uncaptured run of each stage: [0] pairs, [1] Utot, [2] Z+B, [3] dB/dr. */
+int mdp_fit_profile(mdp_fit *h, double *ms4);
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Part 4: learned surrogate kernels (no upstream counterpart)                                     */
+/* ---------------------------------------------------------------------------------------------- */
+
+/* Per-atom MLP x -> 64 -> 64 -> 1 (SiLU) over nin <= 16 standardized bispectrum components, BF16
+ * weights and activations on the matrix cores with FP32 accumulation.  Parameters are an FP32 array
+ * W1[16][64] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3; mdp_mlp_pack turns them into the BF16
+ * operand tiles the kernels read.  aff = mu[16] | 1/sigma[16] | e_scale | e_shift (doubles):
+ * E_atom = e_shift + e_scale * f((B - mu)/sigma).  Descriptor-shaped arrays are component-major
+ * (x[k*n + atom]).  All pointers are device pointers owned by the caller; calls are asynchronous on
+ * `stream` (a hipStream_t, NULL = the library stream) and allocate nothing. */
+int mdp_mlp_sizes(int *nparam, int *npack, int *nin_max, int *nhidden);
+int mdp_mlp_pack(const float *params, unsigned short *packed, void *stream);
+/* e[n] = E_atom; g[k*n + a] = dE_atom/dB_k (g may be NULL) */
+int mdp_mlp_eval(const unsigned short *packed, const float *params, const double *aff, int nin, const double *x,
+                 int n, double *e, double *g, void *stream);
+/* grad[nparam] = d/dparams sum_a (alpha_a E_a + u_a . g_a); partials: nwaves * nparam floats */
+int mdp_mlp_train(const unsigned short *packed, const float *params, const double *aff, int nin, const double *x,
+                  const double *u, const double *alpha, int n, float *partials, int nwaves, float *grad,
+                  void *stream);
+/* Adam step `step` (>= 1) with decoupled weight decay on the weight matrices */
+int mdp_mlp_adam(float *params, const float *grad, float *m, float *v, float lr, float beta1, float beta2,
+                 float eps, int step, float weight_decay, void *stream);
+
+/* Descriptor chain rule (pairs sorted by centre atom; dbdr[K][3][P] = dB_k(i)/dr_ij, r_ij = x_j - x_i):
+ * s[c][p] = sum_k g[k][pi[p]] dbdr[k][c][p]; F_a = sum over own pairs of s - sum over incoming pairs
+ * (in_idx lists, per atom, the pairs whose neighbour is that atom); R = F - fref. */
+int mdp_sur_pair_s(const double *g, const double *dbdr, const int *pi, int K, int n, long P, double *s,
+                   void *stream);
+int mdp_sur_atom_f(const double *s, const int *own_off, const int *in_off, const int *in_idx, long P, int n,
+                   const double *fref, double *F, double *R, void *stream);
+/* U[k][a] = coef sum_{pairs p of a} (R_a - R_pj) . dbdr[k][:][p]  (adjoint of coef/2 sum |R|^2), K <= 16 */
+int mdp_sur_u(const double *R, const double *dbdr, const int *own_off, const int *pj, int K, int n, long P,
+              double coef, double *U, void *stream);
+/* Ec[c] = sum of e over configuration c (atoms cfg_off[c]..cfg_off[c+1]-1); alpha_a (may be NULL) =
+ * coef 2 (Ec - eref_c) / N_c^2 */
+int mdp_sur_energy(const double *e, const int *cfg_off, const double *eref, int C, double coef, double *Ec,
+                   double *alpha, void *stream);
+/* partials[b] (b < nblocks): block sums of v^2 */
+int mdp_sumsq(const double *v, long n, double *partials, int nblocks, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,395 @@
+// Batched fitting descriptors: host engine and C ABI (mdp_fit_* in include/mdpsnap.h).
+//
+// Plays the role of the upstream per-configuration descriptor loop (snapcpp.jl:539-665 for one
+// configuration, repeated by Regression.cpp:68-95 / the Julia fitting drivers for every configuration
+// of a training set): B and dB/dr of many independent periodic configurations, computed for the
+// whole batch at once.  The stages (pair build, Utot, Z + B, dB/dr) are captured once into a
+// hipGraph and replayed by every mdp_fit_compute call; the dB/dr stage is a persistent kernel whose
+// waves take atoms from a work counter, so configurations of very different sizes share the GPU
+// without per-configuration launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fit_launch.hpp"
+#include "mdpsnap.h"
+#include "runtime.hpp"
+#include "snap_launch.hpp"
+
+using namespace mdp;
+
+struct mdp_fit {
+    int C = 0, N = 0, J = 0, ntypes = 0;
+    int bnorm = 0, bzeroflag = 1;
+    hipStream_t st = nullptr;
+    DevTables tab;
+    std::vector<int> a0_h;   // [C+1]
+    DevBuf x, type, cfg_of, cfg_a0, cfg_L, cfg_p0, rc2, pcnt, poff, scan_tmp, flags, counter;
+    DevBuf pi, pj, pti, ptj, rij, Ur, Ui, zt, blist, dblist, bzero, t_lo, t_b, t_z, t_w;
+    long cap = 0;            // pair capacity of the pair and dB/dr buffers
+    long npairs = 0;         // pairs of the last compute
+    FitTerms terms;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int dbdr_groups = 1024;
+    bool tuned = false;
+    std::string tuning_json = "{}";
+    double stage_ms[4] = {0, 0, 0, 0};
+
+    ~mdp_fit() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    void init(const mdp_fit_params* p, int nconfig, const int* natoms, const double* boxes, const double* x0,
+              const int* types) {
+        if (!p) throw Error("mdp_fit_create: null parameters");
+        if (!templated_j(p->twojmax)) throw Error("mdp_fit_create: twojmax must be 2, 4, 6 or 8");
+        if (nconfig <= 0) throw Error("mdp_fit_create: nconfig must be positive");
+        if (p->ntypes <= 0) throw Error("mdp_fit_create: ntypes must be positive");
+        J = p->twojmax;
+        C = nconfig;
+        ntypes = p->ntypes;
+        bnorm = p->bnormflag;
+        bzeroflag = p->bzeroflag;
+        a0_h.assign(C + 1, 0);
+        for (int c = 0; c < C; ++c) {
+            if (natoms[c] <= 0) throw Error("mdp_fit_create: every configuration needs at least one atom");
+            for (int d = 0; d < 3; ++d)
+                if (!(boxes[3 * c + d] > 0.0)) throw Error("mdp_fit_create: box lengths must be positive");
+            a0_h[c + 1] = a0_h[c] + natoms[c];
+        }
+        N = a0_h[C];
+        for (int g = 0; g < N; ++g)
+            if (types[g] < 1 || types[g] > ntypes) throw Error("mdp_fit_create: atom types must be 1..ntypes");
+        MDP_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+
+        SnapIndex ix;
+        ix.build(J, nullptr);
+        std::vector<double> radelem(ntypes + 1, 0.0), wjelem(ntypes + 1, 0.0);
+        std::vector<int> map(ntypes + 1, 0);
+        for (int t = 1; t <= ntypes; ++t) {
+            radelem[t] = p->elemradius[t - 1];
+            wjelem[t] = p->elemweight[t - 1];
+        }
+        tab.upload(ix, nullptr, nullptr, radelem.data(), wjelem.data(), map.data(), ntypes, st);
+        tab.view.rcutfac = p->rcutfac;
+        tab.view.rfac0 = p->rfac0;
+        tab.view.rmin0 = p->rmin0;
+        tab.view.switchflag = p->switchflag;
+
+        std::vector<double> rc2h((ntypes + 1) * (ntypes + 1), 0.0);
+        for (int a = 1; a <= ntypes; ++a)
+            for (int b = 1; b <= ntypes; ++b) {
+                const double rc = (radelem[a] + radelem[b]) * p->rcutfac;
+                rc2h[a * (ntypes + 1) + b] = rc * rc;
+            }
+        // bzero[j] = wself^3 (j+1), wself = 1 (snapcpp.jl:290-298); 1 under bnorm
+        std::vector<double> bz(J + 1);
+        for (int j = 0; j <= J; ++j) bz[j] = bnorm ? 1.0 : j + 1.0;
+        std::vector<int> cof(N);
+        for (int c = 0; c < C; ++c)
+            for (int g = a0_h[c]; g < a0_h[c + 1]; ++g) cof[g] = c;
+        std::vector<int> lo, tb, tz;
+        std::vector<double> tw;
+        fit_term_table(J, bnorm, lo, tb, tz, tw);
+
+        auto put = [&](DevBuf& b, const void* h, size_t bytes) {
+            b.ensure(bytes);
+            MDP_HIP(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, st));
+        };
+        const size_t n = N;
+        put(x, x0, sizeof(double) * 3 * n);
+        put(type, types, sizeof(int) * n);
+        put(cfg_of, cof.data(), sizeof(int) * n);
+        put(cfg_a0, a0_h.data(), sizeof(int) * (C + 1));
+        put(cfg_L, boxes, sizeof(double) * 3 * C);
+        put(rc2, rc2h.data(), sizeof(double) * rc2h.size());
+        put(bzero, bz.data(), sizeof(double) * bz.size());
+        put(t_lo, lo.data(), sizeof(int) * lo.size());
+        put(t_b, tb.data(), sizeof(int) * tb.size());
+        put(t_z, tz.data(), sizeof(int) * tz.size());
+        put(t_w, tw.data(), sizeof(double) * tw.size());
+        terms.lo = t_lo.as<int>();
+        terms.b = t_b.as<int>();
+        terms.z = t_z.as<int>();
+        terms.w = t_w.as<double>();
+        cfg_p0.ensure(sizeof(int) * (C + 1));
+        pcnt.ensure(sizeof(int) * n);
+        poff.ensure(sizeof(int) * (n + 1));
+        scan_tmp.ensure(sizeof(int) * (scan_tmp_ints(N) + 8));
+        flags.ensure(sizeof(int) * 4);
+        counter.ensure(sizeof(int) * 4);
+        const SnapIndex& I = tab.idx;
+        Ur.ensure(sizeof(double) * n * I.idxu_max);
+        Ui.ensure(sizeof(double) * n * I.idxu_max);
+        zt.ensure(sizeof(double) * 2 * n * I.idxz_max);
+        blist.ensure(sizeof(double) * n * I.idxb_max);
+        MDP_HIP(hipStreamSynchronize(st));   // host staging vectors go out of scope
+    }
+
+    void set_positions(const double* xh) {
+        MDP_HIP(hipMemcpyAsync(x.p, xh, sizeof(double) * 3 * N, hipMemcpyHostToDevice, st));
+        MDP_HIP(hipStreamSynchronize(st));
+    }
+
+    void ensure_pairs(long need) {
+        if (need <= cap) return;
+        const long c = need + need / 8 + 64;
+        pi.ensure(sizeof(int) * c);
+        pj.ensure(sizeof(int) * c);
+        pti.ensure(sizeof(int) * c);
+        ptj.ensure(sizeof(int) * c);
+        rij.ensure(sizeof(double) * 3 * c);
+        dblist.ensure(sizeof(double) * 3 * c * tab.idx.idxb_max);
+        cap = c;
+        if (gexec) {   // buffers moved: the captured graph refers to the old ones
+            MDP_HIP(hipGraphExecDestroy(gexec));
+            MDP_HIP(hipGraphDestroy(graph));
+            gexec = nullptr;
+            graph = nullptr;
+        }
+    }
+
+    // stage k of the pipeline (0 pairs, 1 Utot, 2 Z+B, 3 dB/dr)
+    void stage(int k) {
+        const SnapDev& s = tab.view;
+        switch (k) {
+            case 0:
+                launch_fill_zero_int(flags.as<int>(), 4, st);
+                launch_fit_pairs(false, x.as<double>(), type.as<int>(), cfg_of.as<int>(), cfg_a0.as<int>(),
+                                 cfg_L.as<double>(), rc2.as<double>(), ntypes, N, pcnt.as<int>(), nullptr, 0, nullptr,
+                                 nullptr, nullptr, nullptr, nullptr, flags.as<int>(), st);
+                exclusive_scan(pcnt.as<int>(), poff.as<int>(), N, scan_tmp.as<int>(), st);
+                launch_fit_cfg_p0(poff.as<int>(), cfg_a0.as<int>(), cfg_p0.as<int>(), C, st);
+                launch_fit_pairs(true, x.as<double>(), type.as<int>(), cfg_of.as<int>(), cfg_a0.as<int>(),
+                                 cfg_L.as<double>(), rc2.as<double>(), ntypes, N, pcnt.as<int>(), poff.as<int>(), cap,
+                                 rij.as<double>(), pi.as<int>(), pj.as<int>(), pti.as<int>(), ptj.as<int>(),
+                                 flags.as<int>(), st);
+                break;
+            case 1:
+                launch_ui(s, UI_FULL_WSELF, Ur.as<double>(), Ui.as<double>(), rij.as<double>(), poff.as<int>(), nullptr,
+                          pti.as<int>(), ptj.as<int>(), N, 1.0, -1, st);
+                break;
+            case 2:
+                launch_fit_zb(s, Ur.as<double>(), Ui.as<double>(), zt.as<double>(), blist.as<double>(),
+                              bzero.as<double>(), cfg_of.as<int>(), cfg_a0.as<int>(), N, bnorm, bzeroflag, st);
+                break;
+            default:
+                launch_fill_zero_int(counter.as<int>(), 1, st);
+                launch_fit_dbdr(s, zt.as<double>(), terms, rij.as<double>(), pti.as<int>(), ptj.as<int>(),
+                                poff.as<int>(), cfg_of.as<int>(), cfg_p0.as<int>(), dblist.as<double>(),
+                                counter.as<int>(), N, dbdr_groups, st);
+                break;
+        }
+    }
+
+    long count_pairs() {
+        launch_fit_pairs(false, x.as<double>(), type.as<int>(), cfg_of.as<int>(), cfg_a0.as<int>(), cfg_L.as<double>(),
+                         rc2.as<double>(), ntypes, N, pcnt.as<int>(), nullptr, 0, nullptr, nullptr, nullptr, nullptr,
+                         nullptr, flags.as<int>(), st);
+        exclusive_scan(pcnt.as<int>(), poff.as<int>(), N, scan_tmp.as<int>(), st);
+        int tot = 0;
+        MDP_HIP(hipMemcpyAsync(&tot, poff.as<int>() + N, sizeof(int), hipMemcpyDeviceToHost, st));
+        MDP_HIP(hipStreamSynchronize(st));
+        return tot;
+    }
+
+    void capture() {
+        MDP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < 4; ++k) stage(k);
+        MDP_HIP(hipStreamEndCapture(st, &graph));
+        MDP_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    }
+
+    float time_stage(int k, int reps) {
+        hipEvent_t a, b;
+        MDP_HIP(hipEventCreate(&a));
+        MDP_HIP(hipEventCreate(&b));
+        MDP_HIP(hipEventRecord(a, st));
+        for (int r = 0; r < reps; ++r) stage(k);
+        MDP_HIP(hipEventRecord(b, st));
+        MDP_HIP(hipEventSynchronize(b));
+        float ms = 0;
+        MDP_HIP(hipEventElapsedTime(&ms, a, b));
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return ms / reps;
+    }
+
+    // Time the persistent dB/dr kernel at a few grid sizes on the live data and keep the fastest
+    // (the kernel is a pure function of its inputs; its counter is reset inside the stage).
+    void autotune() {
+        const char* env = std::getenv("MDP_AUTOTUNE");
+        tuned = true;
+        if (env && std::string(env) == "0") return;
+        int props_cu = 256;
+        hipDeviceProp_t prop;
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            props_cu = prop.multiProcessorCount;
+        const int cand[] = {props_cu * 2, props_cu * 4, props_cu * 5, props_cu * 8};
+        double best = 1e30;
+        int pick = dbdr_groups;
+        std::string js = "{\"fit_dbdr_groups\": {";
+        for (int k = 0; k < 4; ++k) {
+            dbdr_groups = cand[k];
+            stage(3);   // warm-up
+            const double t = time_stage(3, 2);
+            js += (k ? ", " : "") + std::string("\"") + std::to_string(cand[k]) + "\": " + std::to_string(t);
+            if (t < best) { best = t; pick = cand[k]; }
+        }
+        dbdr_groups = pick;
+        js += "}, \"chosen\": {\"fit_dbdr_groups\": " + std::to_string(pick) + "}}";
+        tuning_json = js;
+    }
+
+    void compute() {
+        if (cap == 0) ensure_pairs(count_pairs());
+        if (!tuned) {
+            for (int k = 0; k < 4; ++k) stage(k);
+            MDP_HIP(hipGetLastError());
+            autotune();
+        }
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            if (!gexec) capture();
+            MDP_HIP(hipGraphLaunch(gexec, st));
+            int fl = 0, tot = 0;
+            MDP_HIP(hipMemcpyAsync(&fl, flags.p, sizeof(int), hipMemcpyDeviceToHost, st));
+            MDP_HIP(hipMemcpyAsync(&tot, poff.as<int>() + N, sizeof(int), hipMemcpyDeviceToHost, st));
+            MDP_HIP(hipStreamSynchronize(st));
+            npairs = tot;
+            if (!(fl & 1)) return;
+            ensure_pairs(tot);   // positions changed and the pair list outgrew its buffers
+        }
+        throw Error("mdp_fit_compute: pair buffers could not be sized");
+    }
+
+    // replay without the readback (for timing loops); returns nothing, errors surface in sync()
+    void replay() {
+        if (!gexec) throw Error("mdp_fit_replay: call mdp_fit_compute first");
+        MDP_HIP(hipGraphLaunch(gexec, st));
+    }
+
+    void profile() {
+        if (!gexec) compute();
+        for (int k = 0; k < 4; ++k) stage_ms[k] = time_stage(k, 1);
+    }
+};
+
+namespace {
+template <class F>
+int api(F&& f) {
+    clear_error();
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return 1;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int mdp_fit_create(mdp_fit** h, const mdp_fit_params* p, int nconfig, const int* natoms, const double* boxes,
+                   const double* x, const int* types) {
+    return api([&] {
+        if (!h) throw Error("mdp_fit_create: null handle pointer");
+        *h = nullptr;
+        auto e = std::make_unique<mdp_fit>();
+        e->init(p, nconfig, natoms, boxes, x, types);
+        *h = e.release();
+    });
+}
+
+int mdp_fit_destroy(mdp_fit* h) {
+    return api([&] { delete h; });
+}
+
+int mdp_fit_set_positions(mdp_fit* h, const double* x) {
+    return api([&] { h->set_positions(x); });
+}
+
+int mdp_fit_compute(mdp_fit* h) {
+    return api([&] { h->compute(); });
+}
+
+int mdp_fit_replay(mdp_fit* h, int nreps) {
+    return api([&] {
+        for (int r = 0; r < nreps; ++r) h->replay();
+    });
+}
+
+int mdp_fit_sync(mdp_fit* h) {
+    return api([&] {
+        MDP_HIP(hipStreamSynchronize(h->st));
+        MDP_HIP(hipGetLastError());
+    });
+}
+
+int mdp_fit_layout(mdp_fit* h, long* atom_off, long* pair_off) {
+    return api([&] {
+        if (atom_off)
+            for (int c = 0; c <= h->C; ++c) atom_off[c] = h->a0_h[c];
+        if (pair_off) {
+            std::vector<int> p(h->C + 1);
+            MDP_HIP(hipMemcpy(p.data(), h->cfg_p0.p, sizeof(int) * (h->C + 1), hipMemcpyDeviceToHost));
+            for (int c = 0; c <= h->C; ++c) pair_off[c] = p[c];
+        }
+    });
+}
+
+int mdp_fit_get(mdp_fit* h, double* blist, double* dblist) {
+    return api([&] {
+        const int ib = h->tab.idx.idxb_max;
+        if (blist)
+            MDP_HIP(hipMemcpy(blist, h->blist.p, sizeof(double) * static_cast<size_t>(h->N) * ib,
+                              hipMemcpyDeviceToHost));
+        if (dblist)
+            MDP_HIP(hipMemcpy(dblist, h->dblist.p, sizeof(double) * 3 * static_cast<size_t>(h->npairs) * ib,
+                              hipMemcpyDeviceToHost));
+    });
+}
+
+int mdp_fit_pairs(mdp_fit* h, int* ai, int* aj, double* rij) {
+    return api([&] {
+        const size_t P = h->npairs;
+        std::vector<int> gi(P), gj(P);
+        MDP_HIP(hipMemcpy(gi.data(), h->pi.p, sizeof(int) * P, hipMemcpyDeviceToHost));
+        MDP_HIP(hipMemcpy(gj.data(), h->pj.p, sizeof(int) * P, hipMemcpyDeviceToHost));
+        std::vector<int> cof(h->N);
+        for (int c = 0; c < h->C; ++c)
+            for (int g = h->a0_h[c]; g < h->a0_h[c + 1]; ++g) cof[g] = c;
+        for (size_t q = 0; q < P; ++q) {
+            const int a0 = h->a0_h[cof[gi[q]]];
+            if (ai) ai[q] = gi[q] - a0;
+            if (aj) aj[q] = gj[q] - a0;
+        }
+        if (rij) MDP_HIP(hipMemcpy(rij, h->rij.p, sizeof(double) * 3 * P, hipMemcpyDeviceToHost));
+    });
+}
+
+int mdp_fit_tuning(mdp_fit* h, char* buf, int len) {
+    return api([&] {
+        if (!buf || len <= 0) return;
+        std::strncpy(buf, h->tuning_json.c_str(), static_cast<size_t>(len) - 1);
+        buf[len - 1] = 0;
+    });
+}
+
+int mdp_fit_profile(mdp_fit* h, double* ms4) {
+    return api([&] {
+        h->profile();
+        for (int k = 0; k < 4; ++k) ms4[k] = h->stage_ms[k];
+    });
+}
+
+}  // extern "C"

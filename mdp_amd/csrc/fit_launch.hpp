@@ -1,0 +1,44 @@
+// Launchers of the batched fitting-descriptor kernels (k_fit.hip).  All pointers are device pointers.
+// The batch concatenates the atoms of C configurations (config c owns atoms cfg_a0[c]..cfg_a0[c+1]-1,
+// orthogonal periodic box cfg_L[3c..3c+2]); pairs are one CSR list over all atoms.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "snap_launch.hpp"
+
+namespace mdp {
+
+// Pair build by per-configuration brute force over periodic images.  fill = false writes the pair
+// count of every atom to pcnt; fill = true writes rij/pi/pj/pti/ptj at the CSR offsets poff (N+1
+// entries) and sets bit 0 of flags (leaving the outputs unwritten) when poff[N] exceeds cap.
+void launch_fit_pairs(bool fill, const double* x, const int* type, const int* cfg_of, const int* cfg_a0,
+                      const double* cfg_L, const double* rc2, int ntypes, int N, int* pcnt, const int* poff, long cap,
+                      double* rij, int* pi, int* pj, int* pti, int* ptj, int* flags, hipStream_t st);
+// cfg_p0[c] = poff[cfg_a0[c]] for c = 0..C (first pair of each configuration, total at C)
+void launch_fit_cfg_p0(const int* poff, const int* cfg_a0, int* cfg_p0, int C, hipStream_t st);
+// p[0..n) = 0 by a kernel (safe inside captured graphs)
+void launch_fill_zero_int(int* p, int n, hipStream_t st);
+// Z of every coupling triple (atom-major, complex-interleaved, zt[(g*idxz_max + jjz)*2 + {0,1}]) and B
+// (per-configuration layout blist[a0_c*idxb_max + ii + N_c*jjb]) from the full-layout Utot.
+void launch_fit_zb(const SnapDev& s, const double* Ur, const double* Ui, double* zt, double* blist,
+                   const double* bzero, const int* cfg_of, const int* cfg_a0, int N, int bnorm, int bzeroflag,
+                   hipStream_t st);
+// Device copy of the dB/dr term table (see fit_term_table)
+struct FitTerms {
+    const int* lo = nullptr;      // [J+2]
+    const int* b = nullptr;       // bispectrum index of each term
+    const int* z = nullptr;       // Z block offset (idxz ordering)
+    const double* w = nullptr;    // weight
+};
+// Host term table of the dB/dr contraction, grouped by the layer whose dU it contracts.
+void fit_term_table(int J, int bnorm, std::vector<int>& lo, std::vector<int>& b, std::vector<int>& z,
+                    std::vector<double>& w);
+// dB/dr of every pair (per-configuration layout dblist[p0_c*3*idxb_max + lp + P_c*k + 3*P_c*jjb]) by a
+// persistent kernel of `groups` single-wave workgroups; counter must be 0 at launch.
+void launch_fit_dbdr(const SnapDev& s, const double* zt, const FitTerms& tt, const double* rij, const int* pti,
+                     const int* ptj, const int* poff, const int* cfg_of, const int* cfg_p0, double* dblist,
+                     int* counter, int N, int groups, hipStream_t st);
+
+}  // namespace mdp

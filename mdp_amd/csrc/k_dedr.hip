@@ -103,10 +103,75 @@ __global__ __launch_bounds__(64, WPE) void k_dedr_split(SnapDev s, const double*
     dedr[3 * static_cast<size_t>(p) + comp] = 2.0 * d;
 }
 
+// One wave per centre atom (grid = atoms), lanes (pair, Cartesian component) as in k_dedr_split for up
+// to 21 pairs per pass.  All pairs of the wave share the atom, so its Y rows are wave-uniform and come in
+// through scalar loads instead of one gather per lane.
+template <int J>
+__global__ __launch_bounds__(64, 2) void k_dedr_atom(const double* __restrict__ rootpq, const double* __restrict__ Yr,
+                                                    const double* __restrict__ Yi, const double* __restrict__ radelem,
+                                                    const double* __restrict__ wjelem, double rcutfac, double rmin0,
+                                                    double rfac0, int switchflag, const double* __restrict__ rij,
+                                                    const int* __restrict__ ti, const int* __restrict__ tj,
+                                                    const int* __restrict__ poff, double* __restrict__ dedr, int N) {
+    const int atom = blockIdx.x;
+    const int lane = threadIdx.x, comp = lane % 3, slot = lane / 3;
+    const int q0 = poff[atom], q1 = poff[atom + 1];
+    const size_t n = static_cast<size_t>(N);
+    const double* __restrict__ yr = Yr + atom;
+    const double* __restrict__ yi = Yi + atom;
+    for (int qb = q0; qb < q1; qb += 21) {
+        const int q = qb + slot;
+        const bool active = lane < 63 && q < q1;
+        const int p = active ? q : q0;   // idle lanes recompute a valid pair and store nothing
+        const int tjp = tj[p];
+        const double rc = (radelem[ti[p]] + radelem[tjp]) * rcutfac;
+        PairGeom g;
+        pair_geometry<true>(rij[3 * p], rij[3 * p + 1], rij[3 * p + 2], rc, rmin0, rfac0, switchflag, wjelem[tjp], g);
+        const double uk = comp == 0 ? g.u[0] : (comp == 1 ? g.u[1] : g.u[2]);
+        g.dar[0] = comp == 0 ? g.dar[0] : (comp == 1 ? g.dar[1] : g.dar[2]);
+        g.dai[0] = comp == 0 ? g.dai[0] : (comp == 1 ? g.dai[1] : g.dai[2]);
+        g.dbr[0] = comp == 0 ? g.dbr[0] : (comp == 1 ? g.dbr[1] : g.dbr[2]);
+        g.dbi[0] = comp == 0 ? g.dbi[0] : (comp == 1 ? g.dbi[1] : g.dbi[2]);
+        const double dsu = g.dsfac * uk;
+        double d = 0.0;
+        row_chain<J, 1, 0, J / 2>(g, rootpq, [&](auto Jc, auto MBc, auto& pr, auto& pi, auto& dpr, auto& dpi) {
+            constexpr int JJ = decltype(Jc)::value, MB = decltype(MBc)::value;
+            if constexpr (JJ == 0) {
+                d = dsu * (0.5 * yr[0]);
+            } else {
+                constexpr int LAST = (JJ == 2 * MB) ? MB : JJ;
+                constexpr int row = h_block(JJ) + (JJ + 1) * MB;
+#pragma unroll
+                for (int ma = 0; ma <= LAST; ++ma) {
+                    double a = yr[(row + ma) * n], b = yi[(row + ma) * n];
+                    if (JJ == 2 * MB && ma == MB) { a *= 0.5; b *= 0.5; }
+                    d += (dsu * pr[ma] + g.sfac * dpr[0][ma]) * a + (dsu * pi[ma] + g.sfac * dpi[0][ma]) * b;
+                }
+            }
+        });
+        if (active) dedr[3 * static_cast<size_t>(q) + comp] = 2.0 * d;
+    }
+}
+
 void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const double* rij, const int* yatom,
                  const int* ti, const int* tj, const long* ybase, double* dedr, int N, int npairs_host,
-                 const int* npairs_dev, hipStream_t st, bool half, int variant) {
+                 const int* npairs_dev, hipStream_t st, bool half, int variant, const int* pair_off) {
     if (npairs_host <= 0) return;
+    if (half && variant == 5 && pair_off && !ybase) {
+        switch (s.J) {
+#define DA_CASE(JV)                                                                                                \
+    case JV:                                                                                                       \
+        hipLaunchKernelGGL((k_dedr_atom<JV>), dim3(N), dim3(64), 0, st, s.rootpq, Yr, Yi, s.radelem, s.wjelem,       \
+                           s.rcutfac, s.rmin0, s.rfac0, s.switchflag, rij, ti, tj, pair_off, dedr, N);              \
+        return;
+            DA_CASE(2)
+            DA_CASE(4)
+            DA_CASE(6)
+            DA_CASE(8)
+#undef DA_CASE
+            default: return;
+        }
+    }
     const dim3 g3((npairs_host + 63) / 64), g1((npairs_host + 20) / 21);
 #define DE_ARGS Yr, Yi, rij, yatom, ti, tj, ybase, dedr, N, npairs_host, npairs_dev
 #define DE_CASE(JV)                                                                                            \
@@ -134,9 +199,9 @@ void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const dou
 }
 
 int dedr_variants(int* ids) {
-    static const int v[] = {0, 2, 3, 4};
-    for (int k = 0; k < 4; ++k) ids[k] = v[k];
-    return 4;
+    static const int v[] = {0, 2, 3, 4, 5};
+    for (int k = 0; k < 5; ++k) ids[k] = v[k];
+    return 5;
 }
 
 }  // namespace mdp

@@ -1,0 +1,407 @@
+// Batched fitting descriptors for gfx950: per-atom B and per-pair dB/dr of many independent periodic
+// configurations in one pass.  Upstream runs, per configuration, cpuComputeUij -> cpuZeroUarraytot2 ->
+// cpuAddUarraytot -> cpuComputeZi2 -> cpuComputeBi2 -> cpuComputeDbidrj (cpuSnap.cpp:1620-2263, called
+// from snapcpp.jl:539-665 and snappotential.cpp:762-910 inside the per-configuration loop of
+// Regression.cpp:68-95).  Here all configurations share one atom index space and one pair list, so
+// each stage is a single launch over the whole batch; the per-pair derivative lists dU/dr are never
+// stored (they are recomputed in registers inside the dB/dr kernel and contracted on the fly).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "fit_launch.hpp"
+#include "snap_device.hpp"
+#include "zrow.hpp"
+
+namespace mdp {
+
+// offset of Z(j1,j2,j) in the idxz ordering (cpuSnap.cpp:147-171): triples in canonical order, each
+// holding rows mb = 0..j/2 of j+1 columns
+__host__ __device__ constexpr int z_block(int J, int j1, int j2, int j) {
+    int n = 0;
+    for (int a = 0; a <= J; ++a)
+        for (int b = 0; b <= a; ++b)
+            for (int c = a - b; c <= cx_min(J, a + b); c += 2) {
+                if (a == j1 && b == j2 && c == j) return n;
+                n += (c / 2 + 1) * (c + 1);
+            }
+    return -1;
+}
+__host__ __device__ constexpr int n_z(int J) {
+    int n = 0;
+    for (int a = 0; a <= J; ++a)
+        for (int b = 0; b <= a; ++b)
+            for (int c = a - b; c <= cx_min(J, a + b); c += 2) n += (c / 2 + 1) * (c + 1);
+    return n;
+}
+// bispectrum triples (j >= j1) in idxb order (cpuSnap.cpp:112-120): component k of triple t
+// (0: j1, 1: j2, 2: j) and their number
+__host__ __device__ constexpr int b_comp(int J, int t, int k) {
+    int n = 0;
+    for (int a = 0; a <= J; ++a)
+        for (int b = 0; b <= a; ++b)
+            for (int c = a - b; c <= cx_min(J, a + b); c += 2) {
+                if (c < a) continue;
+                if (n == t) return k == 0 ? a : (k == 1 ? b : c);
+                ++n;
+            }
+    return -1;
+}
+__host__ __device__ constexpr int n_b(int J) {
+    int n = 0;
+    for (int a = 0; a <= J; ++a)
+        for (int b = 0; b <= a; ++b)
+            for (int c = a - b; c <= cx_min(J, a + b); c += 2) n += (c >= a);
+    return n;
+}
+__host__ __device__ constexpr int b_index(int J, int j1, int j2, int j) {
+    for (int k = 0; k < n_b(J); ++k)
+        if (b_comp(J, k, 0) == j1 && b_comp(J, k, 1) == j2 && b_comp(J, k, 2) == j) return k;
+    return -1;
+}
+// ---- batched pair build ------------------------------------------------------------------------
+
+__device__ __forceinline__ double image_delta(double xj, int s, double L, double xi) {
+#pragma clang fp contract(off)
+    const double g = xj + s * L;
+    return g - xi;
+}
+__device__ __forceinline__ double dist2(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// Pairs of atom g: every periodic image x_j + s*L of every atom j of its configuration with
+// |x_j + s*L - x_i|^2 <= rc(ti,tj)^2, the atom itself at s = 0 excluded (neighpairlist.jl:52 with the
+// ghost construction of fullneighborlist.jl).  Order: j ascending, then (sz, sy, sx) ascending.
+// FILL = false counts, FILL = true writes the CSR slot poff[g].. (capacity checked against cap).
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_fit_pairs(const double* __restrict__ x, const int* __restrict__ type,
+                                                   const int* __restrict__ cfg_of, const int* __restrict__ cfg_a0,
+                                                   const double* __restrict__ cfg_L, const double* __restrict__ rc2,
+                                                   int ntypes, int N, int* __restrict__ pcnt,
+                                                   const int* __restrict__ poff, long cap, double* __restrict__ rij,
+                                                   int* __restrict__ pi, int* __restrict__ pj,
+                                                   int* __restrict__ pti, int* __restrict__ ptj,
+                                                   int* __restrict__ flags) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+    const int c = cfg_of[g];
+    const int j0 = cfg_a0[c], j1 = cfg_a0[c + 1];
+    const double Lx = cfg_L[3 * c], Ly = cfg_L[3 * c + 1], Lz = cfg_L[3 * c + 2];
+    const double xi = x[3 * g], yi = x[3 * g + 1], zi = x[3 * g + 2];
+    const int ti = type[g];
+    long q = 0;
+    if constexpr (FILL) {
+        q = poff[g];
+        if (static_cast<long>(poff[N]) > cap) {
+            if (g == 0) atomicOr(flags, 1);
+            return;
+        }
+    }
+    int cnt = 0;
+    for (int j = j0; j < j1; ++j) {
+        const int tj = type[j];
+        const double r2 = rc2[ti * (ntypes + 1) + tj];
+        const double rc = sqrt(r2);
+        const double xj = x[3 * j], yj = x[3 * j + 1], zj = x[3 * j + 2];
+        // image ranges from the real-number bound, widened by one: the exact test below decides
+        const int sx0 = static_cast<int>(floor((xi - xj - rc) / Lx)), sx1 = static_cast<int>(ceil((xi - xj + rc) / Lx));
+        const int sy0 = static_cast<int>(floor((yi - yj - rc) / Ly)), sy1 = static_cast<int>(ceil((yi - yj + rc) / Ly));
+        const int sz0 = static_cast<int>(floor((zi - zj - rc) / Lz)), sz1 = static_cast<int>(ceil((zi - zj + rc) / Lz));
+        for (int sz = sz0; sz <= sz1; ++sz) {
+            const double dz = image_delta(zj, sz, Lz, zi);
+            for (int sy = sy0; sy <= sy1; ++sy) {
+                const double dy = image_delta(yj, sy, Ly, yi);
+                for (int sx = sx0; sx <= sx1; ++sx) {
+                    if (j == g && sx == 0 && sy == 0 && sz == 0) continue;
+                    const double dx = image_delta(xj, sx, Lx, xi);
+                    if (dist2(dx, dy, dz) <= r2) {
+                        if constexpr (FILL) {
+                            rij[3 * q] = dx;
+                            rij[3 * q + 1] = dy;
+                            rij[3 * q + 2] = dz;
+                            pi[q] = g;
+                            pj[q] = j;
+                            pti[q] = ti;
+                            ptj[q] = tj;
+                            ++q;
+                        }
+                        ++cnt;
+                    }
+                }
+            }
+        }
+    }
+    if constexpr (!FILL) pcnt[g] = cnt;
+}
+
+// first pair of every configuration: cfg_p0[c] = poff[cfg_a0[c]]
+__global__ void k_fit_cfg_p0(const int* __restrict__ poff, const int* __restrict__ cfg_a0, int* __restrict__ cfg_p0,
+                             int C) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c <= C) cfg_p0[c] = poff[cfg_a0[c]];
+}
+
+__global__ void k_fill_zero_int(int* __restrict__ p, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0;
+}
+
+// ---- Z (atom-major) and B ----------------------------------------------------------------------
+
+// Z of one coupling triple (blockIdx.y) for 64 atoms, all rows mb <= j/2 and columns ma, stored
+// atom-major and complex-interleaved: zt[(atom*idxz_max + jjz)*2 + {0,1}], so that the dB/dr
+// kernel streams one atom's Z contiguously.  For bispectrum triples (j >= j1) also
+// B = 2 sum' U*(j).Z(j1,j2,j) - bzero[j] (cpuSnap.cpp:1971-2072), written in the per-configuration
+// layout blist[c][ii + N_c*jjb].  U is the full-layout Utot (element jju of atom g at jju*N + g).
+template <int J>
+__global__ __launch_bounds__(64) void k_fit_zb(const double* __restrict__ cg, const double* __restrict__ Ur,
+                                               const double* __restrict__ Ui, double* __restrict__ zt,
+                                               double* __restrict__ blist, const double* __restrict__ bzero,
+                                               const int* __restrict__ cfg_of, const int* __restrict__ cfg_a0,
+                                               int N, int bnorm, int bzeroflag) {
+    const int atom = blockIdx.x * 64 + threadIdx.x;
+    if (atom >= N) return;
+    constexpr int IZ = n_z(J), IB = n_b(J);
+    const int t = blockIdx.y;
+    const size_t n = static_cast<size_t>(N);
+    static_for<0, J + 1>([&](auto J1c) {
+        constexpr int J1 = decltype(J1c)::value;
+        static_for<0, J1 + 1>([&](auto J2c) {
+            constexpr int J2 = decltype(J2c)::value;
+            static_for<0, J + 1>([&](auto JOc) {
+                constexpr int JO = decltype(JOc)::value;
+                if constexpr (coupled(J1, J2, JO)) {
+                    if (t == triple_id(J, J1, J2, JO)) {
+                        constexpr int ZB = z_block(J, J1, J2, JO);
+                        const double scale = bnorm ? 1.0 / (JO + 1) : 1.0;
+                        double bsum = 0.0;
+                        double* __restrict__ zo = zt + (static_cast<size_t>(atom) * IZ + ZB) * 2;
+                        for (int mb = 0; 2 * mb <= JO; ++mb) {
+                            double zr[JO + 1], zi[JO + 1];
+#pragma unroll
+                            for (int m = 0; m <= JO; ++m) { zr[m] = 0.0; zi[m] = 0.0; }
+                            z_row<J, J1, J2, JO>(mb, JO, Ur, Ui, Ur, Ui, n, atom, cg, scale, zr, zi);
+#pragma unroll
+                            for (int ma = 0; ma <= JO; ++ma) {
+                                zo[2 * ((JO + 1) * mb + ma)] = zr[ma];
+                                zo[2 * ((JO + 1) * mb + ma) + 1] = zi[ma];
+                            }
+                            if constexpr (JO >= J1) {
+                                const size_t o = static_cast<size_t>(u_block(JO) + (JO + 1) * mb) * n + atom;
+                                const int malim = (2 * mb == JO) ? mb : JO;
+#pragma unroll
+                                for (int ma = 0; ma <= JO; ++ma) {
+                                    if (ma <= malim) {
+                                        const double w = (2 * mb < JO || ma < mb) ? 1.0 : 0.5;
+                                        bsum += w * (Ur[o + ma * n] * zr[ma] + Ui[o + ma * n] * zi[ma]);
+                                    }
+                                }
+                            }
+                        }
+                        if constexpr (JO >= J1) {
+                            constexpr int jjb = b_index(J, J1, J2, JO);
+                            static_assert(jjb >= 0 && jjb < IB, "bispectrum triple");
+                            const int c = cfg_of[atom];
+                            const int a0 = cfg_a0[c], nc = cfg_a0[c + 1] - a0;
+                            double b = 2.0 * bsum;
+                            if (bzeroflag) b -= bzero[JO];
+                            blist[static_cast<size_t>(a0) * IB + (atom - a0) + static_cast<size_t>(nc) * jjb] = b;
+                        }
+                    }
+                }
+            });
+        });
+    });
+}
+
+// ---- dB/dr ----------------------------------------------------------------------------------------
+
+// Persistent: each wave takes whole atoms from a global counter; lanes are (pair, Cartesian
+// component), 21 pairs x 3 per pass (lane 63 idle), all pairs of the pass belonging to the wave's atom.
+// Each lane runs the U and dU/dr_c row-chain recursion of its pair in registers and, as every
+// half-domain row (l, mb) of d(sfac U)/dr_c = dsfac u_c U + sfac dU/dr_c is finished, contracts it
+// with the Z block of every bispectrum term on layer l (cpuSnap.cpp:2101-2240: middle column of even
+// layers 1/2, term weights 2, 2(j+1)/(j1+1), 2(j+1)/(j2+1), or 2 under bnorm).  The terms of layer l
+// are entries term_lo[l]..term_lo[l+1]-1 of the term table (bispectrum index, Z block offset,
+// weight), walked by a uniform loop; the atom is wave-uniform, so Z values, table entries and
+// weights are scalar loads shared by all 63 lanes.  Each lane's 55 accumulators live in LDS.
+// dblist is written in the per-configuration layout dblist[c][lp + P_c*k + 3 P_c*jjb].
+template <int J>
+__global__ __launch_bounds__(64) void k_fit_dbdr(const double* __restrict__ rootpq, const double* __restrict__ zt,
+                                                 const int* __restrict__ term_lo, const int* __restrict__ term_b,
+                                                 const int* __restrict__ term_z, const double* __restrict__ term_w,
+                                                 const double* __restrict__ radelem, const double* __restrict__ wjelem,
+                                                 double rcutfac, double rmin0, double rfac0, int switchflag,
+                                                 const double* __restrict__ rij, const int* __restrict__ pti,
+                                                 const int* __restrict__ ptj, const int* __restrict__ poff,
+                                                 const int* __restrict__ cfg_of, const int* __restrict__ cfg_p0,
+                                                 double* __restrict__ dblist, int* __restrict__ counter, int N) {
+    constexpr int IZ = n_z(J), IB = n_b(J);
+    __shared__ double acc[IB * 64];
+    const int lane = threadIdx.x;
+    const int comp = lane % 3, slot = lane / 3;
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(counter, 1);
+        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+        if (t >= N) break;
+        const int atom = t;
+        const int q0 = __builtin_amdgcn_readfirstlane(poff[atom]);
+        const int q1 = __builtin_amdgcn_readfirstlane(poff[atom + 1]);
+        const int c = __builtin_amdgcn_readfirstlane(cfg_of[atom]);
+        const int p0 = __builtin_amdgcn_readfirstlane(cfg_p0[c]);
+        const long Pc = __builtin_amdgcn_readfirstlane(cfg_p0[c + 1]) - p0;
+        const double* __restrict__ z = zt + static_cast<size_t>(atom) * IZ * 2;
+        double* __restrict__ db = dblist + static_cast<size_t>(p0) * 3 * IB;
+        for (int qb = q0; qb < q1; qb += 21) {
+            const int q = qb + slot;
+            const bool active = lane < 63 && q < q1;
+            const int p = active ? q : q0;   // idle lanes recompute a valid pair and store nothing
+            const int tjp = ptj[p];
+            const double rc = (radelem[pti[p]] + radelem[tjp]) * rcutfac;
+            PairGeom g;
+            pair_geometry<true>(rij[3 * p], rij[3 * p + 1], rij[3 * p + 2], rc, rmin0, rfac0, switchflag, wjelem[tjp], g);
+            const double uk = comp == 0 ? g.u[0] : (comp == 1 ? g.u[1] : g.u[2]);
+            g.dar[0] = comp == 0 ? g.dar[0] : (comp == 1 ? g.dar[1] : g.dar[2]);
+            g.dai[0] = comp == 0 ? g.dai[0] : (comp == 1 ? g.dai[1] : g.dai[2]);
+            g.dbr[0] = comp == 0 ? g.dbr[0] : (comp == 1 ? g.dbr[1] : g.dbr[2]);
+            g.dbi[0] = comp == 0 ? g.dbi[0] : (comp == 1 ? g.dbi[1] : g.dbi[2]);
+            const double dsu = g.dsfac * uk, sf = g.sfac;
+            for (int k = 0; k < IB; ++k) acc[k * 64 + lane] = 0.0;
+            row_chain<J, 1, 0, J / 2>(g, rootpq, [&](auto Jc, auto MBc, auto& pr, auto& pim, auto& dpr, auto& dpi) {
+                constexpr int L = decltype(Jc)::value, MB = decltype(MBc)::value;
+                constexpr int LAST = (L == 2 * MB) ? MB : L;
+                double dr[LAST + 1], di[LAST + 1];
+#pragma unroll
+                for (int ma = 0; ma <= LAST; ++ma) {
+                    dr[ma] = dsu * pr[ma] + sf * dpr[0][ma];
+                    di[ma] = dsu * pim[ma] + sf * dpi[0][ma];
+                }
+                if constexpr (L == 2 * MB) {   // middle column of an even layer counts 1/2
+                    dr[MB] *= 0.5;
+                    di[MB] *= 0.5;
+                }
+                const int k0 = term_lo[L], k1 = term_lo[L + 1];
+                for (int k = k0; k < k1; ++k) {
+                    const double* __restrict__ zr = z + 2 * (term_z[k] + (L + 1) * MB);
+                    double s = 0.0;
+#pragma unroll
+                    for (int ma = 0; ma <= LAST; ++ma) s += dr[ma] * zr[2 * ma] + di[ma] * zr[2 * ma + 1];
+                    acc[term_b[k] * 64 + lane] += term_w[k] * s;
+                }
+            });
+            if (active) {
+                const size_t lp = static_cast<size_t>(q - p0);
+                for (int k = 0; k < IB; ++k) db[lp + Pc * comp + 3 * Pc * k] = acc[k * 64 + lane];
+            }
+        }
+    }
+}
+
+// Term table of the dB/dr contraction grouped by layer: for every layer l the terms (bispectrum
+// index, Z block offset, weight) that contract dU of layer l; term_lo has J+2 entries.
+void fit_term_table(int J, int bnorm, std::vector<int>& lo, std::vector<int>& tb, std::vector<int>& tz,
+                    std::vector<double>& tw) {
+    struct Term { int l, b, z; double w; };
+    std::vector<Term> all;
+    auto zb = [&](int j1, int j2, int j) {
+        int n = 0;
+        for (int a = 0; a <= J; ++a)
+            for (int b = 0; b <= a; ++b)
+                for (int c = a - b; c <= std::min(J, a + b); c += 2) {
+                    if (a == j1 && b == j2 && c == j) return n;
+                    n += (c / 2 + 1) * (c + 1);
+                }
+        return -1;
+    };
+    int nb = 0;
+    for (int j1 = 0; j1 <= J; ++j1)
+        for (int j2 = 0; j2 <= j1; ++j2)
+            for (int j = j1 - j2; j <= std::min(J, j1 + j2); j += 2) {
+                if (j < j1) continue;
+                const double f1 = bnorm ? 2.0 : 2.0 * (j + 1) / (j1 + 1.0);
+                const double f2 = bnorm ? 2.0 : 2.0 * (j + 1) / (j2 + 1.0);
+                all.push_back({j, nb, zb(j1, j2, j), 2.0});
+                all.push_back({j1, nb, zb(j, j2, j1), f1});
+                all.push_back({j2, nb, zb(j, j1, j2), f2});
+                ++nb;
+            }
+    std::stable_sort(all.begin(), all.end(), [](const Term& a, const Term& b) { return a.l < b.l; });
+    lo.assign(J + 2, 0);
+    tb.clear(); tz.clear(); tw.clear();
+    for (const Term& t : all) {
+        tb.push_back(t.b);
+        tz.push_back(t.z);
+        tw.push_back(t.w);
+        lo[t.l + 1] += 1;
+    }
+    for (int l = 0; l <= J; ++l) lo[l + 1] += lo[l];
+}
+
+// ---- launchers ----------------------------------------------------------------------------------
+
+void launch_fit_pairs(bool fill, const double* x, const int* type, const int* cfg_of, const int* cfg_a0,
+                      const double* cfg_L, const double* rc2, int ntypes, int N, int* pcnt, const int* poff, long cap,
+                      double* rij, int* pi, int* pj, int* pti, int* ptj, int* flags, hipStream_t st) {
+    if (N <= 0) return;
+    const dim3 grid((N + 255) / 256), block(256);
+    if (fill)
+        hipLaunchKernelGGL((k_fit_pairs<true>), grid, block, 0, st, x, type, cfg_of, cfg_a0, cfg_L, rc2, ntypes, N, pcnt,
+                           poff, cap, rij, pi, pj, pti, ptj, flags);
+    else
+        hipLaunchKernelGGL((k_fit_pairs<false>), grid, block, 0, st, x, type, cfg_of, cfg_a0, cfg_L, rc2, ntypes, N,
+                           pcnt, poff, cap, rij, pi, pj, pti, ptj, flags);
+}
+
+void launch_fit_cfg_p0(const int* poff, const int* cfg_a0, int* cfg_p0, int C, hipStream_t st) {
+    hipLaunchKernelGGL(k_fit_cfg_p0, dim3((C + 1 + 255) / 256), dim3(256), 0, st, poff, cfg_a0, cfg_p0, C);
+}
+
+void launch_fill_zero_int(int* p, int n, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_zero_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+}
+
+void launch_fit_zb(const SnapDev& s, const double* Ur, const double* Ui, double* zt, double* blist,
+                   const double* bzero, const int* cfg_of, const int* cfg_a0, int N, int bnorm, int bzeroflag,
+                   hipStream_t st) {
+    if (N <= 0) return;
+#define ZB_CASE(JV)                                                                                            \
+    case JV:                                                                                                   \
+        hipLaunchKernelGGL((k_fit_zb<JV>), dim3((N + 63) / 64, n_triples(JV)), dim3(64), 0, st, s.cg, Ur, Ui, zt, \
+                           blist, bzero, cfg_of, cfg_a0, N, bnorm, bzeroflag);                                 \
+        break;
+    switch (s.J) {
+        ZB_CASE(2)
+        ZB_CASE(4)
+        ZB_CASE(6)
+        ZB_CASE(8)
+        default: break;
+    }
+#undef ZB_CASE
+}
+
+void launch_fit_dbdr(const SnapDev& s, const double* zt, const FitTerms& tt, const double* rij, const int* pti,
+                     const int* ptj, const int* poff, const int* cfg_of, const int* cfg_p0, double* dblist,
+                     int* counter, int N, int groups, hipStream_t st) {
+    if (N <= 0) return;
+    const int G = groups > 0 ? groups : 1280;
+#define DB_CASE(JV)                                                                                            \
+    case JV:                                                                                                   \
+        hipLaunchKernelGGL((k_fit_dbdr<JV>), dim3(G), dim3(64), 0, st, s.rootpq, zt, tt.lo, tt.b, tt.z, tt.w,    \
+                           s.radelem, s.wjelem, s.rcutfac, s.rmin0, s.rfac0, s.switchflag, rij, pti, ptj, poff,  \
+                           cfg_of, cfg_p0, dblist, counter, N);                                                \
+        break;
+    switch (s.J) {
+        DB_CASE(2)
+        DB_CASE(4)
+        DB_CASE(6)
+        DB_CASE(8)
+        default: break;
+    }
+#undef DB_CASE
+}
+
+}  // namespace mdp

@@ -25,6 +25,38 @@ __global__ __launch_bounds__(64) void k_yi(SnapDev s, const double* __restrict__
     });
 }
 
+// Per-atom weights (beta_t[triple*N + atom]), every idxz entry written: the Y of the learned surrogate.
+template <int J>
+__global__ __launch_bounds__(64) void k_yi_pa(SnapDev s, const double* __restrict__ Ur, const double* __restrict__ Ui,
+                                              const double* __restrict__ beta_t, double* __restrict__ Yr,
+                                              double* __restrict__ Yi, int N) {
+    const int atom = blockIdx.x * 64 + threadIdx.x;
+    if (atom >= N) return;
+    const int row = blockIdx.y;
+    const int jo = half_row_j(row), mb = half_row_mb(row);
+    static_for<0, J + 1>([&](auto JOc) {
+        constexpr int JO = decltype(JOc)::value;
+        if (jo == JO)
+            y_row<J, JO, true, false, false, true>(s, row, mb, atom, N, Ur, Ui, Ur, Ui, beta_t, Yr, Yi, nullptr);
+    });
+}
+
+bool launch_yi_peratom(const SnapDev& s, const double* Ur, const double* Ui, const double* beta_t, double* Yr,
+                       double* Yi, int N, hipStream_t st) {
+    if (N <= 0) return true;
+    switch (s.J) {
+        case 2:
+            hipLaunchKernelGGL((k_yi_pa<2>), dim3((N + 63) / 64, n_half_rows(2)), dim3(64), 0, st, s, Ur, Ui, beta_t, Yr,
+                               Yi, N);
+            return true;
+        case 4:
+            hipLaunchKernelGGL((k_yi_pa<4>), dim3((N + 63) / 64, n_half_rows(4)), dim3(64), 0, st, s, Ur, Ui, beta_t, Yr,
+                               Yi, N);
+            return true;
+        default: return false;
+    }
+}
+
 void launch_yi(const SnapDev& s, const double* Ur, const double* Ui, const double* beta_t, double* Yr, double* Yi,
                double* epart, int N, bool full_mid, hipStream_t st) {
     (void)epart;

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/mdpsnap.h"
+#include "mlp_launch.hpp"
 #include "runtime.hpp"
 #include "snap_device.hpp"
 
@@ -38,9 +39,10 @@ constexpr int SHK = 8;         // image shifts are coded in [-SHK, SHK] per axis
 constexpr double FTM2V = 1.0 / 1.0364269e-4;   // metal units, eV/A/amu -> A/ps^2
 constexpr double MVV2E = 1.0364269e-4;
 
-enum Timer { T_VV1, T_REBUILD, T_PAIRS, T_UI, T_YI, T_DEDR, T_TALLY, T_COUNT };
+enum Timer { T_VV1, T_REBUILD, T_PAIRS, T_UI, T_YI, T_DEDR, T_TALLY, T_SUR, T_AUDIT, T_COUNT };
 const char* const kTimerNames[T_COUNT] = {"integrate", "neighbor_build", "pair_list", "compute_ui",
-                                           "compute_yi", "compute_dedr", "tally_integrate"};
+                                           "compute_yi", "compute_dedr", "tally_integrate", "surrogate_force",
+                                           "audit"};
 
 __host__ __device__ inline int shift_code(int sx, int sy, int sz) {
     const int w = 2 * SHK + 1;
@@ -332,7 +334,8 @@ __global__ void k_pair_fill_md(const double* __restrict__ x, const int* __restri
 __global__ void k_tally(const double* __restrict__ dedr, const double* __restrict__ prij, const int* __restrict__ poff,
                         const int* __restrict__ in_cnt, const int* __restrict__ in_e,
                         const int* __restrict__ entry_pair, const double* __restrict__ epart, int nrows, double e0,
-                        double* __restrict__ f, double* __restrict__ vir, double* __restrict__ eatom,
+                        const double* __restrict__ eext, double* __restrict__ f, double* __restrict__ vir,
+                        double* __restrict__ eatom,
                         double* __restrict__ v, const int* __restrict__ type, const double* __restrict__ massinv_t,
                         double dtf, int integrate, int N) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -363,7 +366,8 @@ __global__ void k_tally(const double* __restrict__ dedr, const double* __restric
         }
     }
     double e = 0.0;
-    for (int r = 0; r < nrows; ++r) e += epart[static_cast<size_t>(r) * N + i];
+    if (!eext)
+        for (int r = 0; r < nrows; ++r) e += epart[static_cast<size_t>(r) * N + i];
     f[3 * i] = fx;
     f[3 * i + 1] = fy;
     f[3 * i + 2] = fz;
@@ -373,13 +377,51 @@ __global__ void k_tally(const double* __restrict__ dedr, const double* __restric
     vir[6 * i + 3] = w3;
     vir[6 * i + 4] = w4;
     vir[6 * i + 5] = w5;
-    eatom[i] = e0 + (2.0 / 3.0) * e;
+    eatom[i] = eext ? eext[i] : e0 + (2.0 / 3.0) * e;
     if (integrate) {
         const double dtfm = dtf * massinv_t[type[i]];
         v[3 * i] += dtfm * fx;
         v[3 * i + 1] += dtfm * fy;
         v[3 * i + 2] += dtfm * fz;
     }
+}
+
+// second velocity half-step on its own (steps whose forces are chosen after an audit)
+__global__ void k_vv2(double* __restrict__ v, const double* __restrict__ f, const int* __restrict__ type,
+                      const double* __restrict__ massinv_t, double dtf, int N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double dtfm = dtf * massinv_t[type[i]];
+    v[3 * i] += dtfm * f[3 * i];
+    v[3 * i + 1] += dtfm * f[3 * i + 1];
+    v[3 * i + 2] += dtfm * f[3 * i + 2];
+}
+
+// per-atom, per-coupling-triple weights of the surrogate's Y: beta[t][a] = fac[t] * g[jjb[t]][a]
+__global__ void k_beta_peratom(const double* __restrict__ g, const int* __restrict__ jjb, const double* __restrict__ fac,
+                               int ntrip, int N, double* __restrict__ beta) {
+    const long k = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= static_cast<long>(ntrip) * N) return;
+    const int t = static_cast<int>(k / N), a = static_cast<int>(k % N);
+    beta[k] = fac[t] * g[static_cast<size_t>(jjb[t]) * N + a];
+}
+
+// per-block partial sums of |a - b|^2
+__global__ __launch_bounds__(256) void k_diff_sumsq(const double* __restrict__ a, const double* __restrict__ b, long n,
+                                                    double* __restrict__ part) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (long k = static_cast<long>(blockIdx.x) * 256 + threadIdx.x; k < n; k += static_cast<long>(gridDim.x) * 256) {
+        const double d = a[k] - b[k];
+        s += d * d;
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (static_cast<int>(threadIdx.x) < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
 inline int nb(long n, int b) { return static_cast<int>((n + b - 1) / b); }
@@ -406,6 +448,16 @@ struct mdp_md {
         beta_t, rc2, flags, maxd2, row_order;
     long stats[6] = {0, 0, 0, 0, 0, 0};
     bool forces_current = false;
+    std::vector<double> rad_h, wj_h;
+    // learned surrogate (mdp_md_set_surrogate): forces from a per-atom MLP over low-order bispectrum
+    // descriptors, audited every audit_every steps against plain SNAP
+    bool sur = false;
+    int sur_nin = 0, sur_ntrip = 0, audit_every = 0;
+    double audit_tol = 0.0;
+    DevTables tab_s;
+    DevBuf sUr, sUi, sB, sE, sG, sBeta, sYr, sYi, sParams, sPacked, sAff, sJjb, sFac, fP, vP, eP, ssq;
+    long audits = 0, fallbacks = 0;
+    double rmse_last = 0, rmse_max = 0, rmse_sum = 0;
     // autotuned launch variants (see autotune())
     int dedr_variant = 3, yi_waves = 8;
     bool tuned = false;
@@ -494,6 +546,8 @@ struct mdp_md {
             rcmax = std::max(rcmax, 2.0 * radelem[t] * p->rcutfac);
         }
         tab.upload(ix, nullptr, nullptr, radelem.data(), wjelem.data(), map.data(), ntypes, st);
+        rad_h = radelem;
+        wj_h = wjelem;
         tab.view.rcutfac = p->rcutfac;
         tab.view.rfac0 = p->rfac0;
         tab.view.rmin0 = p->rmin0;
@@ -645,8 +699,12 @@ struct mdp_md {
                            entry_pair.as<int>(), L[0], L[1], L[2], N);
     }
 
-    void snap_and_tally(bool integrate) {
+    // plain SNAP forces; outputs to (fo, vo, eo), by default the engine's f, vir and eatom
+    void snap_and_tally(bool integrate, double* fo = nullptr, double* vo = nullptr, double* eo = nullptr) {
         const SnapDev& s = tab.view;
+        if (!fo) fo = f.as<double>();
+        if (!vo) vo = vir.as<double>();
+        if (!eo) eo = eatom.as<double>();
         timed(T_UI, [&] {
             launch_ui(s, UI_HALF_WSELF, Ur.as<double>(), Ui.as<double>(), prij.as<double>(), poff.as<int>(), nullptr,
                       pti.as<int>(), ptj.as<int>(), N, 1.0, -1, st);
@@ -658,14 +716,142 @@ struct mdp_md {
         timed(T_DEDR, [&] {
             launch_dedr(s, Yr.as<double>(), Yi.as<double>(), prij.as<double>(), pi.as<int>(), pti.as<int>(),
                         ptj.as<int>(), nullptr, dedr.as<double>(), N, static_cast<int>(stats[1]), poff.as<int>() + N,
-                        st, true, dedr_variant);
+                        st, true, dedr_variant, poff.as<int>());
         });
         timed(T_TALLY, [&] {
             hipLaunchKernelGGL(k_tally, dim3(nb(N, 256)), dim3(256), 0, st, dedr.as<double>(), prij.as<double>(),
                                poff.as<int>(), in_cnt.as<int>(), in_e.as<int>(), entry_pair.as<int>(),
-                               epart.as<double>(), nrows, e0, f.as<double>(), vir.as<double>(), eatom.as<double>(),
-                               v.as<double>(), type.as<int>(), massinv.as<double>(), dtf, integrate ? 1 : 0, N);
+                               epart.as<double>(), nrows, e0, nullptr, fo, vo, eo, v.as<double>(), type.as<int>(),
+                               massinv.as<double>(), dtf, integrate ? 1 : 0, N);
         });
+    }
+
+    void set_surrogate(const float* params, const double* aff, int nin, int js, int every, double tol) {
+        if (!params || !aff) throw Error("mdp_md_set_surrogate: null parameters");
+        if (js != 2 && js != 4) throw Error("mdp_md_set_surrogate: descriptor twojmax must be 2 or 4");
+        if (every < 0) throw Error("mdp_md_set_surrogate: audit_every must be >= 0");
+        SnapIndex ix;
+        ix.build(js, nullptr);
+        if (nin != ix.idxb_max)
+            throw Error("mdp_md_set_surrogate: nin must be the number of bispectrum components of twojmax");
+        std::vector<int> map(ntypes + 1, 0);
+        tab_s.upload(ix, nullptr, nullptr, rad_h.data(), wj_h.data(), map.data(), ntypes, st);
+        tab_s.view.rcutfac = tab.view.rcutfac;
+        tab_s.view.rfac0 = tab.view.rfac0;
+        tab_s.view.rmin0 = tab.view.rmin0;
+        tab_s.view.switchflag = tab.view.switchflag;
+        // Y weights of each coupling triple as (bispectrum index, multiplicity factor), read off the
+        // uniform-beta table with one-hot coefficient vectors
+        const int nc = ix.idxb_max + 1;
+        std::vector<int> jjb;
+        std::vector<double> fac;
+        for (int k = 0; k < ix.idxb_max; ++k) {
+            std::vector<double> beta(nc, 0.0);
+            beta[1 + k] = 1.0;
+            const std::vector<double> t = y_beta_table(ix, beta.data(), 0, 0, 0, 1, 0, nc, 0);
+            if (jjb.empty()) {
+                jjb.assign(t.size(), -1);
+                fac.assign(t.size(), 0.0);
+            }
+            for (size_t q = 0; q < t.size(); ++q)
+                if (t[q] != 0.0) {
+                    if (jjb[q] >= 0) throw Error("mdp_md_set_surrogate: coupling triple with two coefficients");
+                    jjb[q] = k;
+                    fac[q] = t[q];
+                }
+        }
+        for (int q : jjb)
+            if (q < 0) throw Error("mdp_md_set_surrogate: coupling triple without a coefficient");
+        sur_ntrip = static_cast<int>(jjb.size());
+        // the model's descriptors are B - bzero_j (wself = 1): fold bzero into the centring
+        std::vector<double> a(aff, aff + 34);
+        for (int k = 0; k < nin; ++k) a[k] += ix.idxb[3 * k + 2] + 1.0;
+        auto put = [&](DevBuf& b, const void* h, size_t bytes) {
+            b.ensure(bytes);
+            MDP_HIP(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, st));
+        };
+        put(sParams, params, sizeof(float) * MLP_NPARAM);
+        put(sAff, a.data(), sizeof(double) * 34);
+        put(sJjb, jjb.data(), sizeof(int) * jjb.size());
+        put(sFac, fac.data(), sizeof(double) * fac.size());
+        sPacked.ensure(sizeof(unsigned short) * MLP_NPACK);
+        launch_mlp_pack(sParams.as<float>(), sPacked.as<unsigned short>(), st);
+        const size_t n = N, iu = ix.idxu_max;
+        sUr.ensure(sizeof(double) * n * iu);
+        sUi.ensure(sizeof(double) * n * iu);
+        sYr.ensure(sizeof(double) * n * iu);
+        sYi.ensure(sizeof(double) * n * iu);
+        sB.ensure(sizeof(double) * n * nin);
+        sG.ensure(sizeof(double) * n * nin);
+        sE.ensure(sizeof(double) * n);
+        sBeta.ensure(sizeof(double) * n * sur_ntrip);
+        fP.ensure(sizeof(double) * 3 * n);
+        vP.ensure(sizeof(double) * 6 * n);
+        eP.ensure(sizeof(double) * n);
+        ssq.ensure(sizeof(double) * 256);
+        MDP_HIP(hipStreamSynchronize(st));
+        sur_nin = nin;
+        audit_every = every;
+        audit_tol = tol;
+        audits = fallbacks = 0;
+        rmse_last = rmse_max = rmse_sum = 0;
+        sur = true;
+        forces_current = false;
+    }
+
+    // surrogate forces: Utot and B at the descriptor twojmax, the MLP's energy and dE/dB per atom, then
+    // dE/dr by the adjoint route (Y with per-atom weights dE/dB, then the dU/dr contraction)
+    void sur_and_tally(bool integrate) {
+        const SnapDev& s4 = tab_s.view;
+        timed(T_SUR, [&] {
+            launch_ui(s4, UI_FULL_WSELF, sUr.as<double>(), sUi.as<double>(), prij.as<double>(), poff.as<int>(), nullptr,
+                      pti.as<int>(), ptj.as<int>(), N, 1.0, -1, st);
+            launch_bi(s4, sUr.as<double>(), sUi.as<double>(), 0, 0, 0, sB.as<double>(), N, 0, st);
+            launch_mlp_eval(sPacked.as<unsigned short>(), sParams.as<float>(), sAff.as<double>(), sur_nin,
+                            sB.as<double>(), N, sE.as<double>(), sG.as<double>(), st);
+            hipLaunchKernelGGL(k_beta_peratom, dim3(nb(static_cast<long>(sur_ntrip) * N, 256)), dim3(256), 0, st,
+                               sG.as<double>(), sJjb.as<int>(), sFac.as<double>(), sur_ntrip, N, sBeta.as<double>());
+            launch_yi_peratom(s4, sUr.as<double>(), sUi.as<double>(), sBeta.as<double>(), sYr.as<double>(),
+                              sYi.as<double>(), N, st);
+            launch_dedr(s4, sYr.as<double>(), sYi.as<double>(), prij.as<double>(), pi.as<int>(), pti.as<int>(),
+                        ptj.as<int>(), nullptr, dedr.as<double>(), N, static_cast<int>(stats[1]), poff.as<int>() + N, st,
+                        false, 0);
+        });
+        timed(T_TALLY, [&] {
+            hipLaunchKernelGGL(k_tally, dim3(nb(N, 256)), dim3(256), 0, st, dedr.as<double>(), prij.as<double>(),
+                               poff.as<int>(), in_cnt.as<int>(), in_e.as<int>(), entry_pair.as<int>(),
+                               epart.as<double>(), 0, 0.0, sE.as<double>(), f.as<double>(), vir.as<double>(),
+                               eatom.as<double>(), v.as<double>(), type.as<int>(), massinv.as<double>(), dtf,
+                               integrate ? 1 : 0, N);
+        });
+    }
+
+    // plain SNAP on the same state; RMS force difference; plain forces replace the surrogate's when the
+    // difference exceeds audit_tol; then the second velocity half-step
+    void audit() {
+        snap_and_tally(false, fP.as<double>(), vP.as<double>(), eP.as<double>());
+        timed(T_AUDIT, [&] {
+            hipLaunchKernelGGL(k_diff_sumsq, dim3(256), dim3(256), 0, st, f.as<double>(), fP.as<double>(),
+                               3L * N, ssq.as<double>());
+        });
+        double part[256];
+        MDP_HIP(hipMemcpyAsync(part, ssq.p, sizeof(part), hipMemcpyDeviceToHost, st));
+        MDP_HIP(hipStreamSynchronize(st));
+        double sum = 0.0;
+        for (double v_ : part) sum += v_;
+        const double rmse = std::sqrt(sum / (3.0 * N));
+        audits += 1;
+        rmse_last = rmse;
+        rmse_max = std::max(rmse_max, rmse);
+        rmse_sum += rmse;
+        if (!(rmse <= audit_tol)) {
+            fallbacks += 1;
+            MDP_HIP(hipMemcpyAsync(f.p, fP.p, sizeof(double) * 3 * N, hipMemcpyDeviceToDevice, st));
+            MDP_HIP(hipMemcpyAsync(vir.p, vP.p, sizeof(double) * 6 * N, hipMemcpyDeviceToDevice, st));
+            MDP_HIP(hipMemcpyAsync(eatom.p, eP.p, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+        }
+        hipLaunchKernelGGL(k_vv2, dim3(nb(N, 256)), dim3(256), 0, st, v.as<double>(), f.as<double>(), type.as<int>(),
+                           massinv.as<double>(), dtf, N);
     }
 
     // Time every compiled variant of the two heaviest kernels on the live data (both are pure
@@ -699,7 +885,7 @@ struct mdp_md {
             const double t = time_launch([&] {
                 launch_dedr(s, Yr.as<double>(), Yi.as<double>(), prij.as<double>(), pi.as<int>(), pti.as<int>(),
                             ptj.as<int>(), nullptr, dedr.as<double>(), N, static_cast<int>(stats[1]),
-                            poff.as<int>() + N, st, true, ids[k]);
+                            poff.as<int>() + N, st, true, ids[k], poff.as<int>());
             }, 3);
             js += (k ? ", " : "") + std::string("\"") + std::to_string(ids[k]) + "\": " + std::to_string(t);
             if (t < best) { best = t; dedr_variant = ids[k]; }
@@ -729,6 +915,7 @@ struct mdp_md {
             autotune();
             snap_and_tally(false);   // final state from the chosen variants
         }
+        if (sur) sur_and_tally(false);
         MDP_HIP(hipGetLastError());
         MDP_HIP(hipStreamSynchronize(st));
         int np = 0;
@@ -747,7 +934,13 @@ struct mdp_md {
             stats[4] += 1;
             if (stats[4] % rebuild_every == 0) timed(T_REBUILD, [&] { rebuild(true); });
             timed(T_PAIRS, [&] { pairs(); });
-            snap_and_tally(true);
+            if (sur) {
+                const bool aud = audit_every > 0 && stats[4] % audit_every == 0;
+                sur_and_tally(!aud);
+                if (aud) audit();
+            } else {
+                snap_and_tally(true);
+            }
         }
         MDP_HIP(hipGetLastError());
         MDP_HIP(hipStreamSynchronize(st));
@@ -870,6 +1063,28 @@ int mdp_md_tuning(mdp_md* md, char* buf, int len) {
         const size_t n = std::min(static_cast<size_t>(len - 1), j.size());
         std::memcpy(buf, j.data(), n);
         buf[n] = '\0';
+    });
+}
+
+int mdp_md_set_surrogate(mdp_md* md, const float* params, const double* aff, int nin, int twojmax, int audit_every,
+                         double audit_tol) {
+    return api([&] { md->set_surrogate(params, aff, nin, twojmax, audit_every, audit_tol); });
+}
+
+int mdp_md_clear_surrogate(mdp_md* md) {
+    return api([&] {
+        md->sur = false;
+        md->forces_current = false;
+    });
+}
+
+int mdp_md_audits(mdp_md* md, double* out5) {
+    return api([&] {
+        out5[0] = static_cast<double>(md->audits);
+        out5[1] = static_cast<double>(md->fallbacks);
+        out5[2] = md->rmse_last;
+        out5[3] = md->rmse_max;
+        out5[4] = md->audits ? md->rmse_sum / md->audits : 0.0;
     });
 }
 

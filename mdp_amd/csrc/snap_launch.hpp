@@ -47,6 +47,11 @@ void launch_ui(const SnapDev& s, UiMode mode, double* Ur, double* Ui, const doub
 void launch_yi(const SnapDev& s, const double* Ur, const double* Ui, const double* beta_t, double* Yr,
                double* Yi, double* epart, int natoms, bool full_mid, hipStream_t st);
 
+// Y with one weight per (coupling triple, atom): beta_t[triple*N + atom]; full layout, every idxz entry.
+// Compiled for twojmax 2 and 4 (the surrogate's descriptors); returns false for other J.
+bool launch_yi_peratom(const SnapDev& s, const double* Ur, const double* Ui, const double* beta_t, double* Yr,
+                       double* Yi, int natoms, hipStream_t st);
+
 // MD path: Y (half layout, entries the force uses) + energy partials from a half-layout U.  The half U
 // of 64 atoms is staged in LDS; the workgroup's waves take output rows from a shared counter in the
 // order row_order (device array of n_half_rows(J) row ids, most expensive first).
@@ -71,9 +76,11 @@ void launch_zi(const SnapDev& s, const double* Ur, const double* Ui, long off1, 
 // element offset (elements) for chemflag.
 void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const double* rij, const int* yatom,
                  const int* ti, const int* tj, const long* ybase, double* dedr, int natoms, int npairs_host,
-                 const int* npairs_dev, hipStream_t st, bool half_layout = false, int variant = 0);
+                 const int* npairs_dev, hipStream_t st, bool half_layout = false, int variant = 0,
+                 const int* pair_off = nullptr);
 // Variants of the half-layout dE/dr kernel (0: all three components per lane; 2, 3, 4: one component per
-// lane with at least that many waves per SIMD).  Writes the ids, returns their number.
+// lane with at least that many waves per SIMD; 5: one atom per wave, pairs given by the CSR offsets
+// pair_off, its Y read through the scalar cache).  Writes the ids, returns their number.
 int dedr_variants(int* ids);
 
 // ---- generic helpers ----------------------------------------------------------------------------

@@ -9,7 +9,9 @@
 
 namespace mdp {
 
-template <int J, int JO, bool FULLMID, bool WITH_E, bool ACCUM>
+// PER_ATOM: beta_t holds one weight per (coupling triple, atom), beta_t[triple*N + atom], instead of one
+// per triple (the learned surrogate's dE/dB differs from atom to atom).
+template <int J, int JO, bool FULLMID, bool WITH_E, bool ACCUM, bool PER_ATOM = false>
 __device__ __forceinline__ void y_row(const SnapDev& s, int row, int mb, int atom, int N, const double* __restrict__ U1r,
                                       const double* __restrict__ U1i, const double* __restrict__ U2r,
                                       const double* __restrict__ U2i, const double* __restrict__ beta_t,
@@ -24,7 +26,8 @@ __device__ __forceinline__ void y_row(const SnapDev& s, int row, int mb, int ato
         static_for<0, J1 + 1>([&](auto J2c) {
             constexpr int J2 = decltype(J2c)::value;
             if constexpr (coupled(J1, J2, JO)) {
-                const double beta = beta_t[triple_id(J, J1, J2, JO)];
+                const double beta = PER_ATOM ? beta_t[static_cast<size_t>(triple_id(J, J1, J2, JO)) * n + atom]
+                                             : beta_t[triple_id(J, J1, J2, JO)];
                 z_row<J, J1, J2, JO>(mb, malim, U1r, U1i, U2r, U2i, n, atom, s.cg, beta, yr, yi);
             }
         });

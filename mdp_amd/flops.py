@@ -12,6 +12,11 @@ enumerated from the loop bounds of the index spaces -- the denominators of the r
                       only up to ma = mb) plus 4 per entry for the energy partial U.Y.
 * ``ei``   per atom : the upstream fused energy (cpuSnap.cpp:544-719): Z over the half domain of each
                       bispectrum triple (10*na*nb + 4*nb) + 4 for the U.Z product + 2 for beta*B.
+* ``zb``   per atom : every Z entry (cpuSnap.cpp:1899-1969: 10*na*nb + 4*nb per idxz entry) plus B of
+                      the bispectrum triples (4 per half-domain U.Z product).
+* ``dbdr`` per pair : the U and dU/dr recursions of ``dedr`` plus, per Cartesian component, 4 per
+                      (bispectrum term, half-domain element) of the three-term contraction of
+                      cpuSnap.cpp:2101-2240 and 2 per term for its weight.
 """
 from functools import lru_cache
 
@@ -43,7 +48,18 @@ def snap_flops(J):
     contracted = sum((mb + 1 if j == 2 * mb else j + 1) for j in range(J + 1) for mb in range(j // 2 + 1))
     ui = _row_updates(J, False) + 4 * emitted
     dedr = _row_updates(J, True) + 32 * contracted
-    yi = yi_md = ei = 0
+    yi = yi_md = ei = zb = 0
+    terms = elems = 0
+    half = lambda l: sum((mb + 1 if l == 2 * mb else l + 1) for mb in range(l // 2 + 1))
+    for j1, j2, j in _triples(J):
+        if j >= j1:
+            terms += 3
+            elems += half(j) + half(j1) + half(j2)
+            zb += 4 * half(j) + 2
+        for mb in range(j // 2 + 1):
+            nb_ = _cg_n(mb, j1, j2, j)
+            for ma in range(j + 1):
+                zb += 10 * _cg_n(ma, j1, j2, j) * nb_ + 4 * nb_
     for j1, j2, j in _triples(J):
         for mb in range(j // 2 + 1):
             nb = _cg_n(mb, j1, j2, j)
@@ -57,7 +73,8 @@ def snap_flops(J):
                         ei += w + 4
         if j >= j1:
             ei += 2
-    return dict(ui=ui, dedr=dedr, yi=yi, yi_md=yi_md, ei=ei)
+    dbdr = _row_updates(J, True) + 3 * (4 * elems + 2 * terms)
+    return dict(ui=ui, dedr=dedr, yi=yi, yi_md=yi_md, ei=ei, zb=zb, dbdr=dbdr)
 
 
 def step_flops_per_atom(J, pairs_per_atom):

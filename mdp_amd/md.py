@@ -54,6 +54,9 @@ def _bind():
     c.mdp_md_num_timers.argtypes = []
     c.mdp_md_energy.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     c.mdp_md_tuning.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
+    c.mdp_md_set_surrogate.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+    c.mdp_md_clear_surrogate.argtypes = [vp]
+    c.mdp_md_audits.argtypes = [vp, vp]
     _bound = True
     return c
 
@@ -151,6 +154,24 @@ class SnapMD:
         buf = ctypes.create_string_buffer(4096)
         self._check(self._c.mdp_md_tuning(self._h, buf, 4096), "mdp_md_tuning")
         return json.loads(buf.value.decode() or "{}")
+
+    def set_surrogate(self, model, audit_every=50, audit_tol=0.05, twojmax=4):
+        """Use the learned surrogate ``model`` (mdp_amd.surrogate.Model) for forces from now on, with a
+        plain-SNAP audit every ``audit_every`` steps (plain forces replace the surrogate's on an audit
+        step whose RMS force difference exceeds ``audit_tol`` eV/A)."""
+        self._sur_keep = (np.ascontiguousarray(model.params, np.float32), np.ascontiguousarray(model.aff, np.float64))
+        self._check(self._c.mdp_md_set_surrogate(self._h, self._sur_keep[0].ctypes.data, self._sur_keep[1].ctypes.data,
+                                                 int(model.nin), int(twojmax), int(audit_every), float(audit_tol)),
+                    "mdp_md_set_surrogate")
+
+    def clear_surrogate(self):
+        self._check(self._c.mdp_md_clear_surrogate(self._h), "mdp_md_clear_surrogate")
+
+    def audits(self):
+        out = np.zeros(5)
+        self._check(self._c.mdp_md_audits(self._h, out.ctypes.data), "mdp_md_audits")
+        return dict(audits=int(out[0]), fallbacks=int(out[1]), last_rmse=float(out[2]), max_rmse=float(out[3]),
+                    mean_rmse=float(out[4]))
 
     def energy(self):
         ep, ek = ctypes.c_double(), ctypes.c_double()

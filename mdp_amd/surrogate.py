@@ -1,0 +1,326 @@
+"""Learned surrogate of plain SNAP: a per-atom MLP over cheap twojmax=4 bispectrum descriptors,
+trained on energies and forces of plain twojmax=8 SNAP.
+
+Everything that touches the data runs on the GPU through libmdpsnap:
+  * labels: plain FP64 twojmax=8 SNAP MD (mdp_md_*) sampled at 300-3000 K;
+  * descriptors: B and dB/dr at twojmax=4 by the batched fitting engine (mdp_fit_*);
+  * model: mdp_mlp_eval (BF16 MFMA forward + input gradient), mdp_mlp_train (parameter gradient of
+    the energy + force loss, including the second-order term of the force loss), mdp_mlp_adam;
+  * descriptor chain rule F_i = -dE/dx_i: mdp_sur_pair_s / mdp_sur_atom_f, and its adjoint mdp_sur_u.
+torch is used only to hold device buffers.  There is no upstream counterpart: the reference's
+potentials are linear/quadratic SNAP fitted by least squares (src/Julia/Potential/linearfit.jl).
+"""
+import ctypes
+import json
+
+import numpy as np
+
+from . import library
+from . import lattice
+
+NPARAM = 5313
+J_SUR = 4
+NIN = 14           # bispectrum components at twojmax = 4
+J_PLAIN = 8
+RCUTFAC, RFAC0 = 4.67637, 0.99363
+MASS = 180.95
+
+_bound = False
+
+
+def _lib():
+    global _bound
+    c = library().cdll
+    if _bound:
+        return c
+    vp, i, l, d, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_float
+    c.mdp_mlp_pack.argtypes = [vp, vp, vp]
+    c.mdp_mlp_eval.argtypes = [vp, vp, vp, i, vp, i, vp, vp, vp]
+    c.mdp_mlp_train.argtypes = [vp, vp, vp, i, vp, vp, vp, i, vp, i, vp, vp]
+    c.mdp_mlp_adam.argtypes = [vp, vp, vp, vp, f, f, f, f, i, f, vp]
+    c.mdp_sur_pair_s.argtypes = [vp, vp, vp, i, i, l, vp, vp]
+    c.mdp_sur_atom_f.argtypes = [vp, vp, vp, vp, l, i, vp, vp, vp, vp]
+    c.mdp_sur_u.argtypes = [vp, vp, vp, vp, i, i, l, d, vp, vp]
+    c.mdp_sur_energy.argtypes = [vp, vp, vp, i, d, vp, vp, vp]
+    c.mdp_sumsq.argtypes = [vp, l, vp, i, vp]
+    _bound = True
+    return c
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what}: {library().cdll.mdp_snap_last_error().decode()}")
+
+
+def _stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def plain_coefficients(seed=20241202 + 2):
+    """The plain twojmax=8 linear coefficients of the headline MD workload."""
+    return lattice.snap_coefficients(55, beta0=-11.7, sigma=0.05, seed=seed)
+
+
+class Model:
+    """Parameters (FP32, the kernels' flat layout) and the descriptor/energy affine map."""
+
+    def __init__(self, params, aff, nin=NIN):
+        self.params = np.ascontiguousarray(params, np.float32)
+        self.aff = np.ascontiguousarray(aff, np.float64)
+        self.nin = nin
+
+    @staticmethod
+    def init(mu, sigma, e_scale, e_shift, seed=0, nin=NIN):
+        rng = np.random.default_rng(seed)
+        p = np.zeros(NPARAM, np.float32)
+        W1 = np.zeros((16, 64))
+        W1[:nin] = rng.normal(0, 1.0 / np.sqrt(nin), (nin, 64))
+        p[0:1024] = W1.ravel()
+        p[1088:5184] = rng.normal(0, 1.0 / 8.0, 64 * 64)
+        p[5248:5312] = rng.normal(0, 1.0 / 8.0, 64)
+        aff = np.zeros(34)
+        aff[:nin] = mu
+        aff[16:16 + nin] = 1.0 / np.maximum(sigma, 1e-12)
+        aff[32], aff[33] = e_scale, e_shift
+        return Model(p, aff, nin)
+
+    def save(self, path):
+        np.savez(path, params=self.params, aff=self.aff, nin=self.nin)
+
+    @staticmethod
+    def load(path):
+        z = np.load(path, allow_pickle=False)
+        return Model(z["params"], z["aff"], int(z["nin"]))
+
+
+def make_training_set(nconfig=4000, ncell=5, tmin=300.0, tmax=3000.0, every=20, equil=50, per_run=100, seed=777,
+                      log=None):
+    """Configurations of 250 atoms (BCC 5^3 cells, a = 3.316 A, and random packings at the BCC density)
+    sampled from plain twojmax=8 SNAP NVE runs started at temperatures spread over [tmin, tmax]; labels
+    are the plain SNAP total energy and forces.  Returns list of dict(x, box, E, F, kind, T)."""
+    from .md import SnapMD
+    coeff = plain_coefficients()
+    out = []
+    nruns = (nconfig + per_run - 1) // per_run
+    for run in range(nruns):
+        kind = "bcc" if run % 2 == 0 else "liquid"
+        T = tmin + (tmax - tmin) * (run // 2) / max(1, (nruns - 1) // 2)
+        if kind == "bcc":
+            x, box = lattice.crystal("bcc", ncell, 3.316, 0.05, seed + run)
+        else:
+            x, box = lattice.random_packing(2 * ncell ** 3, 2 / 3.316 ** 3, 2.2, seed=seed + run)
+        v = lattice.maxwell_boltzmann(len(x), MASS, T, seed=seed + 1000 + run)
+        md = SnapMD(x, box, np.ones(len(x), np.int32), J_PLAIN, coeff, [0.5], [1.0], [MASS], v=v, dt=0.0005,
+                    skin=0.3, rebuild_every=10)
+        md.compute()
+        md.run(equil)
+        for k in range(per_run):
+            if len(out) == nconfig:
+                break
+            md.run(every)
+            st = md.state()
+            out.append(dict(x=np.mod(st["x"], box), box=box.copy(), E=float(st["eatom"].sum()), F=st["f"].copy(),
+                            kind=kind, T=T))
+        md.close()
+        if log:
+            log(f"  run {run}: {kind} at {T:.0f} K -> {len(out)} configurations")
+    return out
+
+
+class DeviceSet:
+    """Descriptors, pair chain-rule arrays and labels of a list of configurations, on the GPU."""
+
+    def __init__(self, configs, chunk=500, log=None):
+        import torch
+        from .fit import SnapFit
+        dev = torch.device("cuda")
+        C = len(configs)
+        nat = np.array([len(c["x"]) for c in configs])
+        a_off = np.concatenate([[0], np.cumsum(nat)])
+        N = int(a_off[-1])
+        X = np.zeros((NIN, N))
+        blocks, pis, pjs = [], [], []
+        p_tot = 0
+        for c0 in range(0, C, chunk):
+            sub = configs[c0:c0 + chunk]
+            fit = SnapFit([(c["x"], c["box"]) for c in sub], J_SUR, rcutfac=RCUTFAC, rfac0=RFAC0)
+            fit.compute()
+            b, db = fit.raw()
+            ai, aj, _ = fit.pairs()
+            la, lp = fit.layout()
+            for k, c in enumerate(sub):
+                g0 = int(a_off[c0 + k])
+                n = int(la[k + 1] - la[k])
+                P = int(lp[k + 1] - lp[k])
+                X[:, g0:g0 + n] = b[int(la[k]) * NIN:int(la[k + 1]) * NIN].reshape(NIN, n)
+                blocks.append(db[int(lp[k]) * 3 * NIN:int(lp[k + 1]) * 3 * NIN].reshape(NIN, 3, P))
+                pis.append(ai[int(lp[k]):int(lp[k + 1])] + g0)
+                pjs.append(aj[int(lp[k]):int(lp[k + 1])] + g0)
+                p_tot += P
+            fit.close()
+            if log:
+                log(f"  descriptors of configurations {c0}..{c0 + len(sub) - 1}")
+        pi = np.concatenate(pis).astype(np.int32)
+        pj = np.concatenate(pjs).astype(np.int32)
+        dbdr = np.concatenate(blocks, axis=2)          # (K, 3, P), pairs sorted by centre atom
+        own_off = np.concatenate([[0], np.cumsum(np.bincount(pi, minlength=N))]).astype(np.int32)
+        in_idx = np.argsort(pj, kind="stable").astype(np.int32)
+        in_off = np.concatenate([[0], np.cumsum(np.bincount(pj, minlength=N))]).astype(np.int32)
+        self.C, self.N, self.P = C, N, int(p_tot)
+        self.nat = nat
+        self.X = torch.from_numpy(np.ascontiguousarray(X)).to(dev)
+        self.dbdr = torch.from_numpy(np.ascontiguousarray(dbdr)).to(dev)
+        del dbdr, blocks
+        self.pi = torch.from_numpy(pi).to(dev)
+        self.pj = torch.from_numpy(pj).to(dev)
+        self.own_off = torch.from_numpy(own_off).to(dev)
+        self.in_off = torch.from_numpy(in_off).to(dev)
+        self.in_idx = torch.from_numpy(in_idx).to(dev)
+        self.cfg_off = torch.from_numpy(a_off.astype(np.int32)).to(dev)
+        self.Eref = torch.tensor([c["E"] for c in configs], dtype=torch.float64, device=dev)
+        self.Fref = torch.from_numpy(np.ascontiguousarray(np.concatenate([c["F"] for c in configs]))).to(dev)
+        self.Xh = X   # host copy for statistics
+
+
+class Evaluator:
+    """Energies and forces of a model on a DeviceSet, and the training step."""
+
+    def __init__(self, ds, model, nwaves=1024):
+        import torch
+        self.c = _lib()
+        self.ds = ds
+        self.model = model
+        dev = torch.device("cuda")
+        f64 = dict(dtype=torch.float64, device=dev)
+        self.params = torch.from_numpy(model.params.copy()).to(dev)
+        self.aff = torch.from_numpy(model.aff.copy()).to(dev)
+        self.packed = torch.zeros(10240, dtype=torch.int16, device=dev)
+        self.e = torch.zeros(ds.N, **f64)
+        self.g = torch.zeros(NIN * ds.N, **f64)
+        self.U = torch.zeros(NIN * ds.N, **f64)
+        self.alpha = torch.zeros(ds.N, **f64)
+        self.s = torch.zeros(3 * max(ds.P, 1), **f64)
+        self.F = torch.zeros(3 * ds.N, **f64)
+        self.R = torch.zeros(3 * ds.N, **f64)
+        self.Ec = torch.zeros(ds.C, **f64)
+        self.nwaves = nwaves
+        self.part = torch.zeros(nwaves * NPARAM, dtype=torch.float32, device=dev)
+        self.grad = torch.zeros(NPARAM, dtype=torch.float32, device=dev)
+        self.m = torch.zeros(NPARAM, dtype=torch.float32, device=dev)
+        self.v = torch.zeros(NPARAM, dtype=torch.float32, device=dev)
+        self.ssq = torch.zeros(256, **f64)
+        self.t = 0
+
+    def forward(self, wE=1.0, need_alpha=False):
+        c, ds, st = self.c, self.ds, _stream()
+        _chk(c.mdp_mlp_pack(_p(self.params), _p(self.packed), st), "mdp_mlp_pack")
+        _chk(c.mdp_mlp_eval(_p(self.packed), _p(self.params), _p(self.aff), self.model.nin, _p(ds.X), ds.N,
+                            _p(self.e), _p(self.g), st), "mdp_mlp_eval")
+        _chk(c.mdp_sur_energy(_p(self.e), _p(ds.cfg_off), _p(ds.Eref), ds.C, wE / ds.C, _p(self.Ec),
+                              _p(self.alpha) if need_alpha else None, st), "mdp_sur_energy")
+        _chk(c.mdp_sur_pair_s(_p(self.g), _p(ds.dbdr), _p(ds.pi), NIN, ds.N, ds.P, _p(self.s), st), "mdp_sur_pair_s")
+        _chk(c.mdp_sur_atom_f(_p(self.s), _p(ds.own_off), _p(ds.in_off), _p(ds.in_idx), ds.P, ds.N, _p(ds.Fref),
+                              _p(self.F), _p(self.R), st), "mdp_sur_atom_f")
+
+    def metrics(self):
+        """(force RMSE eV/A, energy RMSE eV/atom) of the last forward."""
+        import torch
+        c, ds, st = self.c, self.ds, _stream()
+        _chk(c.mdp_sumsq(_p(self.R), 3 * ds.N, _p(self.ssq), 256, st), "mdp_sumsq")
+        torch.cuda.synchronize()
+        frmse = float(np.sqrt(self.ssq.cpu().numpy().sum() / (3 * ds.N)))
+        de = (self.Ec - ds.Eref).cpu().numpy() / ds.nat
+        return frmse, float(np.sqrt(np.mean(de * de)))
+
+    def step(self, lr, wE=1.0, wF=1.0, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+        """One Adam step on L = wE mean_c ((E_c - E_c^ref)/N_c)^2 + wF mean_i,c (F - F^ref)^2."""
+        c, ds, st = self.c, self.ds, _stream()
+        self.forward(wE, need_alpha=True)
+        _chk(c.mdp_sur_u(_p(self.R), _p(ds.dbdr), _p(ds.own_off), _p(ds.pj), NIN, ds.N, ds.P, 2.0 * wF / (3 * ds.N),
+                         _p(self.U), st), "mdp_sur_u")
+        _chk(c.mdp_mlp_train(_p(self.packed), _p(self.params), _p(self.aff), self.model.nin, _p(ds.X), _p(self.U),
+                             _p(self.alpha), ds.N, _p(self.part), self.nwaves, _p(self.grad), st), "mdp_mlp_train")
+        self.t += 1
+        _chk(c.mdp_mlp_adam(_p(self.params), _p(self.grad), _p(self.m), _p(self.v), lr, b1, b2, eps, self.t, wd, st),
+             "mdp_mlp_adam")
+
+    def current_model(self):
+        return Model(self.params.cpu().numpy(), self.model.aff, self.model.nin)
+
+
+def affine_from(ds):
+    """Descriptor standardization and per-atom energy scale of a training set."""
+    mu = ds.Xh.mean(axis=1)
+    sg = ds.Xh.std(axis=1)
+    eat = ds.Eref.cpu().numpy() / ds.nat
+    return mu, sg, max(float(eat.std()), 0.1), float(eat.mean())
+
+
+def train(ds, model, steps=2000, lr=2e-3, lr_end=1e-4, wE=1.0, wF=1.0, log=None, every=100):
+    ev = Evaluator(ds, model)
+    hist = []
+    for t in range(steps):
+        frac = t / max(1, steps - 1)
+        ev.step(lr * (lr_end / lr) ** frac, wE, wF)
+        if log and (t % every == 0 or t == steps - 1):
+            ev.forward(wE)
+            fr, er = ev.metrics()
+            hist.append((t, fr, er))
+            log(f"  step {t}: force RMSE {fr:.4f} eV/A, energy RMSE {er:.4f} eV/atom")
+    return ev.current_model(), hist
+
+
+def evaluate(ds, model):
+    ev = Evaluator(ds, model)
+    ev.forward()
+    return ev.metrics()
+
+
+def linear_baseline(train_ds, test_ds, wE=1.0, wF=1.0):
+    """The method without the network: linear twojmax=4 SNAP (E_atom = b0 + beta . B) fitted by least
+    squares on the same energies and forces; returns (force RMSE, energy RMSE per atom) on test_ds."""
+    import torch
+
+    def columns(ds):
+        c, st = _lib(), _stream()
+        dev = ds.X.device
+        g = torch.zeros(NIN * ds.N, dtype=torch.float64, device=dev)
+        s = torch.zeros(3 * max(ds.P, 1), dtype=torch.float64, device=dev)
+        F = torch.zeros(3 * ds.N, dtype=torch.float64, device=dev)
+        cols = []
+        for k in range(NIN):
+            g.zero_()
+            g[k * ds.N:(k + 1) * ds.N] = 1.0
+            _chk(c.mdp_sur_pair_s(_p(g), _p(ds.dbdr), _p(ds.pi), NIN, ds.N, ds.P, _p(s), st), "mdp_sur_pair_s")
+            _chk(c.mdp_sur_atom_f(_p(s), _p(ds.own_off), _p(ds.in_off), _p(ds.in_idx), ds.P, ds.N, None, _p(F), None,
+                                  st), "mdp_sur_atom_f")
+            cols.append(F.cpu().numpy().copy())
+        Ff = np.stack(cols, axis=1)                      # (3N, K): force per unit beta_k
+        a_off = np.concatenate([[0], np.cumsum(ds.nat)])
+        Bs = np.add.reduceat(ds.Xh, a_off[:-1], axis=1).T  # (C, K): sum of B over each configuration
+        return Ff, Bs
+
+    Ff, Bs = columns(train_ds)
+    nat = train_ds.nat.astype(np.float64)
+    Ae = np.concatenate([nat[:, None], Bs], axis=1) / nat[:, None]
+    be = train_ds.Eref.cpu().numpy() / nat
+    Af = np.concatenate([np.zeros((Ff.shape[0], 1)), Ff], axis=1)
+    bf = train_ds.Fref.cpu().numpy().ravel()
+    A = np.concatenate([np.sqrt(wE / len(be)) * Ae, np.sqrt(wF / len(bf)) * Af])
+    b = np.concatenate([np.sqrt(wE / len(be)) * be, np.sqrt(wF / len(bf)) * bf])
+    coef, *_ = np.linalg.lstsq(A, b, rcond=None)
+    Ff2, Bs2 = columns(test_ds)
+    fpred = Ff2 @ coef[1:]
+    epred = (test_ds.nat * coef[0] + Bs2 @ coef[1:]) / test_ds.nat
+    fr = float(np.sqrt(np.mean((fpred - test_ds.Fref.cpu().numpy().ravel()) ** 2)))
+    er = float(np.sqrt(np.mean((epred - test_ds.Eref.cpu().numpy() / test_ds.nat) ** 2)))
+    return fr, er, coef
+
+
+def summary_json(d):
+    return json.dumps({k: (float(v) if isinstance(v, (np.floating,)) else v) for k, v in d.items()})

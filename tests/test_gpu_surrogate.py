@@ -1,0 +1,171 @@
+"""Learned surrogate on the GPU.
+
+* MLP kernels (BF16 MFMA, FP32 accumulate) against the float64 reference of tests/mlp_ref.py with the
+  same bfloat16 rounding points: energies and descriptor gradients within 1e-2, parameter gradients
+  within 2e-2, relative to the max norm (the residual is FP32 accumulation order and double rounding
+  FP32 -> BF16 of a few activations); Adam exactly as the reference formula (1e-6).
+* Descriptor chain rule: GPU surrogate forces against central finite differences (h = 1e-5 A) of the
+  float64 surrogate energy built from the oracle's twojmax=4 descriptors (within 2e-2 of the max force:
+  the GPU path is BF16), and the float64 chain rule itself against the same differences (1e-6).
+* Training: on plain-SNAP data the trained network's held-out force error is below the untrained
+  network's and below the linear twojmax=4 SNAP least-squares fit (the method without the network).
+"""
+import numpy as np
+import pytest
+
+import mlp_ref as R
+from conftest import relerr
+from mdp_amd import lattice, snapcpp
+
+pytestmark = pytest.mark.gpu
+
+RCUTFAC, RFAC0 = 4.67637, 0.99363
+
+
+def _torch():
+    import torch
+    return torch, torch.device("cuda")
+
+
+def _rand_model(seed, nin=14):
+    rng = np.random.default_rng(seed)
+    p = R.init_params(seed, nin)
+    p[1024:1088] = rng.normal(0, 0.3, 64)
+    p[5184:5248] = rng.normal(0, 0.3, 64)
+    p[5312] = 0.1
+    aff = np.zeros(34)
+    aff[:nin] = rng.normal(0, 1, nin)
+    aff[16:16 + nin] = rng.uniform(0.5, 2.0, nin)
+    aff[32], aff[33] = 1.3, -2.0
+    return p.astype(np.float32), aff, rng
+
+
+def _run_eval(p, aff, nin, X):
+    import ctypes
+    from mdp_amd import surrogate as S
+    torch, dev = _torch()
+    c = S._lib()
+    n = len(X)
+    tp = torch.from_numpy(p).to(dev)
+    ta = torch.from_numpy(aff).to(dev)
+    tq = torch.zeros(10240, dtype=torch.int16, device=dev)
+    tx = torch.from_numpy(np.ascontiguousarray(X.T)).to(dev)
+    te = torch.zeros(n, dtype=torch.float64, device=dev)
+    tg = torch.zeros(nin * n, dtype=torch.float64, device=dev)
+    st = S._stream()
+    S._chk(c.mdp_mlp_pack(S._p(tp), S._p(tq), st), "pack")
+    S._chk(c.mdp_mlp_eval(S._p(tq), S._p(tp), S._p(ta), nin, S._p(tx), n, S._p(te), S._p(tg), st), "eval")
+    torch.cuda.synchronize()
+    return te.cpu().numpy(), tg.cpu().numpy().reshape(nin, n).T, (c, tp, ta, tq, tx, st)
+
+
+@pytest.mark.parametrize("n,nin", [(1003, 14), (16, 5), (4096, 16)])
+def test_mlp_eval_vs_reference(n, nin):
+    p, aff, rng = _rand_model(n + nin, nin)
+    X = rng.normal(0, 2.0, (n, nin))
+    print(f"mlp eval n={n} nin={nin}", flush=True)
+    e, g, _ = _run_eval(p, aff, nin, X)
+    er, gr = R.forward(p, aff, nin, X, quant=True)
+    assert relerr(e, er) < 1e-2
+    assert relerr(g, gr) < 1e-2
+    e64, g64 = R.forward(p, aff, nin, X, quant=False)
+    assert relerr(e, e64) < 5e-2 and relerr(g, g64) < 5e-2
+
+
+def test_mlp_train_grad_and_adam_vs_reference():
+    from mdp_amd import surrogate as S
+    torch, dev = _torch()
+    n, nin, nwaves = 2000, 14, 37
+    p, aff, rng = _rand_model(11, nin)
+    X = rng.normal(0, 2.0, (n, nin))
+    U = rng.normal(0, 1.0, (n, nin))
+    alpha = rng.normal(0, 1.0, n)
+    _, _, (c, tp, ta, tq, tx, st) = _run_eval(p, aff, nin, X)
+    tu = torch.from_numpy(np.ascontiguousarray(U.T)).to(dev)
+    tal = torch.from_numpy(alpha).to(dev)
+    part = torch.zeros(nwaves * S.NPARAM, dtype=torch.float32, device=dev)
+    grad = torch.zeros(S.NPARAM, dtype=torch.float32, device=dev)
+    print("mlp train grad", flush=True)
+    S._chk(c.mdp_mlp_train(S._p(tq), S._p(tp), S._p(ta), nin, S._p(tx), S._p(tu), S._p(tal), n, S._p(part), nwaves,
+                           S._p(grad), st), "train")
+    torch.cuda.synchronize()
+    g = grad.cpu().numpy().astype(np.float64)
+    gr = R.train_grad(p, aff, nin, X, U, alpha, quant=True)
+    for name, (a, b) in dict(W1=(0, 1024), b1=(1024, 1088), W2=(1088, 5184), b2=(5184, 5248), w3=(5248, 5312),
+                             b3=(5312, 5313)).items():
+        assert relerr(g[a:b], gr[a:b]) < 2e-2, name
+    # Adam on the GPU gradient
+    m = torch.zeros_like(grad)
+    v = torch.zeros_like(grad)
+    p0 = tp.cpu().numpy().astype(np.float64)
+    for t in (1, 2):
+        S._chk(c.mdp_mlp_adam(S._p(tp), S._p(grad), S._p(m), S._p(v), 1e-3, 0.9, 0.999, 1e-8, t, 0.01, st), "adam")
+    torch.cuda.synchronize()
+    pr, mr, vr = R.adam(p0, g, np.zeros_like(g), np.zeros_like(g), 1e-3, 0.9, 0.999, 1e-8, 1, 0.01)
+    pr, mr, vr = R.adam(pr, g, mr, vr, 1e-3, 0.9, 0.999, 1e-8, 2, 0.01)
+    assert np.max(np.abs(tp.cpu().numpy() - pr)) < 1e-6
+
+
+def _cpu_energy_and_chain_forces(oracle, x, box, p, aff, nin):
+    """float64 surrogate energy of a configuration from the oracle's twojmax=4 descriptors, and the
+    forces by the descriptor chain rule F_i = sum_pairs(i,j) g_i.dB_i/dr_ij - sum_pairs(a,i) g_a.dB_a/dr_ai."""
+    sna = snapcpp.initsna(oracle, [1, 4, RCUTFAC, RFAC0, 0, 1, 1, 0, 0, 0, 0], [0.5], [1.0])
+    cell = [[box[0], 0, 0], [0, box[1], 0], [0, 0, box[2]]]
+    pr = snapcpp.pairs_from_box(x, np.ones(len(x)), *cell, [1, 1, 1], sna)
+    _, _, _, raw = snapcpp.snapdescriptors_pairs(oracle, sna, pr)
+    n, P, K = len(x), pr["ijnum"], sna.idxb_max
+    B = raw["blist"].reshape(K, n).T
+    db = raw["dblist"].reshape(K, 3, P)
+    e, g = R.forward(p, aff, nin, B)
+    s = np.einsum("kp,kcp->pc", g[pr["ai"]].T, db)
+    F = np.zeros((n, 3))
+    np.add.at(F, pr["ai"], s)
+    np.add.at(F, pr["aj"], -s)
+    return e.sum(), F
+
+
+def test_chain_rule_forces_vs_finite_differences(oracle):
+    from mdp_amd import surrogate as S
+    x, box = lattice.crystal("bcc", 2, 3.316, 0.15, 21)
+    rng = np.random.default_rng(2)
+    p, aff, _ = _rand_model(5)
+    aff[:14] = 0.0
+    aff[16:30] = 0.2
+    E0, Fc = _cpu_energy_and_chain_forces(oracle, x, box, p, aff, 14)
+    h = 1e-5
+    Ffd = np.zeros_like(x)
+    for i in range(len(x)):
+        for c in range(3):
+            d = np.zeros_like(x)
+            d[i, c] = h
+            Ep, _ = _cpu_energy_and_chain_forces(oracle, x + d, box, p, aff, 14)
+            Em, _ = _cpu_energy_and_chain_forces(oracle, x - d, box, p, aff, 14)
+            Ffd[i, c] = -(Ep - Em) / (2 * h)
+    assert relerr(Fc, Ffd) < 1e-6
+    # the GPU path on the same configuration
+    cfg = [dict(x=x, box=box, E=E0, F=Ffd)]
+    ds = S.DeviceSet(cfg)
+    ev = S.Evaluator(ds, S.Model(p, aff, 14))
+    ev.forward()
+    import torch
+    torch.cuda.synchronize()
+    Fg = ev.F.cpu().numpy().reshape(-1, 3)
+    assert relerr(Fg, Ffd) < 2e-2
+    assert abs(float(ev.Ec.cpu().numpy()[0]) - E0) < 2e-2 * max(1.0, abs(E0))
+
+
+def test_training_beats_untrained_and_linear():
+    from mdp_amd import surrogate as S
+    print("generating plain-SNAP training data", flush=True)
+    data = S.make_training_set(nconfig=120, ncell=3, per_run=20, every=10, equil=20, tmin=300, tmax=1500, seed=5)
+    train, test = data[:96], data[96:]
+    ds_tr, ds_te = S.DeviceSet(train), S.DeviceSet(test)
+    mu, sg, es, esh = S.affine_from(ds_tr)
+    m0 = S.Model.init(mu, sg, es, esh, seed=1)
+    f0, _ = S.evaluate(ds_te, m0)
+    m1, hist = S.train(ds_tr, m0, steps=400, lr=3e-3, lr_end=3e-4, log=print, every=100)
+    f1, e1 = S.evaluate(ds_te, m1)
+    fl, el, _ = S.linear_baseline(ds_tr, ds_te)
+    print(f"held-out force RMSE: untrained {f0:.3f}, linear J4 {fl:.3f}, MLP {f1:.3f} eV/A", flush=True)
+    assert np.isfinite(f1) and f1 < f0
+    assert f1 < fl
