@@ -230,7 +230,8 @@ typedef struct {
 } mdp_fit_params;
 
 /* nconfig configurations; configuration c has natoms[c] atoms (concatenated in x[3*sum] AoS and
- * types[sum], 1-based) in the orthogonal periodic box boxes[3c..3c+2].  Pairs of atom i are all
+ * types[sum], 1-based) in the orthogonal periodic box boxes[3c..3c+2]; coordinates must be finite
+ * and within [-L, 2L) of their box.  Pairs of atom i are all
  * periodic images x_j + s*L (j of the same configuration, the atom itself at s = 0 excluded) with
  * |x_j + s*L - x_i|^2 <= ((r_ti + r_tj) rcutfac)^2, ordered by j then (sz, sy, sx). */
 int mdp_fit_create(mdp_fit **h, const mdp_fit_params *p, int nconfig, const int *natoms, const double *boxes,

@@ -29,6 +29,7 @@ struct mdp_fit {
     hipStream_t st = nullptr;
     DevTables tab;
     std::vector<int> a0_h;   // [C+1]
+    std::vector<double> boxes_h;
     DevBuf x, type, cfg_of, cfg_a0, cfg_L, cfg_p0, rc2, pcnt, poff, scan_tmp, flags, counter;
     DevBuf pi, pj, pti, ptj, rij, Ur, Ui, zt, blist, dblist, bzero, t_lo, t_b, t_z, t_w;
     long cap = 0;            // pair capacity of the pair and dB/dr buffers
@@ -66,8 +67,10 @@ struct mdp_fit {
             a0_h[c + 1] = a0_h[c] + natoms[c];
         }
         N = a0_h[C];
+        boxes_h.assign(boxes, boxes + 3 * C);
         for (int g = 0; g < N; ++g)
             if (types[g] < 1 || types[g] > ntypes) throw Error("mdp_fit_create: atom types must be 1..ntypes");
+        check_positions(x0, "mdp_fit_create");
         MDP_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
 
         SnapIndex ix;
@@ -134,7 +137,20 @@ struct mdp_fit {
         MDP_HIP(hipStreamSynchronize(st));   // host staging vectors go out of scope
     }
 
+    // the pair build derives image ranges from the coordinates: they must be finite and within one box
+    // length of the box, which bounds the images each atom pair visits
+    void check_positions(const double* xh, const char* where) const {
+        for (int c = 0; c < C; ++c)
+            for (long g = a0_h[c]; g < a0_h[c + 1]; ++g)
+                for (int d = 0; d < 3; ++d) {
+                    const double v = xh[3 * g + d], L = boxes_h[3 * c + d];
+                    if (!std::isfinite(v)) throw Error(std::string(where) + ": non-finite position");
+                    if (v < -L || v >= 2.0 * L) throw Error(std::string(where) + ": position outside [-L, 2L)");
+                }
+    }
+
     void set_positions(const double* xh) {
+        check_positions(xh, "mdp_fit_set_positions");
         MDP_HIP(hipMemcpyAsync(x.p, xh, sizeof(double) * 3 * N, hipMemcpyHostToDevice, st));
         MDP_HIP(hipStreamSynchronize(st));
     }
@@ -172,6 +188,7 @@ struct mdp_fit {
                                  cfg_L.as<double>(), rc2.as<double>(), ntypes, N, pcnt.as<int>(), poff.as<int>(), cap,
                                  rij.as<double>(), pi.as<int>(), pj.as<int>(), pti.as<int>(), ptj.as<int>(),
                                  flags.as<int>(), st);
+                launch_fit_guard(flags.as<int>(), poff.as<int>(), N, st);
                 break;
             case 1:
                 launch_ui(s, UI_FULL_WSELF, Ur.as<double>(), Ui.as<double>(), rij.as<double>(), poff.as<int>(), nullptr,
@@ -260,13 +277,16 @@ struct mdp_fit {
         for (int attempt = 0; attempt < 2; ++attempt) {
             if (!gexec) capture();
             MDP_HIP(hipGraphLaunch(gexec, st));
-            int fl = 0, tot = 0;
-            MDP_HIP(hipMemcpyAsync(&fl, flags.p, sizeof(int), hipMemcpyDeviceToHost, st));
+            int fl[2] = {0, 0}, tot = 0;
+            MDP_HIP(hipMemcpyAsync(fl, flags.p, sizeof(fl), hipMemcpyDeviceToHost, st));
             MDP_HIP(hipMemcpyAsync(&tot, poff.as<int>() + N, sizeof(int), hipMemcpyDeviceToHost, st));
             MDP_HIP(hipStreamSynchronize(st));
             npairs = tot;
-            if (!(fl & 1)) return;
-            ensure_pairs(tot);   // positions changed and the pair list outgrew its buffers
+            if (!(fl[0] & 1)) return;
+            // positions changed and the pair list outgrew its buffers: the guard emptied the pair list
+            // for the rest of this run; resize to the reported need and run again
+            npairs = 0;
+            ensure_pairs(fl[1]);
         }
         throw Error("mdp_fit_compute: pair buffers could not be sized");
     }

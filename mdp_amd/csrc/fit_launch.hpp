@@ -12,10 +12,13 @@ namespace mdp {
 
 // Pair build by per-configuration brute force over periodic images.  fill = false writes the pair
 // count of every atom to pcnt; fill = true writes rij/pi/pj/pti/ptj at the CSR offsets poff (N+1
-// entries) and sets bit 0 of flags (leaving the outputs unwritten) when poff[N] exceeds cap.
+// entries); when poff[N] exceeds cap it leaves the outputs unwritten, sets bit 0 of flags[0] and
+// stores poff[N] in flags[1].
 void launch_fit_pairs(bool fill, const double* x, const int* type, const int* cfg_of, const int* cfg_a0,
                       const double* cfg_L, const double* rc2, int ntypes, int N, int* pcnt, const int* poff, long cap,
                       double* rij, int* pi, int* pj, int* pti, int* ptj, int* flags, hipStream_t st);
+// after an overflow flagged by the fill pass: poff[0..N] = 0 (no pairs for the remaining stages)
+void launch_fit_guard(const int* flags, int* poff, int N, hipStream_t st);
 // cfg_p0[c] = poff[cfg_a0[c]] for c = 0..C (first pair of each configuration, total at C)
 void launch_fit_cfg_p0(const int* poff, const int* cfg_a0, int* cfg_p0, int C, hipStream_t st);
 // p[0..n) = 0 by a kernel (safe inside captured graphs)

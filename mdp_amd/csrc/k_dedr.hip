@@ -103,22 +103,33 @@ __global__ __launch_bounds__(64, WPE) void k_dedr_split(SnapDev s, const double*
     dedr[3 * static_cast<size_t>(p) + comp] = 2.0 * d;
 }
 
+// Y of the half layout ([element][atom]) copied atom-major and complex-interleaved,
+// yt[(atom*HE + e)*2 + {0,1}], so that one atom's Y is a contiguous block.
+__global__ void k_y_atom_major(const double* __restrict__ Yr, const double* __restrict__ Yi, int HE, int N,
+                               double* __restrict__ yt) {
+    const long k = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= static_cast<long>(HE) * N) return;
+    const int e = static_cast<int>(k / N), atom = static_cast<int>(k % N);
+    const size_t o = (static_cast<size_t>(atom) * HE + e) * 2;
+    yt[o] = Yr[k];
+    yt[o + 1] = Yi[k];
+}
+
 // One wave per centre atom (grid = atoms), lanes (pair, Cartesian component) as in k_dedr_split for up
-// to 21 pairs per pass.  All pairs of the wave share the atom, so its Y rows are wave-uniform and come in
-// through scalar loads instead of one gather per lane.
+// to 21 pairs per pass.  All pairs of the wave share the atom, so its Y block (atom-major copy) is
+// wave-uniform and every element comes in through scalar loads at compile-time offsets.
 template <int J>
-__global__ __launch_bounds__(64, 2) void k_dedr_atom(const double* __restrict__ rootpq, const double* __restrict__ Yr,
-                                                    const double* __restrict__ Yi, const double* __restrict__ radelem,
+__global__ __launch_bounds__(64, 2) void k_dedr_atom(const double* __restrict__ rootpq, const double* __restrict__ yt,
+                                                    const double* __restrict__ radelem,
                                                     const double* __restrict__ wjelem, double rcutfac, double rmin0,
                                                     double rfac0, int switchflag, const double* __restrict__ rij,
                                                     const int* __restrict__ ti, const int* __restrict__ tj,
-                                                    const int* __restrict__ poff, double* __restrict__ dedr, int N) {
+                                                    const int* __restrict__ poff, double* __restrict__ dedr) {
+    constexpr int HE = n_half_elems(J);
     const int atom = blockIdx.x;
     const int lane = threadIdx.x, comp = lane % 3, slot = lane / 3;
     const int q0 = poff[atom], q1 = poff[atom + 1];
-    const size_t n = static_cast<size_t>(N);
-    const double* __restrict__ yr = Yr + atom;
-    const double* __restrict__ yi = Yi + atom;
+    const double* __restrict__ y = yt + static_cast<size_t>(atom) * HE * 2;
     for (int qb = q0; qb < q1; qb += 21) {
         const int q = qb + slot;
         const bool active = lane < 63 && q < q1;
@@ -137,13 +148,13 @@ __global__ __launch_bounds__(64, 2) void k_dedr_atom(const double* __restrict__ 
         row_chain<J, 1, 0, J / 2>(g, rootpq, [&](auto Jc, auto MBc, auto& pr, auto& pi, auto& dpr, auto& dpi) {
             constexpr int JJ = decltype(Jc)::value, MB = decltype(MBc)::value;
             if constexpr (JJ == 0) {
-                d = dsu * (0.5 * yr[0]);
+                d = dsu * (0.5 * y[0]);
             } else {
                 constexpr int LAST = (JJ == 2 * MB) ? MB : JJ;
                 constexpr int row = h_block(JJ) + (JJ + 1) * MB;
 #pragma unroll
                 for (int ma = 0; ma <= LAST; ++ma) {
-                    double a = yr[(row + ma) * n], b = yi[(row + ma) * n];
+                    double a = y[2 * (row + ma)], b = y[2 * (row + ma) + 1];
                     if (JJ == 2 * MB && ma == MB) { a *= 0.5; b *= 0.5; }
                     d += (dsu * pr[ma] + g.sfac * dpr[0][ma]) * a + (dsu * pi[ma] + g.sfac * dpi[0][ma]) * b;
                 }
@@ -153,25 +164,38 @@ __global__ __launch_bounds__(64, 2) void k_dedr_atom(const double* __restrict__ 
     }
 }
 
+void launch_dedr_atom(const SnapDev& s, const double* Yr, const double* Yi, double* yt, const double* rij,
+                      const int* ti, const int* tj, const int* pair_off, double* dedr, int N, hipStream_t st) {
+    if (N <= 0) return;
+    int HE = 0;
+    switch (s.J) {
+        case 2: HE = n_half_elems(2); break;
+        case 4: HE = n_half_elems(4); break;
+        case 6: HE = n_half_elems(6); break;
+        case 8: HE = n_half_elems(8); break;
+        default: return;
+    }
+    const long tot = static_cast<long>(HE) * N;
+    hipLaunchKernelGGL(k_y_atom_major, dim3((tot + 255) / 256), dim3(256), 0, st, Yr, Yi, HE, N, yt);
+    switch (s.J) {
+#define DA_CASE(JV)                                                                                               \
+    case JV:                                                                                                      \
+        hipLaunchKernelGGL((k_dedr_atom<JV>), dim3(N), dim3(64), 0, st, s.rootpq, yt, s.radelem, s.wjelem, s.rcutfac, \
+                           s.rmin0, s.rfac0, s.switchflag, rij, ti, tj, pair_off, dedr);                          \
+        return;
+        DA_CASE(2)
+        DA_CASE(4)
+        DA_CASE(6)
+        DA_CASE(8)
+#undef DA_CASE
+        default: return;
+    }
+}
+
 void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const double* rij, const int* yatom,
                  const int* ti, const int* tj, const long* ybase, double* dedr, int N, int npairs_host,
-                 const int* npairs_dev, hipStream_t st, bool half, int variant, const int* pair_off) {
+                 const int* npairs_dev, hipStream_t st, bool half, int variant) {
     if (npairs_host <= 0) return;
-    if (half && variant == 5 && pair_off && !ybase) {
-        switch (s.J) {
-#define DA_CASE(JV)                                                                                                \
-    case JV:                                                                                                       \
-        hipLaunchKernelGGL((k_dedr_atom<JV>), dim3(N), dim3(64), 0, st, s.rootpq, Yr, Yi, s.radelem, s.wjelem,       \
-                           s.rcutfac, s.rmin0, s.rfac0, s.switchflag, rij, ti, tj, pair_off, dedr, N);              \
-        return;
-            DA_CASE(2)
-            DA_CASE(4)
-            DA_CASE(6)
-            DA_CASE(8)
-#undef DA_CASE
-            default: return;
-        }
-    }
     const dim3 g3((npairs_host + 63) / 64), g1((npairs_host + 20) / 21);
 #define DE_ARGS Yr, Yi, rij, yatom, ti, tj, ybase, dedr, N, npairs_host, npairs_dev
 #define DE_CASE(JV)                                                                                            \
@@ -201,7 +225,7 @@ void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const dou
 int dedr_variants(int* ids) {
     static const int v[] = {0, 2, 3, 4, 5};
     for (int k = 0; k < 5; ++k) ids[k] = v[k];
-    return 5;
+    return 5;   // 5 is launch_dedr_atom
 }
 
 }  // namespace mdp

@@ -75,7 +75,8 @@ __device__ __forceinline__ double dist2(double dx, double dy, double dz) {
 // Pairs of atom g: every periodic image x_j + s*L of every atom j of its configuration with
 // |x_j + s*L - x_i|^2 <= rc(ti,tj)^2, the atom itself at s = 0 excluded (neighpairlist.jl:52 with the
 // ghost construction of fullneighborlist.jl).  Order: j ascending, then (sz, sy, sx) ascending.
-// FILL = false counts, FILL = true writes the CSR slot poff[g].. (capacity checked against cap).
+// FILL = false counts, FILL = true writes the CSR slot poff[g].. when the total fits cap; otherwise
+// it writes nothing, sets bit 0 of flags[0] and stores the needed total in flags[1].
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_fit_pairs(const double* __restrict__ x, const int* __restrict__ type,
                                                    const int* __restrict__ cfg_of, const int* __restrict__ cfg_a0,
@@ -95,8 +96,11 @@ __global__ __launch_bounds__(256) void k_fit_pairs(const double* __restrict__ x,
     long q = 0;
     if constexpr (FILL) {
         q = poff[g];
-        if (static_cast<long>(poff[N]) > cap) {
-            if (g == 0) atomicOr(flags, 1);
+        if (static_cast<long>(poff[N]) > cap) {   // buffers too small: write nothing, report the need
+            if (g == 0) {
+                atomicOr(flags, 1);
+                flags[1] = poff[N];
+            }
             return;
         }
     }
@@ -142,6 +146,13 @@ __global__ void k_fit_cfg_p0(const int* __restrict__ poff, const int* __restrict
                              int C) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c <= C) cfg_p0[c] = poff[cfg_a0[c]];
+}
+
+// after a capacity overflow of the fill pass, empty every atom's pair range so that the remaining
+// stages of the batch touch no pair (the host then resizes and runs the batch again)
+__global__ void k_fit_guard(const int* __restrict__ flags, int* __restrict__ poff, int N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= N && (flags[0] & 1)) poff[i] = 0;
 }
 
 __global__ void k_fill_zero_int(int* __restrict__ p, int n) {
@@ -354,6 +365,10 @@ void launch_fit_pairs(bool fill, const double* x, const int* type, const int* cf
     else
         hipLaunchKernelGGL((k_fit_pairs<false>), grid, block, 0, st, x, type, cfg_of, cfg_a0, cfg_L, rc2, ntypes, N,
                            pcnt, poff, cap, rij, pi, pj, pti, ptj, flags);
+}
+
+void launch_fit_guard(const int* flags, int* poff, int N, hipStream_t st) {
+    hipLaunchKernelGGL(k_fit_guard, dim3((N + 1 + 255) / 256), dim3(256), 0, st, flags, poff, N);
 }
 
 void launch_fit_cfg_p0(const int* poff, const int* cfg_a0, int* cfg_p0, int C, hipStream_t st) {

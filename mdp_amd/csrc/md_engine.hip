@@ -445,7 +445,7 @@ struct mdp_md {
     // device state
     DevBuf x, v, f, type, massinv, xref, cell_of, cell_cnt, cell_start, cell_fill, cell_atoms, scan_tmp, nb_cnt, nb_j,
         nb_s, in_cnt, in_e, pcnt, poff, pi, pj, pti, ptj, prij, entry_pair, Ur, Ui, Yr, Yi, epart, dedr, eatom, vir,
-        beta_t, rc2, flags, maxd2, row_order;
+        beta_t, rc2, flags, maxd2, row_order, yt;
     long stats[6] = {0, 0, 0, 0, 0, 0};
     bool forces_current = false;
     std::vector<double> rad_h, wj_h;
@@ -699,6 +699,19 @@ struct mdp_md {
                            entry_pair.as<int>(), L[0], L[1], L[2], N);
     }
 
+    void dedr_launch(int variant) {
+        const SnapDev& s = tab.view;
+        if (variant == 5) {
+            yt.ensure(sizeof(double) * 2 * static_cast<size_t>(tab.idx.half_count()) * N);
+            launch_dedr_atom(s, Yr.as<double>(), Yi.as<double>(), yt.as<double>(), prij.as<double>(), pti.as<int>(),
+                             ptj.as<int>(), poff.as<int>(), dedr.as<double>(), N, st);
+        } else {
+            launch_dedr(s, Yr.as<double>(), Yi.as<double>(), prij.as<double>(), pi.as<int>(), pti.as<int>(),
+                        ptj.as<int>(), nullptr, dedr.as<double>(), N, static_cast<int>(stats[1]), poff.as<int>() + N,
+                        st, true, variant);
+        }
+    }
+
     // plain SNAP forces; outputs to (fo, vo, eo), by default the engine's f, vir and eatom
     void snap_and_tally(bool integrate, double* fo = nullptr, double* vo = nullptr, double* eo = nullptr) {
         const SnapDev& s = tab.view;
@@ -714,9 +727,7 @@ struct mdp_md {
                            epart.as<double>(), row_order.as<int>(), N, st, yi_waves);
         });
         timed(T_DEDR, [&] {
-            launch_dedr(s, Yr.as<double>(), Yi.as<double>(), prij.as<double>(), pi.as<int>(), pti.as<int>(),
-                        ptj.as<int>(), nullptr, dedr.as<double>(), N, static_cast<int>(stats[1]), poff.as<int>() + N,
-                        st, true, dedr_variant, poff.as<int>());
+            dedr_launch(dedr_variant);
         });
         timed(T_TALLY, [&] {
             hipLaunchKernelGGL(k_tally, dim3(nb(N, 256)), dim3(256), 0, st, dedr.as<double>(), prij.as<double>(),
@@ -883,9 +894,7 @@ struct mdp_md {
         js += "\"compute_dedr\": {";
         for (int k = 0; k < nv; ++k) {
             const double t = time_launch([&] {
-                launch_dedr(s, Yr.as<double>(), Yi.as<double>(), prij.as<double>(), pi.as<int>(), pti.as<int>(),
-                            ptj.as<int>(), nullptr, dedr.as<double>(), N, static_cast<int>(stats[1]),
-                            poff.as<int>() + N, st, true, ids[k], poff.as<int>());
+                dedr_launch(ids[k]);
             }, 3);
             js += (k ? ", " : "") + std::string("\"") + std::to_string(ids[k]) + "\": " + std::to_string(t);
             if (t < best) { best = t; dedr_variant = ids[k]; }

@@ -76,11 +76,14 @@ void launch_zi(const SnapDev& s, const double* Ur, const double* Ui, long off1, 
 // element offset (elements) for chemflag.
 void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const double* rij, const int* yatom,
                  const int* ti, const int* tj, const long* ybase, double* dedr, int natoms, int npairs_host,
-                 const int* npairs_dev, hipStream_t st, bool half_layout = false, int variant = 0,
-                 const int* pair_off = nullptr);
-// Variants of the half-layout dE/dr kernel (0: all three components per lane; 2, 3, 4: one component per
-// lane with at least that many waves per SIMD; 5: one atom per wave, pairs given by the CSR offsets
-// pair_off, its Y read through the scalar cache).  Writes the ids, returns their number.
+                 const int* npairs_dev, hipStream_t st, bool half_layout = false, int variant = 0);
+// dE/dr with one centre atom per wave (pairs sorted by centre atom, CSR offsets pair_off): Y (half
+// layout) is first copied atom-major into yt (2 * n_half_elems(J) * natoms doubles) so that each
+// wave reads its atom's Y through the scalar cache.
+void launch_dedr_atom(const SnapDev& s, const double* Yr, const double* Yi, double* yt, const double* rij,
+                      const int* ti, const int* tj, const int* pair_off, double* dedr, int natoms, hipStream_t st);
+// Variants of the MD path's dE/dr (0: all three components per lane; 2, 3, 4: one component per lane
+// with at least that many waves per SIMD; 5: launch_dedr_atom).  Writes the ids, returns their number.
 int dedr_variants(int* ids);
 
 // ---- generic helpers ----------------------------------------------------------------------------

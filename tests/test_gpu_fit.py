@@ -66,8 +66,8 @@ def _oracle_descriptors(oracle, J, x, ai, aj, rij, bnorm=0, bzero=1):
     return raw["blist"].reshape(K, n).T, raw["dblist"].reshape(K, 3, P).transpose(2, 1, 0)
 
 
-def _check_config(oracle, fit, c, cfg, b, db, ai, aj, rij, J, bnorm=0, bzero=1):
-    x, box = cfg[0], cfg[1]
+def _check_config(oracle, fit, c, cfgs, b, db, ai, aj, rij, J, bnorm=0, bzero=1):
+    x, box = cfgs[c][0], cfgs[c][1]
     bc, dbc, sl = fit.config(c, b, db)
     off, nj, ns = _bruteforce(oracle, x, box)
     # the pair list: same owners and images in the same order, r_ij bit-exact
