@@ -2,12 +2,12 @@
 
 Two subcommands:
 
-``python -m mdp_amd.traffic copy --bytes B --reps R``
-    Runs R device-to-device copies of B bytes (one copy kernel each).  Profiled with
-    ``rocprofv3 --pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE`` it calibrates the counters: a copy of B bytes
-    reads and writes exactly B bytes.
+``python mdp_amd/traffic.py copy --bytes B --reps R``
+    Runs R elementwise copies (dst = 1.0 * src) of B bytes, one kernel each.  Profiled with
+    ``rocprofv3 --pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE`` it calibrates the counters: each launch reads
+    and writes exactly B bytes.
 
-``python -m mdp_amd.traffic summarize --kernel REGEX --fetch DIR --write DIR --copy-fetch DIR
+``python mdp_amd/traffic.py summarize --kernel REGEX --fetch DIR --write DIR --copy-fetch DIR
 --copy-write DIR --copy-bytes B --out FILE``
     Reads the counter CSV files of the four runs, averages FETCH_SIZE and WRITE_SIZE (KiB per
     dispatch) over the dispatches of the kernels matching REGEX, scales each by the factor the same
@@ -55,7 +55,7 @@ def cmd_copy(a):
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
     for _ in range(a.reps):
-        dst.copy_(src)
+        torch.mul(src, 1.0, out=dst)
     torch.cuda.synchronize()
     print(f"copied {a.reps} x {a.bytes} bytes", flush=True)
 
@@ -94,7 +94,7 @@ def main(argv=None):
     s.add_argument("--copy-fetch", required=True)
     s.add_argument("--copy-write", required=True)
     s.add_argument("--copy-bytes", type=int, default=1 << 30)
-    s.add_argument("--copy-kernel", default="copy|Copy|elementwise")
+    s.add_argument("--copy-kernel", default="elementwise|Mul|mul")
     s.add_argument("--algorithmic", type=float, default=0.0, help="algorithmic bytes per launch")
     s.add_argument("--out", required=True)
     a = ap.parse_args(argv)
