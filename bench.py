@@ -302,15 +302,18 @@ def main():
         log(f"  {k:16s} {ms:9.2f} ms total, {cnt:6d} launches, {ms / max(cnt, 1):8.4f} ms avg")
 
     fl = snap_flops(J)
-    ppa = stats["pairs"] / n
-    per_launch = {"compute_yi": fl["yi_md"] * n, "compute_dedr": fl["dedr"] * stats["pairs"],
-                  "compute_ui": fl["ui"] * stats["pairs"]}
+    pairs_mean = md.mean_pairs()          # pairs per step averaged over the timed steps
+    ppa = pairs_mean / n
+    per_launch = {"compute_yi": fl["yi_md"] * n, "compute_dedr": fl["dedr"] * pairs_mean,
+                  "compute_ui": fl["ui"] * pairs_mean}
     dom = max(per_launch, key=lambda k: timers[k][0])
     ms_avg = timers[dom][0] / max(timers[dom][1], 1)
     achieved = per_launch[dom] / (ms_avg * 1e-3) / 1e12
+    # HBM bytes per launch of the dominant kernel, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
+    # workload and calibrated with a copy kernel of known size (profiles/traffic_<kernel>_bcc40_j8.*)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_bcc40_j8.json")
-    if os.path.exists(tfile):
+    if os.path.exists(tfile) and a.ncell == NCELL:
         traffic = json.load(open(tfile)).get("bytes_per_launch")
     step_flops = n * (ppa * (fl["ui"] + fl["dedr"]) + fl["yi_md"])
     log(f"dominant kernel {dom}: {ms_avg:.3f} ms/launch, {achieved:.2f} TFLOP/s "

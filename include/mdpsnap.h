@@ -193,6 +193,9 @@ int mdp_md_num_timers(void);
  * with the measured ms per launch of every variant (set MDP_AUTOTUNE=0 to keep the defaults). */
 int mdp_md_tuning(mdp_md *md, char *buf, int len);
 
+/* Mean number of pairs per force evaluation since timing was last switched (mdp_md_set_timing). */
+int mdp_md_mean_pairs(mdp_md *md, double *mean);
+
 /* Total potential and kinetic energy (eV) of the current state. */
 int mdp_md_energy(mdp_md *md, double *epot, double *ekin);
 

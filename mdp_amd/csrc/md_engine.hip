@@ -337,9 +337,10 @@ __global__ void k_tally(const double* __restrict__ dedr, const double* __restric
                         const double* __restrict__ eext, double* __restrict__ f, double* __restrict__ vir,
                         double* __restrict__ eatom,
                         double* __restrict__ v, const int* __restrict__ type, const double* __restrict__ massinv_t,
-                        double dtf, int integrate, int N) {
+                        double dtf, int integrate, int N, unsigned long long* __restrict__ pair_sum) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
+    if (pair_sum && i == 0) atomicAdd(pair_sum, static_cast<unsigned long long>(poff[N]));   // pairs per step
     double fx = 0, fy = 0, fz = 0, w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0, w5 = 0;
     for (int p = poff[i]; p < poff[i + 1]; ++p) {
         const double gx = dedr[3 * static_cast<size_t>(p)], gy = dedr[3 * static_cast<size_t>(p) + 1],
@@ -445,9 +446,11 @@ struct mdp_md {
     // device state
     DevBuf x, v, f, type, massinv, xref, cell_of, cell_cnt, cell_start, cell_fill, cell_atoms, scan_tmp, nb_cnt, nb_j,
         nb_s, in_cnt, in_e, pcnt, poff, pi, pj, pti, ptj, prij, entry_pair, Ur, Ui, Yr, Yi, epart, dedr, eatom, vir,
-        beta_t, rc2, flags, maxd2, row_order, yt;
+        beta_t, rc2, flags, maxd2, row_order, yt, pair_sum;
+    long pair_steps0 = 0;   // force evaluations counted in pair_sum since the last reset
     long stats[6] = {0, 0, 0, 0, 0, 0};
     bool forces_current = false;
+    long force_evals = 0;   // force evaluations (steps and compute calls) whose pairs were counted
     std::vector<double> rad_h, wj_h;
     // learned surrogate (mdp_md_set_surrogate): forces from a per-atom MLP over low-order bispectrum
     // descriptors, audited every audit_every steps against plain SNAP
@@ -619,6 +622,8 @@ struct mdp_md {
         vir.ensure(sizeof(double) * 6 * n);
         flags.ensure(sizeof(int) * 4);
         maxd2.ensure(sizeof(unsigned long long));
+        pair_sum.ensure(sizeof(unsigned long long));
+        MDP_HIP(hipMemsetAsync(pair_sum.p, 0, sizeof(unsigned long long), st));
         MDP_HIP(hipMemsetAsync(Yr.p, 0, sizeof(double) * n * iu, st));
         MDP_HIP(hipMemsetAsync(Yi.p, 0, sizeof(double) * n * iu, st));
         MDP_HIP(hipStreamSynchronize(st));
@@ -715,6 +720,8 @@ struct mdp_md {
     // plain SNAP forces; outputs to (fo, vo, eo), by default the engine's f, vir and eatom
     void snap_and_tally(bool integrate, double* fo = nullptr, double* vo = nullptr, double* eo = nullptr) {
         const SnapDev& s = tab.view;
+        const bool count = (fo == nullptr);   // audit evaluations do not count as steps
+        if (count) ++force_evals;
         if (!fo) fo = f.as<double>();
         if (!vo) vo = vir.as<double>();
         if (!eo) eo = eatom.as<double>();
@@ -733,7 +740,7 @@ struct mdp_md {
             hipLaunchKernelGGL(k_tally, dim3(nb(N, 256)), dim3(256), 0, st, dedr.as<double>(), prij.as<double>(),
                                poff.as<int>(), in_cnt.as<int>(), in_e.as<int>(), entry_pair.as<int>(),
                                epart.as<double>(), nrows, e0, nullptr, fo, vo, eo, v.as<double>(), type.as<int>(),
-                               massinv.as<double>(), dtf, integrate ? 1 : 0, N);
+                               massinv.as<double>(), dtf, integrate ? 1 : 0, N, count ? pair_sum.as<unsigned long long>() : nullptr);
         });
     }
 
@@ -814,6 +821,7 @@ struct mdp_md {
     // dE/dr by the adjoint route (Y with per-atom weights dE/dB, then the dU/dr contraction)
     void sur_and_tally(bool integrate) {
         const SnapDev& s4 = tab_s.view;
+        ++force_evals;
         timed(T_SUR, [&] {
             launch_ui(s4, UI_FULL_WSELF, sUr.as<double>(), sUi.as<double>(), prij.as<double>(), poff.as<int>(), nullptr,
                       pti.as<int>(), ptj.as<int>(), N, 1.0, -1, st);
@@ -833,7 +841,7 @@ struct mdp_md {
                                poff.as<int>(), in_cnt.as<int>(), in_e.as<int>(), entry_pair.as<int>(),
                                epart.as<double>(), 0, 0.0, sE.as<double>(), f.as<double>(), vir.as<double>(),
                                eatom.as<double>(), v.as<double>(), type.as<int>(), massinv.as<double>(), dtf,
-                               integrate ? 1 : 0, N);
+                               integrate ? 1 : 0, N, pair_sum.as<unsigned long long>());
         });
     }
 
@@ -1047,6 +1055,8 @@ int mdp_md_stats(mdp_md* md, long* out6) {
 int mdp_md_set_timing(mdp_md* md, int on) {
     return api([&] {
         md->flush_timers();
+        MDP_HIP(hipMemsetAsync(md->pair_sum.p, 0, sizeof(unsigned long long), md->st));
+        md->pair_steps0 = md->force_evals;
         md->timing = on != 0;
         for (int k = 0; k < T_COUNT; ++k) {
             md->t_ms[k] = 0;
@@ -1084,6 +1094,16 @@ int mdp_md_clear_surrogate(mdp_md* md) {
     return api([&] {
         md->sur = false;
         md->forces_current = false;
+    });
+}
+
+int mdp_md_mean_pairs(mdp_md* md, double* mean) {
+    return api([&] {
+        unsigned long long sum = 0;
+        MDP_HIP(hipMemcpyAsync(&sum, md->pair_sum.p, sizeof(sum), hipMemcpyDeviceToHost, md->st));
+        MDP_HIP(hipStreamSynchronize(md->st));
+        const long n = md->force_evals - md->pair_steps0;
+        *mean = n > 0 ? static_cast<double>(sum) / n : 0.0;
     });
 }
 

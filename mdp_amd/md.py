@@ -57,6 +57,7 @@ def _bind():
     c.mdp_md_set_surrogate.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
     c.mdp_md_clear_surrogate.argtypes = [vp]
     c.mdp_md_audits.argtypes = [vp, vp]
+    c.mdp_md_mean_pairs.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     _bound = True
     return c
 
@@ -166,6 +167,12 @@ class SnapMD:
 
     def clear_surrogate(self):
         self._check(self._c.mdp_md_clear_surrogate(self._h), "mdp_md_clear_surrogate")
+
+    def mean_pairs(self):
+        """Mean pair count per force evaluation since set_timing() was last called."""
+        m = ctypes.c_double()
+        self._check(self._c.mdp_md_mean_pairs(self._h, ctypes.byref(m)), "mdp_md_mean_pairs")
+        return m.value
 
     def audits(self):
         out = np.zeros(5)
