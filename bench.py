@@ -163,8 +163,9 @@ def bench_surrogate(a):
 
     mdp_amd.library()
     t0 = time.perf_counter()
-    data = S.make_training_set(nconfig=a.train_configs, log=log)
-    log(f"plain-SNAP training data: {len(data)} configurations in {time.perf_counter() - t0:.1f} s")
+    data, dropped = S.make_training_set(nconfig=a.train_configs, log=log)
+    log(f"plain-SNAP training data: {len(data)} configurations in {time.perf_counter() - t0:.1f} s "
+        f"({dropped} samples dropped: a pair under 2 A or a force above 200 eV/A)")
     ntest = max(1, len(data) // 10)
     rng = np.random.default_rng(SEED + 7)
     perm = rng.permutation(len(data))
@@ -230,7 +231,7 @@ def bench_surrogate(a):
         "data": "synthetic (plain-SNAP MD samples for training; seeded BCC lattice with vacancies for MD)",
         "config": {"workload": f"surrogate NVE MD, BCC Ta {a.sur_ncell}^3 cells with 0.5% vacancies ({n} atoms), "
                                "600 K, dt=1 fs, audit every 50 steps", "atoms": n, "parallelism": "1 GPU",
-                   "train_configs": len(train), "train_steps": a.train_steps},
+                   "train_configs": len(train), "train_steps": a.train_steps, "samples_dropped": dropped},
         "surrogate": {"atom_steps_per_s_without_audits": n / t_noaudit, "plain_gpu_atom_steps_per_s": n / t_plain,
                       "speedup_vs_plain_with_audits": (n * a.steps / elapsed) / (n / t_plain),
                       "speedup_vs_plain_without_audits": t_plain / t_noaudit,

@@ -24,6 +24,8 @@
 #ifndef MDPSNAP_H
 #define MDPSNAP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -275,6 +277,14 @@ int mdp_fit_profile(mdp_fit *h, double *ms4);
  * (x[k*n + atom]).  All pointers are device pointers owned by the caller; calls are asynchronous on
  * `stream` (a hipStream_t, NULL = the library stream) and allocate nothing. */
 int mdp_mlp_sizes(int *nparam, int *npack, int *nin_max, int *nhidden);
+/* Device buffers for the part-4 kernels (copies run on the library stream after its pending kernels
+ * and return when done; mdp_dev_zero is asynchronous on it). */
+int mdp_dev_malloc(void **p, size_t bytes);
+int mdp_dev_free(void *p);
+int mdp_dev_h2d(void *dst, const void *src, size_t bytes);
+int mdp_dev_d2h(void *dst, const void *src, size_t bytes);
+int mdp_dev_zero(void *p, size_t bytes);
+int mdp_dev_sync(void);
 int mdp_mlp_pack(const float *params, unsigned short *packed, void *stream);
 /* e[n] = E_atom; g[k*n + a] = dE_atom/dB_k (g may be NULL) */
 int mdp_mlp_eval(const unsigned short *packed, const float *params, const double *aff, int nin, const double *x,

@@ -31,6 +31,49 @@ hipStream_t S(void* s) { return s ? static_cast<hipStream_t>(s) : library_stream
 
 extern "C" {
 
+// Device buffers for callers without a GPU array library of their own (every copy is ordered with the
+// kernels of the library stream and waits for them).
+int mdp_dev_malloc(void** p, size_t bytes) {
+    return api("mdp_dev_malloc", [&] {
+        if (!p) throw Error("mdp_dev_malloc: null pointer");
+        *p = nullptr;
+        MDP_HIP(hipMalloc(p, bytes ? bytes : 8));
+    });
+}
+
+int mdp_dev_free(void* p) {
+    return api("mdp_dev_free", [&] {
+        if (p) {
+            MDP_HIP(hipStreamSynchronize(library_stream()));
+            MDP_HIP(hipFree(p));
+        }
+    });
+}
+
+int mdp_dev_h2d(void* dst, const void* src, size_t bytes) {
+    return api("mdp_dev_h2d", [&] {
+        hipStream_t st = library_stream();
+        MDP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+        MDP_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int mdp_dev_d2h(void* dst, const void* src, size_t bytes) {
+    return api("mdp_dev_d2h", [&] {
+        hipStream_t st = library_stream();
+        MDP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+        MDP_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int mdp_dev_zero(void* p, size_t bytes) {
+    return api("mdp_dev_zero", [&] { MDP_HIP(hipMemsetAsync(p, 0, bytes, library_stream())); });
+}
+
+int mdp_dev_sync(void) {
+    return api("mdp_dev_sync", [&] { MDP_HIP(hipStreamSynchronize(library_stream())); });
+}
+
 int mdp_mlp_sizes(int* nparam, int* npack, int* nin_max, int* nhidden) {
     return api("mdp_mlp_sizes", [&] {
         if (nparam) *nparam = MLP_NPARAM;

@@ -7,7 +7,8 @@ Everything that touches the data runs on the GPU through libmdpsnap:
   * model: mdp_mlp_eval (BF16 MFMA forward + input gradient), mdp_mlp_train (parameter gradient of
     the energy + force loss, including the second-order term of the force loss), mdp_mlp_adam;
   * descriptor chain rule F_i = -dE/dx_i: mdp_sur_pair_s / mdp_sur_atom_f, and its adjoint mdp_sur_u.
-torch is used only to hold device buffers.  There is no upstream counterpart: the reference's
+Device buffers are DevArray (mdp_dev_*), allocated by the library's own HIP runtime.  There is no
+upstream counterpart: the reference's
 potentials are linear/quadratic SNAP fitted by least squares (src/Julia/Potential/linearfit.jl).
 """
 import ctypes
@@ -17,6 +18,7 @@ import numpy as np
 
 from . import library
 from . import lattice
+from .devarray import DevArray
 
 NPARAM = 5313
 J_SUR = 4
@@ -48,7 +50,7 @@ def _lib():
 
 
 def _p(t):
-    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+    return t.ptr if t is not None else None
 
 
 def _chk(rc, what):
@@ -57,8 +59,7 @@ def _chk(rc, what):
 
 
 def _stream():
-    import torch
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return None   # the library stream
 
 
 def plain_coefficients(seed=20241202 + 2):
@@ -98,47 +99,65 @@ class Model:
         return Model(z["params"], z["aff"], int(z["nin"]))
 
 
-def make_training_set(nconfig=4000, ncell=5, tmin=300.0, tmax=3000.0, every=20, equil=50, per_run=100, seed=777,
-                      log=None):
+def make_training_set(nconfig=4000, ncell=5, tmin=300.0, tmax=3000.0, every=10, equil=10, per_run=10, seed=777,
+                      fmax=200.0, rmin=2.0, log=None):
     """Configurations of 250 atoms (BCC 5^3 cells, a = 3.316 A, and random packings at the BCC density)
-    sampled from plain twojmax=8 SNAP NVE runs started at temperatures spread over [tmin, tmax]; labels
-    are the plain SNAP total energy and forces.  Returns list of dict(x, box, E, F, kind, T)."""
+    sampled from short plain twojmax=8 SNAP NVE runs started at temperatures spread over [tmin, tmax];
+    labels are the plain SNAP total energy and forces.  Short runs keep the samples near their
+    starting structures: with untrained (random) coefficients long hot runs drift into collapsed
+    states.  Samples with a pair closer than ``rmin`` A or a force above ``fmax`` eV/A are dropped and
+    counted.  Returns (list of dict(x, box, E, F, kind, T), number dropped)."""
+    import os
+    from scipy.spatial import cKDTree
     from .md import SnapMD
     coeff = plain_coefficients()
     out = []
-    nruns = (nconfig + per_run - 1) // per_run
-    for run in range(nruns):
-        kind = "bcc" if run % 2 == 0 else "liquid"
-        T = tmin + (tmax - tmin) * (run // 2) / max(1, (nruns - 1) // 2)
-        if kind == "bcc":
-            x, box = lattice.crystal("bcc", ncell, 3.316, 0.05, seed + run)
+    dropped = 0
+    old = os.environ.get("MDP_AUTOTUNE")
+    os.environ["MDP_AUTOTUNE"] = "0"     # small systems: the default launch variants
+    try:
+        nruns_est = max(2, (nconfig + per_run - 1) // per_run)
+        run = 0
+        while len(out) < nconfig:
+            kind = "bcc" if run % 2 == 0 else "liquid"
+            T = tmin + (tmax - tmin) * ((run // 2) % max(1, nruns_est // 2)) / max(1, nruns_est // 2 - 1)
+            if kind == "bcc":
+                x, box = lattice.crystal("bcc", ncell, 3.316, 0.05, seed + run)
+            else:
+                x, box = lattice.random_packing(2 * ncell ** 3, 2 / 3.316 ** 3, 2.2, seed=seed + run)
+            v = lattice.maxwell_boltzmann(len(x), MASS, T, seed=seed + 100000 + run)
+            md = SnapMD(x, box, np.ones(len(x), np.int32), J_PLAIN, coeff, [0.5], [1.0], [MASS], v=v, dt=0.0005,
+                        skin=0.3, rebuild_every=10)
+            md.compute()
+            md.run(equil)
+            for k in range(per_run):
+                if len(out) == nconfig:
+                    break
+                md.run(every)
+                st = md.state()
+                xs = np.mod(st["x"], box)
+                d, _ = cKDTree(xs, boxsize=box).query(xs, k=2)
+                if d[:, 1].min() < rmin or np.abs(st["f"]).max() > fmax:
+                    dropped += 1
+                    continue
+                out.append(dict(x=xs, box=box.copy(), E=float(st["eatom"].sum()), F=st["f"].copy(), kind=kind, T=T))
+            md.close()
+            run += 1
+            if log and run % 50 == 0:
+                log(f"  {run} runs -> {len(out)} configurations ({dropped} dropped)")
+    finally:
+        if old is None:
+            os.environ.pop("MDP_AUTOTUNE", None)
         else:
-            x, box = lattice.random_packing(2 * ncell ** 3, 2 / 3.316 ** 3, 2.2, seed=seed + run)
-        v = lattice.maxwell_boltzmann(len(x), MASS, T, seed=seed + 1000 + run)
-        md = SnapMD(x, box, np.ones(len(x), np.int32), J_PLAIN, coeff, [0.5], [1.0], [MASS], v=v, dt=0.0005,
-                    skin=0.3, rebuild_every=10)
-        md.compute()
-        md.run(equil)
-        for k in range(per_run):
-            if len(out) == nconfig:
-                break
-            md.run(every)
-            st = md.state()
-            out.append(dict(x=np.mod(st["x"], box), box=box.copy(), E=float(st["eatom"].sum()), F=st["f"].copy(),
-                            kind=kind, T=T))
-        md.close()
-        if log:
-            log(f"  run {run}: {kind} at {T:.0f} K -> {len(out)} configurations")
-    return out
+            os.environ["MDP_AUTOTUNE"] = old
+    return out, dropped
 
 
 class DeviceSet:
     """Descriptors, pair chain-rule arrays and labels of a list of configurations, on the GPU."""
 
     def __init__(self, configs, chunk=500, log=None):
-        import torch
         from .fit import SnapFit
-        dev = torch.device("cuda")
         C = len(configs)
         nat = np.array([len(c["x"]) for c in configs])
         a_off = np.concatenate([[0], np.cumsum(nat)])
@@ -173,17 +192,19 @@ class DeviceSet:
         in_off = np.concatenate([[0], np.cumsum(np.bincount(pj, minlength=N))]).astype(np.int32)
         self.C, self.N, self.P = C, N, int(p_tot)
         self.nat = nat
-        self.X = torch.from_numpy(np.ascontiguousarray(X)).to(dev)
-        self.dbdr = torch.from_numpy(np.ascontiguousarray(dbdr)).to(dev)
+        self.X = DevArray.from_numpy(X)
+        self.dbdr = DevArray.from_numpy(dbdr)
         del dbdr, blocks
-        self.pi = torch.from_numpy(pi).to(dev)
-        self.pj = torch.from_numpy(pj).to(dev)
-        self.own_off = torch.from_numpy(own_off).to(dev)
-        self.in_off = torch.from_numpy(in_off).to(dev)
-        self.in_idx = torch.from_numpy(in_idx).to(dev)
-        self.cfg_off = torch.from_numpy(a_off.astype(np.int32)).to(dev)
-        self.Eref = torch.tensor([c["E"] for c in configs], dtype=torch.float64, device=dev)
-        self.Fref = torch.from_numpy(np.ascontiguousarray(np.concatenate([c["F"] for c in configs]))).to(dev)
+        self.pi = DevArray.from_numpy(pi)
+        self.pj = DevArray.from_numpy(pj)
+        self.own_off = DevArray.from_numpy(own_off)
+        self.in_off = DevArray.from_numpy(in_off)
+        self.in_idx = DevArray.from_numpy(in_idx)
+        self.cfg_off = DevArray.from_numpy(a_off.astype(np.int32))
+        self.Eref_h = np.array([c["E"] for c in configs], dtype=np.float64)
+        self.Fref_h = np.ascontiguousarray(np.concatenate([c["F"] for c in configs]), dtype=np.float64)
+        self.Eref = DevArray.from_numpy(self.Eref_h)
+        self.Fref = DevArray.from_numpy(self.Fref_h)
         self.Xh = X   # host copy for statistics
 
 
@@ -191,29 +212,26 @@ class Evaluator:
     """Energies and forces of a model on a DeviceSet, and the training step."""
 
     def __init__(self, ds, model, nwaves=1024):
-        import torch
         self.c = _lib()
         self.ds = ds
         self.model = model
-        dev = torch.device("cuda")
-        f64 = dict(dtype=torch.float64, device=dev)
-        self.params = torch.from_numpy(model.params.copy()).to(dev)
-        self.aff = torch.from_numpy(model.aff.copy()).to(dev)
-        self.packed = torch.zeros(10240, dtype=torch.int16, device=dev)
-        self.e = torch.zeros(ds.N, **f64)
-        self.g = torch.zeros(NIN * ds.N, **f64)
-        self.U = torch.zeros(NIN * ds.N, **f64)
-        self.alpha = torch.zeros(ds.N, **f64)
-        self.s = torch.zeros(3 * max(ds.P, 1), **f64)
-        self.F = torch.zeros(3 * ds.N, **f64)
-        self.R = torch.zeros(3 * ds.N, **f64)
-        self.Ec = torch.zeros(ds.C, **f64)
+        self.params = DevArray.from_numpy(model.params, np.float32)
+        self.aff = DevArray.from_numpy(model.aff, np.float64)
+        self.packed = DevArray(10240, np.int16)
+        self.e = DevArray.zeros(ds.N)
+        self.g = DevArray.zeros(NIN * ds.N)
+        self.U = DevArray.zeros(NIN * ds.N)
+        self.alpha = DevArray.zeros(ds.N)
+        self.s = DevArray.zeros(3 * max(ds.P, 1))
+        self.F = DevArray.zeros(3 * ds.N)
+        self.R = DevArray.zeros(3 * ds.N)
+        self.Ec = DevArray.zeros(ds.C)
         self.nwaves = nwaves
-        self.part = torch.zeros(nwaves * NPARAM, dtype=torch.float32, device=dev)
-        self.grad = torch.zeros(NPARAM, dtype=torch.float32, device=dev)
-        self.m = torch.zeros(NPARAM, dtype=torch.float32, device=dev)
-        self.v = torch.zeros(NPARAM, dtype=torch.float32, device=dev)
-        self.ssq = torch.zeros(256, **f64)
+        self.part = DevArray.zeros(nwaves * NPARAM, np.float32)
+        self.grad = DevArray.zeros(NPARAM, np.float32)
+        self.m = DevArray.zeros(NPARAM, np.float32)
+        self.v = DevArray.zeros(NPARAM, np.float32)
+        self.ssq = DevArray.zeros(256)
         self.t = 0
 
     def forward(self, wE=1.0, need_alpha=False):
@@ -229,12 +247,10 @@ class Evaluator:
 
     def metrics(self):
         """(force RMSE eV/A, energy RMSE eV/atom) of the last forward."""
-        import torch
         c, ds, st = self.c, self.ds, _stream()
         _chk(c.mdp_sumsq(_p(self.R), 3 * ds.N, _p(self.ssq), 256, st), "mdp_sumsq")
-        torch.cuda.synchronize()
-        frmse = float(np.sqrt(self.ssq.cpu().numpy().sum() / (3 * ds.N)))
-        de = (self.Ec - ds.Eref).cpu().numpy() / ds.nat
+        frmse = float(np.sqrt(self.ssq.numpy().sum() / (3 * ds.N)))
+        de = (self.Ec.numpy() - ds.Eref_h) / ds.nat
         return frmse, float(np.sqrt(np.mean(de * de)))
 
     def step(self, lr, wE=1.0, wF=1.0, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
@@ -250,14 +266,14 @@ class Evaluator:
              "mdp_mlp_adam")
 
     def current_model(self):
-        return Model(self.params.cpu().numpy(), self.model.aff, self.model.nin)
+        return Model(self.params.numpy(), self.model.aff, self.model.nin)
 
 
 def affine_from(ds):
     """Descriptor standardization and per-atom energy scale of a training set."""
     mu = ds.Xh.mean(axis=1)
     sg = ds.Xh.std(axis=1)
-    eat = ds.Eref.cpu().numpy() / ds.nat
+    eat = ds.Eref_h / ds.nat
     return mu, sg, max(float(eat.std()), 0.1), float(eat.mean())
 
 
@@ -284,22 +300,20 @@ def evaluate(ds, model):
 def linear_baseline(train_ds, test_ds, wE=1.0, wF=1.0):
     """The method without the network: linear twojmax=4 SNAP (E_atom = b0 + beta . B) fitted by least
     squares on the same energies and forces; returns (force RMSE, energy RMSE per atom) on test_ds."""
-    import torch
-
     def columns(ds):
         c, st = _lib(), _stream()
-        dev = ds.X.device
-        g = torch.zeros(NIN * ds.N, dtype=torch.float64, device=dev)
-        s = torch.zeros(3 * max(ds.P, 1), dtype=torch.float64, device=dev)
-        F = torch.zeros(3 * ds.N, dtype=torch.float64, device=dev)
+        g = DevArray(NIN * ds.N)
+        s = DevArray(3 * max(ds.P, 1))
+        F = DevArray(3 * ds.N)
         cols = []
         for k in range(NIN):
-            g.zero_()
-            g[k * ds.N:(k + 1) * ds.N] = 1.0
+            gh = np.zeros(NIN * ds.N)
+            gh[k * ds.N:(k + 1) * ds.N] = 1.0
+            g.upload(gh)
             _chk(c.mdp_sur_pair_s(_p(g), _p(ds.dbdr), _p(ds.pi), NIN, ds.N, ds.P, _p(s), st), "mdp_sur_pair_s")
             _chk(c.mdp_sur_atom_f(_p(s), _p(ds.own_off), _p(ds.in_off), _p(ds.in_idx), ds.P, ds.N, None, _p(F), None,
                                   st), "mdp_sur_atom_f")
-            cols.append(F.cpu().numpy().copy())
+            cols.append(F.numpy())
         Ff = np.stack(cols, axis=1)                      # (3N, K): force per unit beta_k
         a_off = np.concatenate([[0], np.cumsum(ds.nat)])
         Bs = np.add.reduceat(ds.Xh, a_off[:-1], axis=1).T  # (C, K): sum of B over each configuration
@@ -308,17 +322,17 @@ def linear_baseline(train_ds, test_ds, wE=1.0, wF=1.0):
     Ff, Bs = columns(train_ds)
     nat = train_ds.nat.astype(np.float64)
     Ae = np.concatenate([nat[:, None], Bs], axis=1) / nat[:, None]
-    be = train_ds.Eref.cpu().numpy() / nat
+    be = train_ds.Eref_h / nat
     Af = np.concatenate([np.zeros((Ff.shape[0], 1)), Ff], axis=1)
-    bf = train_ds.Fref.cpu().numpy().ravel()
+    bf = train_ds.Fref_h.ravel()
     A = np.concatenate([np.sqrt(wE / len(be)) * Ae, np.sqrt(wF / len(bf)) * Af])
     b = np.concatenate([np.sqrt(wE / len(be)) * be, np.sqrt(wF / len(bf)) * bf])
     coef, *_ = np.linalg.lstsq(A, b, rcond=None)
     Ff2, Bs2 = columns(test_ds)
     fpred = Ff2 @ coef[1:]
     epred = (test_ds.nat * coef[0] + Bs2 @ coef[1:]) / test_ds.nat
-    fr = float(np.sqrt(np.mean((fpred - test_ds.Fref.cpu().numpy().ravel()) ** 2)))
-    er = float(np.sqrt(np.mean((epred - test_ds.Eref.cpu().numpy() / test_ds.nat) ** 2)))
+    fr = float(np.sqrt(np.mean((fpred - test_ds.Fref_h.ravel()) ** 2)))
+    er = float(np.sqrt(np.mean((epred - test_ds.Eref_h / test_ds.nat) ** 2)))
     return fr, er, coef
 
 

@@ -22,11 +22,6 @@ pytestmark = pytest.mark.gpu
 RCUTFAC, RFAC0 = 4.67637, 0.99363
 
 
-def _torch():
-    import torch
-    return torch, torch.device("cuda")
-
-
 def _rand_model(seed, nin=14):
     rng = np.random.default_rng(seed)
     p = R.init_params(seed, nin)
@@ -41,22 +36,19 @@ def _rand_model(seed, nin=14):
 
 
 def _run_eval(p, aff, nin, X):
-    import ctypes
     from mdp_amd import surrogate as S
-    torch, dev = _torch()
+    from mdp_amd.devarray import DevArray
     c = S._lib()
     n = len(X)
-    tp = torch.from_numpy(p).to(dev)
-    ta = torch.from_numpy(aff).to(dev)
-    tq = torch.zeros(10240, dtype=torch.int16, device=dev)
-    tx = torch.from_numpy(np.ascontiguousarray(X.T)).to(dev)
-    te = torch.zeros(n, dtype=torch.float64, device=dev)
-    tg = torch.zeros(nin * n, dtype=torch.float64, device=dev)
-    st = S._stream()
-    S._chk(c.mdp_mlp_pack(S._p(tp), S._p(tq), st), "pack")
-    S._chk(c.mdp_mlp_eval(S._p(tq), S._p(tp), S._p(ta), nin, S._p(tx), n, S._p(te), S._p(tg), st), "eval")
-    torch.cuda.synchronize()
-    return te.cpu().numpy(), tg.cpu().numpy().reshape(nin, n).T, (c, tp, ta, tq, tx, st)
+    tp = DevArray.from_numpy(p, np.float32)
+    ta = DevArray.from_numpy(aff, np.float64)
+    tq = DevArray(10240, np.int16)
+    tx = DevArray.from_numpy(np.ascontiguousarray(X.T), np.float64)
+    te = DevArray.zeros(n)
+    tg = DevArray.zeros(nin * n)
+    S._chk(c.mdp_mlp_pack(tp.ptr, tq.ptr, None), "pack")
+    S._chk(c.mdp_mlp_eval(tq.ptr, tp.ptr, ta.ptr, nin, tx.ptr, n, te.ptr, tg.ptr, None), "eval")
+    return te.numpy(), tg.numpy().reshape(nin, n).T, (c, tp, ta, tq, tx)
 
 
 @pytest.mark.parametrize("n,nin", [(1003, 14), (16, 5), (4096, 16)])
@@ -74,36 +66,34 @@ def test_mlp_eval_vs_reference(n, nin):
 
 def test_mlp_train_grad_and_adam_vs_reference():
     from mdp_amd import surrogate as S
-    torch, dev = _torch()
+    from mdp_amd.devarray import DevArray
     n, nin, nwaves = 2000, 14, 37
     p, aff, rng = _rand_model(11, nin)
     X = rng.normal(0, 2.0, (n, nin))
     U = rng.normal(0, 1.0, (n, nin))
     alpha = rng.normal(0, 1.0, n)
-    _, _, (c, tp, ta, tq, tx, st) = _run_eval(p, aff, nin, X)
-    tu = torch.from_numpy(np.ascontiguousarray(U.T)).to(dev)
-    tal = torch.from_numpy(alpha).to(dev)
-    part = torch.zeros(nwaves * S.NPARAM, dtype=torch.float32, device=dev)
-    grad = torch.zeros(S.NPARAM, dtype=torch.float32, device=dev)
+    _, _, (c, tp, ta, tq, tx) = _run_eval(p, aff, nin, X)
+    tu = DevArray.from_numpy(np.ascontiguousarray(U.T), np.float64)
+    tal = DevArray.from_numpy(alpha, np.float64)
+    part = DevArray.zeros(nwaves * S.NPARAM, np.float32)
+    grad = DevArray.zeros(S.NPARAM, np.float32)
     print("mlp train grad", flush=True)
-    S._chk(c.mdp_mlp_train(S._p(tq), S._p(tp), S._p(ta), nin, S._p(tx), S._p(tu), S._p(tal), n, S._p(part), nwaves,
-                           S._p(grad), st), "train")
-    torch.cuda.synchronize()
-    g = grad.cpu().numpy().astype(np.float64)
+    S._chk(c.mdp_mlp_train(tq.ptr, tp.ptr, ta.ptr, nin, tx.ptr, tu.ptr, tal.ptr, n, part.ptr, nwaves, grad.ptr, None),
+           "train")
+    g = grad.numpy().astype(np.float64)
     gr = R.train_grad(p, aff, nin, X, U, alpha, quant=True)
     for name, (a, b) in dict(W1=(0, 1024), b1=(1024, 1088), W2=(1088, 5184), b2=(5184, 5248), w3=(5248, 5312),
                              b3=(5312, 5313)).items():
         assert relerr(g[a:b], gr[a:b]) < 2e-2, name
     # Adam on the GPU gradient
-    m = torch.zeros_like(grad)
-    v = torch.zeros_like(grad)
-    p0 = tp.cpu().numpy().astype(np.float64)
+    m = DevArray.zeros(S.NPARAM, np.float32)
+    v = DevArray.zeros(S.NPARAM, np.float32)
+    p0 = tp.numpy().astype(np.float64)
     for t in (1, 2):
-        S._chk(c.mdp_mlp_adam(S._p(tp), S._p(grad), S._p(m), S._p(v), 1e-3, 0.9, 0.999, 1e-8, t, 0.01, st), "adam")
-    torch.cuda.synchronize()
+        S._chk(c.mdp_mlp_adam(tp.ptr, grad.ptr, m.ptr, v.ptr, 1e-3, 0.9, 0.999, 1e-8, t, 0.01, None), "adam")
     pr, mr, vr = R.adam(p0, g, np.zeros_like(g), np.zeros_like(g), 1e-3, 0.9, 0.999, 1e-8, 1, 0.01)
     pr, mr, vr = R.adam(pr, g, mr, vr, 1e-3, 0.9, 0.999, 1e-8, 2, 0.01)
-    assert np.max(np.abs(tp.cpu().numpy() - pr)) < 1e-6
+    assert np.max(np.abs(tp.numpy() - pr)) < 1e-6
 
 
 def _cpu_energy_and_chain_forces(oracle, x, box, p, aff, nin):
@@ -147,18 +137,16 @@ def test_chain_rule_forces_vs_finite_differences(oracle):
     ds = S.DeviceSet(cfg)
     ev = S.Evaluator(ds, S.Model(p, aff, 14))
     ev.forward()
-    import torch
-    torch.cuda.synchronize()
-    Fg = ev.F.cpu().numpy().reshape(-1, 3)
+    Fg = ev.F.numpy().reshape(-1, 3)
     # BF16 activations leave ~1% error on dE/dB; forces sum pair terms that partly cancel
     assert relerr(Fg, Ffd) < 5e-2
-    assert abs(float(ev.Ec.cpu().numpy()[0]) - E0) < 2e-2 * max(1.0, abs(E0))
+    assert abs(float(ev.Ec.numpy()[0]) - E0) < 2e-2 * max(1.0, abs(E0))
 
 
 def test_training_beats_untrained_and_linear():
     from mdp_amd import surrogate as S
     print("generating plain-SNAP training data", flush=True)
-    data = S.make_training_set(nconfig=120, ncell=3, per_run=20, every=10, equil=20, tmin=300, tmax=1500, seed=5)
+    data, _ = S.make_training_set(nconfig=120, ncell=3, per_run=20, every=10, equil=20, tmin=300, tmax=1500, seed=5)
     perm = np.random.default_rng(3).permutation(len(data))
     train, test = [data[i] for i in perm[:96]], [data[i] for i in perm[96:]]
     ds_tr, ds_te = S.DeviceSet(train), S.DeviceSet(test)
