@@ -23,6 +23,14 @@
 
 using namespace mdp;
 
+namespace {
+int n_half_rows_host(int J) {
+    int n = 0;
+    for (int j = 0; j <= J; ++j) n += j / 2 + 1;
+    return n;
+}
+}  // namespace
+
 struct mdp_fit {
     int C = 0, N = 0, J = 0, ntypes = 0;
     int bnorm = 0, bzeroflag = 1;
@@ -31,7 +39,9 @@ struct mdp_fit {
     std::vector<int> a0_h;   // [C+1]
     std::vector<double> boxes_h;
     DevBuf x, type, cfg_of, cfg_a0, cfg_L, cfg_p0, rc2, pcnt, poff, scan_tmp, flags, counter;
-    DevBuf pi, pj, pti, ptj, rij, Ur, Ui, zt, blist, dblist, bzero, t_lo, t_b, t_z, t_w;
+    DevBuf pi, pj, pti, ptj, rij, Ur, Ui, zt, blist, dblist, bzero, t_lo, t_b, t_z, t_w, zorder, t_wtab;
+    int dbdr_variant = 0;
+    bool zb_lds = false;     // Z + B from the LDS-staged half U (twojmax 4, 6, 8)
     long cap = 0;            // pair capacity of the pair and dB/dr buffers
     long npairs = 0;         // pairs of the last compute
     FitTerms terms;
@@ -119,6 +129,25 @@ struct mdp_fit {
         put(t_b, tb.data(), sizeof(int) * tb.size());
         put(t_z, tz.data(), sizeof(int) * tz.size());
         put(t_w, tw.data(), sizeof(double) * tw.size());
+        zb_lds = (J == 4 || J == 6 || J == 8);
+        if (zb_lds) {
+            std::vector<int> order(n_half_rows_host(J));
+            yi_row_order(J, order.data());
+            put(zorder, order.data(), sizeof(int) * order.size());
+        }
+        {   // weights by (bispectrum triple, term) for the unrolled dB/dr variant (cpuSnap.cpp:2101-2240)
+            std::vector<double> wt;
+            for (int j1 = 0; j1 <= J; ++j1)
+                for (int j2 = 0; j2 <= j1; ++j2)
+                    for (int j = j1 - j2; j <= std::min(J, j1 + j2); j += 2) {
+                        if (j < j1) continue;
+                        wt.push_back(2.0);
+                        wt.push_back(bnorm ? 2.0 : 2.0 * (j + 1) / (j1 + 1.0));
+                        wt.push_back(bnorm ? 2.0 : 2.0 * (j + 1) / (j2 + 1.0));
+                    }
+            put(t_wtab, wt.data(), sizeof(double) * wt.size());
+        }
+        terms.wtab = t_wtab.as<double>();
         terms.lo = t_lo.as<int>();
         terms.b = t_b.as<int>();
         terms.z = t_z.as<int>();
@@ -191,18 +220,23 @@ struct mdp_fit {
                 launch_fit_guard(flags.as<int>(), poff.as<int>(), N, st);
                 break;
             case 1:
-                launch_ui(s, UI_FULL_WSELF, Ur.as<double>(), Ui.as<double>(), rij.as<double>(), poff.as<int>(), nullptr,
-                          pti.as<int>(), ptj.as<int>(), N, 1.0, -1, st);
+                launch_ui(s, zb_lds ? UI_HALF_WSELF : UI_FULL_WSELF, Ur.as<double>(), Ui.as<double>(), rij.as<double>(),
+                          poff.as<int>(), nullptr, pti.as<int>(), ptj.as<int>(), N, 1.0, -1, st);
                 break;
             case 2:
-                launch_fit_zb(s, Ur.as<double>(), Ui.as<double>(), zt.as<double>(), blist.as<double>(),
-                              bzero.as<double>(), cfg_of.as<int>(), cfg_a0.as<int>(), N, bnorm, bzeroflag, st);
+                if (zb_lds)
+                    launch_fit_zb_lds(s, Ur.as<double>(), Ui.as<double>(), zt.as<double>(), blist.as<double>(),
+                                      bzero.as<double>(), cfg_of.as<int>(), cfg_a0.as<int>(), zorder.as<int>(), N, bnorm,
+                                      bzeroflag, st);
+                else
+                    launch_fit_zb(s, Ur.as<double>(), Ui.as<double>(), zt.as<double>(), blist.as<double>(),
+                                  bzero.as<double>(), cfg_of.as<int>(), cfg_a0.as<int>(), N, bnorm, bzeroflag, st);
                 break;
             default:
                 launch_fill_zero_int(counter.as<int>(), 1, st);
                 launch_fit_dbdr(s, zt.as<double>(), terms, rij.as<double>(), pti.as<int>(), ptj.as<int>(),
                                 poff.as<int>(), cfg_of.as<int>(), cfg_p0.as<int>(), dblist.as<double>(),
-                                counter.as<int>(), N, dbdr_groups, st);
+                                counter.as<int>(), N, dbdr_groups, st, dbdr_variant);
                 break;
         }
     }
@@ -245,25 +279,40 @@ struct mdp_fit {
     void autotune() {
         const char* env = std::getenv("MDP_AUTOTUNE");
         tuned = true;
-        if (env && std::string(env) == "0") return;
+        const char* fv = std::getenv("MDP_FIT_DBDR_VARIANT");   // pin the dB/dr variant (tests)
+        if (fv) {
+            const std::string v(fv);
+            dbdr_variant = (v == "1" && (J == 4 || J == 6 || J == 8)) ? 1 : ((v == "2" && (J == 6 || J == 8)) ? 2 : 0);
+        }
+        if ((env && std::string(env) == "0") || fv) return;
         int props_cu = 256;
         hipDeviceProp_t prop;
         int dev = 0;
         if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
             props_cu = prop.multiProcessorCount;
         const int cand[] = {props_cu * 2, props_cu * 4, props_cu * 5, props_cu * 8};
+        const int nvar = (J == 6 || J == 8) ? 3 : ((J == 4) ? 2 : 1);
         double best = 1e30;
-        int pick = dbdr_groups;
-        std::string js = "{\"fit_dbdr_groups\": {";
-        for (int k = 0; k < 4; ++k) {
-            dbdr_groups = cand[k];
-            stage(3);   // warm-up
-            const double t = time_stage(3, 2);
-            js += (k ? ", " : "") + std::string("\"") + std::to_string(cand[k]) + "\": " + std::to_string(t);
-            if (t < best) { best = t; pick = cand[k]; }
-        }
+        int pick = dbdr_groups, pickv = 0;
+        std::string js = "{\"fit_dbdr\": {";
+        for (int v = 0; v < nvar; ++v)
+            for (int k = 0; k < 4; ++k) {
+                dbdr_variant = v;
+                dbdr_groups = cand[k];
+                stage(3);   // warm-up
+                const double t = time_stage(3, 2);
+                js += (v || k ? ", " : "") + std::string("\"") + std::to_string(v) + "/" + std::to_string(cand[k]) +
+                      "\": " + std::to_string(t);
+                if (t < best) {
+                    best = t;
+                    pick = cand[k];
+                    pickv = v;
+                }
+            }
         dbdr_groups = pick;
-        js += "}, \"chosen\": {\"fit_dbdr_groups\": " + std::to_string(pick) + "}}";
+        dbdr_variant = pickv;
+        js += "}, \"chosen\": {\"fit_dbdr_variant\": " + std::to_string(pickv) + ", \"fit_dbdr_groups\": " +
+              std::to_string(pick) + "}}";
         tuning_json = js;
     }
 

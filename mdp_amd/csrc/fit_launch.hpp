@@ -28,20 +28,29 @@ void launch_fill_zero_int(int* p, int n, hipStream_t st);
 void launch_fit_zb(const SnapDev& s, const double* Ur, const double* Ui, double* zt, double* blist,
                    const double* bzero, const int* cfg_of, const int* cfg_a0, int N, int bnorm, int bzeroflag,
                    hipStream_t st);
+// Z + B with the half U (rows 2mb <= j, [e][N] layout) of 64 atoms staged in LDS per workgroup; row_order
+// = the half rows (j, mb) by decreasing work (yi_row_order).  Compiled for twojmax 4, 6, 8 (returns
+// false otherwise); same outputs as launch_fit_zb.
+bool launch_fit_zb_lds(const SnapDev& s, const double* Uhr, const double* Uhi, double* zt, double* blist,
+                       const double* bzero, const int* cfg_of, const int* cfg_a0, const int* row_order, int N,
+                       int bnorm, int bzeroflag, hipStream_t st);
 // Device copy of the dB/dr term table (see fit_term_table)
 struct FitTerms {
     const int* lo = nullptr;      // [J+2]
     const int* b = nullptr;       // bispectrum index of each term
     const int* z = nullptr;       // Z block offset (idxz ordering)
     const double* w = nullptr;    // weight
+    const double* wtab = nullptr; // weights by (bispectrum index, term): wtab[3*t + term]
 };
 // Host term table of the dB/dr contraction, grouped by the layer whose dU it contracts.
 void fit_term_table(int J, int bnorm, std::vector<int>& lo, std::vector<int>& b, std::vector<int>& z,
                     std::vector<double>& w);
 // dB/dr of every pair (per-configuration layout dblist[p0_c*3*idxb_max + lp + P_c*k + 3*P_c*jjb]) by a
-// persistent kernel of `groups` single-wave workgroups; counter must be 0 at launch.
+// persistent kernel of `groups` single-wave workgroups; counter must be 0 at launch.  variant 0 walks the
+// layer-grouped term table; variant 1 (twojmax 4, 6, 8) has every term unrolled at compile time;
+// variant 2 (twojmax 6, 8) is variant 0 limited to 256 registers (two waves per SIMD).
 void launch_fit_dbdr(const SnapDev& s, const double* zt, const FitTerms& tt, const double* rij, const int* pti,
                      const int* ptj, const int* poff, const int* cfg_of, const int* cfg_p0, double* dblist,
-                     int* counter, int N, int groups, hipStream_t st);
+                     int* counter, int N, int groups, hipStream_t st, int variant = 0);
 
 }  // namespace mdp

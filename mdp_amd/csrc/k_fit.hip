@@ -60,6 +60,15 @@ __host__ __device__ constexpr int b_index(int J, int j1, int j2, int j) {
         if (b_comp(J, k, 0) == j1 && b_comp(J, k, 1) == j2 && b_comp(J, k, 2) == j) return k;
     return -1;
 }
+// The three terms of dB_t/dr (cpuSnap.cpp:2101-2240): term 0 contracts dU of layer j with Z(j1,j2,j),
+// term 1 dU of layer j1 with Z(j,j2,j1), term 2 dU of layer j2 with Z(j,j1,j2).
+__host__ __device__ constexpr int term_layer(int J, int t, int w) { return b_comp(J, t, w == 0 ? 2 : (w == 1 ? 0 : 1)); }
+__host__ __device__ constexpr int term_zblock(int J, int t, int w) {
+    return w == 0 ? z_block(J, b_comp(J, t, 0), b_comp(J, t, 1), b_comp(J, t, 2))
+                  : (w == 1 ? z_block(J, b_comp(J, t, 2), b_comp(J, t, 1), b_comp(J, t, 0))
+                            : z_block(J, b_comp(J, t, 2), b_comp(J, t, 0), b_comp(J, t, 1)));
+}
+
 // ---- batched pair build ------------------------------------------------------------------------
 
 __device__ __forceinline__ double image_delta(double xj, int s, double L, double xi) {
@@ -240,8 +249,8 @@ __global__ __launch_bounds__(64) void k_fit_zb(const double* __restrict__ cg, co
 // weight), walked by a uniform loop; the atom is wave-uniform, so Z values, table entries and
 // weights are scalar loads shared by all 63 lanes.  Each lane's 55 accumulators live in LDS.
 // dblist is written in the per-configuration layout dblist[c][lp + P_c*k + 3 P_c*jjb].
-template <int J>
-__global__ __launch_bounds__(64) void k_fit_dbdr(const double* __restrict__ rootpq, const double* __restrict__ zt,
+template <int J, int WPE = 1>
+__global__ __launch_bounds__(64, WPE) void k_fit_dbdr(const double* __restrict__ rootpq, const double* __restrict__ zt,
                                                  const int* __restrict__ term_lo, const int* __restrict__ term_b,
                                                  const int* __restrict__ term_z, const double* __restrict__ term_w,
                                                  const double* __restrict__ radelem, const double* __restrict__ wjelem,
@@ -303,6 +312,91 @@ __global__ __launch_bounds__(64) void k_fit_dbdr(const double* __restrict__ root
                     for (int ma = 0; ma <= LAST; ++ma) s += dr[ma] * zr[2 * ma] + di[ma] * zr[2 * ma + 1];
                     acc[term_b[k] * 64 + lane] += term_w[k] * s;
                 }
+            });
+            if (active) {
+                const size_t lp = static_cast<size_t>(q - p0);
+                for (int k = 0; k < IB; ++k) db[lp + Pc * comp + 3 * Pc * k] = acc[k * 64 + lane];
+            }
+        }
+    }
+}
+
+// As k_fit_dbdr with the terms of every row enumerated at compile time: each term's Z row is a
+// contiguous run of the atom's Z block at a constant offset, so it arrives in wide scalar loads, and
+// the accumulator of each term sits at a constant LDS offset.  wtab[3*t + w] = weight of term w of t.
+template <int J>
+__global__ __launch_bounds__(64) void k_fit_dbdr_u(const double* __restrict__ rootpq, const double* __restrict__ zt,
+                                                   const double* __restrict__ wtab, const double* __restrict__ radelem,
+                                                   const double* __restrict__ wjelem, double rcutfac, double rmin0,
+                                                   double rfac0, int switchflag, const double* __restrict__ rij,
+                                                   const int* __restrict__ pti, const int* __restrict__ ptj,
+                                                   const int* __restrict__ poff, const int* __restrict__ cfg_of,
+                                                   const int* __restrict__ cfg_p0, double* __restrict__ dblist,
+                                                   int* __restrict__ counter, int N) {
+    constexpr int IZ = n_z(J), IB = n_b(J);
+    __shared__ double acc[IB * 64];
+    const int lane = threadIdx.x;
+    const int comp = lane % 3, slot = lane / 3;
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(counter, 1);
+        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+        if (t >= N) break;
+        const int atom = t;
+        const int q0 = __builtin_amdgcn_readfirstlane(poff[atom]);
+        const int q1 = __builtin_amdgcn_readfirstlane(poff[atom + 1]);
+        const int c = __builtin_amdgcn_readfirstlane(cfg_of[atom]);
+        const int p0 = __builtin_amdgcn_readfirstlane(cfg_p0[c]);
+        const long Pc = __builtin_amdgcn_readfirstlane(cfg_p0[c + 1]) - p0;
+        const double* zbase = zt + static_cast<size_t>(atom) * IZ * 2;
+        double* __restrict__ db = dblist + static_cast<size_t>(p0) * 3 * IB;
+        for (int qb = q0; qb < q1; qb += 21) {
+            // an opaque copy of the Z pointer per pass keeps its thousands of loads from being hoisted
+            // out of the pass loop (where they would all stay live at once)
+            const double* zp = zbase;
+            asm volatile("" : "+s"(zp));
+            const double* __restrict__ z = zp;
+            const int q = qb + slot;
+            const bool active = lane < 63 && q < q1;
+            const int p = active ? q : q0;
+            const int tjp = ptj[p];
+            const double rc = (radelem[pti[p]] + radelem[tjp]) * rcutfac;
+            PairGeom g;
+            pair_geometry<true>(rij[3 * p], rij[3 * p + 1], rij[3 * p + 2], rc, rmin0, rfac0, switchflag, wjelem[tjp], g);
+            const double uk = comp == 0 ? g.u[0] : (comp == 1 ? g.u[1] : g.u[2]);
+            g.dar[0] = comp == 0 ? g.dar[0] : (comp == 1 ? g.dar[1] : g.dar[2]);
+            g.dai[0] = comp == 0 ? g.dai[0] : (comp == 1 ? g.dai[1] : g.dai[2]);
+            g.dbr[0] = comp == 0 ? g.dbr[0] : (comp == 1 ? g.dbr[1] : g.dbr[2]);
+            g.dbi[0] = comp == 0 ? g.dbi[0] : (comp == 1 ? g.dbi[1] : g.dbi[2]);
+            const double dsu = g.dsfac * uk, sf = g.sfac;
+            for (int k = 0; k < IB; ++k) acc[k * 64 + lane] = 0.0;
+            row_chain<J, 1, 0, J / 2>(g, rootpq, [&](auto Jc, auto MBc, auto& pr, auto& pim, auto& dpr, auto& dpi) {
+                constexpr int L = decltype(Jc)::value, MB = decltype(MBc)::value;
+                constexpr int LAST = (L == 2 * MB) ? MB : L;
+                double dr[LAST + 1], di[LAST + 1];
+#pragma unroll
+                for (int ma = 0; ma <= LAST; ++ma) {
+                    dr[ma] = dsu * pr[ma] + sf * dpr[0][ma];
+                    di[ma] = dsu * pim[ma] + sf * dpi[0][ma];
+                }
+                if constexpr (L == 2 * MB) {
+                    dr[MB] *= 0.5;
+                    di[MB] *= 0.5;
+                }
+                static_for<0, IB>([&](auto Tc) {
+                    constexpr int T = decltype(Tc)::value;
+                    static_for<0, 3>([&](auto Wc) {
+                        constexpr int WT = decltype(Wc)::value;
+                        if constexpr (term_layer(J, T, WT) == L) {
+                            constexpr int ZO = 2 * (term_zblock(J, T, WT) + (L + 1) * MB);
+                            double sacc = 0.0;
+#pragma unroll
+                            for (int ma = 0; ma <= LAST; ++ma) sacc += dr[ma] * z[ZO + 2 * ma] + di[ma] * z[ZO + 2 * ma + 1];
+                            acc[T * 64 + lane] += wtab[3 * T + WT] * sacc;
+                        }
+                    });
+                    __builtin_amdgcn_sched_barrier(0);
+                });
             });
             if (active) {
                 const size_t lp = static_cast<size_t>(q - p0);
@@ -400,9 +494,38 @@ void launch_fit_zb(const SnapDev& s, const double* Ur, const double* Ui, double*
 
 void launch_fit_dbdr(const SnapDev& s, const double* zt, const FitTerms& tt, const double* rij, const int* pti,
                      const int* ptj, const int* poff, const int* cfg_of, const int* cfg_p0, double* dblist,
-                     int* counter, int N, int groups, hipStream_t st) {
+                     int* counter, int N, int groups, hipStream_t st, int variant) {
     if (N <= 0) return;
     const int G = groups > 0 ? groups : 1280;
+    if (variant == 2) {   // the table-walking kernel limited to 256 registers (two waves per SIMD)
+        switch (s.J) {
+#define D2_CASE(JV)                                                                                             \
+    case JV:                                                                                                    \
+        hipLaunchKernelGGL((k_fit_dbdr<JV, 2>), dim3(G), dim3(64), 0, st, s.rootpq, zt, tt.lo, tt.b, tt.z, tt.w,   \
+                           s.radelem, s.wjelem, s.rcutfac, s.rmin0, s.rfac0, s.switchflag, rij, pti, ptj, poff,    \
+                           cfg_of, cfg_p0, dblist, counter, N);                                                  \
+        return;
+            D2_CASE(6)
+            D2_CASE(8)
+#undef D2_CASE
+            default: break;
+        }
+    }
+    if (variant == 1 && tt.wtab) {
+        switch (s.J) {
+#define DU_CASE(JV)                                                                                             \
+    case JV:                                                                                                    \
+        hipLaunchKernelGGL((k_fit_dbdr_u<JV>), dim3(G), dim3(64), 0, st, s.rootpq, zt, tt.wtab, s.radelem, s.wjelem, \
+                           s.rcutfac, s.rmin0, s.rfac0, s.switchflag, rij, pti, ptj, poff, cfg_of, cfg_p0, dblist,  \
+                           counter, N);                                                                        \
+        return;
+            DU_CASE(4)
+            DU_CASE(6)
+            DU_CASE(8)
+#undef DU_CASE
+            default: break;
+        }
+    }
 #define DB_CASE(JV)                                                                                            \
     case JV:                                                                                                   \
         hipLaunchKernelGGL((k_fit_dbdr<JV>), dim3(G), dim3(64), 0, st, s.rootpq, zt, tt.lo, tt.b, tt.z, tt.w,    \

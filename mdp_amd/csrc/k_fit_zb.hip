@@ -1,0 +1,185 @@
+// Z + B of the batched fitting path with the half-domain U of 64 atoms staged in LDS (the structure of
+// the MD path's compute_yi, k_yih.hpp): each workgroup copies its atoms' half U (158,720 B at
+// twojmax = 8) from HBM once, then its waves take whole coupling triples, heaviest first, from a shared
+// counter.  A triple's rows mb <= j/2 are produced with every U row read from LDS (mirrored rows through
+// the symmetry of U), written atom-major to zt, and for bispectrum triples (j >= j1) contracted with
+// U(j) into B (cpuSnap.cpp:1899-2072 semantics, as k_fit_zb).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "fit_launch.hpp"
+#include "k_yih.hpp"
+
+namespace mdp {
+
+namespace {
+
+__host__ __device__ constexpr int zb_z_block(int J, int j1, int j2, int j) {
+    int n = 0;
+    for (int a = 0; a <= J; ++a)
+        for (int b = 0; b <= a; ++b)
+            for (int c = a - b; c <= cx_min(J, a + b); c += 2) {
+                if (a == j1 && b == j2 && c == j) return n;
+                n += (c / 2 + 1) * (c + 1);
+            }
+    return -1;
+}
+__host__ __device__ constexpr int zb_n_z(int J) {
+    int n = 0;
+    for (int a = 0; a <= J; ++a)
+        for (int b = 0; b <= a; ++b)
+            for (int c = a - b; c <= cx_min(J, a + b); c += 2) n += (c / 2 + 1) * (c + 1);
+    return n;
+}
+__host__ __device__ constexpr int zb_b_index(int J, int j1, int j2, int j) {
+    int n = 0;
+    for (int a = 0; a <= J; ++a)
+        for (int b = 0; b <= a; ++b)
+            for (int c = a - b; c <= cx_min(J, a + b); c += 2) {
+                if (c < a) continue;
+                if (a == j1 && b == j2 && c == j) return n;
+                ++n;
+            }
+    return -1;
+}
+__host__ __device__ constexpr int zb_n_b(int J) {
+    int n = 0;
+    for (int a = 0; a <= J; ++a)
+        for (int b = 0; b <= a; ++b)
+            for (int c = a - b; c <= cx_min(J, a + b); c += 2) n += (c >= a);
+    return n;
+}
+
+}  // namespace
+
+// Z rows (JO, mb) of every coupling (J1, J2) -> JO for this lane's atom, from the LDS half U, stored
+// atom-major.  The middle row of an even layer stops at ma = mb: the dB/dr contraction and B read no
+// entry beyond it.
+template <int J, int JO>
+__device__ __forceinline__ void z_rows_lds(int mb, int atom, bool live, const double* __restrict__ lr,
+                                           const double* __restrict__ li, int lane, const double* __restrict__ cg,
+                                           double scale, double* __restrict__ zt) {
+    constexpr int IZ = zb_n_z(J);
+    const int malim = (2 * mb == JO) ? mb : JO;
+    static_for<0, J + 1>([&](auto J1c) {
+        constexpr int J1 = decltype(J1c)::value;
+        static_for<0, J1 + 1>([&](auto J2c) {
+            constexpr int J2 = decltype(J2c)::value;
+            if constexpr (coupled(J1, J2, JO)) {
+                constexpr int ZB = zb_z_block(J, J1, J2, JO);
+                double zr[JO + 1], zi[JO + 1];
+#pragma unroll
+                for (int m = 0; m <= JO; ++m) { zr[m] = 0.0; zi[m] = 0.0; }
+                z_row_lds<J, J1, J2, JO>(mb, malim, lr, li, lane, cg, scale, zr, zi);
+                if (live) {
+                    double* __restrict__ zo = zt + (static_cast<size_t>(atom) * IZ + ZB + (JO + 1) * mb) * 2;
+#pragma unroll
+                    for (int ma = 0; ma <= JO; ++ma)
+                        if (ma <= malim) {
+                            zo[2 * ma] = zr[ma];
+                            zo[2 * ma + 1] = zi[ma];
+                        }
+                }
+                // couplings are independent: keep the scheduler from interleaving them (register pressure)
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        });
+    });
+}
+
+template <int J, int W>
+__global__ __launch_bounds__(64 * W) void k_fit_z_lds(const double* __restrict__ cg, const double* __restrict__ Ur,
+                                                      const double* __restrict__ Ui, double* __restrict__ zt,
+                                                      const int* __restrict__ order, int N, int bnorm) {
+    constexpr int HE = n_half_elems(J), NR = n_half_rows(J);
+    __shared__ double lr[HE * 64], li[HE * 64];
+    __shared__ int next_row;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int atom = blockIdx.x * 64 + lane;
+    const bool live = atom < N;
+    const size_t n = static_cast<size_t>(N);
+    if (threadIdx.x == 0) next_row = 0;
+    for (int e = wave; e < HE; e += W) {
+        lr[e * 64 + lane] = live ? Ur[static_cast<size_t>(e) * n + atom] : 0.0;
+        li[e * 64 + lane] = live ? Ui[static_cast<size_t>(e) * n + atom] : 0.0;
+    }
+    __syncthreads();
+    for (;;) {
+        int r = 0;
+        if (lane == 0) r = atomicAdd(&next_row, 1);
+        r = __builtin_amdgcn_readfirstlane(__shfl(r, 0));
+        if (r >= NR) break;
+        const int row = __builtin_amdgcn_readfirstlane(order[r]);
+        const int jo = half_row_j(row), mb = half_row_mb(row);
+        static_for<0, J + 1>([&](auto JOc) {
+            constexpr int JO = decltype(JOc)::value;
+            if (jo == JO) z_rows_lds<J, JO>(mb, atom, live, lr, li, lane, cg, bnorm ? 1.0 / (JO + 1) : 1.0, zt);
+        });
+    }
+}
+
+// B = 2 sum' U*(j).Z(j1,j2,j) - bzero[j] from the half U ([e][N]) and the atom-major Z; lane = atom
+template <int J>
+__global__ __launch_bounds__(256) void k_fit_b(const double* __restrict__ Ur, const double* __restrict__ Ui,
+                                               const double* __restrict__ zt, double* __restrict__ blist,
+                                               const double* __restrict__ bzero, const int* __restrict__ cfg_of,
+                                               const int* __restrict__ cfg_a0, int N, int bzeroflag) {
+    constexpr int IZ = zb_n_z(J), IB = zb_n_b(J);
+    const int atom = blockIdx.x * blockDim.x + threadIdx.x;
+    if (atom >= N) return;
+    const size_t n = static_cast<size_t>(N);
+    const int c = cfg_of[atom];
+    const int a0 = cfg_a0[c], nc = cfg_a0[c + 1] - a0;
+    const double* __restrict__ z = zt + static_cast<size_t>(atom) * IZ * 2;
+    double* __restrict__ bo = blist + static_cast<size_t>(a0) * IB + (atom - a0);
+    static_for<0, J + 1>([&](auto J1c) {
+        constexpr int J1 = decltype(J1c)::value;
+        static_for<0, J1 + 1>([&](auto J2c) {
+            constexpr int J2 = decltype(J2c)::value;
+            static_for<J1, J + 1>([&](auto JOc) {
+                constexpr int JO = decltype(JOc)::value;
+                if constexpr (coupled(J1, J2, JO)) {
+                    constexpr int ZB = zb_z_block(J, J1, J2, JO), jjb = zb_b_index(J, J1, J2, JO);
+                    double sum = 0.0;
+#pragma unroll
+                    for (int mb = 0; 2 * mb <= JO; ++mb) {
+                        const int malim = (2 * mb == JO) ? mb : JO;
+#pragma unroll
+                        for (int ma = 0; ma <= malim; ++ma) {
+                            const double w = (2 * mb < JO || ma < mb) ? 1.0 : 0.5;
+                            const size_t u = static_cast<size_t>(h_block(JO) + (JO + 1) * mb + ma) * n + atom;
+                            const int zo = 2 * (ZB + (JO + 1) * mb + ma);
+                            sum += w * (Ur[u] * z[zo] + Ui[u] * z[zo + 1]);
+                        }
+                    }
+                    double b = 2.0 * sum;
+                    if (bzeroflag) b -= bzero[JO];
+                    bo[static_cast<size_t>(nc) * jjb] = b;
+                }
+            });
+        });
+    });
+}
+
+bool launch_fit_zb_lds(const SnapDev& s, const double* Uhr, const double* Uhi, double* zt, double* blist,
+                       const double* bzero, const int* cfg_of, const int* cfg_a0, const int* row_order, int N,
+                       int bnorm, int bzeroflag, hipStream_t st) {
+    if (N <= 0) return true;
+    const dim3 grid((N + 63) / 64), block(64 * 8), gb((N + 255) / 256), bb(256);
+    switch (s.J) {
+#define ZB_CASE(JV)                                                                                            \
+    case JV:                                                                                                   \
+        hipLaunchKernelGGL((k_fit_z_lds<JV, 8>), grid, block, 0, st, s.cg, Uhr, Uhi, zt, row_order, N, bnorm);    \
+        hipLaunchKernelGGL((k_fit_b<JV>), gb, bb, 0, st, Uhr, Uhi, zt, blist, bzero, cfg_of, cfg_a0, N, bzeroflag); \
+        return true;
+        ZB_CASE(4)
+        ZB_CASE(6)
+        ZB_CASE(8)
+#undef ZB_CASE
+        default: return false;
+    }
+}
+
+}  // namespace mdp

@@ -82,11 +82,12 @@ def _check_config(oracle, fit, c, cfgs, b, db, ai, aj, rij, J, bnorm=0, bzero=1)
         assert relerr(dbc, odb) < TOL, c
 
 
-@pytest.mark.parametrize("J", [4, 6, 8])
-def test_small_batch_vs_oracle(oracle, J):
+@pytest.mark.parametrize("J,variant", [(4, "0"), (6, "0"), (8, "0"), (4, "1"), (6, "1"), (8, "1"), (8, "2")])
+def test_small_batch_vs_oracle(oracle, monkeypatch, J, variant):
     from mdp_amd.fit import SnapFit
+    monkeypatch.setenv("MDP_FIT_DBDR_VARIANT", variant)   # each dB/dr kernel variant in turn
     cfg = _small_batch()
-    print(f"small batch J={J}: {[len(c[0]) for c in cfg]} atoms", flush=True)
+    print(f"small batch J={J} dB/dr variant {variant}: {[len(c[0]) for c in cfg]} atoms", flush=True)
     fit = SnapFit(cfg, J)
     fit.compute()
     b, db = fit.raw()
