@@ -59,15 +59,17 @@ __global__ __launch_bounds__(64) void k_dedr(SnapDev s, const double* __restrict
 // Same contraction with lane = (pair, Cartesian component): a wave holds 21 pairs x 3 components
 // (lane 63 idle).  Each lane carries U and one derivative recursion, about half the register state of
 // k_dedr, which lets several waves share a SIMD; U is recomputed by the three lanes of a pair.
-template <int J, bool HALF, int WPE>
-__global__ __launch_bounds__(64, WPE) void k_dedr_split(SnapDev s, const double* __restrict__ Yr,
+// NW waves per workgroup take NW consecutive 21-pair tiles: the pairs of a few neighbouring centre
+// atoms then share one CU, so their Y rows are reused from that CU's L1.
+template <int J, bool HALF, int WPE, int NW = 1>
+__global__ __launch_bounds__(64 * NW, WPE) void k_dedr_split(SnapDev s, const double* __restrict__ Yr,
                                                    const double* __restrict__ Yi, const double* __restrict__ rij,
                                                    const int* __restrict__ yatom, const int* __restrict__ ti,
                                                    const int* __restrict__ tj, const long* __restrict__ ybase,
                                                    double* __restrict__ dedr, int N, int npairs_host,
                                                    const int* __restrict__ npairs_dev) {
-    const int lane = threadIdx.x;
-    const int p = blockIdx.x * 21 + lane / 3, comp = lane % 3;
+    const int lane = threadIdx.x & 63;
+    const int p = (blockIdx.x * NW + static_cast<int>(threadIdx.x >> 6)) * 21 + lane / 3, comp = lane % 3;
     const int np = npairs_dev ? *npairs_dev : npairs_host;
     if (lane >= 63 || p >= np) return;
     const int tjp = tj[p];
@@ -196,7 +198,8 @@ void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const dou
                  const int* ti, const int* tj, const long* ybase, double* dedr, int N, int npairs_host,
                  const int* npairs_dev, hipStream_t st, bool half, int variant) {
     if (npairs_host <= 0) return;
-    const dim3 g3((npairs_host + 63) / 64), g1((npairs_host + 20) / 21);
+    const dim3 g3((npairs_host + 63) / 64), g1((npairs_host + 20) / 21), g2((npairs_host + 41) / 42),
+        g4((npairs_host + 83) / 84);
 #define DE_ARGS Yr, Yi, rij, yatom, ti, tj, ybase, dedr, N, npairs_host, npairs_dev
 #define DE_CASE(JV)                                                                                            \
     case JV:                                                                                                   \
@@ -208,6 +211,10 @@ void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const dou
             hipLaunchKernelGGL((k_dedr_split<JV, true, 2>), g1, dim3(64), 0, st, s, DE_ARGS);                  \
         else if (variant == 3)                                                                                 \
             hipLaunchKernelGGL((k_dedr_split<JV, true, 3>), g1, dim3(64), 0, st, s, DE_ARGS);                  \
+        else if (variant == 6)                                                                                 \
+            hipLaunchKernelGGL((k_dedr_split<JV, true, 2, 2>), g2, dim3(128), 0, st, s, DE_ARGS);              \
+        else if (variant == 7)                                                                                 \
+            hipLaunchKernelGGL((k_dedr_split<JV, true, 2, 4>), g4, dim3(256), 0, st, s, DE_ARGS);              \
         else                                                                                                   \
             hipLaunchKernelGGL((k_dedr_split<JV, true, 4>), g1, dim3(64), 0, st, s, DE_ARGS);                  \
         break;
@@ -223,9 +230,9 @@ void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const dou
 }
 
 int dedr_variants(int* ids) {
-    static const int v[] = {0, 2, 3, 4, 5};
-    for (int k = 0; k < 5; ++k) ids[k] = v[k];
-    return 5;   // 5 is launch_dedr_atom
+    static const int v[] = {0, 2, 3, 4, 5, 6, 7};
+    for (int k = 0; k < 7; ++k) ids[k] = v[k];
+    return 7;   // 5 is launch_dedr_atom; 6 and 7 are variant 2 with 2 and 4 waves per workgroup
 }
 
 }  // namespace mdp

@@ -898,6 +898,16 @@ struct mdp_md {
         std::string js = "{";
         int ids[8];
         int nv = dedr_variants(ids);
+        // MDP_DEDR_VARIANT pins the dE/dr kernel variant (one of dedr_variants) instead of timing them all
+        const char* pin = std::getenv("MDP_DEDR_VARIANT");
+        if (pin) {
+            const int v = std::atoi(pin);
+            bool known = false;
+            for (int k = 0; k < nv; ++k) known |= (ids[k] == v);
+            if (!known) throw Error("MDP_DEDR_VARIANT is not a dE/dr kernel variant");
+            ids[0] = v;
+            nv = 1;
+        }
         double best = 1e30;
         js += "\"compute_dedr\": {";
         for (int k = 0; k < nv; ++k) {

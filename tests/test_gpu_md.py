@@ -78,6 +78,24 @@ def test_first_step_vs_oracle(oracle, J, ncell, seed):
     assert relerr(st["vatom"], v) < TOL
 
 
+@pytest.mark.parametrize("variant", ["0", "2", "3", "4", "5", "6", "7"])
+def test_every_dedr_variant_vs_oracle(oracle, monkeypatch, variant):
+    """Each dE/dr kernel variant the autotuner may pick, pinned in turn (twojmax = 8, 3,456 atoms)."""
+    monkeypatch.setenv("MDP_DEDR_VARIANT", variant)
+    x, box = lattice.crystal("bcc", 12, 3.316, 0.1, 31)
+    coeff = _coeff(oracle, 8, 31)
+    md = _engine(x, box, 8, coeff)
+    md.compute()
+    st = md.state()
+    s = snapcpp.initsna(oracle, [1, 8, RCUTFAC, RFAC0, 0, 1, 1, 0, 0, 0, 0], [0.5], [1.0], coeff)
+    B = np.diag(box)
+    p = snapcpp.pairs_from_box(x, np.ones(len(x), np.int32), B[0], B[1], B[2], [1, 1, 1], s)
+    e, f, v = snapcpp.snappotential_pairs(oracle, s, p)
+    assert relerr(st["eatom"], e) < TOL
+    assert relerr(st["f"], f) < TOL
+    assert relerr(st["vatom"], v) < TOL
+
+
 def test_first_step_128k_twojmax8_sampled(oracle):
     """The headline system: BCC Ta 40^3 cells (128,000 atoms), sigma = 0.1 A, twojmax = 8."""
     import sampled_oracle
