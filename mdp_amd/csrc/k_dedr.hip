@@ -105,6 +105,78 @@ __global__ __launch_bounds__(64 * NW, WPE) void k_dedr_split(SnapDev s, const do
     dedr[3 * static_cast<size_t>(p) + comp] = 2.0 * d;
 }
 
+// sqrt of a non-negative integer-valued argument at compile time (Newton from above)
+__host__ __device__ constexpr double cx_sqrt(double x) {
+    if (x <= 0.0) return 0.0;
+    double r = x > 1.0 ? x : 1.0;
+    for (int k = 0; k < 64; ++k) r = 0.5 * (r + x / r);
+    return r;
+}
+
+// dE/dr_ij from U alone.  U is a representation of SU(2) on the unit quaternion q = (a_r, a_i, b_r, b_i):
+// along the fields V_0 = q*i, V_1 = q*j, V_2 = q*k its derivative is L_g U, with constant tridiagonal
+// generators that mix ma within a row,
+//   (L_0 U)(ma) = i (2ma - j) U(ma),
+//   (L_1 U)(ma) = c(ma) U(ma+1) - c(ma-1) U(ma-1),
+//   (L_2 U)(ma) = i (c(ma) U(ma+1) + c(ma-1) U(ma-1)),      c(m) = sqrt((m+1)(j-m)).
+// |q| = 1 for every r_ij, so dq/dx_k = sum_g (dq/dx_k . V_g) V_g and
+//   dF/dx_k = sum_g (dq/dx_k . V_g) T_g,   T_g = sum' Re(conj(Y) L_g U),   F = sum' Re(conj(Y) U)
+// (half domain, middle column weighted 1/2, j = 0 term Y_0/2).  The result equals the contraction of
+// compute_duidrj's dU/dr with Y (cpuSnap.cpp:1091-1376) up to rounding: one U recursion per pair and four
+// running sums replace its three derivative recursions.
+template <int J, bool HALF, int NW>
+__global__ __launch_bounds__(64 * NW) void k_dedr_gen(SnapDev s, const double* __restrict__ Yr,
+                                                      const double* __restrict__ Yi, const double* __restrict__ rij,
+                                                      const int* __restrict__ yatom, const int* __restrict__ ti,
+                                                      const int* __restrict__ tj, const long* __restrict__ ybase,
+                                                      double* __restrict__ dedr, int N, int npairs_host,
+                                                      const int* __restrict__ npairs_dev) {
+    const int p = (blockIdx.x * NW + static_cast<int>(threadIdx.x >> 6)) * 64 + static_cast<int>(threadIdx.x & 63);
+    const int np = npairs_dev ? *npairs_dev : npairs_host;
+    if (p >= np) return;
+    const int tjp = tj[p];
+    const double rc = (s.radelem[ti[p]] + s.radelem[tjp]) * s.rcutfac;
+    PairGeom g;
+    pair_geometry<true>(rij[3 * p], rij[3 * p + 1], rij[3 * p + 2], rc, s.rmin0, s.rfac0, s.switchflag, s.wjelem[tjp], g);
+    const size_t n = static_cast<size_t>(N);
+    const size_t yb = static_cast<size_t>(yatom[p]) + (ybase ? static_cast<size_t>(ybase[p]) : 0);
+    double f = 0.5 * Yr[yb], t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    row_chain<J, 0, 0, J / 2>(g, s.rootpq, [&](auto Jc, auto MBc, auto& pr, auto& pi, auto&, auto&) {
+        constexpr int JJ = decltype(Jc)::value, MB = decltype(MBc)::value;
+        if constexpr (JJ > 0) {
+            constexpr bool MID = (JJ == 2 * MB);
+            constexpr int LAST = MID ? MB : JJ;
+            constexpr int row = (HALF ? h_block(JJ) : u_block(JJ)) + (JJ + 1) * MB;
+            const size_t o = static_cast<size_t>(row) * n + yb;
+            static_for<0, LAST + 1>([&](auto MAc) {
+                constexpr int MA = decltype(MAc)::value;
+                constexpr double w = (MID && MA == MB) ? 0.5 : 1.0;
+                const double yr = w * Yr[o + MA * n], yi = w * Yi[o + MA * n];
+                f += yr * pr[MA] + yi * pi[MA];
+                t0 += static_cast<double>(2 * MA - JJ) * (yi * pr[MA] - yr * pi[MA]);
+                if constexpr (MA < JJ) {
+                    constexpr double c = cx_sqrt(static_cast<double>((MA + 1) * (JJ - MA)));
+                    t1 += c * (yr * pr[MA + 1] + yi * pi[MA + 1]);
+                    t2 += c * (yi * pr[MA + 1] - yr * pi[MA + 1]);
+                }
+                if constexpr (MA > 0) {
+                    constexpr double c = cx_sqrt(static_cast<double>(MA * (JJ - MA + 1)));
+                    t1 -= c * (yr * pr[MA - 1] + yi * pi[MA - 1]);
+                    t2 += c * (yi * pr[MA - 1] - yr * pi[MA - 1]);
+                }
+            });
+        }
+    });
+    // V_0 = q*i = (-a_i, a_r, b_i, -b_r), V_1 = q*j = (-b_r, -b_i, a_r, a_i), V_2 = q*k = (-b_i, b_r, -a_i, a_r)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double c0 = -g.ai * g.dar[k] + g.ar * g.dai[k] + g.bi * g.dbr[k] - g.br * g.dbi[k];
+        const double c1 = -g.br * g.dar[k] - g.bi * g.dai[k] + g.ar * g.dbr[k] + g.ai * g.dbi[k];
+        const double c2 = -g.bi * g.dar[k] + g.br * g.dai[k] - g.ai * g.dbr[k] + g.ar * g.dbi[k];
+        dedr[3 * static_cast<size_t>(p) + k] = 2.0 * (g.dsfac * g.u[k] * f + g.sfac * (c0 * t0 + c1 * t1 + c2 * t2));
+    }
+}
+
 // Y of the half layout ([element][atom]) copied atom-major and complex-interleaved,
 // yt[(atom*HE + e)*2 + {0,1}], so that one atom's Y is a contiguous block.
 __global__ void k_y_atom_major(const double* __restrict__ Yr, const double* __restrict__ Yi, int HE, int N,
@@ -199,7 +271,7 @@ void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const dou
                  const int* npairs_dev, hipStream_t st, bool half, int variant) {
     if (npairs_host <= 0) return;
     const dim3 g3((npairs_host + 63) / 64), g1((npairs_host + 20) / 21), g2((npairs_host + 41) / 42),
-        g4((npairs_host + 83) / 84);
+        g4((npairs_host + 83) / 84), g256((npairs_host + 255) / 256);
 #define DE_ARGS Yr, Yi, rij, yatom, ti, tj, ybase, dedr, N, npairs_host, npairs_dev
 #define DE_CASE(JV)                                                                                            \
     case JV:                                                                                                   \
@@ -215,6 +287,10 @@ void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const dou
             hipLaunchKernelGGL((k_dedr_split<JV, true, 2, 2>), g2, dim3(128), 0, st, s, DE_ARGS);              \
         else if (variant == 7)                                                                                 \
             hipLaunchKernelGGL((k_dedr_split<JV, true, 2, 4>), g4, dim3(256), 0, st, s, DE_ARGS);              \
+        else if (variant == 8)                                                                                 \
+            hipLaunchKernelGGL((k_dedr_gen<JV, true, 1>), g3, dim3(64), 0, st, s, DE_ARGS);                    \
+        else if (variant == 9)                                                                                 \
+            hipLaunchKernelGGL((k_dedr_gen<JV, true, 4>), g256, dim3(256), 0, st, s, DE_ARGS);                 \
         else                                                                                                   \
             hipLaunchKernelGGL((k_dedr_split<JV, true, 4>), g1, dim3(64), 0, st, s, DE_ARGS);                  \
         break;
@@ -230,9 +306,9 @@ void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const dou
 }
 
 int dedr_variants(int* ids) {
-    static const int v[] = {0, 2, 3, 4, 5, 6, 7};
-    for (int k = 0; k < 7; ++k) ids[k] = v[k];
-    return 7;   // 5 is launch_dedr_atom; 6 and 7 are variant 2 with 2 and 4 waves per workgroup
+    static const int v[] = {0, 2, 3, 4, 5, 6, 7, 8, 9};
+    for (int k = 0; k < 9; ++k) ids[k] = v[k];
+    return 9;   // 5 is launch_dedr_atom; 6, 7: variant 2 with 2 and 4 waves per workgroup; 8, 9: k_dedr_gen
 }
 
 }  // namespace mdp

@@ -896,7 +896,7 @@ struct mdp_md {
             return;
         }
         std::string js = "{";
-        int ids[8];
+        int ids[16];
         int nv = dedr_variants(ids);
         // MDP_DEDR_VARIANT pins the dE/dr kernel variant (one of dedr_variants) instead of timing them all
         const char* pin = std::getenv("MDP_DEDR_VARIANT");
