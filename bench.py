@@ -274,7 +274,7 @@ def main():
         return
 
     import mdp_amd
-    from mdp_amd.flops import snap_flops
+    from mdp_amd.flops import dedr_key, snap_flops
     from mdp_amd.md import SnapMD
 
     mdp_amd.library()   # fails loudly without the HIP library
@@ -305,7 +305,8 @@ def main():
     fl = snap_flops(J)
     pairs_mean = md.mean_pairs()          # pairs per step averaged over the timed steps
     ppa = pairs_mean / n
-    per_launch = {"compute_yi": fl["yi_md"] * n, "compute_dedr": fl["dedr"] * pairs_mean,
+    dkey = dedr_key(int(md.tuning().get("chosen", {}).get("compute_dedr", 9)))
+    per_launch = {"compute_yi": fl["yi_md"] * n, "compute_dedr": fl[dkey] * pairs_mean,
                   "compute_ui": fl["ui"] * pairs_mean}
     dom = max(per_launch, key=lambda k: timers[k][0])
     ms_avg = timers[dom][0] / max(timers[dom][1], 1)
@@ -316,7 +317,7 @@ def main():
     tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_bcc40_j8.json")
     if os.path.exists(tfile) and a.ncell == NCELL:
         traffic = json.load(open(tfile)).get("bytes_per_launch")
-    step_flops = n * (ppa * (fl["ui"] + fl["dedr"]) + fl["yi_md"])
+    step_flops = n * (ppa * (fl["ui"] + fl[dkey]) + fl["yi_md"])
     log(f"dominant kernel {dom}: {ms_avg:.3f} ms/launch, {achieved:.2f} TFLOP/s "
         f"({100 * achieved / FP64_PEAK_TFLOPS:.1f}% of FP64 peak); whole step "
         f"{step_flops / (elapsed / a.steps) / 1e12:.2f} TFLOP/s")

@@ -124,7 +124,7 @@ __host__ __device__ constexpr double cx_sqrt(double x) {
 // (half domain, middle column weighted 1/2, j = 0 term Y_0/2).  The result equals the contraction of
 // compute_duidrj's dU/dr with Y (cpuSnap.cpp:1091-1376) up to rounding: one U recursion per pair and four
 // running sums replace its three derivative recursions.
-template <int J, bool HALF, int NW>
+template <int J, bool HALF, int NW, bool PREFETCH>
 __global__ __launch_bounds__(64 * NW) void k_dedr_gen(SnapDev s, const double* __restrict__ Yr,
                                                       const double* __restrict__ Yi, const double* __restrict__ rij,
                                                       const int* __restrict__ yatom, const int* __restrict__ ti,
@@ -141,17 +141,38 @@ __global__ __launch_bounds__(64 * NW) void k_dedr_gen(SnapDev s, const double* _
     const size_t n = static_cast<size_t>(N);
     const size_t yb = static_cast<size_t>(yatom[p]) + (ybase ? static_cast<size_t>(ybase[p]) : 0);
     double f = 0.5 * Yr[yb], t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    // With PREFETCH, Y rows are loaded one emitted row ahead (rows come in the order (1,0) .. (J,0),
+    // (2,1) .. (J,1), ...) so that their latency overlaps the recursion of the current row.
+    double ynr[J + 1], yni[J + 1];
+    auto load_row = [&](auto Jc, auto MBc) {
+        constexpr int JJ = decltype(Jc)::value, MB = decltype(MBc)::value;
+        constexpr int LAST = (JJ == 2 * MB) ? MB : JJ;
+        constexpr int row = (HALF ? h_block(JJ) : u_block(JJ)) + (JJ + 1) * MB;
+        const size_t o = static_cast<size_t>(row) * n + yb;
+#pragma unroll
+        for (int ma = 0; ma <= LAST; ++ma) {
+            ynr[ma] = Yr[o + ma * n];
+            yni[ma] = Yi[o + ma * n];
+        }
+    };
+    if constexpr (PREFETCH) load_row(ic<1>{}, ic<0>{});
     row_chain<J, 0, 0, J / 2>(g, s.rootpq, [&](auto Jc, auto MBc, auto& pr, auto& pi, auto&, auto&) {
         constexpr int JJ = decltype(Jc)::value, MB = decltype(MBc)::value;
         if constexpr (JJ > 0) {
             constexpr bool MID = (JJ == 2 * MB);
             constexpr int LAST = MID ? MB : JJ;
-            constexpr int row = (HALF ? h_block(JJ) : u_block(JJ)) + (JJ + 1) * MB;
-            const size_t o = static_cast<size_t>(row) * n + yb;
+            double ycr[LAST + 1], yci[LAST + 1];
+            if constexpr (!PREFETCH) load_row(Jc, MBc);
+#pragma unroll
+            for (int ma = 0; ma <= LAST; ++ma) { ycr[ma] = ynr[ma]; yci[ma] = yni[ma]; }
+            if constexpr (PREFETCH) {
+                if constexpr (JJ < J) load_row(ic<JJ + 1>{}, ic<MB>{});
+                else if constexpr (2 * (MB + 1) <= J) load_row(ic<2 * (MB + 1)>{}, ic<MB + 1>{});
+            }
             static_for<0, LAST + 1>([&](auto MAc) {
                 constexpr int MA = decltype(MAc)::value;
                 constexpr double w = (MID && MA == MB) ? 0.5 : 1.0;
-                const double yr = w * Yr[o + MA * n], yi = w * Yi[o + MA * n];
+                const double yr = w * ycr[MA], yi = w * yci[MA];
                 f += yr * pr[MA] + yi * pi[MA];
                 t0 += static_cast<double>(2 * MA - JJ) * (yi * pr[MA] - yr * pi[MA]);
                 if constexpr (MA < JJ) {
@@ -288,9 +309,13 @@ void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const dou
         else if (variant == 7)                                                                                 \
             hipLaunchKernelGGL((k_dedr_split<JV, true, 2, 4>), g4, dim3(256), 0, st, s, DE_ARGS);              \
         else if (variant == 8)                                                                                 \
-            hipLaunchKernelGGL((k_dedr_gen<JV, true, 1>), g3, dim3(64), 0, st, s, DE_ARGS);                    \
+            hipLaunchKernelGGL((k_dedr_gen<JV, true, 1, false>), g3, dim3(64), 0, st, s, DE_ARGS);             \
         else if (variant == 9)                                                                                 \
-            hipLaunchKernelGGL((k_dedr_gen<JV, true, 4>), g256, dim3(256), 0, st, s, DE_ARGS);                 \
+            hipLaunchKernelGGL((k_dedr_gen<JV, true, 4, false>), g256, dim3(256), 0, st, s, DE_ARGS);          \
+        else if (variant == 10)                                                                                \
+            hipLaunchKernelGGL((k_dedr_gen<JV, true, 1, true>), g3, dim3(64), 0, st, s, DE_ARGS);              \
+        else if (variant == 11)                                                                                \
+            hipLaunchKernelGGL((k_dedr_gen<JV, true, 4, true>), g256, dim3(256), 0, st, s, DE_ARGS);           \
         else                                                                                                   \
             hipLaunchKernelGGL((k_dedr_split<JV, true, 4>), g1, dim3(64), 0, st, s, DE_ARGS);                  \
         break;
@@ -306,9 +331,9 @@ void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const dou
 }
 
 int dedr_variants(int* ids) {
-    static const int v[] = {0, 2, 3, 4, 5, 6, 7, 8, 9};
-    for (int k = 0; k < 9; ++k) ids[k] = v[k];
-    return 9;   // 5 is launch_dedr_atom; 6, 7: variant 2 with 2 and 4 waves per workgroup; 8, 9: k_dedr_gen
+    static const int v[] = {0, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11};
+    for (int k = 0; k < 11; ++k) ids[k] = v[k];
+    return 11;   // 5: launch_dedr_atom; 6, 7: variant 2 with 2 and 4 waves per workgroup; 8-11: k_dedr_gen
 }
 
 }  // namespace mdp

@@ -6,6 +6,10 @@ enumerated from the loop bounds of the index spaces -- the denominators of the r
 * ``dedr`` per pair : U recursion as above plus three derivative recursions (48 / 102 / 48 per row
                       update) plus the contraction with Y (32 per contracted element: 2 for dsfac*U and
                       10 per Cartesian component).
+* ``dedr_gen`` per pair: dE/dr through the SU(2) generators: the U recursion as in ``ui`` plus, per
+                      contracted element, 4 for Re(conj(Y) U), 5 for the diagonal generator and 10 for each
+                      existing neighbour ma +- 1 (two generators), plus 90 per pair for the projection of
+                      dq/dr onto the three fields and the combination.
 * ``yi``   per atom : the upstream compute_yi work (cpuSnap.cpp:891-989): sum over the idxz entries of
                       10*na*nb (complex product and CG weight) + 4*nb (row weight) + 4 (beta * Z).
 * ``yi_md`` per atom: the same restricted to the Y entries the force needs (middle row of even layers
@@ -48,6 +52,11 @@ def snap_flops(J):
     contracted = sum((mb + 1 if j == 2 * mb else j + 1) for j in range(J + 1) for mb in range(j // 2 + 1))
     ui = _row_updates(J, False) + 4 * emitted
     dedr = _row_updates(J, True) + 32 * contracted
+    dedr_gen = _row_updates(J, False) + 90
+    for j in range(1, J + 1):
+        for mb in range(j // 2 + 1):
+            for ma in range((mb if j == 2 * mb else j) + 1):
+                dedr_gen += 4 + 5 + 10 * (ma < j) + 10 * (ma > 0)
     yi = yi_md = ei = zb = 0
     terms = elems = 0
     half = lambda l: sum((mb + 1 if l == 2 * mb else l + 1) for mb in range(l // 2 + 1))
@@ -74,10 +83,15 @@ def snap_flops(J):
         if j >= j1:
             ei += 2
     dbdr = _row_updates(J, True) + 3 * (4 * elems + 2 * terms)
-    return dict(ui=ui, dedr=dedr, yi=yi, yi_md=yi_md, ei=ei, zb=zb, dbdr=dbdr)
+    return dict(ui=ui, dedr=dedr, dedr_gen=dedr_gen, yi=yi, yi_md=yi_md, ei=ei, zb=zb, dbdr=dbdr)
 
 
-def step_flops_per_atom(J, pairs_per_atom):
+def dedr_key(variant):
+    """Operation count that matches an MD dE/dr kernel variant (8 to 11 use the SU(2) generators)."""
+    return "dedr_gen" if 8 <= variant <= 11 else "dedr"
+
+
+def step_flops_per_atom(J, pairs_per_atom, dedr_variant=9):
     """Plain MD step (Ui + Yi with energy + dE/dr) per atom."""
     f = snap_flops(J)
-    return pairs_per_atom * (f["ui"] + f["dedr"]) + f["yi_md"]
+    return pairs_per_atom * (f["ui"] + f[dedr_key(dedr_variant)]) + f["yi_md"]
