@@ -297,7 +297,7 @@ void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const dou
 #define DE_CASE(JV)                                                                                            \
     case JV:                                                                                                   \
         if (!half)                                                                                             \
-            hipLaunchKernelGGL((k_dedr<JV, false>), g3, dim3(64), 0, st, s, DE_ARGS);                          \
+            hipLaunchKernelGGL((k_dedr_gen<JV, false, 1, true>), g3, dim3(64), 0, st, s, DE_ARGS);             \
         else if (variant == 0)                                                                                 \
             hipLaunchKernelGGL((k_dedr<JV, true>), g3, dim3(64), 0, st, s, DE_ARGS);                           \
         else if (variant == 2)                                                                                 \

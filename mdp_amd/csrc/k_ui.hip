@@ -4,8 +4,8 @@
 // rows of one "row group" (rows RLO..RHI of the half domain) are accumulated in registers over all
 // pairs, so there are no atomics and the result is independent of scheduling.  Rows below RLO are
 // advanced only up to the seed of the next row.  At twojmax = 8 the 155 half-U elements are split
-// into four groups ({0}, {1}, {2}, {3,4}: 45/42/35/33 elements) run as four launches.  U is stored
-// atom-fastest, so every store of a wave is one contiguous 512-byte segment.
+// into four groups ({0}, {1}, {2}, {3,4}: 45/42/35/33 elements) run by separate workgroups of one
+// launch.  U is stored atom-fastest, so every store of a wave is one contiguous 512-byte segment.
 #include <hip/hip_runtime.h>
 
 #include "snap_device.hpp"
@@ -16,26 +16,36 @@ namespace mdp {
 bool templated_j(int J) { return J == 2 || J == 4 || J == 6 || J == 8; }
 
 template <int J, int RLO, int RHI, int MODE>
-__global__ __launch_bounds__(64) void k_ui(SnapDev s, double* __restrict__ Ur, double* __restrict__ Ui,
-                                           const double* __restrict__ rij, const int* __restrict__ pair_off,
-                                           const int* __restrict__ perm, const int* __restrict__ ti,
-                                           const int* __restrict__ tj, int N, double wself, int elem_sel) {
-    const int atom = blockIdx.x * 64 + threadIdx.x;
-    if (atom >= N) return;
+__device__ __forceinline__ void ui_rows(const SnapDev& s, double* __restrict__ Ur, double* __restrict__ Ui,
+                                        const double* __restrict__ rij, const int* __restrict__ pair_off,
+                                        const int* __restrict__ perm, const int* __restrict__ ti,
+                                        const int* __restrict__ tj, int N, double wself, int elem_sel, int atom) {
     constexpr int NE = rows_elems(J, RLO, RHI);
     double accr[NE], acci[NE];
 #pragma unroll
     for (int k = 0; k < NE; ++k) { accr[k] = 0.0; acci[k] = 0.0; }
 
+    // the next pair's r_ij and types are loaded while the current pair's recursion runs
     const int q0 = pair_off[atom], q1 = pair_off[atom + 1];
+    double nx = 0.0, ny = 0.0, nz = 0.0;
+    int nti = 0, ntj = 0;
+    if (q0 < q1) {
+        const int p = perm ? perm[q0] : q0;
+        nx = rij[3 * p]; ny = rij[3 * p + 1]; nz = rij[3 * p + 2];
+        nti = ti[p]; ntj = tj[p];
+    }
     for (int q = q0; q < q1; ++q) {
-        const int p = perm ? perm[q] : q;
-        const int tjp = tj[p];
+        const double x = nx, y = ny, z = nz;
+        const int tip = nti, tjp = ntj;
+        if (q + 1 < q1) {
+            const int p = perm ? perm[q + 1] : q + 1;
+            nx = rij[3 * p]; ny = rij[3 * p + 1]; nz = rij[3 * p + 2];
+            nti = ti[p]; ntj = tj[p];
+        }
         if (elem_sel >= 0 && s.map[tjp] != elem_sel) continue;
-        const double rc = (s.radelem[ti[p]] + s.radelem[tjp]) * s.rcutfac;
+        const double rc = (s.radelem[tip] + s.radelem[tjp]) * s.rcutfac;
         PairGeom g;
-        pair_geometry<false>(rij[3 * p], rij[3 * p + 1], rij[3 * p + 2], rc, s.rmin0, s.rfac0, s.switchflag,
-                             s.wjelem[tjp], g);
+        pair_geometry<false>(x, y, z, rc, s.rmin0, s.rfac0, s.switchflag, s.wjelem[tjp], g);
         const double sf = g.sfac;
         row_chain<J, 0, RLO, RHI>(g, s.rootpq, [&](auto Jc, auto MBc, auto& pr, auto& pi, auto&, auto&) {
             constexpr int JJ = decltype(Jc)::value, MB = decltype(MBc)::value;
@@ -83,23 +93,43 @@ __global__ __launch_bounds__(64) void k_ui(SnapDev s, double* __restrict__ Ur, d
     });
 }
 
+// The row groups of twojmax = 8 ({0}, {1}, {2}, {3,4}) and 6 ({0}, {1}, {2,3}) as one launch: workgroup
+// b runs group (b / 8) % NG on atom block (b / 8 / NG) * 8 + b % 8, so the groups of one atom block go to
+// the same XCD (workgroups are dealt to the 8 XCDs round-robin) and share its L2 for r_ij.
+template <int J>
+constexpr int ui_ngroups() { return J == 8 ? 4 : (J == 6 ? 3 : 1); }
+
+template <int J, int MODE>
+__global__ __launch_bounds__(64) void k_ui(SnapDev s, double* __restrict__ Ur, double* __restrict__ Ui,
+                                           const double* __restrict__ rij, const int* __restrict__ pair_off,
+                                           const int* __restrict__ perm, const int* __restrict__ ti,
+                                           const int* __restrict__ tj, int N, double wself, int elem_sel) {
+    constexpr int NG = ui_ngroups<J>();
+    const int b = blockIdx.x, k = b / 8;
+    const int group = k % NG, ablock = (k / NG) * 8 + b % 8;
+    const int atom = ablock * 64 + threadIdx.x;
+    if (atom >= N) return;
+    if constexpr (J == 8) {
+        if (group == 0) ui_rows<8, 0, 0, MODE>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom);
+        else if (group == 1) ui_rows<8, 1, 1, MODE>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom);
+        else if (group == 2) ui_rows<8, 2, 2, MODE>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom);
+        else ui_rows<8, 3, 4, MODE>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom);
+    } else if constexpr (J == 6) {
+        if (group == 0) ui_rows<6, 0, 0, MODE>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom);
+        else if (group == 1) ui_rows<6, 1, 1, MODE>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom);
+        else ui_rows<6, 2, 3, MODE>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom);
+    } else {
+        ui_rows<J, 0, J / 2, MODE>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom);
+    }
+}
+
 template <int J, int MODE>
 static void ui_groups(const SnapDev& s, double* Ur, double* Ui, const double* rij, const int* off, const int* perm,
                       const int* ti, const int* tj, int N, double wself, int esel, hipStream_t st) {
-    dim3 grid((N + 63) / 64), block(64);
-    if constexpr (J == 8) {
-        hipLaunchKernelGGL((k_ui<8, 0, 0, MODE>), grid, block, 0, st, s, Ur, Ui, rij, off, perm, ti, tj, N, wself, esel);
-        hipLaunchKernelGGL((k_ui<8, 1, 1, MODE>), grid, block, 0, st, s, Ur, Ui, rij, off, perm, ti, tj, N, wself, esel);
-        hipLaunchKernelGGL((k_ui<8, 2, 2, MODE>), grid, block, 0, st, s, Ur, Ui, rij, off, perm, ti, tj, N, wself, esel);
-        hipLaunchKernelGGL((k_ui<8, 3, 4, MODE>), grid, block, 0, st, s, Ur, Ui, rij, off, perm, ti, tj, N, wself, esel);
-    } else if constexpr (J == 6) {
-        hipLaunchKernelGGL((k_ui<6, 0, 0, MODE>), grid, block, 0, st, s, Ur, Ui, rij, off, perm, ti, tj, N, wself, esel);
-        hipLaunchKernelGGL((k_ui<6, 1, 1, MODE>), grid, block, 0, st, s, Ur, Ui, rij, off, perm, ti, tj, N, wself, esel);
-        hipLaunchKernelGGL((k_ui<6, 2, 3, MODE>), grid, block, 0, st, s, Ur, Ui, rij, off, perm, ti, tj, N, wself, esel);
-    } else {
-        hipLaunchKernelGGL((k_ui<J, 0, J / 2, MODE>), grid, block, 0, st, s, Ur, Ui, rij, off, perm, ti, tj, N, wself,
-                           esel);
-    }
+    constexpr int NG = ui_ngroups<J>();
+    const int ablocks = ((N + 63) / 64 + 7) / 8 * 8;   // padded to whole XCD rounds; excess blocks return
+    hipLaunchKernelGGL((k_ui<J, MODE>), dim3(ablocks * NG), dim3(64), 0, st, s, Ur, Ui, rij, off, perm, ti, tj, N,
+                       wself, esel);
 }
 
 void launch_ui(const SnapDev& s, UiMode mode, double* Ur, double* Ui, const double* rij, const int* pair_off,

@@ -73,7 +73,8 @@ void launch_zi(const SnapDev& s, const double* Ur, const double* Ui, long off1, 
 
 // dE/dr_ij per pair: 2 sum' d(sfac U_ij)/dr . Y(atom of the pair); pairs p in [0, *npairs) (device count)
 // or [0, npairs_host) when npairs_dev == nullptr.  yatom[p] = row of Y used (aii), ybase[p] optional
-// element offset (elements) for chemflag.
+// element offset (elements) for chemflag.  The full layout always runs the SU(2)-generator kernel; the
+// half layout runs the given variant (dedr_variants).
 void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const double* rij, const int* yatom,
                  const int* ti, const int* tj, const long* ybase, double* dedr, int natoms, int npairs_host,
                  const int* npairs_dev, hipStream_t st, bool half_layout = false, int variant = 0);
