@@ -299,91 +299,123 @@ __global__ void k_pair_count_md(const double* __restrict__ x, const int* __restr
     pcnt[i] = c;
 }
 
-__global__ void k_pair_fill_md(const double* __restrict__ x, const int* __restrict__ type,
-                               const int* __restrict__ nb_cnt, const int* __restrict__ nb_j,
-                               const int* __restrict__ nb_s, const double* __restrict__ rc2_tt, int ntypes,
-                               const int* __restrict__ poff, int* __restrict__ pi_, int* __restrict__ pj,
-                               int* __restrict__ pti, int* __restrict__ ptj, double* __restrict__ prij,
-                               int* __restrict__ entry_pair, double Lx, double Ly, double Lz, int N) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const int ti = type[i];
-    int p = poff[i];
-    for (int k = 0; k < nb_cnt[i]; ++k) {
-        const int e = k * N + i, j = nb_j[e];
-        double dx, dy, dz;
-        entry_disp(x, i, j, nb_s[e], Lx, Ly, Lz, dx, dy, dz);
-        const double d2 = dist2_nc(dx, dy, dz);
-        if (d2 < rc2_tt[ti * (ntypes + 1) + type[j]] && d2 > 1e-20) {
-            pi_[p] = i;
-            pj[p] = j;
-            pti[p] = ti;
-            ptj[p] = type[j];
-            prij[3 * static_cast<size_t>(p)] = dx;
-            prij[3 * static_cast<size_t>(p) + 1] = dy;
-            prij[3 * static_cast<size_t>(p) + 2] = dz;
-            entry_pair[e] = p;
-            ++p;
-        } else {
-            entry_pair[e] = -1;
+// 16 lanes per atom: each pass tests 16 consecutive entries of the atom, and the accepted ones get
+// consecutive pair slots from a ballot, so a wave's stores cover a few contiguous pair ranges (one per
+// atom) instead of 64 scattered slots.  Pair order within an atom is the entry order, as with one lane
+// per atom.
+constexpr int PSUB = 16;
+
+__global__ __launch_bounds__(256) void k_pair_fill_md(const double* __restrict__ x, const int* __restrict__ type,
+                                                      const int* __restrict__ nb_cnt, const int* __restrict__ nb_j,
+                                                      const int* __restrict__ nb_s, const double* __restrict__ rc2_tt,
+                                                      int ntypes, const int* __restrict__ poff, int* __restrict__ pi_,
+                                                      int* __restrict__ pj, int* __restrict__ pti,
+                                                      int* __restrict__ ptj, double* __restrict__ prij,
+                                                      int* __restrict__ entry_pair, double Lx, double Ly, double Lz,
+                                                      int N) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / PSUB, sub = t % PSUB, lane = threadIdx.x & 63;
+    const unsigned long long group = 0xFFFFull << (lane & ~(PSUB - 1));
+    const unsigned long long below = (1ull << lane) - 1;
+    const bool live = i < N;
+    const int ti = live ? type[i] : 0;
+    const int n = live ? nb_cnt[i] : 0;
+    int p = live ? poff[i] : 0;
+    for (int k0 = 0; __ballot(k0 < n) != 0; k0 += PSUB) {   // wave-uniform trip count
+        const int k = k0 + sub;
+        const bool valid = k < n;
+        int j = 0, tjv = 0;
+        double dx = 0.0, dy = 0.0, dz = 0.0;
+        bool acc = false;
+        const int e = k * N + i;
+        if (valid) {
+            j = nb_j[e];
+            tjv = type[j];
+            entry_disp(x, i, j, nb_s[e], Lx, Ly, Lz, dx, dy, dz);
+            const double d2 = dist2_nc(dx, dy, dz);
+            acc = d2 < rc2_tt[ti * (ntypes + 1) + tjv] && d2 > 1e-20;
         }
+        const unsigned long long m = __ballot(acc) & group;
+        if (valid) {
+            if (acc) {
+                const int q = p + __popcll(m & below);
+                pi_[q] = i;
+                pj[q] = j;
+                pti[q] = ti;
+                ptj[q] = tjv;
+                prij[3 * static_cast<size_t>(q)] = dx;
+                prij[3 * static_cast<size_t>(q) + 1] = dy;
+                prij[3 * static_cast<size_t>(q) + 2] = dz;
+                entry_pair[e] = q;
+            } else {
+                entry_pair[e] = -1;
+            }
+        }
+        p += __popcll(m);
     }
 }
 
-// f, per-atom virial and energy of atom i without atomics; optionally the second velocity half-step
-__global__ void k_tally(const double* __restrict__ dedr, const double* __restrict__ prij, const int* __restrict__ poff,
-                        const int* __restrict__ in_cnt, const int* __restrict__ in_e,
-                        const int* __restrict__ entry_pair, const double* __restrict__ epart, int nrows, double e0,
-                        const double* __restrict__ eext, double* __restrict__ f, double* __restrict__ vir,
-                        double* __restrict__ eatom,
-                        double* __restrict__ v, const int* __restrict__ type, const double* __restrict__ massinv_t,
-                        double dtf, int integrate, int N, unsigned long long* __restrict__ pair_sum) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    if (pair_sum && i == 0) atomicAdd(pair_sum, static_cast<unsigned long long>(poff[N]));   // pairs per step
-    double fx = 0, fy = 0, fz = 0, w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0, w5 = 0;
-    for (int p = poff[i]; p < poff[i + 1]; ++p) {
-        const double gx = dedr[3 * static_cast<size_t>(p)], gy = dedr[3 * static_cast<size_t>(p) + 1],
-                     gz = dedr[3 * static_cast<size_t>(p) + 2];
-        const double rx = prij[3 * static_cast<size_t>(p)], ry = prij[3 * static_cast<size_t>(p) + 1],
-                     rz = prij[3 * static_cast<size_t>(p) + 2];
-        fx += gx;
-        fy += gy;
-        fz += gz;
-        w0 -= rx * gx;
-        w1 -= ry * gy;
-        w2 -= rz * gz;
-        w3 -= rx * gy;
-        w4 -= rx * gz;
-        w5 -= ry * gz;
-    }
-    const int nin = in_cnt[i];
-    for (int k = 0; k < nin; ++k) {
-        const int p = entry_pair[in_e[static_cast<size_t>(k) * N + i]];
-        if (p >= 0) {
-            fx -= dedr[3 * static_cast<size_t>(p)];
-            fy -= dedr[3 * static_cast<size_t>(p) + 1];
-            fz -= dedr[3 * static_cast<size_t>(p) + 2];
+// f, per-atom virial and energy of atom i without atomics; optionally the second velocity half-step.
+// 16 lanes per atom: lane s takes the atom's pairs s, s+16, ... and incoming entries s, s+16, ... (the
+// pair reads of a wave are then contiguous), and the 16 partial sums are combined by a butterfly.
+__global__ __launch_bounds__(256) void k_tally(const double* __restrict__ dedr, const double* __restrict__ prij,
+                                               const int* __restrict__ poff, const int* __restrict__ in_cnt,
+                                               const int* __restrict__ in_e, const int* __restrict__ entry_pair,
+                                               const double* __restrict__ epart, int nrows, double e0,
+                                               const double* __restrict__ eext, double* __restrict__ f,
+                                               double* __restrict__ vir, double* __restrict__ eatom,
+                                               double* __restrict__ v, const int* __restrict__ type,
+                                               const double* __restrict__ massinv_t, double dtf, int integrate, int N,
+                                               unsigned long long* __restrict__ pair_sum) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / PSUB, sub = t % PSUB;
+    const bool live = i < N;
+    if (pair_sum && t == 0) atomicAdd(pair_sum, static_cast<unsigned long long>(poff[N]));   // pairs per step
+    double a[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};   // f x, y, z; virial 0..5; energy
+    if (live) {
+        for (int p = poff[i] + sub; p < poff[i + 1]; p += PSUB) {
+            const double gx = dedr[3 * static_cast<size_t>(p)], gy = dedr[3 * static_cast<size_t>(p) + 1],
+                         gz = dedr[3 * static_cast<size_t>(p) + 2];
+            const double rx = prij[3 * static_cast<size_t>(p)], ry = prij[3 * static_cast<size_t>(p) + 1],
+                         rz = prij[3 * static_cast<size_t>(p) + 2];
+            a[0] += gx;
+            a[1] += gy;
+            a[2] += gz;
+            a[3] -= rx * gx;
+            a[4] -= ry * gy;
+            a[5] -= rz * gz;
+            a[6] -= rx * gy;
+            a[7] -= rx * gz;
+            a[8] -= ry * gz;
         }
+        const int nin = in_cnt[i];
+        for (int k = sub; k < nin; k += PSUB) {
+            const int p = entry_pair[in_e[static_cast<size_t>(k) * N + i]];
+            if (p >= 0) {
+                a[0] -= dedr[3 * static_cast<size_t>(p)];
+                a[1] -= dedr[3 * static_cast<size_t>(p) + 1];
+                a[2] -= dedr[3 * static_cast<size_t>(p) + 2];
+            }
+        }
+        if (!eext)
+            for (int r = sub; r < nrows; r += PSUB) a[9] += epart[static_cast<size_t>(r) * N + i];
     }
-    double e = 0.0;
-    if (!eext)
-        for (int r = 0; r < nrows; ++r) e += epart[static_cast<size_t>(r) * N + i];
-    f[3 * i] = fx;
-    f[3 * i + 1] = fy;
-    f[3 * i + 2] = fz;
-    vir[6 * i] = w0;
-    vir[6 * i + 1] = w1;
-    vir[6 * i + 2] = w2;
-    vir[6 * i + 3] = w3;
-    vir[6 * i + 4] = w4;
-    vir[6 * i + 5] = w5;
-    eatom[i] = eext ? eext[i] : e0 + (2.0 / 3.0) * e;
+#pragma unroll
+    for (int w = PSUB / 2; w >= 1; w /= 2)
+#pragma unroll
+        for (int c = 0; c < 10; ++c) a[c] += __shfl_xor(a[c], w);
+    if (!live || sub != 0) return;
+    f[3 * i] = a[0];
+    f[3 * i + 1] = a[1];
+    f[3 * i + 2] = a[2];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) vir[6 * i + c] = a[3 + c];
+    eatom[i] = eext ? eext[i] : e0 + (2.0 / 3.0) * a[9];
     if (integrate) {
         const double dtfm = dtf * massinv_t[type[i]];
-        v[3 * i] += dtfm * fx;
-        v[3 * i + 1] += dtfm * fy;
-        v[3 * i + 2] += dtfm * fz;
+        v[3 * i] += dtfm * a[0];
+        v[3 * i + 1] += dtfm * a[1];
+        v[3 * i + 2] += dtfm * a[2];
     }
 }
 
@@ -698,7 +730,7 @@ struct mdp_md {
                            nb_cnt.as<int>(), nb_j.as<int>(), nb_s.as<int>(), rc2.as<double>(), ntypes, pcnt.as<int>(),
                            L[0], L[1], L[2], N);
         exclusive_scan(pcnt.as<int>(), poff.as<int>(), N, scan_tmp.as<int>(), st);
-        hipLaunchKernelGGL(k_pair_fill_md, dim3(nb(N, 256)), dim3(256), 0, st, x.as<double>(), type.as<int>(),
+        hipLaunchKernelGGL(k_pair_fill_md, dim3(nb(static_cast<long>(N) * PSUB, 256)), dim3(256), 0, st, x.as<double>(), type.as<int>(),
                            nb_cnt.as<int>(), nb_j.as<int>(), nb_s.as<int>(), rc2.as<double>(), ntypes, poff.as<int>(),
                            pi.as<int>(), pj.as<int>(), pti.as<int>(), ptj.as<int>(), prij.as<double>(),
                            entry_pair.as<int>(), L[0], L[1], L[2], N);
@@ -737,7 +769,7 @@ struct mdp_md {
             dedr_launch(dedr_variant);
         });
         timed(T_TALLY, [&] {
-            hipLaunchKernelGGL(k_tally, dim3(nb(N, 256)), dim3(256), 0, st, dedr.as<double>(), prij.as<double>(),
+            hipLaunchKernelGGL(k_tally, dim3(nb(static_cast<long>(N) * PSUB, 256)), dim3(256), 0, st, dedr.as<double>(), prij.as<double>(),
                                poff.as<int>(), in_cnt.as<int>(), in_e.as<int>(), entry_pair.as<int>(),
                                epart.as<double>(), nrows, e0, nullptr, fo, vo, eo, v.as<double>(), type.as<int>(),
                                massinv.as<double>(), dtf, integrate ? 1 : 0, N, count ? pair_sum.as<unsigned long long>() : nullptr);
@@ -837,7 +869,7 @@ struct mdp_md {
                         false, 0);
         });
         timed(T_TALLY, [&] {
-            hipLaunchKernelGGL(k_tally, dim3(nb(N, 256)), dim3(256), 0, st, dedr.as<double>(), prij.as<double>(),
+            hipLaunchKernelGGL(k_tally, dim3(nb(static_cast<long>(N) * PSUB, 256)), dim3(256), 0, st, dedr.as<double>(), prij.as<double>(),
                                poff.as<int>(), in_cnt.as<int>(), in_e.as<int>(), entry_pair.as<int>(),
                                epart.as<double>(), 0, 0.0, sE.as<double>(), f.as<double>(), vir.as<double>(),
                                eatom.as<double>(), v.as<double>(), type.as<int>(), massinv.as<double>(), dtf,
