@@ -34,7 +34,7 @@ using namespace mdp;
 namespace {
 
 constexpr int NBMAX = 96;      // neighbour slots per atom at rcut+skin
-constexpr int CELLCAP = 48;    // atoms per cell the LDS tile accepts
+constexpr int CELLCAP = 48;    // atoms per cell accepted (bounds the per-cell insertion sort)
 constexpr int SHK = 8;         // image shifts are coded in [-SHK, SHK] per axis
 constexpr double FTM2V = 1.0 / 1.0364269e-4;   // metal units, eV/A/amu -> A/ps^2
 constexpr double MVV2E = 1.0364269e-4;
@@ -143,67 +143,59 @@ __global__ void k_cell_sort(const int* __restrict__ cell_start, int* __restrict_
     }
 }
 
-// One workgroup (64 lanes) per cell: the atoms of the 27 surrounding cells (periodic images applied)
-// are staged in LDS in a fixed order, then each lane scans the tile for its centre atom.
-__global__ __launch_bounds__(64) void k_neighbors_cell(const double* __restrict__ x, const int* __restrict__ cell_start,
-                                                       const int* __restrict__ cell_atoms, int* __restrict__ nb_cnt,
-                                                       int* __restrict__ nb_j, int* __restrict__ nb_s, double Lx,
-                                                       double Ly, double Lz, int ncx, int ncy, int ncz, double rcs2,
-                                                       int N, int* __restrict__ overflow) {
-    __shared__ double tx[27 * CELLCAP], ty[27 * CELLCAP], tz[27 * CELLCAP];
-    __shared__ int tj[27 * CELLCAP], ts[27 * CELLCAP];
-    __shared__ int toff[28];
-    const int c = blockIdx.x;
+// Eight lanes per centre atom, atoms taken in cell order (so the atoms of a wave share their 27
+// surrounding cells in L1).  The group walks the 27 cells (periodic images applied) in a fixed order and
+// each cell's atoms in ascending order, eight candidates per pass; accepted candidates are ranked by a
+// ballot, which keeps the list in candidate order.
+constexpr int NSUB = 8;
+
+__global__ __launch_bounds__(256) void k_neighbors_cell(const double* __restrict__ x, const int* __restrict__ cell_of,
+                                                        const int* __restrict__ cell_start,
+                                                        const int* __restrict__ cell_atoms, int* __restrict__ nb_cnt,
+                                                        int* __restrict__ nb_j, int* __restrict__ nb_s, double Lx,
+                                                        double Ly, double Lz, int ncx, int ncy, int ncz, double rcs2,
+                                                        int N, int* __restrict__ overflow) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = t / NSUB, sub = t % NSUB, lane = threadIdx.x & 63;
+    const unsigned long long group = 0xFFull << (lane & ~(NSUB - 1));
+    const unsigned long long below = (1ull << lane) - 1;
+    const bool live = g < N;
+    const int i = live ? cell_atoms[g] : 0;
+    const int c = live ? cell_of[i] : 0;
     const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int k = 0; k < 27; ++k) {
-            const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
-            const int nx = (cx + dx + ncx) % ncx, ny = (cy + dy + ncy) % ncy, nz = (cz + dz + ncz) % ncz;
-            const int nc = nx + ncx * (ny + ncy * nz);
-            toff[k] = acc;
-            acc += min(cell_start[nc + 1] - cell_start[nc], CELLCAP);
-        }
-        toff[27] = acc;
-    }
-    __syncthreads();
+    const double xi = live ? x[3 * i] : 0.0, yi = live ? x[3 * i + 1] : 0.0, zi = live ? x[3 * i + 2] : 0.0;
+    const int self_code = shift_code(0, 0, 0);
+    int cnt = 0;
     for (int k = 0; k < 27; ++k) {
-        const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
-        const int ux = cx + dx, uy = cy + dy, uz = cz + dz;
+        const int ux = cx + k % 3 - 1, uy = cy + (k / 3) % 3 - 1, uz = cz + k / 9 - 1;
         const int sx = (ux < 0) ? -1 : (ux >= ncx ? 1 : 0);
         const int sy = (uy < 0) ? -1 : (uy >= ncy ? 1 : 0);
         const int sz = (uz < 0) ? -1 : (uz >= ncz ? 1 : 0);
         const int nc = (ux - sx * ncx) + ncx * ((uy - sy * ncy) + ncy * (uz - sz * ncz));
-        const int b = cell_start[nc], cnt = toff[k + 1] - toff[k];
-        for (int m = threadIdx.x; m < cnt; m += 64) {
-            const int j = cell_atoms[b + m];
-            const int t = toff[k] + m;
-            tx[t] = image_nc(x[3 * j], sx, Lx, 0.0);
-            ty[t] = image_nc(x[3 * j + 1], sy, Ly, 0.0);
-            tz[t] = image_nc(x[3 * j + 2], sz, Lz, 0.0);
-            tj[t] = j;
-            ts[t] = shift_code(sx, sy, sz);
+        const int code = shift_code(sx, sy, sz);
+        const int b = live ? cell_start[nc] : 0, n = live ? cell_start[nc + 1] - b : 0;
+        for (int m0 = 0; __ballot(m0 < n) != 0; m0 += NSUB) {   // wave-uniform trip count
+            const int m = m0 + sub;
+            int j = 0;
+            bool acc = false;
+            if (m < n) {
+                j = cell_atoms[b + m];
+                const double d2 = dist2_nc(image_nc(x[3 * j], sx, Lx, xi), image_nc(x[3 * j + 1], sy, Ly, yi),
+                                           image_nc(x[3 * j + 2], sz, Lz, zi));
+                acc = d2 <= rcs2 && !(j == i && code == self_code);
+            }
+            const unsigned long long mk = __ballot(acc) & group;
+            if (acc) {
+                const int q = cnt + __popcll(mk & below);
+                if (q < NBMAX) {
+                    nb_j[static_cast<size_t>(q) * N + i] = j;
+                    nb_s[static_cast<size_t>(q) * N + i] = code;
+                }
+            }
+            cnt += __popcll(mk);
         }
     }
-    __syncthreads();
-    const int b0 = cell_start[c], n0 = cell_start[c + 1] - b0;
-    const int ntile = toff[27];
-    const int self_code = shift_code(0, 0, 0);
-    for (int m = threadIdx.x; m < n0; m += 64) {
-        const int i = cell_atoms[b0 + m];
-        const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
-        int cnt = 0;
-        for (int t = 0; t < ntile; ++t) {
-            if (tj[t] == i && ts[t] == self_code) continue;
-            const double d2 = dist2_nc(tx[t] - xi, ty[t] - yi, tz[t] - zi);
-            if (d2 <= rcs2) {
-                if (cnt < NBMAX) {
-                    nb_j[static_cast<size_t>(cnt) * N + i] = tj[t];
-                    nb_s[static_cast<size_t>(cnt) * N + i] = ts[t];
-                }
-                ++cnt;
-            }
-        }
+    if (live && sub == 0) {
         if (cnt > NBMAX) atomicOr(overflow, 1);
         nb_cnt[i] = min(cnt, NBMAX);
     }
@@ -281,22 +273,33 @@ __device__ __forceinline__ void entry_disp(const double* __restrict__ x, int i, 
     dz = image_nc(x[3 * j + 2], sz, Lz, x[3 * i + 2]);
 }
 
-__global__ void k_pair_count_md(const double* __restrict__ x, const int* __restrict__ type,
-                                const int* __restrict__ nb_cnt, const int* __restrict__ nb_j,
-                                const int* __restrict__ nb_s, const double* __restrict__ rc2_tt, int ntypes,
-                                int* __restrict__ pcnt, double Lx, double Ly, double Lz, int N) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const int ti = type[i];
+// pairs per atom (entries within the pair cutoff); 16 lanes per atom as in k_pair_fill_md below
+__global__ __launch_bounds__(256) void k_pair_count_md(const double* __restrict__ x, const int* __restrict__ type,
+                                                       const int* __restrict__ nb_cnt, const int* __restrict__ nb_j,
+                                                       const int* __restrict__ nb_s, const double* __restrict__ rc2_tt,
+                                                       int ntypes, int* __restrict__ pcnt, double Lx, double Ly,
+                                                       double Lz, int N) {
+    constexpr int SUB = 16;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / SUB, sub = t % SUB, lane = threadIdx.x & 63;
+    const unsigned long long group = 0xFFFFull << (lane & ~(SUB - 1));
+    const bool live = i < N;
+    const int ti = live ? type[i] : 0;
+    const int n = live ? nb_cnt[i] : 0;
     int c = 0;
-    for (int k = 0; k < nb_cnt[i]; ++k) {
-        const int e = k * N + i, j = nb_j[e];
-        double dx, dy, dz;
-        entry_disp(x, i, j, nb_s[e], Lx, Ly, Lz, dx, dy, dz);
-        const double d2 = dist2_nc(dx, dy, dz);
-        if (d2 < rc2_tt[ti * (ntypes + 1) + type[j]] && d2 > 1e-20) ++c;
+    for (int k0 = 0; __ballot(k0 < n) != 0; k0 += SUB) {   // wave-uniform trip count
+        const int k = k0 + sub;
+        bool acc = false;
+        if (k < n) {
+            const int e = k * N + i, j = nb_j[e];
+            double dx, dy, dz;
+            entry_disp(x, i, j, nb_s[e], Lx, Ly, Lz, dx, dy, dz);
+            const double d2 = dist2_nc(dx, dy, dz);
+            acc = d2 < rc2_tt[ti * (ntypes + 1) + type[j]] && d2 > 1e-20;
+        }
+        c += __popcll(__ballot(acc) & group);
     }
-    pcnt[i] = c;
+    if (live && sub == 0) pcnt[i] = c;
 }
 
 // 16 lanes per atom: each pass tests 16 consecutive entries of the atom, and the accepted ones get
@@ -689,8 +692,9 @@ struct mdp_md {
                                cell_fill.as<int>(), cell_atoms.as<int>(), N);
             hipLaunchKernelGGL(k_cell_sort, dim3(nb(static_cast<long>(ncells), 256)), dim3(256), 0, st,
                                cell_start.as<int>(), cell_atoms.as<int>(), static_cast<int>(ncells), flags.as<int>());
-            hipLaunchKernelGGL(k_neighbors_cell, dim3(static_cast<unsigned>(ncells)), dim3(64), 0, st, x.as<double>(),
-                               cell_start.as<int>(), cell_atoms.as<int>(), nb_cnt.as<int>(), nb_j.as<int>(),
+            hipLaunchKernelGGL(k_neighbors_cell, dim3(nb(static_cast<long>(N) * NSUB, 256)), dim3(256), 0, st,
+                               x.as<double>(), cell_of.as<int>(), cell_start.as<int>(), cell_atoms.as<int>(),
+                               nb_cnt.as<int>(), nb_j.as<int>(),
                                nb_s.as<int>(), L[0], L[1], L[2], ncell[0], ncell[1], ncell[2], rcs * rcs, N,
                                flags.as<int>());
         } else {
@@ -726,7 +730,7 @@ struct mdp_md {
     }
 
     void pairs() {
-        hipLaunchKernelGGL(k_pair_count_md, dim3(nb(N, 256)), dim3(256), 0, st, x.as<double>(), type.as<int>(),
+        hipLaunchKernelGGL(k_pair_count_md, dim3(nb(static_cast<long>(N) * PSUB, 256)), dim3(256), 0, st, x.as<double>(), type.as<int>(),
                            nb_cnt.as<int>(), nb_j.as<int>(), nb_s.as<int>(), rc2.as<double>(), ntypes, pcnt.as<int>(),
                            L[0], L[1], L[2], N);
         exclusive_scan(pcnt.as<int>(), poff.as<int>(), N, scan_tmp.as<int>(), st);
