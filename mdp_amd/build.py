@@ -28,6 +28,13 @@ FLAGS = ["-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC", "-Wall", "-Wno-
          "-I", os.path.join(ROOT, "include")]
 
 
+# Per-file flags.  The Z/Y row kernels read their inner Clebsch-Gordan coefficients as compile-time
+# constants; machine-level loop-invariant code motion would hoist thousands of them out of the row loop
+# into SGPRs and spill those to VGPR lanes, so it is disabled for these translation units.
+FILE_FLAGS = {f: ["-mllvm", "-disable-machine-licm"]
+              for f in ("k_yih_8_8.hip", "k_yih_8_12.hip", "k_yih_8_4.hip", "k_yih_small.hip", "k_fit_zb.hip")}
+
+
 def hipcc():
     for cand in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
         if cand and os.path.exists(cand):
@@ -46,7 +53,7 @@ def _compile(src, force, hdr_time):
     if not force and os.path.exists(o) and os.path.getmtime(o) >= max(os.path.getmtime(s), hdr_time):
         return src, 0.0, ""
     t0 = time.time()
-    cmd = [hipcc()] + FLAGS + ["-c", s, "-o", o]
+    cmd = [hipcc()] + FLAGS + FILE_FLAGS.get(src, []) + ["-c", s, "-o", o]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:

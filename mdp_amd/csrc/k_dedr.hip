@@ -105,14 +105,6 @@ __global__ __launch_bounds__(64 * NW, WPE) void k_dedr_split(SnapDev s, const do
     dedr[3 * static_cast<size_t>(p) + comp] = 2.0 * d;
 }
 
-// sqrt of a non-negative integer-valued argument at compile time (Newton from above)
-__host__ __device__ constexpr double cx_sqrt(double x) {
-    if (x <= 0.0) return 0.0;
-    double r = x > 1.0 ? x : 1.0;
-    for (int k = 0; k < 64; ++k) r = 0.5 * (r + x / r);
-    return r;
-}
-
 // dE/dr_ij from U alone.  U is a representation of SU(2) on the unit quaternion q = (a_r, a_i, b_r, b_i):
 // along the fields V_0 = q*i, V_1 = q*j, V_2 = q*k its derivative is L_g U, with constant tridiagonal
 // generators that mix ma within a row,

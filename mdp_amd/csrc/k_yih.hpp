@@ -62,7 +62,8 @@ __device__ __forceinline__ void z_row_lds(int mb, int malim, const double* __res
                 static_for<0, NA>([&](auto IAc) {
                     constexpr int IA = decltype(IAc)::value;
                     constexpr int M1 = LO + IA, M2 = HI2 - IA;
-                    const double cga = c[M1 * (J2 + 1) + M2];
+                    // compile-time coefficient: no scalar load, whose wait would also drain the LDS reads
+                    constexpr double cga = cg_const(J1, J2, JO, M1, M2);
                     sr += cga * (ar[M1] * br[M2] - ai[M1] * bi[M2]);
                     si += cga * (ar[M1] * bi[M2] + ai[M1] * br[M2]);
                 });

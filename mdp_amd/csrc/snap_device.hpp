@@ -110,6 +110,46 @@ __host__ __device__ constexpr int half_row_mb(int r) {
     return r;
 }
 
+// ---- compile-time Clebsch-Gordan coefficients ------------------------------------------------------
+
+// sqrt of a non-negative argument at compile time (Newton from above; within an ulp of sqrt)
+__host__ __device__ constexpr double cx_sqrt(double x) {
+    if (x <= 0.0) return 0.0;
+    double r = x > 1.0 ? x : 1.0;
+    for (int k = 0; k < 200; ++k) {
+        const double nr = 0.5 * (r + x / r);
+        if (nr >= r) break;
+        r = nr;
+    }
+    return r;
+}
+__host__ __device__ constexpr double cx_fact(int n) {
+    double f = 1.0;
+    for (int k = 2; k <= n; ++k) f *= k;
+    return f;
+}
+// <j1 m1 j2 m2 | j m> with doubled j and m1, m2 counted from -j1/2, -j2/2 (the layout of the CG table:
+// entry m1 * (j2 + 1) + m2 of block cg_block(j1, j2, j)), by Racah's formula (Varshalovich 8.2.1), the
+// same expression the host tables evaluate
+__host__ __device__ constexpr double cg_const(int j1, int j2, int j, int m1, int m2) {
+    const int a2 = 2 * m1 - j1, b2 = 2 * m2 - j2, m = (a2 + b2 + j) / 2;
+    if (m < 0 || m > j) return 0.0;
+    const int zmin = cx_max(0, cx_max(-(j - j2 + a2) / 2, -(j - j1 - b2) / 2));
+    const int zmax = cx_min((j1 + j2 - j) / 2, cx_min((j1 - a2) / 2, (j2 + b2) / 2));
+    double s = 0.0;
+    for (int z = zmin; z <= zmax; ++z) {
+        const double sg = (z & 1) ? -1.0 : 1.0;
+        s += sg / (cx_fact(z) * cx_fact((j1 + j2 - j) / 2 - z) * cx_fact((j1 - a2) / 2 - z) *
+                   cx_fact((j2 + b2) / 2 - z) * cx_fact((j - j2 + a2) / 2 + z) * cx_fact((j - j1 - b2) / 2 + z));
+    }
+    const int c2 = 2 * m - j;
+    const double tri = cx_sqrt(cx_fact((j1 + j2 - j) / 2) * cx_fact((j1 - j2 + j) / 2) * cx_fact((-j1 + j2 + j) / 2) /
+                               cx_fact((j1 + j2 + j) / 2 + 1));
+    const double nrm = cx_sqrt(cx_fact((j1 + a2) / 2) * cx_fact((j1 - a2) / 2) * cx_fact((j2 + b2) / 2) *
+                               cx_fact((j2 - b2) / 2) * cx_fact((j + c2) / 2) * cx_fact((j - c2) / 2) * (j + 1));
+    return s * tri * nrm;
+}
+
 // ---- per-pair geometry ------------------------------------------------------------------------
 
 struct PairGeom {
