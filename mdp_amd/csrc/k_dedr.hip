@@ -116,15 +116,27 @@ __global__ __launch_bounds__(64 * NW, WPE) void k_dedr_split(SnapDev s, const do
 // (half domain, middle column weighted 1/2, j = 0 term Y_0/2).  The result equals the contraction of
 // compute_duidrj's dU/dr with Y (cpuSnap.cpp:1091-1376) up to rounding: one U recursion per pair and four
 // running sums replace its three derivative recursions.
-template <int J, bool HALF, int NW, bool PREFETCH>
+// With XCD, workgroup b runs tile (b % 8) * C + b / 8, C = ceil(tiles / 8) for the pairs present:
+// workgroups are dealt to the 8 XCDs round-robin, so each XCD then walks one contiguous eighth of the
+// pair list and the Y lines of its atoms (one 128-byte line holds an element of 16 consecutive atoms)
+// are fetched into one L2 only.  The grid is a multiple of 8 covering the pair capacity; workgroups with
+// b / 8 >= C and tiles past the end return.
+template <int J, bool HALF, int NW, bool PREFETCH, bool XCD = false>
 __global__ __launch_bounds__(64 * NW) void k_dedr_gen(SnapDev s, const double* __restrict__ Yr,
                                                       const double* __restrict__ Yi, const double* __restrict__ rij,
                                                       const int* __restrict__ yatom, const int* __restrict__ ti,
                                                       const int* __restrict__ tj, const long* __restrict__ ybase,
                                                       double* __restrict__ dedr, int N, int npairs_host,
                                                       const int* __restrict__ npairs_dev) {
-    const int p = (blockIdx.x * NW + static_cast<int>(threadIdx.x >> 6)) * 64 + static_cast<int>(threadIdx.x & 63);
     const int np = npairs_dev ? *npairs_dev : npairs_host;
+    int tile = static_cast<int>(blockIdx.x);
+    if constexpr (XCD) {
+        const int chunk = ((np + 64 * NW - 1) / (64 * NW) + 7) / 8;
+        const int k = static_cast<int>(blockIdx.x / 8);
+        if (k >= chunk) return;
+        tile = static_cast<int>(blockIdx.x % 8) * chunk + k;
+    }
+    const int p = (tile * NW + static_cast<int>(threadIdx.x >> 6)) * 64 + static_cast<int>(threadIdx.x & 63);
     if (p >= np) return;
     const int tjp = tj[p];
     const double rc = (s.radelem[ti[p]] + s.radelem[tjp]) * s.rcutfac;
@@ -284,7 +296,8 @@ void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const dou
                  const int* npairs_dev, hipStream_t st, bool half, int variant) {
     if (npairs_host <= 0) return;
     const dim3 g3((npairs_host + 63) / 64), g1((npairs_host + 20) / 21), g2((npairs_host + 41) / 42),
-        g4((npairs_host + 83) / 84), g256((npairs_host + 255) / 256);
+        g4((npairs_host + 83) / 84), g256((npairs_host + 255) / 256),
+        g3x(((npairs_host + 63) / 64 + 7) / 8 * 8), g256x(((npairs_host + 255) / 256 + 7) / 8 * 8);
 #define DE_ARGS Yr, Yi, rij, yatom, ti, tj, ybase, dedr, N, npairs_host, npairs_dev
 #define DE_CASE(JV)                                                                                            \
     case JV:                                                                                                   \
@@ -308,6 +321,10 @@ void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const dou
             hipLaunchKernelGGL((k_dedr_gen<JV, true, 1, true>), g3, dim3(64), 0, st, s, DE_ARGS);              \
         else if (variant == 11)                                                                                \
             hipLaunchKernelGGL((k_dedr_gen<JV, true, 4, true>), g256, dim3(256), 0, st, s, DE_ARGS);           \
+        else if (variant == 12)                                                                                \
+            hipLaunchKernelGGL((k_dedr_gen<JV, true, 1, true, true>), g3x, dim3(64), 0, st, s, DE_ARGS);       \
+        else if (variant == 13)                                                                                \
+            hipLaunchKernelGGL((k_dedr_gen<JV, true, 4, true, true>), g256x, dim3(256), 0, st, s, DE_ARGS);    \
         else                                                                                                   \
             hipLaunchKernelGGL((k_dedr_split<JV, true, 4>), g1, dim3(64), 0, st, s, DE_ARGS);                  \
         break;
@@ -323,9 +340,9 @@ void launch_dedr(const SnapDev& s, const double* Yr, const double* Yi, const dou
 }
 
 int dedr_variants(int* ids) {
-    static const int v[] = {0, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11};
-    for (int k = 0; k < 11; ++k) ids[k] = v[k];
-    return 11;   // 5: launch_dedr_atom; 6, 7: variant 2 with 2 and 4 waves per workgroup; 8-11: k_dedr_gen
+    static const int v[] = {0, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13};
+    for (int k = 0; k < 13; ++k) ids[k] = v[k];
+    return 13;   // 5: launch_dedr_atom; 6, 7: variant 2 with 2 and 4 waves per workgroup; 8-13: k_dedr_gen
 }
 
 }  // namespace mdp

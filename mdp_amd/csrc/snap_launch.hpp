@@ -86,7 +86,8 @@ void launch_dedr_atom(const SnapDev& s, const double* Yr, const double* Yi, doub
 // Variants of the MD path's dE/dr (0: all three components per lane; 2, 3, 4: one component per lane
 // with at least that many waves per SIMD; 5: launch_dedr_atom; 6, 7: variant 2 with 2 or 4 waves per
 // workgroup on consecutive pair tiles; 8, 9: lane = pair, derivatives from U through the SU(2)
-// generators, 1 or 4 waves per workgroup; 10, 11: the same with Y rows loaded one row ahead).
+// generators, 1 or 4 waves per workgroup; 10, 11: the same with Y rows loaded one row ahead; 12, 13:
+// 10, 11 with each XCD walking one contiguous eighth of the pairs).
 // Writes at most 16 ids, returns their number.
 int dedr_variants(int* ids);
 

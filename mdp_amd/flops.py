@@ -87,8 +87,8 @@ def snap_flops(J):
 
 
 def dedr_key(variant):
-    """Operation count that matches an MD dE/dr kernel variant (8 to 11 use the SU(2) generators)."""
-    return "dedr_gen" if 8 <= variant <= 11 else "dedr"
+    """Operation count that matches an MD dE/dr kernel variant (8 to 13 use the SU(2) generators)."""
+    return "dedr_gen" if 8 <= variant <= 13 else "dedr"
 
 
 def step_flops_per_atom(J, pairs_per_atom, dedr_variant=9):

@@ -78,7 +78,7 @@ def test_first_step_vs_oracle(oracle, J, ncell, seed):
     assert relerr(st["vatom"], v) < TOL
 
 
-@pytest.mark.parametrize("variant", ["0", "2", "3", "4", "5", "6", "7", "8", "9", "10", "11"])
+@pytest.mark.parametrize("variant", ["0", "2", "3", "4", "5", "6", "7", "8", "9", "10", "11", "12", "13"])
 def test_every_dedr_variant_vs_oracle(oracle, monkeypatch, variant):
     """Each dE/dr kernel variant the autotuner may pick, pinned in turn (twojmax = 8, 3,456 atoms)."""
     monkeypatch.setenv("MDP_DEDR_VARIANT", variant)

@@ -2,8 +2,8 @@
 
 Closed-form table sizes, Clebsch-Gordan values against SymPy and their orthogonality, unitarity of
 the Wigner-U layers, rotation/translation invariance, the isolated-atom limit, finite differences of
-the energy, the Euler identity of the cubic bispectrum energy, and the brute-force neighbour list
-against the host builder.
+the energy, the Euler identity of the cubic bispectrum energy, the SU(2)-generator form of dU/dr that
+the MD dE/dr kernel uses, and the brute-force neighbour list against the host builder.
 """
 import numpy as np
 import pytest
@@ -216,3 +216,62 @@ def test_sampled_oracle_matches_full_evaluation(oracle):
     assert relerr(es, e[sample]) < 1e-13
     assert relerr(fs, f[sample]) < 1e-11
     assert relerr(vs, v[sample]) < 1e-11
+
+
+def _cayley_klein(r, rc, rfac0):
+    """(a_r, a_i, b_r, b_i) of a pair vector (cpuSnap.cpp:996-1067 geometry, rmin0 = 0)."""
+    x, y, z = r
+    rr = np.sqrt(x * x + y * y + z * z)
+    theta0 = rr * rfac0 * np.pi / rc
+    z0 = rr / np.tan(theta0)
+    r0 = np.sqrt(rr * rr + z0 * z0)
+    return np.array([z0 / r0, -z / r0, y / r0, -x / r0])
+
+
+def test_su2_generator_derivative_identity(oracle):
+    """The identity behind the MD dE/dr kernel: along the fields V_0 = q*i, V_1 = q*j, V_2 = q*k of the
+    unit quaternion q = (a_r, a_i, b_r, b_i), the derivative of every U row is a fixed tridiagonal
+    combination of the row itself, so dU/dx_k = sum_g (dq/dx_k . V_g) (L_g U).  Checked against the
+    oracle's derivative recursion (cpuComputeUij, switching off) with dq/dx_k from central differences
+    of the Cayley-Klein map, on every half-domain row of five random pairs at twojmax = 8."""
+    J = 8
+    s = snapcpp.initsna(oracle, [1, J, 4.67637, 0.99363, 0, 1, 0, 0, 0, 0, 0], [0.5], [1.0])
+    rc = 2 * 0.5 * 4.67637
+    rng = np.random.default_rng(11)
+    P = 5
+    rij = rng.normal(size=(P, 3))
+    rij = rij / np.linalg.norm(rij, axis=1)[:, None] * rng.uniform(1.5, 4.5, size=(P, 1))
+    ur, ui = np.zeros(s.idxu_max * P), np.zeros(s.idxu_max * P)
+    dur, dui = np.zeros(3 * s.idxu_max * P), np.zeros(3 * s.idxu_max * P)
+    one = np.ones(P, np.int32)
+    oracle.cpuComputeUij(ur, ui, dur, dui, s.rootpqarray, np.ascontiguousarray(rij).ravel(), s.wjelem,
+                         s.radelem, 0.0, s.rfac0, s.rcutfac, s.idxu_block, one, one, J, s.idxu_max, P, 0)
+    U = (ur + 1j * ui).reshape(s.idxu_max, P)
+    dU = (dur + 1j * dui).reshape(3, s.idxu_max, P)
+    h = 1e-6
+    worst = 0.0
+    for p in range(P):
+        q = _cayley_klein(rij[p], rc, s.rfac0)
+        ar, ai, br, bi = q
+        V = np.array([[-ai, ar, bi, -br], [-br, -bi, ar, ai], [-bi, br, -ai, ar]])
+        for k in range(3):
+            e = np.zeros(3)
+            e[k] = h
+            dq = (_cayley_klein(rij[p] + e, rc, s.rfac0) - _cayley_klein(rij[p] - e, rc, s.rfac0)) / (2 * h)
+            c = V @ dq
+            for j in range(1, J + 1):
+                cm = np.sqrt([(m + 1) * (j - m) for m in range(j)])
+                for mb in range(j // 2 + 1):
+                    o = s.idxu_block[j] + (j + 1) * mb
+                    row = U[o:o + j + 1, p]
+                    up = np.append(row[1:], 0.0)            # U(ma + 1)
+                    dn = np.insert(row[:-1], 0, 0.0)        # U(ma - 1)
+                    cu = np.append(cm, 0.0)                 # c(ma)
+                    cd = np.insert(cm, 0, 0.0)              # c(ma - 1)
+                    l0 = 1j * (2 * np.arange(j + 1) - j) * row
+                    l1 = cu * up - cd * dn
+                    l2 = 1j * (cu * up + cd * dn)
+                    pred = c[0] * l0 + c[1] * l1 + c[2] * l2
+                    ref = dU[k, o:o + j + 1, p]
+                    worst = max(worst, np.max(np.abs(pred - ref)))
+    assert worst < 1e-7, worst
