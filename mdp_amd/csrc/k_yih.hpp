@@ -74,12 +74,15 @@ __device__ __forceinline__ void z_row_lds(int mb, int malim, const double* __res
     }
 }
 
-template <int J, int JO>
+// PA: per-atom weights beta_t[triple * N + atom] (the learned surrogate's dE/dB) and no energy partials;
+// otherwise wave-uniform weights beta_t[triple]
+template <int J, int JO, bool PA = false>
 __device__ __forceinline__ void y_row_lds(const double* __restrict__ cg, int row, int mb, int atom, bool live, int N,
                                           const double* __restrict__ lr, const double* __restrict__ li, int lane,
                                           const double* __restrict__ beta_t, double* __restrict__ Yr,
                                           double* __restrict__ Yi, double* __restrict__ epart) {
     const int malim = (2 * mb == JO) ? mb : JO;
+    const size_t n = static_cast<size_t>(N);
     double yr[JO + 1], yi[JO + 1];
 #pragma unroll
     for (int m = 0; m <= JO; ++m) { yr[m] = 0.0; yi[m] = 0.0; }
@@ -88,13 +91,13 @@ __device__ __forceinline__ void y_row_lds(const double* __restrict__ cg, int row
         static_for<0, J1 + 1>([&](auto J2c) {
             constexpr int J2 = decltype(J2c)::value;
             if constexpr (coupled(J1, J2, JO)) {
-                const double beta = beta_t[triple_id(J, J1, J2, JO)];
+                constexpr int T = triple_id(J, J1, J2, JO);
+                const double beta = PA ? (live ? beta_t[static_cast<size_t>(T) * n + atom] : 0.0) : beta_t[T];
                 z_row_lds<J, J1, J2, JO>(mb, malim, lr, li, lane, cg, beta, yr, yi);
             }
         });
     });
     if (!live) return;
-    const size_t n = static_cast<size_t>(N);
     const int hrow = h_block(JO) + (JO + 1) * mb;
     const size_t o = static_cast<size_t>(hrow) * n + atom;
     double e = 0.0;
@@ -107,12 +110,12 @@ __device__ __forceinline__ void y_row_lds(const double* __restrict__ cg, int row
             e += w * (lr[(hrow + ma) * 64 + lane] * yr[ma] + li[(hrow + ma) * 64 + lane] * yi[ma]);
         }
     }
-    epart[static_cast<size_t>(row) * n + atom] = e;
+    if constexpr (!PA) epart[static_cast<size_t>(row) * n + atom] = e;
 }
 
 // The coefficient tables (cg, beta_t, order) are separate read-only, non-aliased kernel arguments so
 // that their wave-uniform loads go through the scalar cache into SGPRs.
-template <int J, int W>
+template <int J, int W, bool PA = false>
 __global__ __launch_bounds__(64 * W) void k_yi_half(const double* __restrict__ cg, const double* __restrict__ Ur,
                                                     const double* __restrict__ Ui, const double* __restrict__ beta_t,
                                                     double* __restrict__ Yr, double* __restrict__ Yi,
@@ -141,7 +144,68 @@ __global__ __launch_bounds__(64 * W) void k_yi_half(const double* __restrict__ c
         const int jo = half_row_j(row), mb = half_row_mb(row);
         static_for<0, J + 1>([&](auto JOc) {
             constexpr int JO = decltype(JOc)::value;
-            if (jo == JO) y_row_lds<J, JO>(cg, row, mb, atom, live, N, lr, li, lane, beta_t, Yr, Yi, epart);
+            if (jo == JO) y_row_lds<J, JO, PA>(cg, row, mb, atom, live, N, lr, li, lane, beta_t, Yr, Yi, epart);
+        });
+    }
+}
+
+// B of every bispectrum triple (j >= j1) from the half U of 64 atoms staged in LDS, as k_yi_half stages
+// it: B = 2 sum' U*(j).Z(j1,j2,j) (cpuSnap.cpp:1899-2072 semantics, no bzero subtraction), Z rows from
+// z_row_lds, the middle column of even layers weighted 1/2.  Waves take whole triples from a shared
+// counter; blist[jjb * N + atom].
+template <int J, int W>
+__global__ __launch_bounds__(64 * W) void k_b_half(const double* __restrict__ cg, const double* __restrict__ Ur,
+                                                   const double* __restrict__ Ui, double* __restrict__ blist, int N,
+                                                   int bnorm) {
+    constexpr int HE = n_half_elems(J), NB = n_bisp(J);
+    __shared__ double lr[HE * 64], li[HE * 64];
+    __shared__ int next_task;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int atom = blockIdx.x * 64 + lane;
+    const bool live = atom < N;
+    const size_t n = static_cast<size_t>(N);
+    if (threadIdx.x == 0) next_task = 0;
+    for (int e = wave; e < HE; e += W) {
+        lr[e * 64 + lane] = live ? Ur[static_cast<size_t>(e) * n + atom] : 0.0;
+        li[e * 64 + lane] = live ? Ui[static_cast<size_t>(e) * n + atom] : 0.0;
+    }
+    __syncthreads();
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(&next_task, 1);
+        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+        if (t >= NB) break;
+        static_for<0, J + 1>([&](auto J1c) {
+            constexpr int J1 = decltype(J1c)::value;
+            static_for<0, J1 + 1>([&](auto J2c) {
+                constexpr int J2 = decltype(J2c)::value;
+                static_for<J1, J + 1>([&](auto JOc) {
+                    constexpr int JO = decltype(JOc)::value;
+                    if constexpr (coupled(J1, J2, JO)) {
+                        constexpr int JJB = bisp_index(J, J1, J2, JO);
+                        if (t == JJB) {
+                            double sum = 0.0;
+                            for (int mb = 0; 2 * mb <= JO; ++mb) {
+                                const int malim = (2 * mb == JO) ? mb : JO;
+                                double zr[JO + 1], zi[JO + 1];
+#pragma unroll
+                                for (int m = 0; m <= JO; ++m) { zr[m] = 0.0; zi[m] = 0.0; }
+                                z_row_lds<J, J1, J2, JO>(mb, malim, lr, li, lane, cg, bnorm ? 1.0 / (JO + 1) : 1.0, zr,
+                                                         zi);
+                                const int hrow = h_block(JO) + (JO + 1) * mb;
+#pragma unroll
+                                for (int ma = 0; ma <= JO; ++ma)
+                                    if (ma <= malim) {
+                                        const double w = (2 * mb < JO || ma < mb) ? 1.0 : 0.5;
+                                        sum += w * (lr[(hrow + ma) * 64 + lane] * zr[ma] +
+                                                    li[(hrow + ma) * 64 + lane] * zi[ma]);
+                                    }
+                            }
+                            if (live) blist[static_cast<size_t>(JJB) * n + atom] = 2.0 * sum;
+                        }
+                    }
+                });
+            });
         });
     }
 }

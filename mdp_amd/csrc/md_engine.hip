@@ -493,7 +493,7 @@ struct mdp_md {
     int sur_nin = 0, sur_ntrip = 0, audit_every = 0;
     double audit_tol = 0.0;
     DevTables tab_s;
-    DevBuf sUr, sUi, sB, sE, sG, sBeta, sYr, sYi, sParams, sPacked, sAff, sJjb, sFac, fP, vP, eP, ssq;
+    DevBuf sUr, sUi, sB, sE, sG, sBeta, sYr, sYi, sParams, sPacked, sAff, sJjb, sFac, sRowOrder, fP, vP, eP, ssq;
     long audits = 0, fallbacks = 0;
     double rmse_last = 0, rmse_max = 0, rmse_sum = 0;
     // autotuned launch variants (see autotune())
@@ -827,6 +827,9 @@ struct mdp_md {
         put(sParams, params, sizeof(float) * MLP_NPARAM);
         put(sAff, a.data(), sizeof(double) * 34);
         put(sJjb, jjb.data(), sizeof(int) * jjb.size());
+        std::vector<int> sorder((js / 2 + 1) * (js / 2 + 1));   // half-U rows (j, mb), 2mb <= j <= js
+        yi_row_order(js, sorder.data());
+        put(sRowOrder, sorder.data(), sizeof(int) * sorder.size());
         put(sFac, fac.data(), sizeof(double) * fac.size());
         sPacked.ensure(sizeof(unsigned short) * MLP_NPACK);
         launch_mlp_pack(sParams.as<float>(), sPacked.as<unsigned short>(), st);
@@ -859,18 +862,18 @@ struct mdp_md {
         const SnapDev& s4 = tab_s.view;
         ++force_evals;
         timed(T_SUR, [&] {
-            launch_ui(s4, UI_FULL_WSELF, sUr.as<double>(), sUi.as<double>(), prij.as<double>(), poff.as<int>(), nullptr,
+            launch_ui(s4, UI_HALF_WSELF, sUr.as<double>(), sUi.as<double>(), prij.as<double>(), poff.as<int>(), nullptr,
                       pti.as<int>(), ptj.as<int>(), N, 1.0, -1, st);
-            launch_bi(s4, sUr.as<double>(), sUi.as<double>(), 0, 0, 0, sB.as<double>(), N, 0, st);
+            launch_b_half(s4, sUr.as<double>(), sUi.as<double>(), sB.as<double>(), N, 0, st);
             launch_mlp_eval(sPacked.as<unsigned short>(), sParams.as<float>(), sAff.as<double>(), sur_nin,
                             sB.as<double>(), N, sE.as<double>(), sG.as<double>(), st);
             hipLaunchKernelGGL(k_beta_peratom, dim3(nb(static_cast<long>(sur_ntrip) * N, 256)), dim3(256), 0, st,
                                sG.as<double>(), sJjb.as<int>(), sFac.as<double>(), sur_ntrip, N, sBeta.as<double>());
-            launch_yi_peratom(s4, sUr.as<double>(), sUi.as<double>(), sBeta.as<double>(), sYr.as<double>(),
-                              sYi.as<double>(), N, st);
+            launch_yi_half_pa(s4, sUr.as<double>(), sUi.as<double>(), sBeta.as<double>(), sYr.as<double>(),
+                              sYi.as<double>(), sRowOrder.as<int>(), N, st);
             launch_dedr(s4, sYr.as<double>(), sYi.as<double>(), prij.as<double>(), pi.as<int>(), pti.as<int>(),
                         ptj.as<int>(), nullptr, dedr.as<double>(), N, static_cast<int>(stats[1]), poff.as<int>() + N, st,
-                        false, 0);
+                        true, 10);
         });
         timed(T_TALLY, [&] {
             hipLaunchKernelGGL(k_tally, dim3(nb(static_cast<long>(N) * PSUB, 256)), dim3(256), 0, st, dedr.as<double>(), prij.as<double>(),

@@ -76,6 +76,26 @@ __host__ __device__ constexpr int triple_id(int J, int j1, int j2, int j) {
     return -1;
 }
 
+// ordinal of (j1, j2, j) among the bispectrum triples (j >= j1) in canonical order, and their number
+__host__ __device__ constexpr int bisp_index(int J, int j1, int j2, int j) {
+    int n = 0;
+    for (int a = 0; a <= J; ++a)
+        for (int b = 0; b <= a; ++b)
+            for (int c = a - b; c <= cx_min(J, a + b); c += 2) {
+                if (c < a) continue;
+                if (a == j1 && b == j2 && c == j) return n;
+                ++n;
+            }
+    return -1;
+}
+__host__ __device__ constexpr int n_bisp(int J) {
+    int n = 0;
+    for (int a = 0; a <= J; ++a)
+        for (int b = 0; b <= a; ++b)
+            for (int c = a - b; c <= cx_min(J, a + b); c += 2) n += (c >= a);
+    return n;
+}
+
 __host__ __device__ constexpr int n_triples(int J) {
     int n = 0;
     for (int a = 0; a <= J; ++a)

@@ -61,6 +61,14 @@ void launch_yi_half(const SnapDev& s, const double* Uhr, const double* Uhi, cons
 int yi_half_variants(int J, int* ids);
 // row ids of the half domain sorted by decreasing compute_yi work
 void yi_row_order(int J, int* out);
+// The learned surrogate's twojmax 2 / 4 descriptor path on the half layout (rows 2mb <= j, [e][N]): Y
+// with per-atom weights beta_pa[triple * N + atom] (no energy partials; order = yi_row_order(J)), and
+// every B (bispectrum triples j >= j1, blist[jjb * N + atom], no bzero subtraction).  Both stage the
+// half U of 64 atoms in LDS.  Return false for other twojmax.
+bool launch_yi_half_pa(const SnapDev& s, const double* Ur, const double* Ui, const double* beta_pa, double* Yr,
+                       double* Yi, const int* order, int N, hipStream_t st);
+bool launch_b_half(const SnapDev& s, const double* Ur, const double* Ui, double* blist, int N, int bnorm,
+                   hipStream_t st);
 
 // B of the idxb triples: blist[ii + N*jjb] = 2 sum' U*(j).Z(j1,j2,j)  (before bzero; Z/(j+1) if bnorm).
 // U1/U2/U3 element blocks given by their offsets (elements) into Ur/Ui.

@@ -159,3 +159,28 @@ def test_training_beats_untrained_and_linear():
     print(f"held-out force RMSE: untrained {f0:.3f}, linear J4 {fl:.3f}, MLP {f1:.3f} eV/A", flush=True)
     assert np.isfinite(f1) and f1 < f0
     assert f1 < fl
+
+
+@pytest.mark.parametrize("kind", ["bcc", "packing"])
+def test_md_engine_surrogate_vs_chain_rule(oracle, kind):
+    """The MD engine's surrogate force call (twojmax=4 descriptors through the LDS-staged B and per-atom
+    Y kernels, BF16 MLP, generator dE/dr) against the float64 chain rule on the oracle's descriptors."""
+    from mdp_amd import surrogate as S
+    from mdp_amd.md import SnapMD
+    if kind == "bcc":
+        x, box = lattice.crystal("bcc", 3, 3.316, 0.15, 21)
+    else:
+        x, box = lattice.random_packing(40, 0.0554, 2.2, seed=9)
+    p, aff, _ = _rand_model(5)
+    aff[:14] = 0.0
+    aff[16:30] = 0.2
+    E0, Fc = _cpu_energy_and_chain_forces(oracle, x, box, p, aff, 14)
+    s8 = snapcpp.initsna(oracle, [1, 8, RCUTFAC, RFAC0, 0, 1, 1, 0, 0, 0, 0], [0.5], [1.0])
+    coeff = lattice.snap_coefficients(s8.ncoeff, seed=3)
+    md = SnapMD(x, box, np.ones(len(x), np.int32), 8, coeff, [0.5], [1.0], [180.95])
+    md.set_surrogate(S.Model(p, aff, 14), audit_every=0)
+    print(f"engine surrogate, {kind}: {len(x)} atoms", flush=True)
+    md.compute()
+    st = md.state()
+    assert relerr(st["f"], Fc) < 5e-2
+    assert abs(st["eatom"].sum() - E0) < 2e-2 * max(1.0, abs(E0))
