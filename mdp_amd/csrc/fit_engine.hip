@@ -282,7 +282,8 @@ struct mdp_fit {
         const char* fv = std::getenv("MDP_FIT_DBDR_VARIANT");   // pin the dB/dr variant (tests)
         if (fv) {
             const std::string v(fv);
-            dbdr_variant = (v == "1" && (J == 4 || J == 6 || J == 8)) ? 1 : ((v == "2" && (J == 6 || J == 8)) ? 2 : 0);
+            const bool j468 = (J == 4 || J == 6 || J == 8);
+            dbdr_variant = (v == "1" && j468) ? 1 : ((v == "2" && (J == 6 || J == 8)) ? 2 : ((v == "3" && j468) ? 3 : 0));
         }
         if ((env && std::string(env) == "0") || fv) return;
         int props_cu = 256;
@@ -291,11 +292,15 @@ struct mdp_fit {
         if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
             props_cu = prop.multiProcessorCount;
         const int cand[] = {props_cu * 2, props_cu * 4, props_cu * 5, props_cu * 8};
-        const int nvar = (J == 6 || J == 8) ? 3 : ((J == 4) ? 2 : 1);
+        // dB/dr variants: 0 table walk, 1 unrolled terms, 2 table walk at two waves per SIMD, 3 variant 0
+        // with the term loop pipelined
+        std::vector<int> vars = {0};
+        if (J == 4) vars = {0, 1, 3};
+        if (J == 6 || J == 8) vars = {0, 1, 2, 3};
         double best = 1e30;
         int pick = dbdr_groups, pickv = 0;
         std::string js = "{\"fit_dbdr\": {";
-        for (int v = 0; v < nvar; ++v)
+        for (int v : vars)
             for (int k = 0; k < 4; ++k) {
                 dbdr_variant = v;
                 dbdr_groups = cand[k];

@@ -113,16 +113,18 @@ __global__ __launch_bounds__(256) void k_fit_pairs(const double* __restrict__ x,
             return;
         }
     }
+    // image ranges from the real-number bound, rounded outwards: the exact test below decides, so the
+    // reciprocal (an ulp off at worst) only ever adds a candidate image
+    const double iLx = 1.0 / Lx, iLy = 1.0 / Ly, iLz = 1.0 / Lz;
     int cnt = 0;
     for (int j = j0; j < j1; ++j) {
         const int tj = type[j];
         const double r2 = rc2[ti * (ntypes + 1) + tj];
         const double rc = sqrt(r2);
         const double xj = x[3 * j], yj = x[3 * j + 1], zj = x[3 * j + 2];
-        // image ranges from the real-number bound, widened by one: the exact test below decides
-        const int sx0 = static_cast<int>(floor((xi - xj - rc) / Lx)), sx1 = static_cast<int>(ceil((xi - xj + rc) / Lx));
-        const int sy0 = static_cast<int>(floor((yi - yj - rc) / Ly)), sy1 = static_cast<int>(ceil((yi - yj + rc) / Ly));
-        const int sz0 = static_cast<int>(floor((zi - zj - rc) / Lz)), sz1 = static_cast<int>(ceil((zi - zj + rc) / Lz));
+        const int sx0 = static_cast<int>(floor((xi - xj - rc) * iLx)), sx1 = static_cast<int>(ceil((xi - xj + rc) * iLx));
+        const int sy0 = static_cast<int>(floor((yi - yj - rc) * iLy)), sy1 = static_cast<int>(ceil((yi - yj + rc) * iLy));
+        const int sz0 = static_cast<int>(floor((zi - zj - rc) * iLz)), sz1 = static_cast<int>(ceil((zi - zj + rc) * iLz));
         for (int sz = sz0; sz <= sz1; ++sz) {
             const double dz = image_delta(zj, sz, Lz, zi);
             for (int sy = sy0; sy <= sy1; ++sy) {
@@ -249,7 +251,9 @@ __global__ __launch_bounds__(64) void k_fit_zb(const double* __restrict__ cg, co
 // weight), walked by a uniform loop; the atom is wave-uniform, so Z values, table entries and
 // weights are scalar loads shared by all 63 lanes.  Each lane's 55 accumulators live in LDS.
 // dblist is written in the per-configuration layout dblist[c][lp + P_c*k + 3 P_c*jjb].
-template <int J, int WPE = 1>
+// PF: the term loop is software-pipelined, the table entry and Z row of term k + 1 being loaded while
+// term k is contracted (a term otherwise waits for two dependent scalar loads).
+template <int J, int WPE = 1, bool PF = false>
 __global__ __launch_bounds__(64, WPE) void k_fit_dbdr(const double* __restrict__ rootpq, const double* __restrict__ zt,
                                                  const int* __restrict__ term_lo, const int* __restrict__ term_b,
                                                  const int* __restrict__ term_z, const double* __restrict__ term_w,
@@ -305,12 +309,37 @@ __global__ __launch_bounds__(64, WPE) void k_fit_dbdr(const double* __restrict__
                     di[MB] *= 0.5;
                 }
                 const int k0 = term_lo[L], k1 = term_lo[L + 1];
-                for (int k = k0; k < k1; ++k) {
-                    const double* __restrict__ zr = z + 2 * (term_z[k] + (L + 1) * MB);
-                    double s = 0.0;
+                if constexpr (!PF) {
+                    for (int k = k0; k < k1; ++k) {
+                        const double* __restrict__ zr = z + 2 * (term_z[k] + (L + 1) * MB);
+                        double s = 0.0;
 #pragma unroll
-                    for (int ma = 0; ma <= LAST; ++ma) s += dr[ma] * zr[2 * ma] + di[ma] * zr[2 * ma + 1];
-                    acc[term_b[k] * 64 + lane] += term_w[k] * s;
+                        for (int ma = 0; ma <= LAST; ++ma) s += dr[ma] * zr[2 * ma] + di[ma] * zr[2 * ma + 1];
+                        acc[term_b[k] * 64 + lane] += term_w[k] * s;
+                    }
+                } else if (k0 < k1) {
+                    double zn[2 * (LAST + 1)], wn;
+                    int bn;
+                    auto fetch = [&](int k) {
+                        const double* __restrict__ zr = z + 2 * (term_z[k] + (L + 1) * MB);
+#pragma unroll
+                        for (int m = 0; m < 2 * (LAST + 1); ++m) zn[m] = zr[m];
+                        bn = term_b[k];
+                        wn = term_w[k];
+                    };
+                    fetch(k0);
+                    for (int k = k0; k < k1; ++k) {
+                        double zc[2 * (LAST + 1)];
+#pragma unroll
+                        for (int m = 0; m < 2 * (LAST + 1); ++m) zc[m] = zn[m];
+                        const int bc = bn;
+                        const double wc = wn;
+                        if (k + 1 < k1) fetch(k + 1);
+                        double s = 0.0;
+#pragma unroll
+                        for (int ma = 0; ma <= LAST; ++ma) s += dr[ma] * zc[2 * ma] + di[ma] * zc[2 * ma + 1];
+                        acc[bc * 64 + lane] += wc * s;
+                    }
                 }
             });
             if (active) {
@@ -508,6 +537,21 @@ void launch_fit_dbdr(const SnapDev& s, const double* zt, const FitTerms& tt, con
             D2_CASE(6)
             D2_CASE(8)
 #undef D2_CASE
+            default: break;
+        }
+    }
+    if (variant == 3) {   // variant 0 with the pipelined term loop
+        switch (s.J) {
+#define D3_CASE(JV)                                                                                             \
+    case JV:                                                                                                    \
+        hipLaunchKernelGGL((k_fit_dbdr<JV, 1, true>), dim3(G), dim3(64), 0, st, s.rootpq, zt, tt.lo, tt.b, tt.z,   \
+                           tt.w, s.radelem, s.wjelem, s.rcutfac, s.rmin0, s.rfac0, s.switchflag, rij, pti, ptj, poff, \
+                           cfg_of, cfg_p0, dblist, counter, N);                                                  \
+        return;
+            D3_CASE(4)
+            D3_CASE(6)
+            D3_CASE(8)
+#undef D3_CASE
             default: break;
         }
     }
