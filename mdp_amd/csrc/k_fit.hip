@@ -116,12 +116,26 @@ __global__ __launch_bounds__(256) void k_fit_pairs(const double* __restrict__ x,
     // image ranges from the real-number bound, rounded outwards: the exact test below decides, so the
     // reciprocal (an ulp off at worst) only ever adds a candidate image
     const double iLx = 1.0 / Lx, iLy = 1.0 / Ly, iLz = 1.0 / Lz;
+    // With every box length above twice the largest cutoff at most the minimum image of j can lie within
+    // the cutoff, so j is skipped when that image is farther (with a relative margin far above rounding)
+    // and the exact enumeration below runs only for the survivors: same pairs, same order.
+    double rcmax2 = 0.0;
+    for (int t = 1; t <= ntypes; ++t) rcmax2 = fmax(rcmax2, rc2[ti * (ntypes + 1) + t]);
+    const double reject2 = rcmax2 * (1.0 + 1e-9) + 1e-12;
+    const bool single = 4.0 * rcmax2 < Lx * Lx && 4.0 * rcmax2 < Ly * Ly && 4.0 * rcmax2 < Lz * Lz;
     int cnt = 0;
     for (int j = j0; j < j1; ++j) {
+        const double xj = x[3 * j], yj = x[3 * j + 1], zj = x[3 * j + 2];
+        if (single) {
+            double ex = xj - xi, ey = yj - yi, ez = zj - zi;
+            ex -= Lx * rint(ex * iLx);
+            ey -= Ly * rint(ey * iLy);
+            ez -= Lz * rint(ez * iLz);
+            if (ex * ex + ey * ey + ez * ez > reject2) continue;
+        }
         const int tj = type[j];
         const double r2 = rc2[ti * (ntypes + 1) + tj];
         const double rc = sqrt(r2);
-        const double xj = x[3 * j], yj = x[3 * j + 1], zj = x[3 * j + 2];
         const int sx0 = static_cast<int>(floor((xi - xj - rc) * iLx)), sx1 = static_cast<int>(ceil((xi - xj + rc) * iLx));
         const int sy0 = static_cast<int>(floor((yi - yj - rc) * iLy)), sy1 = static_cast<int>(ceil((yi - yj + rc) * iLy));
         const int sz0 = static_cast<int>(floor((zi - zj - rc) * iLz)), sz1 = static_cast<int>(ceil((zi - zj + rc) * iLz));
