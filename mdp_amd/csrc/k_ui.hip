@@ -15,38 +15,20 @@ namespace mdp {
 
 bool templated_j(int J) { return J == 2 || J == 4 || J == 6 || J == 8; }
 
-template <int J, int RLO, int RHI, int MODE>
+// GEO: the pair's Cayley-Klein parameters and weighted switching factor come precomputed from
+// pgeo[5q..5q+4] = (a_r, a_i, b_r, b_i, sfac * w_j) (the MD pair list writes them once per step with the
+// same pair_geometry), instead of being derived here from r_ij in every row group.
+template <int J, int RLO, int RHI, int MODE, bool GEO = false>
 __device__ __forceinline__ void ui_rows(const SnapDev& s, double* __restrict__ Ur, double* __restrict__ Ui,
                                         const double* __restrict__ rij, const int* __restrict__ pair_off,
                                         const int* __restrict__ perm, const int* __restrict__ ti,
-                                        const int* __restrict__ tj, int N, double wself, int elem_sel, int atom) {
+                                        const int* __restrict__ tj, int N, double wself, int elem_sel, int atom,
+                                        const double* __restrict__ pgeo = nullptr) {
     constexpr int NE = rows_elems(J, RLO, RHI);
     double accr[NE], acci[NE];
 #pragma unroll
     for (int k = 0; k < NE; ++k) { accr[k] = 0.0; acci[k] = 0.0; }
-
-    // the next pair's r_ij and types are loaded while the current pair's recursion runs
-    const int q0 = pair_off[atom], q1 = pair_off[atom + 1];
-    double nx = 0.0, ny = 0.0, nz = 0.0;
-    int nti = 0, ntj = 0;
-    if (q0 < q1) {
-        const int p = perm ? perm[q0] : q0;
-        nx = rij[3 * p]; ny = rij[3 * p + 1]; nz = rij[3 * p + 2];
-        nti = ti[p]; ntj = tj[p];
-    }
-    for (int q = q0; q < q1; ++q) {
-        const double x = nx, y = ny, z = nz;
-        const int tip = nti, tjp = ntj;
-        if (q + 1 < q1) {
-            const int p = perm ? perm[q + 1] : q + 1;
-            nx = rij[3 * p]; ny = rij[3 * p + 1]; nz = rij[3 * p + 2];
-            nti = ti[p]; ntj = tj[p];
-        }
-        if (elem_sel >= 0 && s.map[tjp] != elem_sel) continue;
-        const double rc = (s.radelem[tip] + s.radelem[tjp]) * s.rcutfac;
-        PairGeom g;
-        pair_geometry<false>(x, y, z, rc, s.rmin0, s.rfac0, s.switchflag, s.wjelem[tjp], g);
-        const double sf = g.sfac;
+    auto accumulate = [&](const PairGeom& g, double sf) {
         row_chain<J, 0, RLO, RHI>(g, s.rootpq, [&](auto Jc, auto MBc, auto& pr, auto& pi, auto&, auto&) {
             constexpr int JJ = decltype(Jc)::value, MB = decltype(MBc)::value;
             constexpr int base = rows_slot(J, RLO, JJ, MB, 0);
@@ -56,6 +38,49 @@ __device__ __forceinline__ void ui_rows(const SnapDev& s, double* __restrict__ U
                 acci[base + ma] += sf * pi[ma];
             }
         });
+    };
+
+    // the next pair's data are loaded while the current pair's recursion runs
+    const int q0 = pair_off[atom], q1 = pair_off[atom + 1];
+    if constexpr (GEO) {
+        double n[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (q0 < q1)
+#pragma unroll
+            for (int k = 0; k < 5; ++k) n[k] = pgeo[5 * static_cast<size_t>(q0) + k];
+        for (int q = q0; q < q1; ++q) {
+            PairGeom g;
+            g.ar = n[0];
+            g.ai = n[1];
+            g.br = n[2];
+            g.bi = n[3];
+            const double sf = n[4];
+            if (q + 1 < q1)
+#pragma unroll
+                for (int k = 0; k < 5; ++k) n[k] = pgeo[5 * static_cast<size_t>(q + 1) + k];
+            accumulate(g, sf);
+        }
+    } else {
+        double nx = 0.0, ny = 0.0, nz = 0.0;
+        int nti = 0, ntj = 0;
+        if (q0 < q1) {
+            const int p = perm ? perm[q0] : q0;
+            nx = rij[3 * p]; ny = rij[3 * p + 1]; nz = rij[3 * p + 2];
+            nti = ti[p]; ntj = tj[p];
+        }
+        for (int q = q0; q < q1; ++q) {
+            const double x = nx, y = ny, z = nz;
+            const int tip = nti, tjp = ntj;
+            if (q + 1 < q1) {
+                const int p = perm ? perm[q + 1] : q + 1;
+                nx = rij[3 * p]; ny = rij[3 * p + 1]; nz = rij[3 * p + 2];
+                nti = ti[p]; ntj = tj[p];
+            }
+            if (elem_sel >= 0 && s.map[tjp] != elem_sel) continue;
+            const double rc = (s.radelem[tip] + s.radelem[tjp]) * s.rcutfac;
+            PairGeom g;
+            pair_geometry<false>(x, y, z, rc, s.rmin0, s.rfac0, s.switchflag, s.wjelem[tjp], g);
+            accumulate(g, g.sfac);
+        }
     }
 
     const size_t n = static_cast<size_t>(N);
@@ -99,37 +124,39 @@ __device__ __forceinline__ void ui_rows(const SnapDev& s, double* __restrict__ U
 template <int J>
 constexpr int ui_ngroups() { return J == 8 ? 4 : (J == 6 ? 3 : 1); }
 
-template <int J, int MODE>
+template <int J, int MODE, bool GEO = false>
 __global__ __launch_bounds__(64) void k_ui(SnapDev s, double* __restrict__ Ur, double* __restrict__ Ui,
                                            const double* __restrict__ rij, const int* __restrict__ pair_off,
                                            const int* __restrict__ perm, const int* __restrict__ ti,
-                                           const int* __restrict__ tj, int N, double wself, int elem_sel) {
+                                           const int* __restrict__ tj, int N, double wself, int elem_sel,
+                                           const double* __restrict__ pgeo) {
     constexpr int NG = ui_ngroups<J>();
     const int b = blockIdx.x, k = b / 8;
     const int group = k % NG, ablock = (k / NG) * 8 + b % 8;
     const int atom = ablock * 64 + threadIdx.x;
     if (atom >= N) return;
     if constexpr (J == 8) {
-        if (group == 0) ui_rows<8, 0, 0, MODE>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom);
-        else if (group == 1) ui_rows<8, 1, 1, MODE>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom);
-        else if (group == 2) ui_rows<8, 2, 2, MODE>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom);
-        else ui_rows<8, 3, 4, MODE>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom);
+        if (group == 0) ui_rows<8, 0, 0, MODE, GEO>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom, pgeo);
+        else if (group == 1) ui_rows<8, 1, 1, MODE, GEO>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom, pgeo);
+        else if (group == 2) ui_rows<8, 2, 2, MODE, GEO>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom, pgeo);
+        else ui_rows<8, 3, 4, MODE, GEO>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom, pgeo);
     } else if constexpr (J == 6) {
-        if (group == 0) ui_rows<6, 0, 0, MODE>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom);
-        else if (group == 1) ui_rows<6, 1, 1, MODE>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom);
-        else ui_rows<6, 2, 3, MODE>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom);
+        if (group == 0) ui_rows<6, 0, 0, MODE, GEO>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom, pgeo);
+        else if (group == 1) ui_rows<6, 1, 1, MODE, GEO>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom, pgeo);
+        else ui_rows<6, 2, 3, MODE, GEO>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom, pgeo);
     } else {
-        ui_rows<J, 0, J / 2, MODE>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom);
+        ui_rows<J, 0, J / 2, MODE, GEO>(s, Ur, Ui, rij, pair_off, perm, ti, tj, N, wself, elem_sel, atom, pgeo);
     }
 }
 
-template <int J, int MODE>
+template <int J, int MODE, bool GEO = false>
 static void ui_groups(const SnapDev& s, double* Ur, double* Ui, const double* rij, const int* off, const int* perm,
-                      const int* ti, const int* tj, int N, double wself, int esel, hipStream_t st) {
+                      const int* ti, const int* tj, int N, double wself, int esel, hipStream_t st,
+                      const double* pgeo = nullptr) {
     constexpr int NG = ui_ngroups<J>();
     const int ablocks = ((N + 63) / 64 + 7) / 8 * 8;   // padded to whole XCD rounds; excess blocks return
-    hipLaunchKernelGGL((k_ui<J, MODE>), dim3(ablocks * NG), dim3(64), 0, st, s, Ur, Ui, rij, off, perm, ti, tj, N,
-                       wself, esel);
+    hipLaunchKernelGGL((k_ui<J, MODE, GEO>), dim3(ablocks * NG), dim3(64), 0, st, s, Ur, Ui, rij, off, perm, ti, tj,
+                       N, wself, esel, pgeo);
 }
 
 void launch_ui(const SnapDev& s, UiMode mode, double* Ur, double* Ui, const double* rij, const int* pair_off,
@@ -152,6 +179,18 @@ void launch_ui(const SnapDev& s, UiMode mode, double* Ur, double* Ui, const doub
         default: break;
     }
 #undef UI_CASE
+}
+
+void launch_ui_geo(const SnapDev& s, double* Ur, double* Ui, const double* pgeo, const int* pair_off, int N,
+                   double wself, hipStream_t st) {
+    if (N <= 0) return;
+    switch (s.J) {
+        case 2: ui_groups<2, UI_HALF_WSELF, true>(s, Ur, Ui, nullptr, pair_off, nullptr, nullptr, nullptr, N, wself, -1, st, pgeo); break;
+        case 4: ui_groups<4, UI_HALF_WSELF, true>(s, Ur, Ui, nullptr, pair_off, nullptr, nullptr, nullptr, N, wself, -1, st, pgeo); break;
+        case 6: ui_groups<6, UI_HALF_WSELF, true>(s, Ur, Ui, nullptr, pair_off, nullptr, nullptr, nullptr, N, wself, -1, st, pgeo); break;
+        case 8: ui_groups<8, UI_HALF_WSELF, true>(s, Ur, Ui, nullptr, pair_off, nullptr, nullptr, nullptr, N, wself, -1, st, pgeo); break;
+        default: break;
+    }
 }
 
 }  // namespace mdp

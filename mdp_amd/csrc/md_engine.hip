@@ -315,7 +315,7 @@ __global__ __launch_bounds__(256) void k_pair_fill_md(const double* __restrict__
                                                       int* __restrict__ pj, int* __restrict__ pti,
                                                       int* __restrict__ ptj, double* __restrict__ prij,
                                                       int* __restrict__ entry_pair, double Lx, double Ly, double Lz,
-                                                      int N) {
+                                                      int N, SnapDev sd, double* __restrict__ pgeo) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = t / PSUB, sub = t % PSUB, lane = threadIdx.x & 63;
     const unsigned long long group = 0xFFFFull << (lane & ~(PSUB - 1));
@@ -349,6 +349,16 @@ __global__ __launch_bounds__(256) void k_pair_fill_md(const double* __restrict__
                 prij[3 * static_cast<size_t>(q)] = dx;
                 prij[3 * static_cast<size_t>(q) + 1] = dy;
                 prij[3 * static_cast<size_t>(q) + 2] = dz;
+                // Cayley-Klein parameters and weighted switching factor, read by compute_ui
+                PairGeom g;
+                pair_geometry<false>(dx, dy, dz, (sd.radelem[ti] + sd.radelem[tjv]) * sd.rcutfac, sd.rmin0, sd.rfac0,
+                                     sd.switchflag, sd.wjelem[tjv], g);
+                double* __restrict__ o = pgeo + 5 * static_cast<size_t>(q);
+                o[0] = g.ar;
+                o[1] = g.ai;
+                o[2] = g.br;
+                o[3] = g.bi;
+                o[4] = g.sfac;
                 entry_pair[e] = q;
             } else {
                 entry_pair[e] = -1;
@@ -481,7 +491,7 @@ struct mdp_md {
     // device state
     DevBuf x, v, f, type, massinv, xref, cell_of, cell_cnt, cell_start, cell_fill, cell_atoms, scan_tmp, nb_cnt, nb_j,
         nb_s, in_cnt, in_e, pcnt, poff, pi, pj, pti, ptj, prij, entry_pair, Ur, Ui, Yr, Yi, epart, dedr, eatom, vir,
-        beta_t, rc2, flags, maxd2, row_order, yt, pair_sum;
+        beta_t, rc2, flags, maxd2, row_order, yt, pair_sum, pgeo;
     long pair_steps0 = 0;   // force evaluations counted in pair_sum since the last reset
     long stats[6] = {0, 0, 0, 0, 0, 0};
     bool forces_current = false;
@@ -672,6 +682,7 @@ struct mdp_md {
         pti.ensure(sizeof(int) * c);
         ptj.ensure(sizeof(int) * c);
         prij.ensure(sizeof(double) * 3 * c);
+        pgeo.ensure(sizeof(double) * 5 * c);
         dedr.ensure(sizeof(double) * 3 * c);
     }
 
@@ -737,7 +748,7 @@ struct mdp_md {
         hipLaunchKernelGGL(k_pair_fill_md, dim3(nb(static_cast<long>(N) * PSUB, 256)), dim3(256), 0, st, x.as<double>(), type.as<int>(),
                            nb_cnt.as<int>(), nb_j.as<int>(), nb_s.as<int>(), rc2.as<double>(), ntypes, poff.as<int>(),
                            pi.as<int>(), pj.as<int>(), pti.as<int>(), ptj.as<int>(), prij.as<double>(),
-                           entry_pair.as<int>(), L[0], L[1], L[2], N);
+                           entry_pair.as<int>(), L[0], L[1], L[2], N, tab.view, pgeo.as<double>());
     }
 
     void dedr_launch(int variant) {
@@ -762,8 +773,7 @@ struct mdp_md {
         if (!vo) vo = vir.as<double>();
         if (!eo) eo = eatom.as<double>();
         timed(T_UI, [&] {
-            launch_ui(s, UI_HALF_WSELF, Ur.as<double>(), Ui.as<double>(), prij.as<double>(), poff.as<int>(), nullptr,
-                      pti.as<int>(), ptj.as<int>(), N, 1.0, -1, st);
+            launch_ui_geo(s, Ur.as<double>(), Ui.as<double>(), pgeo.as<double>(), poff.as<int>(), N, 1.0, st);
         });
         timed(T_YI, [&] {
             launch_yi_half(s, Ur.as<double>(), Ui.as<double>(), beta_t.as<double>(), Yr.as<double>(), Yi.as<double>(),
@@ -862,8 +872,7 @@ struct mdp_md {
         const SnapDev& s4 = tab_s.view;
         ++force_evals;
         timed(T_SUR, [&] {
-            launch_ui(s4, UI_HALF_WSELF, sUr.as<double>(), sUi.as<double>(), prij.as<double>(), poff.as<int>(), nullptr,
-                      pti.as<int>(), ptj.as<int>(), N, 1.0, -1, st);
+            launch_ui_geo(s4, sUr.as<double>(), sUi.as<double>(), pgeo.as<double>(), poff.as<int>(), N, 1.0, st);
             launch_b_half(s4, sUr.as<double>(), sUi.as<double>(), sB.as<double>(), N, 0, st);
             launch_mlp_eval(sPacked.as<unsigned short>(), sParams.as<float>(), sAff.as<double>(), sur_nin,
                             sB.as<double>(), N, sE.as<double>(), sG.as<double>(), st);

@@ -39,6 +39,11 @@ enum UiMode { UI_ACCUMULATE = 0, UI_FULL_WSELF = 1, UI_HALF_WSELF = 2 };
 void launch_ui(const SnapDev& s, UiMode mode, double* Ur, double* Ui, const double* rij, const int* pair_off,
                const int* perm, const int* ti, const int* tj, int natoms, double wself, int elem_sel,
                hipStream_t st);
+// compute_ui on the half layout with wself (UI_HALF_WSELF) from per-pair geometry written by the MD
+// pair list: pgeo[5p..5p+4] = (a_r, a_i, b_r, b_i, sfac * w_j) of pair p, pairs of atom i in
+// [pair_off[i], pair_off[i+1]).
+void launch_ui_geo(const SnapDev& s, double* Ur, double* Ui, const double* pgeo, const int* pair_off, int natoms,
+                   double wself, hipStream_t st);
 
 // Y[half rows] of every atom (Y zeroed by the caller where untouched).  beta_t[triple] holds the
 // effective weight of each coupling triple (multiplicity, (j1+1)/(j+1) and 1/(j+1) factors folded
