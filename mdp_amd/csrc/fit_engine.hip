@@ -39,7 +39,8 @@ struct mdp_fit {
     std::vector<int> a0_h;   // [C+1]
     std::vector<double> boxes_h;
     DevBuf x, type, cfg_of, cfg_a0, cfg_L, cfg_p0, rc2, pcnt, poff, scan_tmp, flags, counter;
-    DevBuf pi, pj, pti, ptj, rij, Ur, Ui, zt, blist, dblist, bzero, t_lo, t_b, t_z, t_w, zorder, t_wtab;
+    DevBuf pi, pj, pti, ptj, rij, Ur, Ui, zt, blist, dblist, bzero, t_lo, t_b, t_z, t_w, zorder, t_wtab, t_hlo, t_hb, t_hz,
+        t_hw;
     int dbdr_variant = 0;
     bool zb_lds = false;     // Z + B from the LDS-staged half U (twojmax 4, 6, 8)
     long cap = 0;            // pair capacity of the pair and dB/dr buffers
@@ -147,6 +148,19 @@ struct mdp_fit {
                     }
             put(t_wtab, wt.data(), sizeof(double) * wt.size());
         }
+        {   // the term table split into two halves of the bispectrum index range (dB/dr variant 4)
+            std::vector<int> hl, hbv, hz;
+            std::vector<double> hw;
+            terms.hb = fit_term_table_halves(J, bnorm, hl, hbv, hz, hw);
+            put(t_hlo, hl.data(), sizeof(int) * hl.size());
+            put(t_hb, hbv.data(), sizeof(int) * hbv.size());
+            put(t_hz, hz.data(), sizeof(int) * hz.size());
+            put(t_hw, hw.data(), sizeof(double) * hw.size());
+            terms.h_lo = t_hlo.as<int>();
+            terms.h_b = t_hb.as<int>();
+            terms.h_z = t_hz.as<int>();
+            terms.h_w = t_hw.as<double>();
+        }
         terms.wtab = t_wtab.as<double>();
         terms.lo = t_lo.as<int>();
         terms.b = t_b.as<int>();
@@ -233,7 +247,7 @@ struct mdp_fit {
                                   bzero.as<double>(), cfg_of.as<int>(), cfg_a0.as<int>(), N, bnorm, bzeroflag, st);
                 break;
             default:
-                launch_fill_zero_int(counter.as<int>(), 1, st);
+                launch_fill_zero_int(counter.as<int>(), 2, st);
                 launch_fit_dbdr(s, zt.as<double>(), terms, rij.as<double>(), pti.as<int>(), ptj.as<int>(),
                                 poff.as<int>(), cfg_of.as<int>(), cfg_p0.as<int>(), dblist.as<double>(),
                                 counter.as<int>(), N, dbdr_groups, st, dbdr_variant);
@@ -284,6 +298,7 @@ struct mdp_fit {
             const std::string v(fv);
             const bool j468 = (J == 4 || J == 6 || J == 8);
             dbdr_variant = (v == "1" && j468) ? 1 : ((v == "2" && (J == 6 || J == 8)) ? 2 : ((v == "3" && j468) ? 3 : 0));
+            if (v == "4" && j468) dbdr_variant = 4;
         }
         if ((env && std::string(env) == "0") || fv) return;
         int props_cu = 256;
@@ -291,12 +306,12 @@ struct mdp_fit {
         int dev = 0;
         if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
             props_cu = prop.multiProcessorCount;
-        const int cand[] = {props_cu * 2, props_cu * 4, props_cu * 5, props_cu * 8};
+        const int cand[] = {props_cu * 2, props_cu * 4, props_cu * 8, props_cu * 12};
         // dB/dr variants: 0 table walk, 1 unrolled terms, 2 table walk at two waves per SIMD, 3 variant 0
-        // with the term loop pipelined
+        // with the term loop pipelined, 4 generator form with the bispectrum indices split over two waves
         std::vector<int> vars = {0};
-        if (J == 4) vars = {0, 1, 3};
-        if (J == 6 || J == 8) vars = {0, 1, 2, 3};
+        if (J == 4) vars = {0, 1, 3, 4};
+        if (J == 6 || J == 8) vars = {0, 1, 2, 3, 4};
         double best = 1e30;
         int pick = dbdr_groups, pickv = 0;
         std::string js = "{\"fit_dbdr\": {";

@@ -41,7 +41,17 @@ struct FitTerms {
     const int* z = nullptr;       // Z block offset (idxz ordering)
     const double* w = nullptr;    // weight
     const double* wtab = nullptr; // weights by (bispectrum index, term): wtab[3*t + term]
+    // the same terms split into two halves of the bispectrum index range (b < hb and b >= hb, of about
+    // equal contraction work), each grouped by layer: half h's terms of layer l are h_lo[h*(J+2) + l] ..
+    const int* h_lo = nullptr;
+    const int* h_b = nullptr;
+    const int* h_z = nullptr;
+    const double* h_w = nullptr;
+    int hb = 0;
 };
+// Host term table split into two halves of the bispectrum index range (FitTerms::h_*): returns hb.
+int fit_term_table_halves(int J, int bnorm, std::vector<int>& lo, std::vector<int>& b, std::vector<int>& z,
+                          std::vector<double>& w);
 // Host term table of the dB/dr contraction, grouped by the layer whose dU it contracts.
 void fit_term_table(int J, int bnorm, std::vector<int>& lo, std::vector<int>& b, std::vector<int>& z,
                     std::vector<double>& w);

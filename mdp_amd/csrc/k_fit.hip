@@ -364,6 +364,118 @@ __global__ __launch_bounds__(64, WPE) void k_fit_dbdr(const double* __restrict__
     }
 }
 
+// dB/dr with dU/dr_c built from U alone through the SU(2) generators (as the MD dE/dr):
+// dU/dx_c = sum_g (dq/dx_c . V_g) L_g U, so each lane runs only the U recursion.  The bispectrum indices
+// are split into two halves (FitTerms::hb); workgroup b (one wave) serves half b % 2 and takes atoms from
+// that half's counter, so its accumulators need half the LDS and more waves fit on a CU.  Lanes, passes,
+// term walk (pipelined) and output layout as k_fit_dbdr.
+template <int J>
+__global__ __launch_bounds__(64) void k_fit_dbdr_g(const double* __restrict__ rootpq, const double* __restrict__ zt,
+                                                   const int* __restrict__ h_lo, const int* __restrict__ h_b,
+                                                   const int* __restrict__ h_z, const double* __restrict__ h_w, int hb,
+                                                   const double* __restrict__ radelem,
+                                                   const double* __restrict__ wjelem, double rcutfac, double rmin0,
+                                                   double rfac0, int switchflag, const double* __restrict__ rij,
+                                                   const int* __restrict__ pti, const int* __restrict__ ptj,
+                                                   const int* __restrict__ poff, const int* __restrict__ cfg_of,
+                                                   const int* __restrict__ cfg_p0, double* __restrict__ dblist,
+                                                   int* __restrict__ counter, int N) {
+    constexpr int IZ = n_z(J), IB = n_b(J);
+    extern __shared__ double acc[];   // max(hb, IB - hb) * 64 doubles, sized at launch
+    const int lane = threadIdx.x;
+    const int comp = lane % 3, slot = lane / 3;
+    const int half = static_cast<int>(blockIdx.x & 1);
+    const int b0 = half ? hb : 0, nbh = half ? IB - hb : hb;
+    const int* __restrict__ lo = h_lo + half * (J + 2);
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(counter + half, 1);
+        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+        if (t >= N) break;
+        const int atom = t;
+        const int q0 = __builtin_amdgcn_readfirstlane(poff[atom]);
+        const int q1 = __builtin_amdgcn_readfirstlane(poff[atom + 1]);
+        const int c = __builtin_amdgcn_readfirstlane(cfg_of[atom]);
+        const int p0 = __builtin_amdgcn_readfirstlane(cfg_p0[c]);
+        const long Pc = __builtin_amdgcn_readfirstlane(cfg_p0[c + 1]) - p0;
+        const double* __restrict__ z = zt + static_cast<size_t>(atom) * IZ * 2;
+        double* __restrict__ db = dblist + static_cast<size_t>(p0) * 3 * IB;
+        for (int qb = q0; qb < q1; qb += 21) {
+            const int q = qb + slot;
+            const bool active = lane < 63 && q < q1;
+            const int p = active ? q : q0;   // idle lanes recompute a valid pair and store nothing
+            const int tjp = ptj[p];
+            const double rc = (radelem[pti[p]] + radelem[tjp]) * rcutfac;
+            PairGeom g;
+            pair_geometry<true>(rij[3 * p], rij[3 * p + 1], rij[3 * p + 2], rc, rmin0, rfac0, switchflag, wjelem[tjp], g);
+            // projections of dq/dx_comp on V_0 = q*i, V_1 = q*j, V_2 = q*k
+            const double dar = comp == 0 ? g.dar[0] : (comp == 1 ? g.dar[1] : g.dar[2]);
+            const double dai = comp == 0 ? g.dai[0] : (comp == 1 ? g.dai[1] : g.dai[2]);
+            const double dbr = comp == 0 ? g.dbr[0] : (comp == 1 ? g.dbr[1] : g.dbr[2]);
+            const double dbi = comp == 0 ? g.dbi[0] : (comp == 1 ? g.dbi[1] : g.dbi[2]);
+            const double uk = comp == 0 ? g.u[0] : (comp == 1 ? g.u[1] : g.u[2]);
+            const double sf = g.sfac, dsu = g.dsfac * uk;
+            const double k0 = sf * (-g.ai * dar + g.ar * dai + g.bi * dbr - g.br * dbi);
+            const double k1 = sf * (-g.br * dar - g.bi * dai + g.ar * dbr + g.ai * dbi);
+            const double k2 = sf * (-g.bi * dar + g.br * dai - g.ai * dbr + g.ar * dbi);
+            for (int k = 0; k < nbh; ++k) acc[k * 64 + lane] = 0.0;
+            row_chain<J, 0, 0, J / 2>(g, rootpq, [&](auto Jc, auto MBc, auto& pr, auto& pim, auto&, auto&) {
+                constexpr int L = decltype(Jc)::value, MB = decltype(MBc)::value;
+                constexpr int LAST = (L == 2 * MB) ? MB : L;
+                double dr[LAST + 1], di[LAST + 1];
+                static_for<0, LAST + 1>([&](auto MAc) {
+                    constexpr int MA = decltype(MAc)::value;
+                    // (L_0 U) = i (2ma - l) U; (L_1 U) = c(ma) U+ - c(ma-1) U-; (L_2 U) = i (c(ma) U+ + c(ma-1) U-)
+                    constexpr double m0 = static_cast<double>(2 * MA - L);
+                    constexpr double cu = (MA < L) ? cx_sqrt(static_cast<double>((MA + 1) * (L - MA))) : 0.0;
+                    constexpr double cd = (MA > 0) ? cx_sqrt(static_cast<double>(MA * (L - MA + 1))) : 0.0;
+                    const double upr = (MA < L) ? pr[MA < L ? MA + 1 : MA] : 0.0;
+                    const double upi = (MA < L) ? pim[MA < L ? MA + 1 : MA] : 0.0;
+                    const double dnr = (MA > 0) ? pr[MA > 0 ? MA - 1 : 0] : 0.0;
+                    const double dni = (MA > 0) ? pim[MA > 0 ? MA - 1 : 0] : 0.0;
+                    const double l1r = cu * upr - cd * dnr, l1i = cu * upi - cd * dni;
+                    const double l2r = -(cu * upi + cd * dni), l2i = cu * upr + cd * dnr;
+                    dr[MA] = dsu * pr[MA] + k0 * (-m0 * pim[MA]) + k1 * l1r + k2 * l2r;
+                    di[MA] = dsu * pim[MA] + k0 * (m0 * pr[MA]) + k1 * l1i + k2 * l2i;
+                });
+                if constexpr (L == 2 * MB) {   // middle column of an even layer counts 1/2
+                    dr[MB] *= 0.5;
+                    di[MB] *= 0.5;
+                }
+                const int ka = lo[L], kb = lo[L + 1];
+                if (ka < kb) {
+                    double zn[2 * (LAST + 1)], wn;
+                    int bn;
+                    auto fetch = [&](int k) {
+                        const double* __restrict__ zr = z + 2 * (h_z[k] + (L + 1) * MB);
+#pragma unroll
+                        for (int m = 0; m < 2 * (LAST + 1); ++m) zn[m] = zr[m];
+                        bn = h_b[k] - b0;
+                        wn = h_w[k];
+                    };
+                    fetch(ka);
+                    for (int k = ka; k < kb; ++k) {
+                        double zc[2 * (LAST + 1)];
+#pragma unroll
+                        for (int m = 0; m < 2 * (LAST + 1); ++m) zc[m] = zn[m];
+                        const int bc = bn;
+                        const double wc = wn;
+                        if (k + 1 < kb) fetch(k + 1);
+                        double sacc = 0.0;
+#pragma unroll
+                        for (int ma = 0; ma <= LAST; ++ma) sacc += dr[ma] * zc[2 * ma] + di[ma] * zc[2 * ma + 1];
+                        acc[bc * 64 + lane] += wc * sacc;
+                    }
+                }
+            });
+            if (active) {
+                const size_t lp = static_cast<size_t>(q - p0);
+                for (int k = 0; k < nbh; ++k) db[lp + Pc * comp + 3 * Pc * (b0 + k)] = acc[k * 64 + lane];
+            }
+        }
+    }
+}
+
 // As k_fit_dbdr with the terms of every row enumerated at compile time: each term's Z row is a
 // contiguous run of the atom's Z block at a constant offset, so it arrives in wide scalar loads, and
 // the accumulator of each term sits at a constant LDS offset.  wtab[3*t + w] = weight of term w of t.
@@ -489,6 +601,40 @@ void fit_term_table(int J, int bnorm, std::vector<int>& lo, std::vector<int>& tb
     for (int l = 0; l <= J; ++l) lo[l + 1] += lo[l];
 }
 
+int fit_term_table_halves(int J, int bnorm, std::vector<int>& lo, std::vector<int>& tb, std::vector<int>& tz,
+                          std::vector<double>& tw) {
+    std::vector<int> lo1, b1, z1;
+    std::vector<double> w1;
+    fit_term_table(J, bnorm, lo1, b1, z1, w1);
+    // contraction work of each bispectrum index: the half-domain size of each of its terms' layers
+    int nb = 0;
+    for (int v : b1) nb = std::max(nb, v + 1);
+    std::vector<long> work(nb, 0);
+    for (int l = 0; l <= J; ++l)
+        for (int k = lo1[l]; k < lo1[l + 1]; ++k) work[b1[k]] += static_cast<long>(l / 2 + 1) * (l + 1);
+    long tot = 0;
+    for (long v : work) tot += v;
+    int hb = 0;
+    long acc = 0;
+    while (hb < nb && 2 * (acc + work[hb]) <= tot) acc += work[hb++];
+    if (hb == 0) hb = 1;
+    lo.assign(2 * (J + 2), 0);
+    tb.clear(); tz.clear(); tw.clear();
+    for (int h = 0; h < 2; ++h) {
+        lo[h * (J + 2)] = static_cast<int>(tb.size());
+        for (int l = 0; l <= J; ++l) {
+            for (int k = lo1[l]; k < lo1[l + 1]; ++k)
+                if ((b1[k] >= hb) == (h == 1)) {
+                    tb.push_back(b1[k]);
+                    tz.push_back(z1[k]);
+                    tw.push_back(w1[k]);
+                }
+            lo[h * (J + 2) + l + 1] = static_cast<int>(tb.size());
+        }
+    }
+    return hb;
+}
+
 // ---- launchers ----------------------------------------------------------------------------------
 
 void launch_fit_pairs(bool fill, const double* x, const int* type, const int* cfg_of, const int* cfg_a0,
@@ -551,6 +697,23 @@ void launch_fit_dbdr(const SnapDev& s, const double* zt, const FitTerms& tt, con
             D2_CASE(6)
             D2_CASE(8)
 #undef D2_CASE
+            default: break;
+        }
+    }
+    if (variant == 4 && tt.h_lo) {   // generator form, bispectrum indices split over two wave halves
+        const int G2 = (G + 1) / 2 * 2;
+        const size_t lds = sizeof(double) * 64 * static_cast<size_t>(std::max(tt.hb, s.idxb_max - tt.hb));
+        switch (s.J) {
+#define D4_CASE(JV)                                                                                             \
+    case JV:                                                                                                    \
+        hipLaunchKernelGGL((k_fit_dbdr_g<JV>), dim3(G2), dim3(64), lds, st, s.rootpq, zt, tt.h_lo, tt.h_b, tt.h_z, \
+                           tt.h_w, tt.hb, s.radelem, s.wjelem, s.rcutfac, s.rmin0, s.rfac0, s.switchflag, rij, pti, \
+                           ptj, poff, cfg_of, cfg_p0, dblist, counter, N);                                       \
+        return;
+            D4_CASE(4)
+            D4_CASE(6)
+            D4_CASE(8)
+#undef D4_CASE
             default: break;
         }
     }

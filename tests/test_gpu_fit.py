@@ -83,7 +83,7 @@ def _check_config(oracle, fit, c, cfgs, b, db, ai, aj, rij, J, bnorm=0, bzero=1)
 
 
 @pytest.mark.parametrize("J,variant", [(4, "0"), (6, "0"), (8, "0"), (4, "1"), (6, "1"), (8, "1"), (8, "2"),
-                                       (4, "3"), (8, "3")])
+                                       (4, "3"), (8, "3"), (4, "4"), (6, "4"), (8, "4")])
 def test_small_batch_vs_oracle(oracle, monkeypatch, J, variant):
     from mdp_amd.fit import SnapFit
     monkeypatch.setenv("MDP_FIT_DBDR_VARIANT", variant)   # each dB/dr kernel variant in turn
