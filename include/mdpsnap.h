@@ -16,6 +16,8 @@
  *
  * Part 3 computes fitting descriptors (B and dB/dr) of many configurations in one batch.
  * Part 4 holds the kernels of the learned surrogate (per-atom MLP over bispectrum descriptors).
+ * Part 5 is the backend-3 (HIP) branch of the upstream C++ core dispatch (src/C++/Common/mdpcore.h) for
+ * the SNAP force path: device arrays in and out; include/mdphipcore.h wraps it as the hip* templates.
  *
  * Errors: the drop-in functions return void like upstream; a failure leaves a message readable
  * with mdp_snap_last_error() (empty string when the last call succeeded).  Part 2 returns 0 on
@@ -313,6 +315,39 @@ int mdp_sur_energy(const double *e, const int *cfg_off, const double *eref, int 
                    double *alpha, void *stream);
 /* partials[b] (b < nblocks): block sums of v^2 */
 int mdp_sumsq(const double *v, long n, double *partials, int nblocks, void *stream);
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Part 5: backend-3 entry points of src/C++/Common/mdpcore.h (SNAP force path, device arrays)      */
+/* ---------------------------------------------------------------------------------------------- */
+/* Same argument lists as the dispatch's `backend == 3` branches (without `backend`), dstype = double,
+ * every pointer a device pointer.  Semantics of the upstream C++ core (src/C++/Core/cpuSnap4.cpp):
+ * Ui and AddWself2Ui, Ei and Fi accumulate into their outputs, Yi zeroes and writes; the element of
+ * atom i is map[type[i]] in Ei and Yi; Fi writes forces AoS and the virial SoA vatom[k*anum + atom]
+ * with -r(x)dE/2 added to both atoms of each pair.  Small index and table arrays are read back to the
+ * host once per call; the large arrays are used in place.  Errors: mdp_snap_last_error(). */
+/* replaces hipSnapComputeUi in SnapComputeUi (mdpcore.h:3093-3116) */
+void mdp_hipSnapComputeUi(double *Stotr, double *Stoti, double *rootpqarray, double *rij, double *wjelem,
+                          double *radelem, double rmin0, double rfac0, double rcutfac, int *idxu_block, int *map,
+                          int *ai, int *ti, int *tj, int twojmax, int idxu_max, int inum, int ijnum, int switchflag,
+                          int chemflag);
+/* replaces hipAddWself2Ui in AddWself2Ui (mdpcore.h:3118-3140) */
+void mdp_hipAddWself2Ui(double *ulisttot_r, double *ulisttot_i, double wself, int *idxu_block, int *type, int *map,
+                        int *ai, int wselfall_flag, int chemflag, int idxu_max, int nelements, int twojmax, int inum);
+/* replaces hipSnapComputeEi in SnapComputeEi (mdpcore.h:3167-3194) */
+void mdp_hipSnapComputeEi(double *eatom, double *ulisttot_r, double *ulisttot_i, double *cglist, double *bzero,
+                          double *coeffelem, int *ilist, int *map, int *type, int *idxb, int *idxcg_block,
+                          int *idxu_block, int twojmax, int idxb_max, int idxu_max, int nelements, int ncoeffall,
+                          int bnorm_flag, int bzero_flag, int wselfall_flag, int quadraticflag, int inum);
+/* replaces hipSnapComputeYi in the coefficient form of SnapComputeYi (mdpcore.h:3221-3244) */
+void mdp_hipSnapComputeYi(double *ylist_r, double *ylist_i, double *ulisttot_r, double *ulisttot_i, double *cglist,
+                          double *betalist, int *map, int *type, int *idxz, int *idxb_block, int *idxu_block,
+                          int *idxcg_block, int twojmax, int idxb_max, int idxu_max, int idxz_max, int nelements,
+                          int ncoeffall, int bnorm_flag, int inum);
+/* replaces hipSnapComputeFi in SnapComputeFi (mdpcore.h:3246-3269) */
+void mdp_hipSnapComputeFi(double *fatom, double *vatom, double *ylist_r, double *ylist_i, double *rootpqarray,
+                          double *rij, double *wjelem, double *radelem, double rmin0, double rfac0, double rcutfac,
+                          int *idxu_block, int *map, int *aii, int *ai, int *aj, int *ti, int *tj, int twojmax,
+                          int idxu_max, int inum, int anum, int ijnum, int switchflag, int chemflag);
 
 #ifdef __cplusplus
 }

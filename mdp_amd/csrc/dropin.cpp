@@ -36,6 +36,27 @@ template <class T>
 void get(T* h, const T* d, size_t n, hipStream_t st) {
     if (n) MDP_HIP(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, st));
 }
+
+// Argument I/O of the shared bodies below.  The cpuSnap.h entry points pass host arrays, staged through
+// the per-slot device buffers.  The backend-3 entry points (hip*, mdpcore.h) pass device arrays: their
+// large arrays (U, Y, r_ij, forces, virials, energies) are then used in place, while the small index
+// and table arrays arrive as host copies and are staged as usual.
+thread_local bool g_devio = false;
+// virial layout of the force tally: R1 AoS vatom[6i + k] (g_vsoa < 0) or, for the backend-3 SnapComputeFi,
+// SoA vatom[k * anum + i] with -r(x)dE/2 added to both atoms of the pair (g_vsoa = anum)
+thread_local int g_vsoa = -1;
+template <class T>
+T* put_arg(int slot, const T* p, size_t n, hipStream_t st) {
+    return g_devio ? const_cast<T*>(p) : put(slot, p, n, st);
+}
+template <class T>
+T* out_arg(int slot, T* p, size_t n) {
+    return g_devio ? p : dev<T>(slot, n);
+}
+template <class T>
+void get_arg(T* h, const T* d, size_t n, hipStream_t st) {
+    if (!g_devio) get(h, d, n, st);
+}
 void finish(hipStream_t st) {
     MDP_HIP(hipGetLastError());
     MDP_HIP(hipStreamSynchronize(st));
@@ -215,9 +236,9 @@ void cpuSnapComputeUi(double* Utotr, double* Utoti, double* rootpqarray, double*
         std::vector<int> off, perm;
         pairs_by_atom(aii, ijnum, inum, off, perm);
         const size_t nu = static_cast<size_t>(inum) * idxu_max * nelem;
-        double* d_ur = put(0, Utotr, nu, st);
-        double* d_ui = put(1, Utoti, nu, st);
-        double* d_rij = put(2, rij, 3 * static_cast<size_t>(ijnum), st);
+        double* d_ur = put_arg(0, Utotr, nu, st);
+        double* d_ui = put_arg(1, Utoti, nu, st);
+        double* d_rij = put_arg(2, rij, 3 * static_cast<size_t>(ijnum), st);
         int* d_ti = put(3, ti, ijnum, st);
         int* d_tj = put(4, tj, ijnum, st);
         int* d_off = put(5, off.data(), off.size(), st);
@@ -230,8 +251,8 @@ void cpuSnapComputeUi(double* Utotr, double* Utoti, double* rootpqarray, double*
                 launch_ui(s, UI_ACCUMULATE, d_ur + o, d_ui + o, d_rij, d_off, d_perm, d_ti, d_tj, inum, 0.0, e, st);
             }
         }
-        get(Utotr, d_ur, nu, st);
-        get(Utoti, d_ui, nu, st);
+        get_arg(Utotr, d_ur, nu, st);
+        get_arg(Utoti, d_ui, nu, st);
         finish(st);
     });
 }
@@ -246,12 +267,12 @@ void cpuAddWself2Ui(double* Utotr, double* Utoti, double wself, int* idxu_block,
         SnapDev s = tables(twojmax, nullptr, nullptr, nullptr, nullptr, map, ntypes, st);
         if (idxu_max != s.idxu_max) throw Error("idxu_max does not match twojmax");
         const size_t nu = static_cast<size_t>(inum) * idxu_max * nelements;
-        double* d_ur = put(0, Utotr, nu, st);
-        double* d_ui = put(1, Utoti, nu, st);
+        double* d_ur = put_arg(0, Utotr, nu, st);
+        double* d_ui = put_arg(1, Utoti, nu, st);
         int* d_t = put(2, type, inum, st);
         launch_add_wself(s, d_ur, d_ui, wself, d_t, wselfall_flag, chemflag, nelements, inum, st);
-        get(Utotr, d_ur, nu, st);
-        get(Utoti, d_ui, nu, st);
+        get_arg(Utotr, d_ur, nu, st);
+        get_arg(Utoti, d_ui, nu, st);
         finish(st);
     });
 }
@@ -285,18 +306,18 @@ void cpuSnapComputeEi(double* eatom, double* Utotr, double* Utoti, double* cglis
         const size_t nu = static_cast<size_t>(inum) * idxu_max * ne;
         const int ncoef_rows = chemflag ? max_of(map + 1, ntypes, 0) + 1 : 1;
         const int eext = max_of(ilist, inum) + 1;
-        double* d_ur = put(0, Utotr, nu, st);
-        double* d_ui = put(1, Utoti, nu, st);
+        double* d_ur = put_arg(0, Utotr, nu, st);
+        double* d_ui = put_arg(1, Utoti, nu, st);
         double* d_bz = put(2, bzero, twojmax + 1, st);
         double* d_c = put(3, coeffelem, static_cast<size_t>(ncoeffall) * ncoef_rows, st);
         int* d_il = put(4, ilist, inum, st);
         int* d_t = put(5, type, inum, st);
-        double* d_e = put(6, eatom, eext, st);
+        double* d_e = put_arg(6, eatom, eext, st);
         double* d_b = dev<double>(7, static_cast<size_t>(inum) * idxb_max * ne * ne * ne);
         all_triples_b(s, d_ur, d_ui, d_b, ne, inum, bnorm_flag, st);
         if (bzero_flag) launch_bzero(s, d_b, d_bz, d_il, d_t, wselfall_flag, chemflag, ne, inum, 0, st);
         launch_ei_from_b(s, d_e, d_b, d_c, d_il, d_t, chemflag, ne, ncoeffall, inum, st);
-        get(eatom, d_e, eext, st);
+        get_arg(eatom, d_e, eext, st);
         finish(st);
     });
 }
@@ -338,10 +359,10 @@ void cpuSnapComputeYi(double* ylist_r, double* ylist_i, double* Utotr, double* U
         const SnapIndex& ix = index_for(twojmax);
         const int ne = nelements;
         const size_t nu = static_cast<size_t>(inum) * idxu_max * ne;
-        double* d_ur = put(0, Utotr, nu, st);
-        double* d_ui = put(1, Utoti, nu, st);
-        double* d_yr = dev<double>(2, nu);
-        double* d_yi = dev<double>(3, nu);
+        double* d_ur = put_arg(0, Utotr, nu, st);
+        double* d_ui = put_arg(1, Utoti, nu, st);
+        double* d_yr = out_arg(2, ylist_r, nu);
+        double* d_yi = out_arg(3, ylist_i, nu);
         MDP_HIP(hipMemsetAsync(d_yr, 0, sizeof(double) * nu, st));
         MDP_HIP(hipMemsetAsync(d_yi, 0, sizeof(double) * nu, st));
         if (ne == 1 && !chemflag) {
@@ -369,8 +390,8 @@ void cpuSnapComputeYi(double* ylist_r, double* ylist_i, double* Utotr, double* U
                         }
             MDP_HIP(hipStreamSynchronize(st));
         }
-        get(ylist_r, d_yr, nu, st);
-        get(ylist_i, d_yi, nu, st);
+        get_arg(ylist_r, d_yr, nu, st);
+        get_arg(ylist_i, d_yi, nu, st);
         finish(st);
     });
 }
@@ -399,9 +420,9 @@ void cpuSnapComputeFi(double* fatom, double* vatom, double* ylist_r, double* yli
             yb.resize(ijnum);
             for (int p = 0; p < ijnum; ++p) yb[p] = static_cast<long>(inum) * idxu_max * map[tj[p]];
         }
-        double* d_yr = put(0, ylist_r, nu, st);
-        double* d_yi = put(1, ylist_i, nu, st);
-        double* d_rij = put(2, rij, 3 * static_cast<size_t>(ijnum), st);
+        double* d_yr = put_arg(0, ylist_r, nu, st);
+        double* d_yi = put_arg(1, ylist_i, nu, st);
+        double* d_rij = put_arg(2, rij, 3 * static_cast<size_t>(ijnum), st);
         int* d_aii = put(3, aii, ijnum, st);
         int* d_ai = put(4, ai, ijnum, st);
         int* d_aj = put(5, aj, ijnum, st);
@@ -409,12 +430,15 @@ void cpuSnapComputeFi(double* fatom, double* vatom, double* ylist_r, double* yli
         int* d_tj = put(7, tj, ijnum, st);
         long* d_yb = chemflag ? put(8, yb.data(), yb.size(), st) : nullptr;
         double* d_de = dev<double>(9, 3 * static_cast<size_t>(ijnum));
-        double* d_f = put(10, fatom, 3 * static_cast<size_t>(fext), st);
-        double* d_v = put(11, vatom, 6 * static_cast<size_t>(vext), st);
+        double* d_f = put_arg(10, fatom, 3 * static_cast<size_t>(fext), st);
+        double* d_v = put_arg(11, vatom, 6 * static_cast<size_t>(vext), st);
         launch_dedr(s, d_yr, d_yi, d_rij, d_aii, d_ti, d_tj, d_yb, d_de, inum, ijnum, nullptr, st);
-        launch_tally_force_atomic(d_f, d_v, d_de, d_rij, d_ai, d_aj, ijnum, st);
-        get(fatom, d_f, 3 * static_cast<size_t>(fext), st);
-        get(vatom, d_v, 6 * static_cast<size_t>(vext), st);
+        if (g_vsoa >= 0)
+            launch_tally_force_atomic_soa(d_f, d_v, d_de, d_rij, d_ai, d_aj, ijnum, g_vsoa, st);
+        else
+            launch_tally_force_atomic(d_f, d_v, d_de, d_rij, d_ai, d_aj, ijnum, st);
+        get_arg(fatom, d_f, 3 * static_cast<size_t>(fext), st);
+        get_arg(vatom, d_v, 6 * static_cast<size_t>(vext), st);
         finish(st);
     });
 }
@@ -677,6 +701,165 @@ void cpuSnapTallyBispectrumVirial(double* bv, double* bispectrum, double* dbdr, 
         get(bv, d_bv, nbv, st);
         finish(st);
     });
+}
+
+
+}  // extern "C"
+
+// ---- backend-3 entry points of the C++ core dispatch (src/C++/Common/mdpcore.h) -----------------
+// Device arrays in and out, same argument lists as the dispatch's backend branches (without backend).
+// The large arrays are used in place; the small index and table arrays are read back once to drive the
+// same host-side set-up as the cpuSnap.h entry points above.
+
+namespace {
+// device -> host copy of a small argument array (after the caller's device work has finished)
+template <class T>
+std::vector<T> fetch(const T* d, size_t n) {
+    std::vector<T> h(std::max<size_t>(n, 1));
+    if (n) MDP_HIP(hipMemcpy(h.data(), d, sizeof(T) * n, hipMemcpyDeviceToHost));
+    return h;
+}
+// device-I/O scope for one backend-3 call
+struct DevIo {
+    explicit DevIo(int vsoa = -1) { g_devio = true; g_vsoa = vsoa; }
+    ~DevIo() { g_devio = false; g_vsoa = -1; }
+};
+// runs the read-back; returns false (with the library error set) when it fails
+template <class F>
+bool fetch_args(F&& f) {
+    bool ok = false;
+    guarded([&] {
+        MDP_HIP(hipDeviceSynchronize());
+        f();
+        ok = true;
+    });
+    return ok;
+}
+int max_type(const std::vector<int>& a, size_t n) { return std::max(1, max_of(a.data(), static_cast<long>(n), 1)); }
+}  // namespace
+
+extern "C" {
+
+// SnapComputeUi backend 3 (mdpcore.h:3093-3116; cpuSnapComputeUi src/C++/Core/cpuSnap4.cpp:11-191)
+void mdp_hipSnapComputeUi(double* Stotr, double* Stoti, double* rootpqarray, double* rij, double* wjelem,
+                          double* radelem, double rmin0, double rfac0, double rcutfac, int* idxu_block, int* map, int* ai,
+                          int* ti, int* tj, int twojmax, int idxu_max, int inum, int ijnum, int switchflag, int chemflag) {
+    (void)idxu_block;
+    std::vector<int> h_ai, h_ti, h_tj, h_map;
+    std::vector<double> h_rp, h_wj, h_rad;
+    if (!fetch_args([&] {
+            const size_t np = static_cast<size_t>(std::max(ijnum, 0));
+            h_ai = fetch(ai, np);
+            h_ti = fetch(ti, np);
+            h_tj = fetch(tj, np);
+            const int nt = std::max(max_type(h_ti, np), max_type(h_tj, np));
+            h_map = fetch(map, nt + 1);
+            h_rp = fetch(rootpqarray, static_cast<size_t>(twojmax + 2) * (twojmax + 2));
+            h_wj = fetch(wjelem, nt + 1);
+            h_rad = fetch(radelem, nt + 1);
+        }))
+        return;
+    DevIo io;
+    cpuSnapComputeUi(Stotr, Stoti, h_rp.data(), rij, h_wj.data(), h_rad.data(), rmin0, rfac0, rcutfac, h_map.data(),
+                     h_ai.data(), h_ti.data(), h_tj.data(), twojmax, idxu_max, inum, ijnum, switchflag, chemflag);
+}
+
+// AddWself2Ui backend 3 (mdpcore.h:3118-3140; src/C++/Core/cpuSnap4.cpp:193-247; ai is unused there too)
+void mdp_hipAddWself2Ui(double* ulisttot_r, double* ulisttot_i, double wself, int* idxu_block, int* type, int* map,
+                        int* ai, int wselfall_flag, int chemflag, int idxu_max, int nelements, int twojmax, int inum) {
+    (void)ai;
+    std::vector<int> h_type, h_map;
+    if (!fetch_args([&] {
+            h_type = fetch(type, static_cast<size_t>(std::max(inum, 0)));
+            h_map = fetch(map, max_type(h_type, std::max(inum, 0)) + 1);
+        }))
+        return;
+    DevIo io;
+    cpuAddWself2Ui(ulisttot_r, ulisttot_i, wself, idxu_block, h_type.data(), h_map.data(), wselfall_flag, chemflag,
+                   idxu_max, nelements, twojmax, inum);
+}
+
+// SnapComputeEi backend 3 (mdpcore.h:3167-3194; src/C++/Core/cpuSnap4.cpp:250-424).  The element of atom i
+// is map[type[i]] there whatever the flags (cpuSnap4.cpp:277-278), so the shared body runs with the
+// element mapping on; quadraticflag is unused there.
+void mdp_hipSnapComputeEi(double* eatom, double* ulisttot_r, double* ulisttot_i, double* cglist, double* bzero,
+                          double* coeffelem, int* ilist, int* map, int* type, int* idxb, int* idxcg_block,
+                          int* idxu_block, int twojmax, int idxb_max, int idxu_max, int nelements, int ncoeffall,
+                          int bnorm_flag, int bzero_flag, int wselfall_flag, int quadraticflag, int inum) {
+    (void)quadraticflag;
+    std::vector<int> h_il, h_type, h_map;
+    std::vector<double> h_cg, h_bz, h_c;
+    if (!fetch_args([&] {
+            const size_t n = static_cast<size_t>(std::max(inum, 0));
+            h_il = fetch(ilist, n);
+            h_type = fetch(type, n);
+            const int nt = max_type(h_type, n);
+            h_map = fetch(map, nt + 1);
+            const int rows = max_of(h_map.data() + 1, nt, 0) + 1;
+            h_cg = fetch(cglist, static_cast<size_t>(index_for(twojmax).idxcg_max));
+            h_bz = fetch(bzero, twojmax + 1);
+            h_c = fetch(coeffelem, static_cast<size_t>(ncoeffall) * rows);
+        }))
+        return;
+    DevIo io;
+    cpuSnapComputeEi(eatom, ulisttot_r, ulisttot_i, h_cg.data(), h_bz.data(), h_c.data(), h_il.data(), h_map.data(),
+                     h_type.data(), idxb, idxcg_block, idxu_block, twojmax, idxb_max, idxu_max, nelements, ncoeffall,
+                     bnorm_flag, bzero_flag, wselfall_flag, 1, inum);
+}
+
+// SnapComputeYi backend 3, coefficient form (mdpcore.h:3221-3244; src/C++/Core/cpuSnap4.cpp:697-796):
+// element of atom i is map[type[i]]
+void mdp_hipSnapComputeYi(double* ylist_r, double* ylist_i, double* ulisttot_r, double* ulisttot_i, double* cglist,
+                          double* betalist, int* map, int* type, int* idxz, int* idxb_block, int* idxu_block,
+                          int* idxcg_block, int twojmax, int idxb_max, int idxu_max, int idxz_max, int nelements,
+                          int ncoeffall, int bnorm_flag, int inum) {
+    std::vector<int> h_type, h_map;
+    std::vector<double> h_cg, h_c;
+    if (!fetch_args([&] {
+            const size_t n = static_cast<size_t>(std::max(inum, 0));
+            h_type = fetch(type, n);
+            const int nt = max_type(h_type, n);
+            h_map = fetch(map, nt + 1);
+            const int rows = max_of(h_map.data() + 1, nt, 0) + 1;
+            h_cg = fetch(cglist, static_cast<size_t>(index_for(twojmax).idxcg_max));
+            h_c = fetch(betalist, static_cast<size_t>(ncoeffall) * rows);
+        }))
+        return;
+    DevIo io;
+    cpuSnapComputeYi(ylist_r, ylist_i, ulisttot_r, ulisttot_i, h_cg.data(), h_c.data(), h_map.data(), h_type.data(),
+                     idxz, idxb_block, idxu_block, idxcg_block, twojmax, idxb_max, idxu_max, idxz_max, nelements,
+                     ncoeffall, bnorm_flag, 1, inum);
+}
+
+// SnapComputeFi backend 3 (mdpcore.h:3246-3269; src/C++/Core/cpuSnap4.cpp:798-1210): forces to ai/aj,
+// virial in the SoA layout vatom[k * anum + atom] with -r(x)dE/2 to both atoms
+void mdp_hipSnapComputeFi(double* fatom, double* vatom, double* ylist_r, double* ylist_i, double* rootpqarray,
+                          double* rij, double* wjelem, double* radelem, double rmin0, double rfac0, double rcutfac,
+                          int* idxu_block, int* map, int* aii, int* ai, int* aj, int* ti, int* tj, int twojmax,
+                          int idxu_max, int inum, int anum, int ijnum, int switchflag, int chemflag) {
+    (void)idxu_block;
+    std::vector<int> h_aii, h_ai, h_aj, h_ti, h_tj, h_map;
+    std::vector<double> h_rp, h_wj, h_rad;
+    if (!fetch_args([&] {
+            const size_t np = static_cast<size_t>(std::max(ijnum, 0));
+            h_aii = fetch(aii, np);
+            h_ai = fetch(ai, np);
+            h_aj = fetch(aj, np);
+            h_ti = fetch(ti, np);
+            h_tj = fetch(tj, np);
+            const int nt = std::max(max_type(h_ti, np), max_type(h_tj, np));
+            h_map = fetch(map, nt + 1);
+            h_rp = fetch(rootpqarray, static_cast<size_t>(twojmax + 2) * (twojmax + 2));
+            h_wj = fetch(wjelem, nt + 1);
+            h_rad = fetch(radelem, nt + 1);
+            if (max_of(h_ai.data(), static_cast<long>(np)) >= anum || max_of(h_aj.data(), static_cast<long>(np)) >= anum)
+                throw Error("SnapComputeFi: a pair atom index is not below anum");
+        }))
+        return;
+    DevIo io(anum);
+    cpuSnapComputeFi(fatom, vatom, ylist_r, ylist_i, h_rp.data(), rij, h_wj.data(), h_rad.data(), rmin0, rfac0,
+                     rcutfac, h_map.data(), h_aii.data(), h_ai.data(), h_aj.data(), h_ti.data(), h_tj.data(), twojmax,
+                     idxu_max, inum, anum, ijnum, switchflag, chemflag);
 }
 
 }  // extern "C"

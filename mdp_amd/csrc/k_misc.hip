@@ -571,6 +571,38 @@ void launch_tally_force_atomic(double* f, double* v, const double* dedr, const d
     hipLaunchKernelGGL(k_tally_force_atomic, dim3(nblk(ijnum, 256)), dim3(256), 0, st, f, v, dedr, rij, ai, aj, ijnum);
 }
 
+// The backend-3 SnapComputeFi tally (src/C++/Core/cpuSnap4.cpp:1176-1209): forces as above, virial in
+// the SoA layout v[k * anum + atom] with -r(x)dE/2 added to both atoms of the pair, components
+// (xx, yy, zz, xy, xz, yz).
+__global__ void k_tally_force_atomic_soa(double* __restrict__ f, double* __restrict__ v,
+                                         const double* __restrict__ dedr, const double* __restrict__ rij,
+                                         const int* __restrict__ ai, const int* __restrict__ aj, int ijnum, int anum) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= ijnum) return;
+    const double dx = dedr[3 * p], dy = dedr[3 * p + 1], dz = dedr[3 * p + 2];
+    const double x = rij[3 * p], y = rij[3 * p + 1], z = rij[3 * p + 2];
+    const int a = ai[p], b = aj[p];
+    atomic_add(&f[3 * a], dx);
+    atomic_add(&f[3 * a + 1], dy);
+    atomic_add(&f[3 * a + 2], dz);
+    atomic_add(&f[3 * b], -dx);
+    atomic_add(&f[3 * b + 1], -dy);
+    atomic_add(&f[3 * b + 2], -dz);
+    const double w[6] = {-0.5 * x * dx, -0.5 * y * dy, -0.5 * z * dz, -0.5 * x * dy, -0.5 * x * dz, -0.5 * y * dz};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        atomic_add(&v[static_cast<size_t>(k) * anum + a], w[k]);
+        atomic_add(&v[static_cast<size_t>(k) * anum + b], w[k]);
+    }
+}
+
+void launch_tally_force_atomic_soa(double* f, double* v, const double* dedr, const double* rij, const int* ai,
+                                   const int* aj, int ijnum, int anum, hipStream_t st) {
+    if (ijnum <= 0) return;
+    hipLaunchKernelGGL(k_tally_force_atomic_soa, dim3(nblk(ijnum, 256)), dim3(256), 0, st, f, v, dedr, rij, ai, aj,
+                       ijnum, anum);
+}
+
 // ================================================================================================
 // pair lists from a CSR neighbour list (cpuSnap.cpp:258-310)
 // ================================================================================================

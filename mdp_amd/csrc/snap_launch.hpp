@@ -159,6 +159,9 @@ void launch_tally_bv(double* bv, const double* b, const double* dbdr, const doub
 // fatom[3*ai] += dE, fatom[3*aj] -= dE, vatom[6*ai + (xx,yy,zz,xy,xz,yz)] += -r (x) dE  (atomic)
 void launch_tally_force_atomic(double* fatom, double* vatom, const double* dedr, const double* rij, const int* ai,
                                const int* aj, int ijnum, hipStream_t st);
+// same forces; virial in the SoA layout vatom[k * anum + atom], -r(x)dE/2 to both atoms of the pair
+void launch_tally_force_atomic_soa(double* fatom, double* vatom, const double* dedr, const double* rij, const int* ai,
+                                   const int* aj, int ijnum, int anum, hipStream_t st);
 
 // cpuNeighPairList: pairnum[ii] = count of neighbours j with d2 < rcutsq(type pair), d2 > 1e-20
 void launch_pair_count(int* pairnum, const double* x, const double* rcutsq, const int* ilist, const int* neighlist,
