@@ -71,7 +71,7 @@ __host__ __device__ constexpr int term_zblock(int J, int t, int w) {
 
 // ---- batched pair build ------------------------------------------------------------------------
 
-__device__ __forceinline__ double image_delta(double xj, int s, double L, double xi) {
+__device__ __forceinline__ double image_delta(double xj, double s, double L, double xi) {
 #pragma clang fp contract(off)
     const double g = xj + s * L;
     return g - xi;
@@ -113,51 +113,91 @@ __global__ __launch_bounds__(256) void k_fit_pairs(const double* __restrict__ x,
             return;
         }
     }
-    // image ranges from the real-number bound, rounded outwards: the exact test below decides, so the
-    // reciprocal (an ulp off at worst) only ever adds a candidate image
     const double iLx = 1.0 / Lx, iLy = 1.0 / Ly, iLz = 1.0 / Lz;
-    // With every box length above twice the largest cutoff at most the minimum image of j can lie within
-    // the cutoff, so j is skipped when that image is farther (with a relative margin far above rounding)
-    // and the exact enumeration below runs only for the survivors: same pairs, same order.
-    double rcmax2 = 0.0;
-    for (int t = 1; t <= ntypes; ++t) rcmax2 = fmax(rcmax2, rc2[ti * (ntypes + 1) + t]);
-    const double reject2 = rcmax2 * (1.0 + 1e-9) + 1e-12;
-    const bool single = 4.0 * rcmax2 < Lx * Lx && 4.0 * rcmax2 < Ly * Ly && 4.0 * rcmax2 < Lz * Lz;
+    double rcmax2 = 0.0, rcmin2 = 1e300;
+    for (int t = 1; t <= ntypes; ++t) {
+        rcmax2 = fmax(rcmax2, rc2[ti * (ntypes + 1) + t]);
+        rcmin2 = fmin(rcmin2, rc2[ti * (ntypes + 1) + t]);
+    }
+    const bool rconst = rcmin2 == rcmax2;   // one cutoff for every partner type (always with one type)
+    // With every box length above twice the largest cutoff (with a relative margin far above rounding)
+    // two images of j are more than 2 rc apart, so at most one lies within the cutoff, and in every
+    // dimension it is the nearest image s = rint((x_i - x_j) / L) (|s L + x_j - x_i| <= rc < L/2).  The
+    // single-image test below evaluates exactly the r_ij and |r_ij|^2 the full enumeration would for
+    // that image: same pairs, same order, same bits.
+    const double wide2 = 4.0 * rcmax2 * (1.0 + 1e-9);
+    const bool single = wide2 < Lx * Lx && wide2 < Ly * Ly && wide2 < Lz * Lz;
     int cnt = 0;
-    for (int j = j0; j < j1; ++j) {
-        const double xj = x[3 * j], yj = x[3 * j + 1], zj = x[3 * j + 2];
-        if (single) {
-            double ex = xj - xi, ey = yj - yi, ez = zj - zi;
-            ex -= Lx * rint(ex * iLx);
-            ey -= Ly * rint(ey * iLy);
-            ez -= Lz * rint(ez * iLz);
-            if (ex * ex + ey * ey + ez * ez > reject2) continue;
+    auto emit = [&](int j, int tj, double dx, double dy, double dz) __attribute__((always_inline)) {
+        if constexpr (FILL) {
+            rij[3 * q] = dx;
+            rij[3 * q + 1] = dy;
+            rij[3 * q + 2] = dz;
+            pi[q] = g;
+            pj[q] = j;
+            pti[q] = ti;
+            ptj[q] = tj;
+            ++q;
         }
-        const int tj = type[j];
-        const double r2 = rc2[ti * (ntypes + 1) + tj];
-        const double rc = sqrt(r2);
-        const int sx0 = static_cast<int>(floor((xi - xj - rc) * iLx)), sx1 = static_cast<int>(ceil((xi - xj + rc) * iLx));
-        const int sy0 = static_cast<int>(floor((yi - yj - rc) * iLy)), sy1 = static_cast<int>(ceil((yi - yj + rc) * iLy));
-        const int sz0 = static_cast<int>(floor((zi - zj - rc) * iLz)), sz1 = static_cast<int>(ceil((zi - zj + rc) * iLz));
-        for (int sz = sz0; sz <= sz1; ++sz) {
-            const double dz = image_delta(zj, sz, Lz, zi);
-            for (int sy = sy0; sy <= sy1; ++sy) {
-                const double dy = image_delta(yj, sy, Ly, yi);
-                for (int sx = sx0; sx <= sx1; ++sx) {
-                    if (j == g && sx == 0 && sy == 0 && sz == 0) continue;
-                    const double dx = image_delta(xj, sx, Lx, xi);
-                    if (dist2(dx, dy, dz) <= r2) {
-                        if constexpr (FILL) {
-                            rij[3 * q] = dx;
-                            rij[3 * q + 1] = dy;
-                            rij[3 * q + 2] = dz;
-                            pi[q] = g;
-                            pj[q] = j;
-                            pti[q] = ti;
-                            ptj[q] = tj;
-                            ++q;
-                        }
-                        ++cnt;
+        ++cnt;
+    };
+    // RC: the cutoff is rcmax2 for every partner (a compile-time choice, so no per-candidate table read)
+    auto nearest = [&](int j, double xj, double yj, double zj, int tj, auto rc) __attribute__((always_inline)) {
+        const double dx = image_delta(xj, rint((xi - xj) * iLx), Lx, xi);
+        const double dy = image_delta(yj, rint((yi - yj) * iLy), Ly, yi);
+        const double dz = image_delta(zj, rint((zi - zj) * iLz), Lz, zi);
+        double r2;
+        if constexpr (decltype(rc)::value) r2 = rcmax2;
+        else r2 = rc2[ti * (ntypes + 1) + tj];
+        if (j != g && dist2(dx, dy, dz) <= r2) emit(j, tj, dx, dy, dz);
+    };
+    // candidates four at a time: their loads are issued together, the tests and emits run in j order
+    auto scan = [&](int ja, int jb, auto rc) __attribute__((always_inline)) {
+        int j = ja;
+        for (; j + 4 <= jb; j += 4) {
+            double px[4], py[4], pz[4];
+            int pt[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                px[u] = x[3 * (j + u)];
+                py[u] = x[3 * (j + u) + 1];
+                pz[u] = x[3 * (j + u) + 2];
+                pt[u] = type[j + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) nearest(j + u, px[u], py[u], pz[u], pt[u], rc);
+        }
+        for (; j < jb; ++j) nearest(j, x[3 * j], x[3 * j + 1], x[3 * j + 2], type[j], rc);
+    };
+    auto scan_rc = [&](int ja, int jb) __attribute__((always_inline)) {
+        if (rconst) scan(ja, jb, std::true_type{});
+        else scan(ja, jb, std::false_type{});
+    };
+    if (single) {
+        const int cu = __builtin_amdgcn_readfirstlane(c);
+        if (__all(c == cu))   // the wave's atoms share one configuration: j, x_j and t_j are wave-uniform
+            scan_rc(__builtin_amdgcn_readfirstlane(j0), __builtin_amdgcn_readfirstlane(j1));
+        else
+            scan_rc(j0, j1);
+    } else {
+        // every image x_j + s L within the cutoff, s from the real-number bound rounded outwards (the
+        // exact test decides, so the reciprocal, an ulp off at worst, only ever adds a candidate)
+        for (int j = j0; j < j1; ++j) {
+            const double xj = x[3 * j], yj = x[3 * j + 1], zj = x[3 * j + 2];
+            const int tj = type[j];
+            const double r2 = rc2[ti * (ntypes + 1) + tj];
+            const double rc = sqrt(r2);
+            const int sx0 = static_cast<int>(floor((xi - xj - rc) * iLx)), sx1 = static_cast<int>(ceil((xi - xj + rc) * iLx));
+            const int sy0 = static_cast<int>(floor((yi - yj - rc) * iLy)), sy1 = static_cast<int>(ceil((yi - yj + rc) * iLy));
+            const int sz0 = static_cast<int>(floor((zi - zj - rc) * iLz)), sz1 = static_cast<int>(ceil((zi - zj + rc) * iLz));
+            for (int sz = sz0; sz <= sz1; ++sz) {
+                const double dz = image_delta(zj, sz, Lz, zi);
+                for (int sy = sy0; sy <= sy1; ++sy) {
+                    const double dy = image_delta(yj, sy, Ly, yi);
+                    for (int sx = sx0; sx <= sx1; ++sx) {
+                        if (j == g && sx == 0 && sy == 0 && sz == 0) continue;
+                        const double dx = image_delta(xj, sx, Lx, xi);
+                        if (dist2(dx, dy, dz) <= r2) emit(j, tj, dx, dy, dz);
                     }
                 }
             }

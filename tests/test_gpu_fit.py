@@ -1,7 +1,8 @@
 """Batched fitting descriptors on the GPU (mdp_fit_*).
 
 * Pair lists: bit-exact against the oracle's brute-force periodic builder (same (j, image) order),
-  r_ij bit-exact against x_j + s*L - x_i.
+  r_ij bit-exact against x_j + s*L - x_i, for boxes below, at and above twice the cutoff (the GPU
+  takes a single-image path above it).
 * B and dB/dr of every configuration: against the oracle's per-configuration chain cpuComputeUij ->
   cpuZeroUarraytot2 -> cpuAddUarraytot -> cpuComputeZi2 -> cpuComputeBi2 -> cpuComputeDbidrj on the
   same pairs, within 1e-10 relative to the max norm of the oracle array.
@@ -28,7 +29,10 @@ TOL = 1e-10
 def _small_batch(seed=7):
     cfg = []
     for k, (kind, n, a) in enumerate([("bcc", 2, 3.3), ("fcc", (2, 2, 3), 4.1), ("bcc", 3, 3.25),
-                                      ("fcc", 1, 4.2), ("bcc", (1, 2, 2), 3.4)]):
+                                      ("fcc", 1, 4.2), ("bcc", (1, 2, 2), 3.4),
+                                      # box side 9.36 just above twice the cutoff 9.3527: one image per
+                                      # pair, with pairs near half the box and near the cutoff
+                                      ("bcc", 3, 3.12)]):
         x, box = lattice.crystal(kind, n, a, 0.1, seed + k)
         cfg.append((x, box))
     for k, n in enumerate((40, 9)):
