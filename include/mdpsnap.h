@@ -17,7 +17,8 @@
  * Part 3 computes fitting descriptors (B and dB/dr) of many configurations in one batch.
  * Part 4 holds the kernels of the learned surrogate (per-atom MLP over bispectrum descriptors).
  * Part 5 is the backend-3 (HIP) branch of the upstream C++ core dispatch (src/C++/Common/mdpcore.h) for
- * the SNAP force path: device arrays in and out; include/mdphipcore.h wraps it as the hip* templates.
+ * the SNAP force path and the descriptor path of the upstream fitting driver: device arrays in and out;
+ * include/mdphipcore.h wraps it as the hip* templates.
  *
  * Errors: the drop-in functions return void like upstream; a failure leaves a message readable
  * with mdp_snap_last_error() (empty string when the last call succeeded).  Part 2 returns 0 on
@@ -317,7 +318,7 @@ int mdp_sur_energy(const double *e, const int *cfg_off, const double *eref, int 
 int mdp_sumsq(const double *v, long n, double *partials, int nblocks, void *stream);
 
 /* ---------------------------------------------------------------------------------------------- */
-/* Part 5: backend-3 entry points of src/C++/Common/mdpcore.h (SNAP force path, device arrays)      */
+/* Part 5: backend-3 entry points of src/C++/Common/mdpcore.h (SNAP force and descriptor paths)   */
 /* ---------------------------------------------------------------------------------------------- */
 /* Same argument lists as the dispatch's `backend == 3` branches (without `backend`), dstype = double,
  * every pointer a device pointer.  Semantics of the upstream C++ core (src/C++/Core/cpuSnap4.cpp):
@@ -348,6 +349,69 @@ void mdp_hipSnapComputeFi(double *fatom, double *vatom, double *ylist_r, double 
                           double *rij, double *wjelem, double *radelem, double rmin0, double rfac0, double rcutfac,
                           int *idxu_block, int *map, int *aii, int *ai, int *aj, int *ti, int *tj, int twojmax,
                           int idxu_max, int inum, int anum, int ijnum, int switchflag, int chemflag);
+
+/* The descriptor chain of the upstream fitting driver ComputeSnap (snappotential.cpp:762-910):
+ * NeighborPairList -> Cumsum -> NeighborPairs -> ComputeSij -> ZeroUarraytot2 -> AddUarraytot ->
+ * ComputeZi2 -> ComputeBi2 -> ComputeDbidrj -> SnapTallyBispectrum{,Deriv,Virial}, same conventions
+ * (src/C++/Core/cpuSnap.cpp:1883-1935, src/C++/Core/cpuSnap2.cpp).  Neighbour and pair lists use the
+ * fixed stride jnum: neighbours of atom i at neighlist[l + jnum*i] (l < neighnum[i]), kept pairs of
+ * the ii-th listed atom at pairlist[l + jnum*ii] (l < pairnum[ii]).  ComputeSij, ZeroUarraytot2,
+ * ComputeZi2, ComputeBi2, ComputeDbidrj and the three tallies overwrite their outputs; AddUarraytot
+ * accumulates. */
+/* replaces hipNeighPairList in the typed-cutoff NeighborPairList (mdpcore.h:3830-3851): pair kept when
+ * 1e-20 < |x_j - x_i|^2 < rcutsq[atomtype[alist[j]] + atomtype[i]*(ntypes+1)] */
+void mdp_hipNeighPairList(int *pairnum, int *pairlist, double *x, double *rcutsq, int *ilist, int *neighlist,
+                          int *neighnum, int *atomtype, int *alist, int inum, int jnum, int dim, int ntypes);
+/* replaces hipCumsum in Cumsum (mdpcore.h:1114-1131): d_out[0] = 0, d_out[j] = sum of d_in[0..j-1] for
+ * j < length; d_sums and d_incr (caller scratch upstream) are not used */
+void mdp_hipCumsum(int *d_out, int *d_in, int *d_sums, int *d_incr, int length);
+/* replaces hipNeighPairs in NeighborPairs (mdpcore.h:3853-3875): per pair k = pairnumsum[ii] + l,
+ * xij = x_j - x_i, aii = ii, ai = i, aj = alist[j], ti = atomtype[i], tj = atomtype[alist[j]] */
+void mdp_hipNeighPairs(double *xij, double *x, int *aii, int *ai, int *aj, int *ti, int *tj, int *pairnum,
+                       int *pairlist, int *pairnumsum, int *ilist, int *atomtype, int *alist, int inum, int jnum,
+                       int dim);
+/* replaces hipComputeSij in ComputeSij (mdpcore.h:2837-2861) */
+void mdp_hipComputeSij(double *Sr, double *Si, double *Srx, double *Six, double *Sry, double *Siy, double *Srz,
+                       double *Siz, double *rootpqarray, double *rij, double *wjelem, double *radelem, double rmin0,
+                       double rfac0, double rcutfac, int *idxu_block, int *ti, int *tj, int twojmax, int idxu_max,
+                       int ijnum, int switchflag);
+/* replaces hipZeroUarraytot2 in ZeroUarraytot2 (mdpcore.h:2887-2909) */
+void mdp_hipZeroUarraytot2(double *ulisttot_r, double *ulisttot_i, double wself, int *idxu_block, int *type, int *map,
+                           int *ai, int wselfall_flag, int chemflag, int idxu_max, int nelements, int twojmax,
+                           int inum);
+/* replaces hipAddUarraytot in the (map, ai, tj) form of AddUarraytot (mdpcore.h:2964-2986) */
+void mdp_hipAddUarraytot(double *ulisttot_r, double *ulisttot_i, double *ulist_r, double *ulist_i, int *map, int *ai,
+                         int *tj, int idxu_max, int inum, int ijnum, int chemflag);
+/* replaces hipAddUarraytot in the (pairnum, pairnumsum) form of AddUarraytot (mdpcore.h:2940-2962):
+ * the pairs of the ii-th atom are pairnumsum[ii] .. pairnumsum[ii] + pairnum[ii] - 1 */
+void mdp_hipAddUarraytotPairs(double *ulisttot_r, double *ulisttot_i, double *ulist_r, double *ulist_i, int *pairnum,
+                              int *pairnumsum, int *map, int *tj, int idxu_max, int nelements, int inum, int ijnum,
+                              int chemflag);
+/* replaces hipComputeZi2 in ComputeZi2 (mdpcore.h:3011-3033) */
+void mdp_hipComputeZi2(double *zlist_r, double *zlist_i, double *ulisttot_r, double *ulisttot_i, double *cglist,
+                       int *idxz, int *idxu_block, int *idxcg_block, int twojmax, int idxu_max, int idxz_max,
+                       int nelements, int bnorm_flag, int inum);
+/* replaces hipComputeBi2 in ComputeBi2 (mdpcore.h:3064-3090) */
+void mdp_hipComputeBi2(double *blist, double *zlist_r, double *zlist_i, double *ulisttot_r, double *ulisttot_i,
+                       double *bzero, int *ilist, int *type, int *map, int *idxb, int *idxu_block, int *idxz_block,
+                       int twojmax, int idxb_max, int idxu_max, int idxz_max, int nelements, int bzero_flag,
+                       int wselfall_flag, int chemflag, int inum);
+/* replaces hipComputeDbidrj in the (inum, ijnum) form of ComputeDbidrj (mdpcore.h:3395-3420) */
+void mdp_hipComputeDbidrj(double *dblist, double *zlist_r, double *zlist_i, double *dulist_r, double *dulist_i,
+                          int *idxb, int *idxu_block, int *idxz_block, int *map, int *ai, int *tj, int twojmax,
+                          int idxb_max, int idxu_max, int idxz_max, int nelements, int bnorm_flag, int chemflag,
+                          int inum, int ijnum);
+/* replaces hipSnapTallyBispectrum in SnapTallyBispectrum (mdpcore.h:3750-3767) */
+void mdp_hipSnapTallyBispectrum(double *bi, double *bispectrum, int *ilist, int *type, int inum, int ncoeff,
+                                int nperdim, int ntype, int quadraticflag);
+/* replaces hipSnapTallyBispectrumDeriv in SnapTallyBispectrumDeriv (mdpcore.h:3769-3786) */
+void mdp_hipSnapTallyBispectrumDeriv(double *db, double *bispectrum, double *dbdr, int *aii, int *ai, int *aj, int *ti,
+                                     int inum, int ijnum, int ncoeff, int nperdim, int ntype, int quadraticflag);
+/* replaces hipSnapTallyBispectrumVirial in SnapTallyBispectrumVirial (mdpcore.h:3788-3805); ai and aj
+ * are read but unused upstream too */
+void mdp_hipSnapTallyBispectrumVirial(double *bv, double *bispectrum, double *dbdr, double *rij, int *aii, int *ai,
+                                      int *aj, int *ti, int inum, int ijnum, int ncoeff, int nperdim, int ntype,
+                                      int quadraticflag);
 
 #ifdef __cplusplus
 }

@@ -457,23 +457,23 @@ void uij_common(double* Sr, double* Si, double* dr[3], double* di[3], double* ro
     s.rmin0 = rmin0;
     s.switchflag = switch_flag;
     const size_t n = static_cast<size_t>(idxu_max) * ijnum;
-    double* d_sr = dev<double>(0, n);
-    double* d_si = dev<double>(1, n);
+    double* d_sr = out_arg(0, Sr, n);
+    double* d_si = out_arg(1, Si, n);
     double* ddr[3];
     double* ddi[3];
     for (int k = 0; k < 3; ++k) {   // upstream leaves some derivative rows untouched: stage them
-        ddr[k] = put(2 + k, dr[k], n, st);
-        ddi[k] = put(5 + k, di[k], n, st);
+        ddr[k] = put_arg(2 + k, dr[k], n, st);
+        ddi[k] = put_arg(5 + k, di[k], n, st);
     }
-    double* d_rij = put(8, rij, 3 * static_cast<size_t>(ijnum), st);
+    double* d_rij = put_arg(8, rij, 3 * static_cast<size_t>(ijnum), st);
     int* d_ti = put(9, ti, ijnum, st);
     int* d_tj = put(10, tj, ijnum, st);
     launch_uij(s, d_sr, d_si, ddr, ddi, d_rij, d_ti, d_tj, ijnum, st);
-    get(Sr, d_sr, n, st);
-    get(Si, d_si, n, st);
+    get_arg(Sr, d_sr, n, st);
+    get_arg(Si, d_si, n, st);
     for (int k = 0; k < 3; ++k) {
-        get(dr[k], ddr[k], n, st);
-        get(di[k], ddi[k], n, st);
+        get_arg(dr[k], ddr[k], n, st);
+        get_arg(di[k], ddi[k], n, st);
     }
     finish(st);
 }
@@ -523,11 +523,11 @@ void cpuZeroUarraytot2(double* Stotr, double* Stoti, double wself, int* idxu_blo
         SnapDev s = tables(twojmax, nullptr, nullptr, nullptr, nullptr, map, ntypes, st);
         if (idxu_max != s.idxu_max) throw Error("idxu_max does not match twojmax");
         const size_t nu = static_cast<size_t>(inum) * idxu_max * nelements;
-        double* d_r = dev<double>(0, nu);
-        double* d_i = dev<double>(1, nu);
+        double* d_r = out_arg(0, Stotr, nu);
+        double* d_i = out_arg(1, Stoti, nu);
         launch_zero_utot(s, d_r, d_i, wself, d_t, d_ai, wselfall_flag, chemflag, nelements, inum, st);
-        get(Stotr, d_r, nu, st);
-        get(Stoti, d_i, nu, st);
+        get_arg(Stotr, d_r, nu, st);
+        get_arg(Stoti, d_i, nu, st);
         finish(st);
     });
 }
@@ -544,15 +544,15 @@ void cpuAddUarraytot(double* Stotr, double* Stoti, double* Sr, double* Si, int* 
         if (count_u(J) != idxu_max) throw Error("idxu_max is not a U index-space size");
         SnapDev s = tables(J, nullptr, nullptr, nullptr, nullptr, chemflag ? map : nullptr, ntypes, st);
         const size_t nu = static_cast<size_t>(inum) * idxu_max * nelem, np = static_cast<size_t>(ijnum) * idxu_max;
-        double* d_tr = put(0, Stotr, nu, st);
-        double* d_ti = put(1, Stoti, nu, st);
-        double* d_sr = put(2, Sr, np, st);
-        double* d_si = put(3, Si, np, st);
+        double* d_tr = put_arg(0, Stotr, nu, st);
+        double* d_ti = put_arg(1, Stoti, nu, st);
+        double* d_sr = put_arg(2, Sr, np, st);
+        double* d_si = put_arg(3, Si, np, st);
         int* d_ai = put(4, ai, ijnum, st);
         int* d_tj = put(5, tj, ijnum, st);
         launch_add_utot(s, d_tr, d_ti, d_sr, d_si, d_ai, d_tj, chemflag, inum, ijnum, st);
-        get(Stotr, d_tr, nu, st);
-        get(Stoti, d_ti, nu, st);
+        get_arg(Stotr, d_tr, nu, st);
+        get_arg(Stoti, d_ti, nu, st);
         finish(st);
     });
 }
@@ -568,18 +568,18 @@ void cpuComputeZi2(double* zlist_r, double* zlist_i, double* Stotr, double* Stot
         if (idxu_max != s.idxu_max || idxz_max != s.idxz_max) throw Error("table sizes do not match twojmax");
         const int ne = nelements;
         const size_t nu = static_cast<size_t>(inum) * idxu_max * ne, nz = static_cast<size_t>(inum) * idxz_max * ne * ne;
-        double* d_ur = put(0, Stotr, nu, st);
-        double* d_ui = put(1, Stoti, nu, st);
-        double* d_zr = dev<double>(2, nz);
-        double* d_zi = dev<double>(3, nz);
+        double* d_ur = put_arg(0, Stotr, nu, st);
+        double* d_ui = put_arg(1, Stoti, nu, st);
+        double* d_zr = out_arg(2, zlist_r, nz);
+        double* d_zi = out_arg(3, zlist_i, nz);
         const long nu1 = static_cast<long>(inum) * idxu_max, nz1 = static_cast<long>(inum) * idxz_max;
         for (int e1 = 0; e1 < ne; ++e1)
             for (int e2 = 0; e2 < ne; ++e2) {
                 const long o = nz1 * (e1 * ne + e2);
                 launch_zi(s, d_ur, d_ui, nu1 * e1, nu1 * e2, d_zr + o, d_zi + o, inum, bnorm_flag, st);
             }
-        get(zlist_r, d_zr, nz, st);
-        get(zlist_i, d_zi, nz, st);
+        get_arg(zlist_r, d_zr, nz, st);
+        get_arg(zlist_i, d_zi, nz, st);
         finish(st);
     });
 }
@@ -600,16 +600,16 @@ void cpuComputeBi2(double* blist, double* zlist_r, double* zlist_i, double* Stot
         const int ne = nelements;
         const size_t nu = static_cast<size_t>(inum) * idxu_max * ne, nz = static_cast<size_t>(inum) * idxz_max * ne * ne;
         const size_t nb = static_cast<size_t>(inum) * idxb_max * ne * ne * ne;
-        double* d_zr = put(0, zlist_r, nz, st);
-        double* d_zi = put(1, zlist_i, nz, st);
-        double* d_ur = put(2, Stotr, nu, st);
-        double* d_ui = put(3, Stoti, nu, st);
+        double* d_zr = put_arg(0, zlist_r, nz, st);
+        double* d_zi = put_arg(1, zlist_i, nz, st);
+        double* d_ur = put_arg(2, Stotr, nu, st);
+        double* d_ui = put_arg(3, Stoti, nu, st);
         double* d_bz = put(4, bzero, twojmax + 1, st);
         int* d_il = put(5, ilist, inum, st);
         int* d_t = put(6, type, text, st);
-        double* d_b = dev<double>(7, nb);
+        double* d_b = out_arg(7, blist, nb);
         launch_bi2(s, d_b, d_zr, d_zi, d_ur, d_ui, d_bz, d_il, d_t, bzero_flag, wselfall_flag, chemflag, ne, inum, st);
-        get(blist, d_b, nb, st);
+        get_arg(blist, d_b, nb, st);
         finish(st);
     });
 }
@@ -629,16 +629,16 @@ void cpuComputeDbidrj(double* dblist, double* zlist_r, double* zlist_i, double* 
         const size_t nz = static_cast<size_t>(inum) * idxz_max * ne * ne;
         const size_t ndu = 3 * static_cast<size_t>(ijnum) * idxu_max;
         const size_t ndb = static_cast<size_t>(ijnum) * idxb_max * 3 * ne * ne * ne;
-        double* d_zr = put(0, zlist_r, nz, st);
-        double* d_zi = put(1, zlist_i, nz, st);
-        double* d_dur = put(2, dulist_r, ndu, st);
-        double* d_dui = put(3, dulist_i, ndu, st);
+        double* d_zr = put_arg(0, zlist_r, nz, st);
+        double* d_zi = put_arg(1, zlist_i, nz, st);
+        double* d_dur = put_arg(2, dulist_r, ndu, st);
+        double* d_dui = put_arg(3, dulist_i, ndu, st);
         int* d_ai = put(4, ai, ijnum, st);
         int* d_tj = put(5, tj, ijnum, st);
-        double* d_db = dev<double>(6, ndb);
+        double* d_db = out_arg(6, dblist, ndb);
         MDP_HIP(hipMemsetAsync(d_db, 0, sizeof(double) * ndb, st));
         launch_dbidrj(s, d_db, d_zr, d_zi, d_dur, d_dui, d_ai, d_tj, bnorm_flag, chemflag, ne, inum, ijnum, st);
-        get(dblist, d_db, ndb, st);
+        get_arg(dblist, d_db, ndb, st);
         finish(st);
     });
 }
@@ -648,16 +648,16 @@ void cpuSnapTallyBispectrum(double* bi, double* bispectrum, int* ilist, int* typ
     guarded([&] {
         hipStream_t st = library_stream();
         const size_t nbi = static_cast<size_t>(nperdim) * ntype;
-        double* d_bi = dev<double>(0, nbi);
-        MDP_HIP(hipMemsetAsync(d_bi, 0, sizeof(double) * std::max<size_t>(nbi, 1), st));
+        double* d_bi = out_arg(0, bi, nbi);
+        if (nbi) MDP_HIP(hipMemsetAsync(d_bi, 0, sizeof(double) * nbi, st));
         if (inum > 0) {
             const int text = max_of(ilist, inum) + 1;
-            double* d_b = put(1, bispectrum, static_cast<size_t>(inum) * ncoeff, st);
+            double* d_b = put_arg(1, bispectrum, static_cast<size_t>(inum) * ncoeff, st);
             int* d_il = put(2, ilist, inum, st);
             int* d_t = put(3, type, text, st);
             launch_tally_bi(d_bi, d_b, d_il, d_t, inum, ncoeff, nperdim, quadraticflag, st);
         }
-        get(bi, d_bi, nbi, st);
+        get_arg(bi, d_bi, nbi, st);
         finish(st);
     });
 }
@@ -667,18 +667,18 @@ void cpuSnapTallyBispectrumDeriv(double* db, double* bispectrum, double* dbdr, i
     guarded([&] {
         hipStream_t st = library_stream();
         const size_t ndb = static_cast<size_t>(inum) * 3 * nperdim * ntype;
-        double* d_db = dev<double>(0, ndb);
-        MDP_HIP(hipMemsetAsync(d_db, 0, sizeof(double) * std::max<size_t>(ndb, 1), st));
+        double* d_db = out_arg(0, db, ndb);
+        if (ndb) MDP_HIP(hipMemsetAsync(d_db, 0, sizeof(double) * ndb, st));
         if (ijnum > 0) {
-            double* d_b = put(1, bispectrum, static_cast<size_t>(inum) * ncoeff, st);
-            double* d_dbdr = put(2, dbdr, static_cast<size_t>(ijnum) * 3 * ncoeff, st);
+            double* d_b = put_arg(1, bispectrum, static_cast<size_t>(inum) * ncoeff, st);
+            double* d_dbdr = put_arg(2, dbdr, static_cast<size_t>(ijnum) * 3 * ncoeff, st);
             int* d_aii = put(3, aii, ijnum, st);
             int* d_ai = put(4, ai, ijnum, st);
             int* d_aj = put(5, aj, ijnum, st);
             int* d_ti = put(6, ti, ijnum, st);
             launch_tally_bd(d_db, d_b, d_dbdr, d_aii, d_ai, d_aj, d_ti, inum, ijnum, ncoeff, nperdim, quadraticflag, st);
         }
-        get(db, d_db, ndb, st);
+        get_arg(db, d_db, ndb, st);
         finish(st);
     });
 }
@@ -688,17 +688,17 @@ void cpuSnapTallyBispectrumVirial(double* bv, double* bispectrum, double* dbdr, 
     guarded([&] {
         hipStream_t st = library_stream();
         const size_t nbv = 6 * static_cast<size_t>(nperdim) * ntype;
-        double* d_bv = dev<double>(0, nbv);
-        MDP_HIP(hipMemsetAsync(d_bv, 0, sizeof(double) * std::max<size_t>(nbv, 1), st));
+        double* d_bv = out_arg(0, bv, nbv);
+        if (nbv) MDP_HIP(hipMemsetAsync(d_bv, 0, sizeof(double) * nbv, st));
         if (ijnum > 0) {
-            double* d_b = put(1, bispectrum, static_cast<size_t>(inum) * ncoeff, st);
-            double* d_dbdr = put(2, dbdr, static_cast<size_t>(ijnum) * 3 * ncoeff, st);
-            double* d_rij = put(3, rij, 3 * static_cast<size_t>(ijnum), st);
+            double* d_b = put_arg(1, bispectrum, static_cast<size_t>(inum) * ncoeff, st);
+            double* d_dbdr = put_arg(2, dbdr, static_cast<size_t>(ijnum) * 3 * ncoeff, st);
+            double* d_rij = put_arg(3, rij, 3 * static_cast<size_t>(ijnum), st);
             int* d_aii = put(4, aii, ijnum, st);
             int* d_ti = put(5, ti, ijnum, st);
             launch_tally_bv(d_bv, d_b, d_dbdr, d_rij, d_aii, d_ti, inum, ijnum, ncoeff, nperdim, quadraticflag, st);
         }
-        get(bv, d_bv, nbv, st);
+        get_arg(bv, d_bv, nbv, st);
         finish(st);
     });
 }
@@ -860,6 +860,220 @@ void mdp_hipSnapComputeFi(double* fatom, double* vatom, double* ylist_r, double*
     cpuSnapComputeFi(fatom, vatom, ylist_r, ylist_i, h_rp.data(), rij, h_wj.data(), h_rad.data(), rmin0, rfac0,
                      rcutfac, h_map.data(), h_aii.data(), h_ai.data(), h_aj.data(), h_ti.data(), h_tj.data(), twojmax,
                      idxu_max, inum, anum, ijnum, switchflag, chemflag);
+}
+
+// ---- the descriptor chain of the upstream fitting driver (snappotential.cpp:762-910) --------------
+
+// NeighborPairList backend 3, typed cutoffs (mdpcore.h:3830-3851; src/C++/Core/cpuSnap.cpp:1883-1905)
+void mdp_hipNeighPairList(int* pairnum, int* pairlist, double* x, double* rcutsq, int* ilist, int* neighlist,
+                          int* neighnum, int* atomtype, int* alist, int inum, int jnum, int dim, int ntypes) {
+    guarded([&] {
+        if (inum <= 0) return;
+        if (jnum <= 0 || dim != 3 || ntypes < 1) throw Error("NeighPairList: needs jnum > 0, dim = 3, ntypes >= 1");
+        hipStream_t st = library_stream();
+        launch_pair_list_strided(pairnum, pairlist, x, rcutsq, ilist, neighlist, neighnum, atomtype, alist, inum, jnum,
+                                 dim, ntypes, st);
+        finish(st);
+    });
+}
+
+// Cumsum backend 3 (mdpcore.h:1114-1131; cpuCumsum src/C++/Core/cpuArrayOperations.cpp:22-27)
+void mdp_hipCumsum(int* d_out, int* d_in, int* d_sums, int* d_incr, int length) {
+    (void)d_sums; (void)d_incr;
+    guarded([&] {
+        if (length <= 0) return;
+        hipStream_t st = library_stream();
+        const int n = length - 1;
+        int* d_tmp = dev<int>(0, scan_tmp_ints(std::max(n, 1)));
+        exclusive_scan(d_in, d_out, n, d_tmp, st);   // out[0..n], out[n] = total
+        finish(st);
+    });
+}
+
+// NeighborPairs backend 3 (mdpcore.h:3853-3875; src/C++/Core/cpuSnap.cpp:1912-1935)
+void mdp_hipNeighPairs(double* xij, double* x, int* aii, int* ai, int* aj, int* ti, int* tj, int* pairnum,
+                       int* pairlist, int* pairnumsum, int* ilist, int* atomtype, int* alist, int inum, int jnum,
+                       int dim) {
+    guarded([&] {
+        if (inum <= 0) return;
+        if (jnum <= 0 || dim < 1) throw Error("NeighPairs: needs jnum > 0 and dim >= 1");
+        hipStream_t st = library_stream();
+        launch_pair_arrays_strided(xij, x, aii, ai, aj, ti, tj, pairnum, pairlist, pairnumsum, ilist, atomtype, alist,
+                                   inum, jnum, dim, st);
+        finish(st);
+    });
+}
+
+// ComputeSij backend 3 (mdpcore.h:2837-2861; src/C++/Core/cpuSnap2.cpp:23-233)
+void mdp_hipComputeSij(double* Sr, double* Si, double* Srx, double* Six, double* Sry, double* Siy, double* Srz,
+                       double* Siz, double* rootpqarray, double* rij, double* wjelem, double* radelem, double rmin0,
+                       double rfac0, double rcutfac, int* idxu_block, int* ti, int* tj, int twojmax, int idxu_max,
+                       int ijnum, int switchflag) {
+    std::vector<int> h_ti, h_tj;
+    std::vector<double> h_rp, h_wj, h_rad;
+    if (!fetch_args([&] {
+            const size_t np = static_cast<size_t>(std::max(ijnum, 0));
+            h_ti = fetch(ti, np);
+            h_tj = fetch(tj, np);
+            const int nt = std::max(max_type(h_ti, np), max_type(h_tj, np));
+            h_rp = fetch(rootpqarray, static_cast<size_t>(twojmax + 2) * (twojmax + 2));
+            h_wj = fetch(wjelem, nt + 1);
+            h_rad = fetch(radelem, nt + 1);
+        }))
+        return;
+    DevIo io;
+    cpuComputeSij(Sr, Si, Srx, Six, Sry, Siy, Srz, Siz, h_rp.data(), rij, h_wj.data(), h_rad.data(), rmin0, rfac0,
+                  rcutfac, idxu_block, h_ti.data(), h_tj.data(), twojmax, idxu_max, ijnum, switchflag);
+}
+
+// ZeroUarraytot2 backend 3 (mdpcore.h:2887-2909; src/C++/Core/cpuSnap2.cpp:235-266)
+void mdp_hipZeroUarraytot2(double* ulisttot_r, double* ulisttot_i, double wself, int* idxu_block, int* type, int* map,
+                           int* ai, int wselfall_flag, int chemflag, int idxu_max, int nelements, int twojmax,
+                           int inum) {
+    std::vector<int> h_ai, h_type, h_map;
+    if (!fetch_args([&] {
+            int nt = 1;
+            if (chemflag && inum > 0) {   // ai[ii] and type[ai[ii]] are read only in this case
+                h_ai = fetch(ai, inum);
+                h_type = fetch(type, max_of(h_ai.data(), inum) + 1);
+                nt = max_type(h_type, h_type.size());
+            }
+            h_map = fetch(map, nt + 1);
+        }))
+        return;
+    DevIo io;
+    cpuZeroUarraytot2(ulisttot_r, ulisttot_i, wself, idxu_block, chemflag ? h_type.data() : type, h_map.data(),
+                      chemflag ? h_ai.data() : ai, wselfall_flag, chemflag, idxu_max, nelements, twojmax, inum);
+}
+
+// AddUarraytot backend 3, (map, ai, tj) form (mdpcore.h:2964-2986; src/C++/Core/cpuSnap2.cpp:322-347)
+void mdp_hipAddUarraytot(double* ulisttot_r, double* ulisttot_i, double* ulist_r, double* ulist_i, int* map, int* ai,
+                         int* tj, int idxu_max, int inum, int ijnum, int chemflag) {
+    std::vector<int> h_ai, h_tj, h_map;
+    if (!fetch_args([&] {
+            const size_t np = static_cast<size_t>(std::max(ijnum, 0));
+            h_ai = fetch(ai, np);
+            h_tj = fetch(tj, np);
+            h_map = fetch(map, (chemflag ? max_type(h_tj, np) : 1) + 1);
+        }))
+        return;
+    DevIo io;
+    cpuAddUarraytot(ulisttot_r, ulisttot_i, ulist_r, ulist_i, h_map.data(), h_ai.data(), h_tj.data(), idxu_max, inum,
+                    ijnum, chemflag);
+}
+
+// AddUarraytot backend 3, (pairnum, pairnumsum) form (mdpcore.h:2940-2962; src/C++/Core/cpuSnap2.cpp:269-318)
+void mdp_hipAddUarraytotPairs(double* ulisttot_r, double* ulisttot_i, double* ulist_r, double* ulist_i, int* pairnum,
+                              int* pairnumsum, int* map, int* tj, int idxu_max, int nelements, int inum, int ijnum,
+                              int chemflag) {
+    (void)nelements;
+    guarded([&] {
+        if (inum <= 0 || idxu_max <= 0) return;
+        hipStream_t st = library_stream();
+        launch_add_utot_pairs(ulisttot_r, ulisttot_i, ulist_r, ulist_i, pairnum, pairnumsum, map, tj, idxu_max, inum,
+                              ijnum, chemflag, st);
+        finish(st);
+    });
+}
+
+// ComputeZi2 backend 3 (mdpcore.h:3011-3033; src/C++/Core/cpuSnap2.cpp:364-438)
+void mdp_hipComputeZi2(double* zlist_r, double* zlist_i, double* ulisttot_r, double* ulisttot_i, double* cglist,
+                       int* idxz, int* idxu_block, int* idxcg_block, int twojmax, int idxu_max, int idxz_max,
+                       int nelements, int bnorm_flag, int inum) {
+    std::vector<double> h_cg;
+    if (!fetch_args([&] { h_cg = fetch(cglist, static_cast<size_t>(index_for(twojmax).idxcg_max)); })) return;
+    DevIo io;
+    cpuComputeZi2(zlist_r, zlist_i, ulisttot_r, ulisttot_i, h_cg.data(), idxz, idxu_block, idxcg_block, twojmax,
+                  idxu_max, idxz_max, nelements, bnorm_flag, inum);
+}
+
+// ComputeBi2 backend 3 (mdpcore.h:3064-3090; src/C++/Core/cpuSnap2.cpp:440-545)
+void mdp_hipComputeBi2(double* blist, double* zlist_r, double* zlist_i, double* ulisttot_r, double* ulisttot_i,
+                       double* bzero, int* ilist, int* type, int* map, int* idxb, int* idxu_block, int* idxz_block,
+                       int twojmax, int idxb_max, int idxu_max, int idxz_max, int nelements, int bzero_flag,
+                       int wselfall_flag, int chemflag, int inum) {
+    std::vector<int> h_il, h_type, h_map;
+    std::vector<double> h_bz;
+    if (!fetch_args([&] {
+            const size_t n = static_cast<size_t>(std::max(inum, 0));
+            h_il = fetch(ilist, n);
+            h_type = fetch(type, static_cast<size_t>(max_of(h_il.data(), static_cast<long>(n)) + 1));
+            h_map = fetch(map, max_type(h_type, h_type.size()) + 1);
+            h_bz = fetch(bzero, twojmax + 1);
+        }))
+        return;
+    DevIo io;
+    cpuComputeBi2(blist, zlist_r, zlist_i, ulisttot_r, ulisttot_i, h_bz.data(), h_il.data(), h_type.data(),
+                  h_map.data(), idxb, idxu_block, idxz_block, twojmax, idxb_max, idxu_max, idxz_max, nelements,
+                  bzero_flag, wselfall_flag, chemflag, inum);
+}
+
+// ComputeDbidrj backend 3, (inum, ijnum) form (mdpcore.h:3395-3420; src/C++/Core/cpuSnap2.cpp:826-1050)
+void mdp_hipComputeDbidrj(double* dblist, double* zlist_r, double* zlist_i, double* dulist_r, double* dulist_i,
+                          int* idxb, int* idxu_block, int* idxz_block, int* map, int* ai, int* tj, int twojmax,
+                          int idxb_max, int idxu_max, int idxz_max, int nelements, int bnorm_flag, int chemflag,
+                          int inum, int ijnum) {
+    std::vector<int> h_ai, h_tj, h_map;
+    if (!fetch_args([&] {
+            const size_t np = static_cast<size_t>(std::max(ijnum, 0));
+            h_ai = fetch(ai, np);
+            h_tj = fetch(tj, np);
+            h_map = fetch(map, (chemflag ? max_type(h_tj, np) : 1) + 1);
+        }))
+        return;
+    DevIo io;
+    cpuComputeDbidrj(dblist, zlist_r, zlist_i, dulist_r, dulist_i, idxb, idxu_block, idxz_block, h_map.data(),
+                     h_ai.data(), h_tj.data(), twojmax, idxb_max, idxu_max, idxz_max, nelements, bnorm_flag, chemflag,
+                     inum, ijnum);
+}
+
+// SnapTallyBispectrum backend 3 (mdpcore.h:3750-3767; src/C++/Core/cpuSnap2.cpp:1297-1322)
+void mdp_hipSnapTallyBispectrum(double* bi, double* bispectrum, int* ilist, int* type, int inum, int ncoeff,
+                                int nperdim, int ntype, int quadraticflag) {
+    std::vector<int> h_il, h_type;
+    if (!fetch_args([&] {
+            const size_t n = static_cast<size_t>(std::max(inum, 0));
+            h_il = fetch(ilist, n);
+            h_type = fetch(type, n ? static_cast<size_t>(max_of(h_il.data(), static_cast<long>(n)) + 1) : 0);
+        }))
+        return;
+    DevIo io;
+    cpuSnapTallyBispectrum(bi, bispectrum, h_il.data(), h_type.data(), inum, ncoeff, nperdim, ntype, quadraticflag);
+}
+
+// SnapTallyBispectrumDeriv backend 3 (mdpcore.h:3769-3786; src/C++/Core/cpuSnap2.cpp:1324-1396)
+void mdp_hipSnapTallyBispectrumDeriv(double* db, double* bispectrum, double* dbdr, int* aii, int* ai, int* aj, int* ti,
+                                     int inum, int ijnum, int ncoeff, int nperdim, int ntype, int quadraticflag) {
+    std::vector<int> h_aii, h_ai, h_aj, h_ti;
+    if (!fetch_args([&] {
+            const size_t np = static_cast<size_t>(std::max(ijnum, 0));
+            h_aii = fetch(aii, np);
+            h_ai = fetch(ai, np);
+            h_aj = fetch(aj, np);
+            h_ti = fetch(ti, np);
+        }))
+        return;
+    DevIo io;
+    cpuSnapTallyBispectrumDeriv(db, bispectrum, dbdr, h_aii.data(), h_ai.data(), h_aj.data(), h_ti.data(), inum, ijnum,
+                                ncoeff, nperdim, ntype, quadraticflag);
+}
+
+// SnapTallyBispectrumVirial backend 3 (mdpcore.h:3788-3805; src/C++/Core/cpuSnap2.cpp:1398-1503; ai and aj
+// are read but unused there)
+void mdp_hipSnapTallyBispectrumVirial(double* bv, double* bispectrum, double* dbdr, double* rij, int* aii, int* ai,
+                                      int* aj, int* ti, int inum, int ijnum, int ncoeff, int nperdim, int ntype,
+                                      int quadraticflag) {
+    (void)ai; (void)aj;
+    std::vector<int> h_aii, h_ti;
+    if (!fetch_args([&] {
+            const size_t np = static_cast<size_t>(std::max(ijnum, 0));
+            h_aii = fetch(aii, np);
+            h_ti = fetch(ti, np);
+        }))
+        return;
+    DevIo io;
+    cpuSnapTallyBispectrumVirial(bv, bispectrum, dbdr, rij, h_aii.data(), h_ti.data(), inum, ijnum, ncoeff, nperdim,
+                                 ntype, quadraticflag);
 }
 
 }  // extern "C"

@@ -689,4 +689,108 @@ void launch_pair_arrays(double* xij, const double* x, int* aii, int* ai, int* aj
                        pairnumsum, ilist, atomtype, alist, inum, dim);
 }
 
+// ================================================================================================
+// fixed-stride pair lists of the upstream C++ core (src/C++/Core/cpuSnap.cpp:1883-1935)
+// ================================================================================================
+
+// neighbours of atom i at neighlist[l + jnum*i] (l < neighnum[i]); kept ones to pairlist[c + jnum*ii]
+__global__ void k_pair_list_strided(int* __restrict__ pairnum, int* __restrict__ pairlist, const double* __restrict__ x,
+                                    const double* __restrict__ rcutsq, const int* __restrict__ ilist,
+                                    const int* __restrict__ neighlist, const int* __restrict__ neighnum,
+                                    const int* __restrict__ atomtype, const int* __restrict__ alist, int inum, int jnum,
+                                    int dim, int ntypes) {
+    const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ii >= inum) return;
+    const int i = ilist[ii], it = atomtype[i];
+    const int* nl = neighlist + static_cast<size_t>(jnum) * i;
+    int* pl = pairlist + static_cast<size_t>(jnum) * ii;
+    const int m = neighnum[i];
+    int c = 0;
+    for (int l = 0; l < m; ++l) {
+        const int j = nl[l];
+        if (in_pair(x, i, j, dim, rcutsq[atomtype[alist[j]] + it * (ntypes + 1)])) pl[c++] = j;
+    }
+    pairnum[ii] = c;
+}
+
+void launch_pair_list_strided(int* pairnum, int* pairlist, const double* x, const double* rcutsq, const int* ilist,
+                              const int* neighlist, const int* neighnum, const int* atomtype, const int* alist,
+                              int inum, int jnum, int dim, int ntypes, hipStream_t st) {
+    if (inum <= 0) return;
+    hipLaunchKernelGGL(k_pair_list_strided, dim3(nblk(inum, 128)), dim3(128), 0, st, pairnum, pairlist, x, rcutsq,
+                       ilist, neighlist, neighnum, atomtype, alist, inum, jnum, dim, ntypes);
+}
+
+// pair k = pairnumsum[ii] + l of the ii-th atom, neighbour pairlist[l + jnum*ii] (l < pairnum[ii])
+__global__ void k_pair_arrays_strided(double* __restrict__ xij, const double* __restrict__ x, int* __restrict__ aii,
+                                      int* __restrict__ ai, int* __restrict__ aj, int* __restrict__ ti,
+                                      int* __restrict__ tj, const int* __restrict__ pairnum,
+                                      const int* __restrict__ pairlist, const int* __restrict__ pairnumsum,
+                                      const int* __restrict__ ilist, const int* __restrict__ atomtype,
+                                      const int* __restrict__ alist, int inum, int jnum, int dim) {
+    const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ii >= inum) return;
+    const int i = ilist[ii], it = atomtype[i];
+    const int m = pairnum[ii], start = pairnumsum[ii];
+    const int* pl = pairlist + static_cast<size_t>(jnum) * ii;
+    for (int l = 0; l < m; ++l) {
+        const int j = pl[l], k = start + l, o = alist[j];
+        aii[k] = ii;
+        ai[k] = i;
+        aj[k] = o;
+        ti[k] = it;
+        tj[k] = atomtype[o];
+        for (int d = 0; d < dim; ++d) xij[static_cast<size_t>(k) * dim + d] = x[j * dim + d] - x[i * dim + d];
+    }
+}
+
+void launch_pair_arrays_strided(double* xij, const double* x, int* aii, int* ai, int* aj, int* ti, int* tj,
+                                const int* pairnum, const int* pairlist, const int* pairnumsum, const int* ilist,
+                                const int* atomtype, const int* alist, int inum, int jnum, int dim, hipStream_t st) {
+    if (inum <= 0) return;
+    hipLaunchKernelGGL(k_pair_arrays_strided, dim3(nblk(inum, 128)), dim3(128), 0, st, xij, x, aii, ai, aj, ti, tj,
+                       pairnum, pairlist, pairnumsum, ilist, atomtype, alist, inum, jnum, dim);
+}
+
+// Stot[ii + inum*jju (+ inum*idxu_max*elem)] += S of the atom's pairs (src/C++/Core/cpuSnap2.cpp:269-318).
+// Thread = (ii, jju) with ii fastest.  Without chemflag the pair sum is formed first and added once,
+// with chemflag each pair is added to its element's block in pair order, as upstream.
+__global__ void k_add_utot_pairs(double* __restrict__ Stotr, double* __restrict__ Stoti, const double* __restrict__ Sr,
+                                 const double* __restrict__ Si, const int* __restrict__ pairnum,
+                                 const int* __restrict__ pairnumsum, const int* __restrict__ map,
+                                 const int* __restrict__ tj, int idxu_max, int inum, int ijnum, int chemflag) {
+    const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long n = static_cast<long>(inum) * idxu_max;
+    if (idx >= n) return;
+    const int ii = static_cast<int>(idx % inum);
+    const long jju = idx / inum;
+    const int m = pairnum[ii], start = pairnumsum[ii];
+    const double* sr = Sr + static_cast<long>(ijnum) * jju + start;
+    const double* si = Si + static_cast<long>(ijnum) * jju + start;
+    if (!chemflag) {
+        double qr = 0.0, qi = 0.0;
+        for (int l = 0; l < m; ++l) {
+            qr += sr[l];
+            qi += si[l];
+        }
+        Stotr[idx] += qr;
+        Stoti[idx] += qi;
+    } else {
+        for (int l = 0; l < m; ++l) {
+            const long kl = idx + n * map[tj[start + l]];
+            Stotr[kl] += sr[l];
+            Stoti[kl] += si[l];
+        }
+    }
+}
+
+void launch_add_utot_pairs(double* Stotr, double* Stoti, const double* Sr, const double* Si, const int* pairnum,
+                           const int* pairnumsum, const int* map, const int* tj, int idxu_max, int inum, int ijnum,
+                           int chemflag, hipStream_t st) {
+    const long n = static_cast<long>(inum) * idxu_max;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_add_utot_pairs, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, Stotr, Stoti,
+                       Sr, Si, pairnum, pairnumsum, map, tj, idxu_max, inum, ijnum, chemflag);
+}
+
 }  // namespace mdp

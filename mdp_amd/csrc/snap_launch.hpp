@@ -175,6 +175,19 @@ void launch_pair_arrays(double* xij, const double* x, int* aii, int* ai, int* aj
                         const int* pairlist, const int* pairnumsum, const int* ilist, const int* atomtype,
                         const int* alist, int inum, int dim, hipStream_t st);
 
+// the fixed-stride forms of the upstream C++ core: neighbours of atom i at neighlist[l + jnum*i],
+// kept pairs of the ii-th atom at pairlist[l + jnum*ii] with pairnum[ii] of them
+void launch_pair_list_strided(int* pairnum, int* pairlist, const double* x, const double* rcutsq, const int* ilist,
+                              const int* neighlist, const int* neighnum, const int* atomtype, const int* alist,
+                              int inum, int jnum, int dim, int ntypes, hipStream_t st);
+void launch_pair_arrays_strided(double* xij, const double* x, int* aii, int* ai, int* aj, int* ti, int* tj,
+                                const int* pairnum, const int* pairlist, const int* pairnumsum, const int* ilist,
+                                const int* atomtype, const int* alist, int inum, int jnum, int dim, hipStream_t st);
+// Stot += S summed over each atom's pair range [pairnumsum[ii], pairnumsum[ii] + pairnum[ii])
+void launch_add_utot_pairs(double* Stotr, double* Stoti, const double* Sr, const double* Si, const int* pairnum,
+                           const int* pairnumsum, const int* map, const int* tj, int idxu_max, int inum, int ijnum,
+                           int chemflag, hipStream_t st);
+
 // AddWself2Ui on device (wself on the diagonal, mirror of rows 2mb < j)
 void launch_add_wself(const SnapDev& s, double* Ur, double* Ui, double wself, const int* type, int wselfall,
                       int chemflag, int nelem, int inum, hipStream_t st);

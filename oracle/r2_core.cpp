@@ -1,7 +1,8 @@
 // Checker only (tests): the upstream C++ core -- src/C++/Core/cpuCore.cpp, which gathers the CPU
 // kernels of the upstream C++ route into one translation unit -- compiled unmodified from its sources
-// where they lie, with C entry points for the double instantiations of the SNAP force path that the
-// backend-3 entry points of libmdpsnap (include/mdpsnap.h, part 5) replace.  Built by the Makefile into
+// where they lie, with C entry points for the double instantiations of the SNAP force path and of the
+// fitting driver's descriptor chain that the backend-3 entry points of libmdpsnap (include/mdpsnap.h,
+// part 5) replace.  Built by the Makefile into
 // _ref/ only when the upstream tree is present; never part of the product path.
 #include "cpuCore.cpp"
 
@@ -46,6 +47,93 @@ void r2_cpuSnapComputeFi(double *fatom, double *vatom, double *ylist_r, double *
     cpuSnapComputeFi(fatom, vatom, ylist_r, ylist_i, rootpqarray, rij, wjelem, radelem, rmin0, rfac0, rcutfac,
                      idxu_block, map, aii, ai, aj, ti, tj, twojmax, idxu_max, inum, anum, ijnum, switchflag,
                      chemflag);
+}
+
+// descriptor chain of ComputeSnap (src/C++/Descriptors/snappotential.cpp:762-910)
+void r2_cpuNeighPairList(int *pairnum, int *pairlist, double *x, double *rcutsq, int *ilist, int *neighlist,
+                         int *neighnum, int *atomtype, int *alist, int inum, int jnum, int dim, int ntypes) {
+    cpuNeighPairList(pairnum, pairlist, x, rcutsq, ilist, neighlist, neighnum, atomtype, alist, inum, jnum, dim,
+                     ntypes);
+}
+
+void r2_cpuCumsum(int *d_out, int *d_in, int *d_sums, int *d_incr, int length) {
+    (void)d_sums; (void)d_incr;
+    cpuCumsum(d_out, d_in, length);
+}
+
+void r2_cpuNeighPairs(double *xij, double *x, int *aii, int *ai, int *aj, int *ti, int *tj, int *pairnum,
+                      int *pairlist, int *pairnumsum, int *ilist, int *atomtype, int *alist, int inum, int jnum,
+                      int dim) {
+    cpuNeighPairs(xij, x, aii, ai, aj, ti, tj, pairnum, pairlist, pairnumsum, ilist, atomtype, alist, inum, jnum, dim);
+}
+
+void r2_cpuComputeSij(double *Sr, double *Si, double *Srx, double *Six, double *Sry, double *Siy, double *Srz,
+                      double *Siz, double *rootpqarray, double *rij, double *wjelem, double *radelem, double rmin0,
+                      double rfac0, double rcutfac, int *idxu_block, int *ti, int *tj, int twojmax, int idxu_max,
+                      int ijnum, int switchflag) {
+    cpuComputeSij(Sr, Si, Srx, Six, Sry, Siy, Srz, Siz, rootpqarray, rij, wjelem, radelem, rmin0, rfac0, rcutfac,
+                  idxu_block, ti, tj, twojmax, idxu_max, ijnum, switchflag);
+}
+
+void r2_cpuZeroUarraytot2(double *ulisttot_r, double *ulisttot_i, double wself, int *idxu_block, int *type, int *map,
+                          int *ai, int wselfall_flag, int chemflag, int idxu_max, int nelements, int twojmax,
+                          int inum) {
+    cpuZeroUarraytot2(ulisttot_r, ulisttot_i, wself, idxu_block, type, map, ai, wselfall_flag, chemflag, idxu_max,
+                      nelements, twojmax, inum);
+}
+
+void r2_cpuAddUarraytot(double *ulisttot_r, double *ulisttot_i, double *ulist_r, double *ulist_i, int *map, int *ai,
+                        int *tj, int idxu_max, int inum, int ijnum, int chemflag) {
+    cpuAddUarraytot(ulisttot_r, ulisttot_i, ulist_r, ulist_i, map, ai, tj, idxu_max, inum, ijnum, chemflag);
+}
+
+void r2_cpuAddUarraytotPairs(double *ulisttot_r, double *ulisttot_i, double *ulist_r, double *ulist_i, int *pairnum,
+                             int *pairnumsum, int *map, int *tj, int idxu_max, int nelements, int inum, int ijnum,
+                             int chemflag) {
+    cpuAddUarraytot(ulisttot_r, ulisttot_i, ulist_r, ulist_i, pairnum, pairnumsum, map, tj, idxu_max, nelements, inum,
+                    ijnum, chemflag);
+}
+
+void r2_cpuComputeZi2(double *zlist_r, double *zlist_i, double *ulisttot_r, double *ulisttot_i, double *cglist,
+                      int *idxz, int *idxu_block, int *idxcg_block, int twojmax, int idxu_max, int idxz_max,
+                      int nelements, int bnorm_flag, int inum) {
+    cpuComputeZi2(zlist_r, zlist_i, ulisttot_r, ulisttot_i, cglist, idxz, idxu_block, idxcg_block, twojmax, idxu_max,
+                  idxz_max, nelements, bnorm_flag, inum);
+}
+
+void r2_cpuComputeBi2(double *blist, double *zlist_r, double *zlist_i, double *ulisttot_r, double *ulisttot_i,
+                      double *bzero, int *ilist, int *type, int *map, int *idxb, int *idxu_block, int *idxz_block,
+                      int twojmax, int idxb_max, int idxu_max, int idxz_max, int nelements, int bzero_flag,
+                      int wselfall_flag, int chemflag, int inum) {
+    cpuComputeBi2(blist, zlist_r, zlist_i, ulisttot_r, ulisttot_i, bzero, ilist, type, map, idxb, idxu_block,
+                  idxz_block, twojmax, idxb_max, idxu_max, idxz_max, nelements, bzero_flag, wselfall_flag, chemflag,
+                  inum);
+}
+
+void r2_cpuComputeDbidrj(double *dblist, double *zlist_r, double *zlist_i, double *dulist_r, double *dulist_i,
+                         int *idxb, int *idxu_block, int *idxz_block, int *map, int *ai, int *tj, int twojmax,
+                         int idxb_max, int idxu_max, int idxz_max, int nelements, int bnorm_flag, int chemflag,
+                         int inum, int ijnum) {
+    cpuComputeDbidrj(dblist, zlist_r, zlist_i, dulist_r, dulist_i, idxb, idxu_block, idxz_block, map, ai, tj, twojmax,
+                     idxb_max, idxu_max, idxz_max, nelements, bnorm_flag, chemflag, inum, ijnum);
+}
+
+void r2_cpuSnapTallyBispectrum(double *bi, double *bispectrum, int *ilist, int *type, int inum, int ncoeff,
+                               int nperdim, int ntype, int quadraticflag) {
+    cpuSnapTallyBispectrum(bi, bispectrum, ilist, type, inum, ncoeff, nperdim, ntype, quadraticflag);
+}
+
+void r2_cpuSnapTallyBispectrumDeriv(double *db, double *bispectrum, double *dbdr, int *aii, int *ai, int *aj, int *ti,
+                                    int inum, int ijnum, int ncoeff, int nperdim, int ntype, int quadraticflag) {
+    cpuSnapTallyBispectrumDeriv(db, bispectrum, dbdr, aii, ai, aj, ti, inum, ijnum, ncoeff, nperdim, ntype,
+                                quadraticflag);
+}
+
+void r2_cpuSnapTallyBispectrumVirial(double *bv, double *bispectrum, double *dbdr, double *rij, int *aii, int *ai,
+                                     int *aj, int *ti, int inum, int ijnum, int ncoeff, int nperdim, int ntype,
+                                     int quadraticflag) {
+    cpuSnapTallyBispectrumVirial(bv, bispectrum, dbdr, rij, aii, ai, aj, ti, inum, ijnum, ncoeff, nperdim, ntype,
+                                 quadraticflag);
 }
 
 }  // extern "C"

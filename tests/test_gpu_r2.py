@@ -5,6 +5,13 @@ SnapComputeYi -> SnapComputeFi runs on the same inputs in both, and every stage 
 Ui and after the self weight, energies, Y, forces and the SoA half/half virial, within 1e-10 of the max
 norm.  The upstream core's energies and forces are also checked against the oracle's (same physics
 through the C route), and its total virial against the oracle's.
+
+The descriptor chain of the upstream fitting driver ComputeSnap (snappotential.cpp:762-910) --
+NeighborPairList -> Cumsum -> NeighborPairs -> ComputeSij -> ZeroUarraytot2 -> AddUarraytot (both the
+(ai, tj) and the (pairnum, pairnumsum) forms) -> ComputeZi2 -> ComputeBi2 -> ComputeDbidrj ->
+SnapTally{Bispectrum, BispectrumDeriv, BispectrumVirial} -- runs the same way on a ghost-extended,
+fixed-stride neighbour list: integer arrays bit for bit, the rest within 1e-10, with quadratic tallies
+and a two-type chemflag case.
 """
 import ctypes
 import os
@@ -114,3 +121,163 @@ def test_backend3_force_chain_vs_upstream_core(oracle, J, ncell, seed):
     assert relerr(ref["f"].reshape(N, 3), f1) < TOL
     vt = ref["v"].reshape(6, N).sum(axis=1)
     assert relerr(vt, v1.sum(axis=0)) < TOL
+
+
+# ---- the descriptor chain of the upstream fitting driver ComputeSnap (snappotential.cpp:762-910) ----
+
+DSIGS = {
+    "NeighPairList": [P] * 9 + [I] * 4,
+    "Cumsum": [P] * 4 + [I],
+    "NeighPairs": [P] * 13 + [I] * 3,
+    "ComputeSij": [P] * 12 + [D] * 3 + [P] * 3 + [I] * 4,
+    "ZeroUarraytot2": [P, P, D, P, P, P, P] + [I] * 6,
+    "AddUarraytot": [P] * 7 + [I] * 4,
+    "AddUarraytotPairs": [P] * 8 + [I] * 5,
+    "ComputeZi2": [P] * 8 + [I] * 6,
+    "ComputeBi2": [P] * 12 + [I] * 9,
+    "ComputeDbidrj": [P] * 11 + [I] * 9,
+    "SnapTallyBispectrum": [P] * 4 + [I] * 5,
+    "SnapTallyBispectrumDeriv": [P] * 7 + [I] * 6,
+    "SnapTallyBispectrumVirial": [P] * 8 + [I] * 6,
+}
+
+
+def _bind_d(lib, prefix):
+    out = {}
+    for name, argt in DSIGS.items():
+        f = getattr(lib, prefix + name)
+        f.argtypes = argt
+        f.restype = None
+        out[name] = f
+    return out
+
+
+def _strided_neighbours(x, box, rcut):
+    """Ghost-extended positions y, owners alist and a fixed-stride full neighbour list (the layout of the
+    upstream C++ neighbour builders): neighbours of atom i at neighlist[l + jnum*i], l < neighnum[i]."""
+    from mdp_amd import neighbors
+    B = np.diag(box)
+    y, alist, nl, nn = neighbors.fullneighborlist(x, B[0], B[1], B[2], [1, 1, 1], rcut)
+    n = len(x)
+    cnt = np.diff(nn)
+    jnum = int(cnt.max()) + 3   # stride above every count: the unused tail must stay untouched
+    neighlist = np.full(n * jnum, -5, np.int32)
+    for i in range(n):
+        neighlist[i * jnum:i * jnum + cnt[i]] = nl[nn[i]:nn[i + 1]]
+    return np.ascontiguousarray(y.ravel()), alist.astype(np.int32), neighlist, cnt.astype(np.int32), jnum
+
+
+def _run_descriptor_chain(fn, ptr, off, s, nb, nelem, nperdim, get_int):
+    """ComputeSnap's call sequence; ptr(name) is the pointer of an array, off(name, k) that pointer
+    advanced by k elements, get_int(name, i) one element of an int array."""
+    n, jnum, ntypes = nb["n"], nb["jnum"], s.ntypes
+    iu, iz, ib = s.idxu_max, s.idxz_max, s.idxb_max
+    fn["NeighPairList"](ptr("pairnum"), ptr("pairlist"), ptr("y"), ptr("rcutsq"), ptr("ilist"), ptr("neighlist"),
+                        ptr("neighnum"), ptr("type"), ptr("alist"), n, jnum, 3, ntypes)
+    fn["Cumsum"](ptr("pairnumsum"), ptr("pairnum"), ptr("scan1"), ptr("scan2"), n + 1)
+    ijnum = get_int("pairnumsum", n)
+    assert ijnum <= nb["maxpairs"]
+    fn["NeighPairs"](ptr("rij"), ptr("y"), ptr("aii"), ptr("ai"), ptr("aj"), ptr("ti"), ptr("tj"), ptr("pairnum"),
+                     ptr("pairlist"), ptr("pairnumsum"), ptr("ilist"), ptr("type"), ptr("alist"), n, jnum, 3)
+    m = iu * ijnum
+    fn["ComputeSij"](ptr("ur"), ptr("ui"), off("dur", 0), off("dui", 0), off("dur", m), off("dui", m),
+                     off("dur", 2 * m), off("dui", 2 * m), ptr("rootpq"), ptr("rij"), ptr("wj"), ptr("rad"), s.rmin0,
+                     s.rfac0, s.rcutfac, ptr("idxu_block"), ptr("ti"), ptr("tj"), s.twojmax, iu, ijnum, s.switchflag)
+    fn["ZeroUarraytot2"](ptr("Ur"), ptr("Ui"), s.wself, ptr("idxu_block"), ptr("type"), ptr("map"), ptr("ai"),
+                         s.wselfallflag, s.chemflag, iu, nelem, s.twojmax, n)
+    fn["AddUarraytot"](ptr("Ur"), ptr("Ui"), ptr("ur"), ptr("ui"), ptr("map"), ptr("ai"), ptr("tj"), iu, n, ijnum,
+                       s.chemflag)
+    # the per-atom gather form of AddUarraytot on a second total
+    fn["ZeroUarraytot2"](ptr("Ur2"), ptr("Ui2"), s.wself, ptr("idxu_block"), ptr("type"), ptr("map"), ptr("ai"),
+                         s.wselfallflag, s.chemflag, iu, nelem, s.twojmax, n)
+    fn["AddUarraytotPairs"](ptr("Ur2"), ptr("Ui2"), ptr("ur"), ptr("ui"), ptr("pairnum"), ptr("pairnumsum"),
+                            ptr("map"), ptr("tj"), iu, nelem, n, ijnum, s.chemflag)
+    fn["ComputeZi2"](ptr("zr"), ptr("zi"), ptr("Ur"), ptr("Ui"), ptr("cg"), ptr("idxz"), ptr("idxu_block"),
+                     ptr("idxcg_block"), s.twojmax, iu, iz, nelem, s.bnormflag, n)
+    fn["ComputeBi2"](ptr("blist"), ptr("zr"), ptr("zi"), ptr("Ur"), ptr("Ui"), ptr("bzero"), ptr("ilist"), ptr("type"),
+                     ptr("map"), ptr("idxb"), ptr("idxu_block"), ptr("idxz_block"), s.twojmax, ib, iu, iz, nelem,
+                     s.bzeroflag, s.wselfallflag, s.chemflag, n)
+    fn["ComputeDbidrj"](ptr("dblist"), ptr("zr"), ptr("zi"), ptr("dur"), ptr("dui"), ptr("idxb"), ptr("idxu_block"),
+                        ptr("idxz_block"), ptr("map"), ptr("ai"), ptr("tj"), s.twojmax, ib, iu, iz, nelem,
+                        s.bnormflag, s.chemflag, n, ijnum)
+    fn["SnapTallyBispectrum"](ptr("bi"), ptr("blist"), ptr("ilist"), ptr("type"), n, s.ncoeff, nperdim, ntypes,
+                              s.quadraticflag)
+    fn["SnapTallyBispectrumDeriv"](ptr("bd"), ptr("blist"), ptr("dblist"), ptr("aii"), ptr("ai"), ptr("aj"), ptr("ti"),
+                                   n, ijnum, s.ncoeff, nperdim, ntypes, s.quadraticflag)
+    fn["SnapTallyBispectrumVirial"](ptr("bv"), ptr("blist"), ptr("dblist"), ptr("rij"), ptr("aii"), ptr("ai"),
+                                    ptr("aj"), ptr("ti"), n, ijnum, s.ncoeff, nperdim, ntypes, s.quadraticflag)
+    return ijnum
+
+
+@pytest.mark.parametrize("J,ncell,seed,ntypes,quadratic", [(4, 3, 4, 1, 0), (8, 3, 5, 1, 1), (6, 2, 6, 1, 0),
+                                                           (4, 3, 7, 2, 1)])
+def test_backend3_descriptor_chain_vs_upstream_core(oracle, J, ncell, seed, ntypes, quadratic):
+    """Every array of the chain against the upstream core: the pair lists, the pair arrays and the
+    prefix sum bit for bit (including the unused tail of the fixed-stride pair list), the floating-point
+    arrays within 1e-10 of the max norm.  With two types, chemflag = 1 (element-resolved U, Z, B)."""
+    if not os.path.exists(R2LIB):
+        pytest.skip("upstream C++ core checker not built (oracle/_ref)")
+    from mdp_amd import library
+    from mdp_amd.devarray import DevArray
+    r2 = _bind_d(ctypes.CDLL(R2LIB), "r2_cpu")
+    gpu = _bind_d(library().cdll, "mdp_hip")
+    chem = 1 if ntypes > 1 else 0
+    param = [ntypes, J, RCUTFAC, RFAC0, 0, 1, 1, quadratic, chem, 0, 0]
+    rad, wj = [0.5, 0.45][:ntypes], [1.0, 0.8][:ntypes]
+    s = snapcpp.initsna(oracle, param, rad, wj)
+    x, box = lattice.crystal("bcc", ncell, 3.316, 0.1, seed)
+    n = len(x)
+    atype = np.ones(n, np.int32) if ntypes == 1 else np.random.default_rng(seed).integers(1, 3, n).astype(np.int32)
+    y, alist, neighlist, neighnum, jnum = _strided_neighbours(x, box, s.rcutmax)
+    typ = np.ascontiguousarray(atype[alist])   # type of every point, ghosts included
+    rcsq = np.zeros((ntypes + 1) ** 2)
+    for a in range(1, ntypes + 1):
+        for b in range(1, ntypes + 1):
+            c = (rad[a - 1] + rad[b - 1]) * RCUTFAC
+            rcsq[b + a * (ntypes + 1)] = c * c
+    nelem = s.nelements
+    ncoeff = s.ncoeff
+    nperdim = ncoeff + (ncoeff * (ncoeff + 1) // 2 if quadratic else 0)
+    maxp = int(neighnum.sum())
+    iu, iz, ib = s.idxu_max, s.idxz_max, s.idxb_max
+    nt = s.ntriples
+    ints = dict(pairnum=n, pairlist=n * jnum, pairnumsum=n + 1, scan1=n + 2, scan2=n + 2, aii=maxp, ai=maxp,
+                aj=maxp, ti=maxp, tj=maxp)
+    dbls = dict(rij=3 * maxp, ur=iu * maxp, ui=iu * maxp, dur=3 * iu * maxp, dui=3 * iu * maxp,
+                Ur=iu * nelem * n, Ui=iu * nelem * n, Ur2=iu * nelem * n, Ui2=iu * nelem * n,
+                zr=iz * s.ndoubles * n, zi=iz * s.ndoubles * n, blist=ib * nt * n, dblist=ib * nt * 3 * maxp,
+                bi=nperdim * ntypes, bd=n * 3 * nperdim * ntypes, bv=6 * nperdim * ntypes)
+    host = dict(y=y, rcutsq=rcsq, ilist=np.arange(n, dtype=np.int32), neighlist=neighlist, neighnum=neighnum,
+                type=typ, alist=alist, rootpq=s.rootpqarray, wj=s.wjelem, rad=s.radelem, cg=s.cglist,
+                bzero=s.bzero, idxu_block=s.idxu_block, idxb=s.idxb, idxz=s.idxz, idxz_block=s.idxz_block,
+                idxcg_block=s.idxcg_block, map=s.map)
+    host = {k: np.ascontiguousarray(v) for k, v in host.items()}
+    for k, size in ints.items():
+        host[k] = np.full(size, -7, np.int32)   # same sentinel on both sides: untouched entries compare equal
+    for k, size in dbls.items():
+        host[k] = np.zeros(size)
+    dev = {k: DevArray.from_numpy(v, v.dtype) for k, v in host.items()}
+    nb = dict(n=n, jnum=jnum, maxpairs=maxp)
+    P_r2 = _run_descriptor_chain(r2, lambda k: host[k].ctypes.data,
+                                 lambda k, o: host[k].ctypes.data + 8 * o, s, nb, nelem, nperdim,
+                                 lambda k, i: int(host[k][i]))
+    print(f"backend-3 descriptor chain J={J} ntypes={ntypes} quadratic={quadratic}: {n} atoms, {len(y)} points, "
+          f"{P_r2} pairs", flush=True)
+    P_gpu = _run_descriptor_chain(gpu, lambda k: dev[k].ptr, lambda k, o: dev[k].ptr.value + 8 * o, s, nb, nelem, nperdim,
+                                  lambda k, i: int(dev[k].numpy()[i]))
+    assert library().cdll.mdp_snap_last_error().decode() == ""
+    assert P_gpu == P_r2 and P_r2 > 0
+    for k in ints:
+        assert np.array_equal(dev[k].numpy(), host[k]), k
+    for k in ("rij",):
+        assert np.array_equal(dev[k].numpy(), host[k]), k
+    m = iu * P_r2
+    trims = dict(ur=m, ui=m, dur=3 * m, dui=3 * m, dblist=ib * nt * 3 * P_r2)
+    for k in dbls:
+        if k == "rij":
+            continue
+        a, b = dev[k].numpy(), host[k]
+        t = trims.get(k, len(b))
+        assert relerr(a[:t], b[:t]) < TOL, k
+    # both forms of AddUarraytot give the same totals
+    assert relerr(host["Ur2"], host["Ur"]) < TOL and relerr(host["Ui2"], host["Ui"]) < TOL
