@@ -299,6 +299,7 @@ struct mdp_fit {
             const bool j468 = (J == 4 || J == 6 || J == 8);
             dbdr_variant = (v == "1" && j468) ? 1 : ((v == "2" && (J == 6 || J == 8)) ? 2 : ((v == "3" && j468) ? 3 : 0));
             if (v == "4" && j468) dbdr_variant = 4;
+            if (v == "5" && j468) dbdr_variant = 5;
         }
         if ((env && std::string(env) == "0") || fv) return;
         int props_cu = 256;
@@ -308,10 +309,11 @@ struct mdp_fit {
             props_cu = prop.multiProcessorCount;
         const int cand[] = {props_cu * 2, props_cu * 4, props_cu * 8, props_cu * 12};
         // dB/dr variants: 0 table walk, 1 unrolled terms, 2 table walk at two waves per SIMD, 3 variant 0
-        // with the term loop pipelined, 4 generator form with the bispectrum indices split over two waves
+        // with the term loop pipelined, 4 generator form with the bispectrum indices split over two waves,
+        // 5 variant 4 with the atom's Z block staged in LDS
         std::vector<int> vars = {0};
-        if (J == 4) vars = {0, 1, 3, 4};
-        if (J == 6 || J == 8) vars = {0, 1, 2, 3, 4};
+        if (J == 4) vars = {0, 1, 3, 4, 5};
+        if (J == 6 || J == 8) vars = {0, 1, 2, 3, 4, 5};
         double best = 1e30;
         int pick = dbdr_groups, pickv = 0;
         std::string js = "{\"fit_dbdr\": {";

@@ -56,9 +56,12 @@ int fit_term_table_halves(int J, int bnorm, std::vector<int>& lo, std::vector<in
 void fit_term_table(int J, int bnorm, std::vector<int>& lo, std::vector<int>& b, std::vector<int>& z,
                     std::vector<double>& w);
 // dB/dr of every pair (per-configuration layout dblist[p0_c*3*idxb_max + lp + P_c*k + 3*P_c*jjb]) by a
-// persistent kernel of `groups` single-wave workgroups; counter must be 0 at launch.  variant 0 walks the
-// layer-grouped term table; variant 1 (twojmax 4, 6, 8) has every term unrolled at compile time;
-// variant 2 (twojmax 6, 8) is variant 0 limited to 256 registers (two waves per SIMD).
+// persistent kernel of `groups` workgroups; counter[0] (and counter[1] for variant 4) must be 0 at
+// launch.  variant 0 walks the layer-grouped term table; variant 1 (twojmax 4, 6, 8) has every term
+// unrolled at compile time; variant 2 (twojmax 6, 8) is variant 0 limited to 256 registers (two waves
+// per SIMD); variant 3 is variant 0 with the term loop pipelined; variant 4 forms dU/dr through the
+// SU(2) generators with the bispectrum indices split over two single-wave workgroups; variant 5 is
+// variant 4 with both halves in one two-wave workgroup that stages the atom's Z block in LDS.
 void launch_fit_dbdr(const SnapDev& s, const double* zt, const FitTerms& tt, const double* rij, const int* pti,
                      const int* ptj, const int* poff, const int* cfg_of, const int* cfg_p0, double* dblist,
                      int* counter, int N, int groups, hipStream_t st, int variant = 0);

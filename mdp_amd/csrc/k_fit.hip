@@ -516,6 +516,160 @@ __global__ __launch_bounds__(64) void k_fit_dbdr_g(const double* __restrict__ ro
     }
 }
 
+// As k_fit_dbdr_g with the atom's Z block staged in LDS.  A workgroup is two waves, one per half of
+// the bispectrum indices, working on the same atom: they copy its Z block (2 IZ doubles) from HBM with
+// wide vector loads once, and every term then reads its Z row from LDS instead of through dependent
+// scalar loads from L2.  LDS: the Z block, then the accumulators of half 0 (hb * 64) and of half 1;
+// two workgroups fit on a CU.
+template <int J>
+__global__ __launch_bounds__(128) void k_fit_dbdr_l(const double* __restrict__ rootpq, const double* __restrict__ zt,
+                                                    const int* __restrict__ h_lo, const int* __restrict__ h_b,
+                                                    const int* __restrict__ h_z, const double* __restrict__ h_w, int hb,
+                                                    const double* __restrict__ radelem,
+                                                    const double* __restrict__ wjelem, double rcutfac, double rmin0,
+                                                    double rfac0, int switchflag, const double* __restrict__ rij,
+                                                    const int* __restrict__ pti, const int* __restrict__ ptj,
+                                                    const int* __restrict__ poff, const int* __restrict__ cfg_of,
+                                                    const int* __restrict__ cfg_p0, double* __restrict__ dblist,
+                                                    int* __restrict__ counter, int N) {
+    constexpr int IZ = n_z(J), IB = n_b(J);
+    __shared__ __align__(16) double sm[2 * IZ + IB * 64];   // 66 KB at twojmax = 8
+    __shared__ int s_atom;
+    double* __restrict__ zs = sm;
+    const int lane = threadIdx.x & 63;
+    const int half = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    const int comp = lane % 3, slot = lane / 3;
+    const int b0 = half ? hb : 0, nbh = half ? IB - hb : hb;
+    double* __restrict__ acc = sm + 2 * IZ + b0 * 64;
+    const int* __restrict__ lo = h_lo + half * (J + 2);
+    for (;;) {
+        if (threadIdx.x == 0) s_atom = atomicAdd(counter, 1);
+        __syncthreads();
+        const int atom = __builtin_amdgcn_readfirstlane(s_atom);
+        if (atom >= N) break;   // the same value in both waves
+        {
+            const double2* __restrict__ src = reinterpret_cast<const double2*>(zt + static_cast<size_t>(atom) * IZ * 2);
+            double2* dst = reinterpret_cast<double2*>(zs);
+            for (int e = threadIdx.x; e < IZ; e += 128) dst[e] = src[e];
+        }
+        __syncthreads();   // Z staged; s_atom read by both waves
+        const int q0 = __builtin_amdgcn_readfirstlane(poff[atom]);
+        const int q1 = __builtin_amdgcn_readfirstlane(poff[atom + 1]);
+        const int c = __builtin_amdgcn_readfirstlane(cfg_of[atom]);
+        const int p0 = __builtin_amdgcn_readfirstlane(cfg_p0[c]);
+        const long Pc = __builtin_amdgcn_readfirstlane(cfg_p0[c + 1]) - p0;
+        double* __restrict__ db = dblist + static_cast<size_t>(p0) * 3 * IB;
+        for (int qb = q0; qb < q1; qb += 21) {
+            const int q = qb + slot;
+            const bool active = lane < 63 && q < q1;
+            const int p = active ? q : q0;   // idle lanes recompute a valid pair and store nothing
+            const int tjp = ptj[p];
+            const double rc = (radelem[pti[p]] + radelem[tjp]) * rcutfac;
+            PairGeom g;
+            pair_geometry<true>(rij[3 * p], rij[3 * p + 1], rij[3 * p + 2], rc, rmin0, rfac0, switchflag, wjelem[tjp], g);
+            const double dar = comp == 0 ? g.dar[0] : (comp == 1 ? g.dar[1] : g.dar[2]);
+            const double dai = comp == 0 ? g.dai[0] : (comp == 1 ? g.dai[1] : g.dai[2]);
+            const double dbr = comp == 0 ? g.dbr[0] : (comp == 1 ? g.dbr[1] : g.dbr[2]);
+            const double dbi = comp == 0 ? g.dbi[0] : (comp == 1 ? g.dbi[1] : g.dbi[2]);
+            const double uk = comp == 0 ? g.u[0] : (comp == 1 ? g.u[1] : g.u[2]);
+            const double sf = g.sfac, dsu = g.dsfac * uk;
+            const double k0 = sf * (-g.ai * dar + g.ar * dai + g.bi * dbr - g.br * dbi);
+            const double k1 = sf * (-g.br * dar - g.bi * dai + g.ar * dbr + g.ai * dbi);
+            const double k2 = sf * (-g.bi * dar + g.br * dai - g.ai * dbr + g.ar * dbi);
+            for (int k = 0; k < nbh; ++k) acc[k * 64 + lane] = 0.0;
+            row_chain<J, 0, 0, J / 2>(g, rootpq, [&](auto Jc, auto MBc, auto& pr, auto& pim, auto&, auto&) {
+                constexpr int L = decltype(Jc)::value, MB = decltype(MBc)::value;
+                constexpr int LAST = (L == 2 * MB) ? MB : L;
+                double dr[LAST + 1], di[LAST + 1];
+                static_for<0, LAST + 1>([&](auto MAc) {
+                    constexpr int MA = decltype(MAc)::value;
+                    constexpr double m0 = static_cast<double>(2 * MA - L);
+                    constexpr double cu = (MA < L) ? cx_sqrt(static_cast<double>((MA + 1) * (L - MA))) : 0.0;
+                    constexpr double cd = (MA > 0) ? cx_sqrt(static_cast<double>(MA * (L - MA + 1))) : 0.0;
+                    const double upr = (MA < L) ? pr[MA < L ? MA + 1 : MA] : 0.0;
+                    const double upi = (MA < L) ? pim[MA < L ? MA + 1 : MA] : 0.0;
+                    const double dnr = (MA > 0) ? pr[MA > 0 ? MA - 1 : 0] : 0.0;
+                    const double dni = (MA > 0) ? pim[MA > 0 ? MA - 1 : 0] : 0.0;
+                    const double l1r = cu * upr - cd * dnr, l1i = cu * upi - cd * dni;
+                    const double l2r = -(cu * upi + cd * dni), l2i = cu * upr + cd * dnr;
+                    dr[MA] = dsu * pr[MA] + k0 * (-m0 * pim[MA]) + k1 * l1r + k2 * l2r;
+                    di[MA] = dsu * pim[MA] + k0 * (m0 * pr[MA]) + k1 * l1i + k2 * l2i;
+                });
+                if constexpr (L == 2 * MB) {   // middle column of an even layer counts 1/2
+                    dr[MB] *= 0.5;
+                    di[MB] *= 0.5;
+                }
+                // Two terms per step (two independent chains), each summed as two FMAs per element.  The
+                // next pair's Z rows and table entries are read from LDS one step ahead, and both
+                // accumulators are read before the chains; when the two terms share a bispectrum index their
+                // updates are merged, in the sequential order.
+                const int ka = lo[L], kb = lo[L + 1];
+                const int zo = 2 * (L + 1) * MB;
+                int k = ka;
+                if (kb - ka >= 2) {
+                    double na[2 * (LAST + 1)], nc[2 * (LAST + 1)], nwa, nwc;
+                    int nba, nbc;
+                    auto load = [&](int kk) {
+                        const double* za = zs + 2 * h_z[kk] + zo;
+                        const double* zc = zs + 2 * h_z[kk + 1] + zo;
+#pragma unroll
+                        for (int m = 0; m < 2 * (LAST + 1); ++m) {
+                            na[m] = za[m];
+                            nc[m] = zc[m];
+                        }
+                        nba = h_b[kk] - b0;
+                        nbc = h_b[kk + 1] - b0;
+                        nwa = h_w[kk];
+                        nwc = h_w[kk + 1];
+                    };
+                    load(k);
+                    for (; k + 2 <= kb; k += 2) {
+                        double za[2 * (LAST + 1)], zc[2 * (LAST + 1)];
+#pragma unroll
+                        for (int m = 0; m < 2 * (LAST + 1); ++m) {
+                            za[m] = na[m];
+                            zc[m] = nc[m];
+                        }
+                        const int ba = nba, bc = nbc;
+                        const double wa = nwa, wc = nwc;
+                        if (k + 4 <= kb) load(k + 2);
+                        double* pa = acc + ba * 64 + lane;
+                        double* pc = acc + bc * 64 + lane;
+                        const double ra = *pa, rc = *pc;
+                        double sa = 0.0, sc = 0.0;
+#pragma unroll
+                        for (int ma = 0; ma <= LAST; ++ma) {
+                            sa = fma(dr[ma], za[2 * ma], sa);
+                            sc = fma(dr[ma], zc[2 * ma], sc);
+                            sa = fma(di[ma], za[2 * ma + 1], sa);
+                            sc = fma(di[ma], zc[2 * ma + 1], sc);
+                        }
+                        const double va = ra + wa * sa;
+                        if (ba == bc) {
+                            *pa = va + wc * sc;
+                        } else {
+                            *pa = va;
+                            *pc = rc + wc * sc;
+                        }
+                    }
+                }
+                if (k < kb) {
+                    const double* za = zs + 2 * h_z[k] + zo;
+                    double sa = 0.0;
+#pragma unroll
+                    for (int ma = 0; ma <= LAST; ++ma) sa = fma(di[ma], za[2 * ma + 1], fma(dr[ma], za[2 * ma], sa));
+                    acc[(h_b[k] - b0) * 64 + lane] += h_w[k] * sa;
+                }
+            });
+            if (active) {
+                const size_t lp = static_cast<size_t>(q - p0);
+                for (int k = 0; k < nbh; ++k) db[lp + Pc * comp + 3 * Pc * (b0 + k)] = acc[k * 64 + lane];
+            }
+        }
+        __syncthreads();   // both waves done with this Z block before the next atom's copy
+    }
+}
+
 // As k_fit_dbdr with the terms of every row enumerated at compile time: each term's Z row is a
 // contiguous run of the atom's Z block at a constant offset, so it arrives in wide scalar loads, and
 // the accumulator of each term sits at a constant LDS offset.  wtab[3*t + w] = weight of term w of t.
@@ -754,6 +908,21 @@ void launch_fit_dbdr(const SnapDev& s, const double* zt, const FitTerms& tt, con
             D4_CASE(6)
             D4_CASE(8)
 #undef D4_CASE
+            default: break;
+        }
+    }
+    if (variant == 5 && tt.h_lo) {   // variant 4 with the atom's Z block staged in LDS, two waves per atom
+        switch (s.J) {
+#define D5_CASE(JV)                                                                                              \
+    case JV:                                                                                                     \
+        hipLaunchKernelGGL((k_fit_dbdr_l<JV>), dim3(G), dim3(128), 0, st, s.rootpq, zt, tt.h_lo, tt.h_b, tt.h_z,    \
+                           tt.h_w, tt.hb, s.radelem, s.wjelem, s.rcutfac, s.rmin0, s.rfac0, s.switchflag, rij, pti,  \
+                           ptj, poff, cfg_of, cfg_p0, dblist, counter, N);                                        \
+        return;
+            D5_CASE(4)
+            D5_CASE(6)
+            D5_CASE(8)
+#undef D5_CASE
             default: break;
         }
     }

@@ -7,8 +7,9 @@
   cpuZeroUarraytot2 -> cpuAddUarraytot -> cpuComputeZi2 -> cpuComputeBi2 -> cpuComputeDbidrj on the
   same pairs, within 1e-10 relative to the max norm of the oracle array.
 * At the full 512-configuration size: a sample of configurations against the oracle, and batch
-  independence (the reversed batch gives bit-identical per-configuration outputs, since nothing is
-  accumulated across atoms or configurations).
+  independence (the reversed batch gives bit-identical per-configuration outputs with the same dB/dr
+  variant, since nothing is accumulated across atoms or configurations).
+* The dB/dr variants the autotuner chooses from agree with each other to rounding.
 * Graph replays after new positions (including pair-list growth past the buffers) equal a fresh run.
 """
 import ctypes
@@ -87,7 +88,8 @@ def _check_config(oracle, fit, c, cfgs, b, db, ai, aj, rij, J, bnorm=0, bzero=1)
 
 
 @pytest.mark.parametrize("J,variant", [(4, "0"), (6, "0"), (8, "0"), (4, "1"), (6, "1"), (8, "1"), (8, "2"),
-                                       (4, "3"), (8, "3"), (4, "4"), (6, "4"), (8, "4")])
+                                       (4, "3"), (8, "3"), (4, "4"), (6, "4"), (8, "4"), (4, "5"), (6, "5"),
+                                       (8, "5")])
 def test_small_batch_vs_oracle(oracle, monkeypatch, J, variant):
     from mdp_amd.fit import SnapFit
     monkeypatch.setenv("MDP_FIT_DBDR_VARIANT", variant)   # each dB/dr kernel variant in turn
@@ -99,6 +101,23 @@ def test_small_batch_vs_oracle(oracle, monkeypatch, J, variant):
     ai, aj, rij = fit.pairs()
     for c in range(len(cfg)):
         _check_config(oracle, fit, c, cfg, b, db, ai, aj, rij, J)
+
+
+@pytest.mark.parametrize("J", [4, 8])
+def test_dbdr_variants_agree(monkeypatch, J):
+    """The autotuner may pick any dB/dr variant: B is the same kernel in all of them (bit for bit), and
+    the dB/dr variants agree to rounding (the compiler contracts their term sums differently)."""
+    from mdp_amd.fit import SnapFit
+    cfg = _small_batch(9)
+    out = {}
+    for v in ("0", "3", "4", "5"):
+        monkeypatch.setenv("MDP_FIT_DBDR_VARIANT", v)
+        fit = SnapFit(cfg, J)
+        fit.compute()
+        out[v] = fit.raw()
+    for v in ("3", "4", "5"):
+        assert np.array_equal(out[v][0], out["0"][0]), v
+        assert relerr(out[v][1], out["0"][1]) < 1e-13, v
 
 
 def test_bnorm_and_no_bzero(oracle):
@@ -138,7 +157,7 @@ def test_replay_after_new_positions_and_pair_growth(oracle):
         _check_config(oracle, fit, c, new, b, db, ai, aj, rij, 6)
 
 
-def test_fitting_batch_512(oracle):
+def test_fitting_batch_512(oracle, monkeypatch):
     from mdp_amd.fit import SnapFit
     batch = lattice.fitting_batch()
     cfg = [(x, box) for x, box, _ in batch]
@@ -159,7 +178,11 @@ def test_fitting_batch_512(oracle):
     for c in sorted(sample):
         print(f"  config {c}: {sizes[c]} atoms, {p_off[c + 1] - p_off[c]} pairs", flush=True)
         _check_config(oracle, fit, c, cfg, b, db, ai, aj, rij, 8)
-    # batch independence: the reversed batch reproduces every configuration bit for bit
+    # batch independence: the reversed batch, with the dB/dr variant the autotuner chose above,
+    # reproduces every configuration bit for bit
+    chosen = fit.tuning().get("chosen", {}).get("fit_dbdr_variant")
+    if chosen is not None:
+        monkeypatch.setenv("MDP_FIT_DBDR_VARIANT", str(chosen))
     rev = SnapFit(cfg[::-1], 8)
     rev.compute()
     rb, rdb = rev.raw()
@@ -168,4 +191,4 @@ def test_fitting_batch_512(oracle):
         b1, d1, _ = fit.config(c, b, db)
         b2, d2, _ = rev.config(C - 1 - c, rb, rdb)
         assert np.array_equal(b1, b2) and np.array_equal(d1, d2), c
-    print("  tuning:", fit.tuning(), "profile:", fit.profile(), flush=True)
+    print("  tuning:", chosen, "profile:", fit.profile(), flush=True)
