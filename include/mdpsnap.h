@@ -267,6 +267,27 @@ int mdp_fit_pairs(mdp_fit *h, int *ai, int *aj, double *rij);
 int mdp_fit_tuning(mdp_fit *h, char *buf, int len);
 /* ms of one uncaptured run of each stage: [0] pairs, [1] Utot, [2] Z+B, [3] dB/dr. */
 int mdp_fit_profile(mdp_fit *h, double *ms4);
+/* Tallies of the last compute, as the upstream descriptor driver returns them for every configuration
+ * (snapcpp.jl:645-664 with cpuSnapTallyBispectrum, ...Deriv, ...Virial, cpuSnap.cpp:2265-2470), with
+ * Kt = ntypes*idxb_max columns, column (t-1)*idxb_max + k for the atoms (pairs) of type t (owner type):
+ * bi[c*Kt + col]; bd block of configuration c at 3*atom_off[c]*Kt, element [comp + 3*ii + 3*N_c*col]
+ * (sign as upstream: + on the pair's owner, - on its neighbour, so forces are bd . beta); bv[c*6*Kt +
+ * 6*col + m], m = xx, yy, zz, xy, xz, yz of -r (x) dB/dr.  Any pointer may be NULL. */
+int mdp_fit_descriptors(mdp_fit *h, double *bi, double *bd, double *bv);
+/* Weighted normal equations of the linear fit (linearfit.jl:168-200) from the last compute:
+ * L[g*Kt*Kt + k*Kt + l] = sum_r w_r^2 a_r[k] a_r[l], hv[g*Kt + k] = sum_r w_r^2 a_r[k] b_r for the groups
+ * g = 0 energy (rows bi_c, targets e[c], both divided by N_c when normalizeenergy), 1 force (the 3 N_c
+ * rows of bd_c, targets f[3*atom + comp] over the batch), 2 stress (the virial rows in the upstream order
+ * xx, yy, zz, yz, xz, xy (snapcpp.jl:660-662), targets s[6c + m]).  Weights per configuration (we, wf,
+ * ws; NULL = 1).  A NULL target array leaves its group zero.  Sums run in a fixed order. */
+int mdp_fit_normal_equations(mdp_fit *h, const double *e, const double *f, const double *s, const double *we,
+                             const double *wf, const double *ws, int normalizeenergy, double *L, double *hv);
+/* lsqsolve (linearfit.jl:29-75) on the normal equations: ce = Le\he, cf = Lf\hf, cef = (Le+Lf)\(he+hf),
+ * cefs = (Le+Lf+Ls)\(he+hf+hs), each when its inputs are not all zero, the diagonal scaled by 1 + 1e-15
+ * (and a(0,0) of the force system raised to 1e-10 when smaller), LU with partial pivoting.  solved[4]
+ * flags which were computed; NULL outputs are skipped.  Host only. */
+int mdp_lsqsolve(int K, const double *Le, const double *Lf, const double *Ls, const double *he, const double *hf,
+                 const double *hs, double *ce, double *cf, double *cef, double *cefs, int *solved);
 
 /* ---------------------------------------------------------------------------------------------- */
 /* Part 4: learned surrogate kernels (no upstream counterpart)                                     */

@@ -48,6 +48,8 @@ struct mdp_fit {
     FitTerms terms;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    DevBuf l_cnt, l_off, l_cur, l_idx, l_bi, l_bd, l_bv, l_as, l_chunks, l_ekl, l_part, l_out, l_T, l_W;
+    int l_ekl_k = 0;         // column count the (k, l) table was built for
     int dbdr_groups = 1024;
     bool tuned = false;
     std::string tuning_json = "{}";
@@ -368,6 +370,105 @@ struct mdp_fit {
         MDP_HIP(hipGraphLaunch(gexec, st));
     }
 
+    // Tallies of the last compute for the linear fit (upstream cpuSnapTallyBispectrum{,Deriv,Virial}
+    // layouts per configuration) and the stress rows of the fit.
+    void lsq_tallies() {
+        if (!gexec) throw Error("mdp_fit_descriptors: call mdp_fit_compute first");
+        const int IB = tab.idx.idxb_max, kt = ntypes * IB;
+        l_cnt.ensure(sizeof(int) * N);
+        l_off.ensure(sizeof(int) * (N + 1));
+        l_cur.ensure(sizeof(int) * N);
+        l_idx.ensure(sizeof(int) * std::max<long>(npairs, 1));
+        l_bi.ensure(sizeof(double) * C * kt);
+        l_bd.ensure(sizeof(double) * 3 * static_cast<size_t>(N) * kt);
+        l_bv.ensure(sizeof(double) * 6 * static_cast<size_t>(C) * kt);
+        l_as.ensure(sizeof(double) * 6 * static_cast<size_t>(C) * kt);
+        launch_lsq_incoming(pj.as<int>(), npairs, N, l_cnt.as<int>(), l_off.as<int>(), l_cur.as<int>(), l_idx.as<int>(),
+                            scan_tmp.as<int>(), st);
+        launch_lsq_tallies(blist.as<double>(), dblist.as<double>(), rij.as<double>(), poff.as<int>(), l_off.as<int>(),
+                           l_idx.as<int>(), pti.as<int>(), type.as<int>(), cfg_of.as<int>(), cfg_a0.as<int>(),
+                           cfg_p0.as<int>(), C, N, IB, ntypes, l_bi.as<double>(), l_bd.as<double>(), l_bv.as<double>(),
+                           l_as.as<double>(), st);
+        MDP_HIP(hipGetLastError());
+    }
+
+    // Weighted normal equations of the energy, force and stress rows (linearfit.jl:168-200):
+    // L[g] = sum_r w_r^2 a_r a_r^T and h[g] = sum_r w_r^2 a_r b_r for g = energy, force, stress.  Energy
+    // rows a = bi_c with target e_c (both / N_c under normalizeenergy, emldescriptors.jl:108-128), force
+    // rows the 3 N_c rows of bd_c with targets f (AoS per atom), stress rows the 6 reordered virial rows
+    // with targets s[6c + m'].  A null target array leaves its group zero; null weights are 1.
+    void lsq_normal(const double* e, const double* f, const double* s, const double* we, const double* wf,
+                    const double* ws, int normalizeenergy, double* L, double* hv) {
+        lsq_tallies();
+        const int K = ntypes * tab.idx.idxb_max;
+        if (lsq_gram_lds(K) > 65536) throw Error("mdp_fit_normal_equations: too many descriptor columns");
+        const int ntri = K * (K + 1) / 2, nout = ntri + K, G = 1024;
+        if (l_ekl_k != K) {
+            std::vector<int2> ekl(nout);
+            int e0 = 0;
+            for (int k = 0; k < K; ++k)
+                for (int l = k; l < K; ++l) ekl[e0++] = make_int2(k, l);
+            for (int k = 0; k < K; ++k) ekl[ntri + k] = make_int2(k, -1);
+            l_ekl.ensure(sizeof(int2) * nout);
+            MDP_HIP(hipMemcpyAsync(l_ekl.p, ekl.data(), sizeof(int2) * nout, hipMemcpyHostToDevice, st));
+            MDP_HIP(hipStreamSynchronize(st));
+            l_ekl_k = K;
+        }
+        l_part.ensure(sizeof(double) * static_cast<size_t>(G) * nout);
+        l_out.ensure(sizeof(double) * nout);
+        std::fill(L, L + 3 * static_cast<size_t>(K) * K, 0.0);
+        std::fill(hv, hv + 3 * static_cast<size_t>(K), 0.0);
+        auto run = [&](int grp, const double* A, const std::vector<LsqChunk>& ch, const double* T, size_t nT,
+                       const std::vector<double>& W) {
+            if (ch.empty()) return;
+            l_chunks.ensure(sizeof(LsqChunk) * ch.size());
+            l_T.ensure(sizeof(double) * nT);
+            l_W.ensure(sizeof(double) * W.size());
+            MDP_HIP(hipMemcpyAsync(l_chunks.p, ch.data(), sizeof(LsqChunk) * ch.size(), hipMemcpyHostToDevice, st));
+            MDP_HIP(hipMemcpyAsync(l_T.p, T, sizeof(double) * nT, hipMemcpyHostToDevice, st));
+            MDP_HIP(hipMemcpyAsync(l_W.p, W.data(), sizeof(double) * W.size(), hipMemcpyHostToDevice, st));
+            launch_lsq_gram(l_chunks.as<LsqChunk>(), static_cast<int>(ch.size()), A, l_T.as<double>(), l_W.as<double>(),
+                            K, l_ekl.as<int2>(), l_part.as<double>(), G, l_out.as<double>(), st);
+            std::vector<double> o(nout);
+            MDP_HIP(hipMemcpyAsync(o.data(), l_out.p, sizeof(double) * nout, hipMemcpyDeviceToHost, st));
+            MDP_HIP(hipStreamSynchronize(st));
+            double* Lg = L + static_cast<size_t>(grp) * K * K;
+            int e0 = 0;
+            for (int k = 0; k < K; ++k)
+                for (int l = k; l < K; ++l, ++e0) Lg[k * K + l] = Lg[l * K + k] = o[e0];
+            for (int k = 0; k < K; ++k) hv[static_cast<size_t>(grp) * K + k] = o[ntri + k];
+        };
+        if (e) {   // one row per configuration, 64 per chunk
+            std::vector<LsqChunk> ch;
+            std::vector<double> W(C);
+            for (int c = 0; c < C; ++c)
+                W[c] = (we ? we[c] : 1.0) / (normalizeenergy ? static_cast<double>(a0_h[c + 1] - a0_h[c]) : 1.0);
+            for (int c0 = 0; c0 < C; c0 += 64)
+                ch.push_back(LsqChunk{static_cast<long>(c0) * K, c0, K, 1, std::min(64, C - c0), c0, 1, 0});
+            run(0, l_bi.as<double>(), ch, e, C, W);
+        }
+        if (f) {   // 3 N_c rows per configuration, column stride 3 N_c
+            std::vector<LsqChunk> ch;
+            std::vector<double> W(C);
+            for (int c = 0; c < C; ++c) {
+                W[c] = wf ? wf[c] : 1.0;
+                const int a0 = a0_h[c], rows = 3 * (a0_h[c + 1] - a0);
+                for (int r0 = 0; r0 < rows; r0 += 64)
+                    ch.push_back(LsqChunk{3L * a0 * K + r0, 3L * a0 + r0, 1, rows, std::min(64, rows - r0), c,
+                                          1 << 30, 0});
+            }
+            run(1, l_bd.as<double>(), ch, f, 3 * static_cast<size_t>(N), W);
+        }
+        if (s) {   // 6 rows per configuration, 60 (ten configurations) per chunk
+            std::vector<LsqChunk> ch;
+            std::vector<double> W(C);
+            for (int c = 0; c < C; ++c) W[c] = ws ? ws[c] : 1.0;
+            for (int r0 = 0; r0 < 6 * C; r0 += 60)
+                ch.push_back(LsqChunk{static_cast<long>(r0) * K, r0, K, 1, std::min(60, 6 * C - r0), r0 / 6, 6, 0});
+            run(2, l_as.as<double>(), ch, s, 6 * static_cast<size_t>(C), W);
+        }
+    }
+
     void profile() {
         if (!gexec) compute();
         for (int k = 0; k < 4; ++k) stage_ms[k] = time_stage(k, 1);
@@ -476,10 +577,104 @@ int mdp_fit_tuning(mdp_fit* h, char* buf, int len) {
     });
 }
 
+int mdp_fit_descriptors(mdp_fit* h, double* bi, double* bd, double* bv) {
+    return api([&] {
+        if (!h) throw Error("mdp_fit_descriptors: null handle");
+        h->lsq_tallies();
+        const size_t kt = static_cast<size_t>(h->ntypes) * h->tab.idx.idxb_max;
+        if (bi) MDP_HIP(hipMemcpyAsync(bi, h->l_bi.p, sizeof(double) * h->C * kt, hipMemcpyDeviceToHost, h->st));
+        if (bd)
+            MDP_HIP(hipMemcpyAsync(bd, h->l_bd.p, sizeof(double) * 3 * h->N * kt, hipMemcpyDeviceToHost, h->st));
+        if (bv) MDP_HIP(hipMemcpyAsync(bv, h->l_bv.p, sizeof(double) * 6 * h->C * kt, hipMemcpyDeviceToHost, h->st));
+        MDP_HIP(hipStreamSynchronize(h->st));
+    });
+}
+
+int mdp_fit_normal_equations(mdp_fit* h, const double* e, const double* f, const double* s, const double* we,
+                             const double* wf, const double* ws, int normalizeenergy, double* L, double* hv) {
+    return api([&] {
+        if (!h || !L || !hv) throw Error("mdp_fit_normal_equations: null handle or output");
+        h->lsq_normal(e, f, s, we, wf, ws, normalizeenergy, L, hv);
+    });
+}
+
 int mdp_fit_profile(mdp_fit* h, double* ms4) {
     return api([&] {
         h->profile();
         for (int k = 0; k < 4; ++k) ms4[k] = h->stage_ms[k];
+    });
+}
+
+// lsqsolve of linearfit.jl:29-75 on the assembled normal equations: each system is solved when its
+// matrices and right-hand sides are not all zero (sum |.| > 1e-15), with the diagonal scaled by
+// (1 + 1e-15) and, for the force-only system, a(0,0) raised to 1e-10 when smaller in magnitude.  The
+// solve is LU with partial pivoting (what the upstream `\` does for a square dense matrix).
+// solved[0..3] says which of ce, cf, cef, cefs were computed.
+int mdp_lsqsolve(int K, const double* Le, const double* Lf, const double* Ls, const double* he, const double* hf,
+                 const double* hs, double* ce, double* cf, double* cef, double* cefs, int* solved) {
+    return api([&] {
+        if (K <= 0) throw Error("mdp_lsqsolve: K must be positive");
+        const size_t kk = static_cast<size_t>(K) * K;
+        auto nonzero = [](const double* a, size_t n) {
+            double t = 0.0;
+            if (a)
+                for (size_t i = 0; i < n; ++i) t += std::fabs(a[i]);
+            return t > 1e-15;
+        };
+        const bool sAe = nonzero(Le, kk), sAf = nonzero(Lf, kk), sAs = nonzero(Ls, kk);
+        const bool sbe = nonzero(he, K), sbf = nonzero(hf, K), sbs = nonzero(hs, K);
+        auto solve = [&](std::initializer_list<const double*> As, std::initializer_list<const double*> bs, bool fguard,
+                         double* x) {
+            std::vector<double> A(kk, 0.0), b(K, 0.0);
+            for (const double* a : As)
+                for (size_t i = 0; i < kk; ++i) A[i] += a[i];
+            for (const double* v : bs)
+                for (int i = 0; i < K; ++i) b[i] += v[i];
+            for (int i = 0; i < K; ++i) A[static_cast<size_t>(i) * K + i] *= 1.0 + 1e-15;
+            if (fguard && std::fabs(A[0]) <= 1e-10) A[0] = 1e-10;
+            std::vector<int> piv(K);
+            for (int c = 0; c < K; ++c) {   // LU with partial pivoting, row-major
+                int pr = c;
+                for (int r = c + 1; r < K; ++r)
+                    if (std::fabs(A[static_cast<size_t>(r) * K + c]) > std::fabs(A[static_cast<size_t>(pr) * K + c])) pr = r;
+                if (A[static_cast<size_t>(pr) * K + c] == 0.0) throw Error("mdp_lsqsolve: singular matrix");
+                if (pr != c) {
+                    for (int j = 0; j < K; ++j) std::swap(A[static_cast<size_t>(c) * K + j], A[static_cast<size_t>(pr) * K + j]);
+                    std::swap(b[c], b[pr]);
+                }
+                const double d = A[static_cast<size_t>(c) * K + c];
+                for (int r = c + 1; r < K; ++r) {
+                    const double m = A[static_cast<size_t>(r) * K + c] / d;
+                    A[static_cast<size_t>(r) * K + c] = m;
+                    for (int j = c + 1; j < K; ++j) A[static_cast<size_t>(r) * K + j] -= m * A[static_cast<size_t>(c) * K + j];
+                    b[r] -= m * b[c];
+                }
+            }
+            for (int r = K - 1; r >= 0; --r) {
+                double t = b[r];
+                for (int j = r + 1; j < K; ++j) t -= A[static_cast<size_t>(r) * K + j] * x[j];
+                x[r] = t / A[static_cast<size_t>(r) * K + r];
+            }
+        };
+        int ok[4] = {0, 0, 0, 0};
+        if (sAe && sAf && sAs && sbe && sbf && sbs && cefs) {
+            solve({Le, Lf, Ls}, {he, hf, hs}, false, cefs);
+            ok[3] = 1;
+        }
+        if (sAe && sAf && sbe && sbf && cef) {
+            solve({Le, Lf}, {he, hf}, false, cef);
+            ok[2] = 1;
+        }
+        if (sAe && sbe && ce) {
+            solve({Le}, {he}, false, ce);
+            ok[0] = 1;
+        }
+        if (sAf && sbf && cf) {
+            solve({Lf}, {hf}, true, cf);
+            ok[1] = 1;
+        }
+        if (solved)
+            for (int i = 0; i < 4; ++i) solved[i] = ok[i];
     });
 }
 

@@ -66,4 +66,27 @@ void launch_fit_dbdr(const SnapDev& s, const double* zt, const FitTerms& tt, con
                      const int* ptj, const int* poff, const int* cfg_of, const int* cfg_p0, double* dblist,
                      int* counter, int N, int groups, hipStream_t st, int variant = 0);
 
+// ---- linear fit over the batched descriptors (k_fitlsq.hip) ----
+// one chunk of rows of a weighted least-squares group: a(r, k) = A[abase + r*rs + k*cs], target
+// T[tbase + r], weight W[wbase + r / wdiv], r < nrows <= 64
+struct LsqChunk {
+    long abase, tbase;
+    int rs, cs, nrows, wbase, wdiv, pad;
+};
+// incoming pairs of every atom, ascending pair index: in_idx[in_off[a] .. in_off[a+1]) (in_cnt, cursor:
+// N ints of scratch; scan_tmp: scan_tmp_ints(N) ints)
+void launch_lsq_incoming(const int* pj, long P, int N, int* in_cnt, int* in_off, int* cursor, int* in_idx,
+                         int* scan_tmp, hipStream_t st);
+// per configuration c: bi[c*Kt + col], bd at 3*a0_c*Kt ([comp + 3 ii + 3 N_c col]), bv[c*6*Kt + 6 col + m]
+// (upstream tally layouts, Kt = ntypes*IB, col = (type-1)*IB + k) and the stress rows as[(6c + m')*Kt + col]
+void launch_lsq_tallies(const double* blist, const double* dblist, const double* rij, const int* poff,
+                        const int* in_off, const int* in_idx, const int* pti, const int* type, const int* cfg_of,
+                        const int* cfg_a0, const int* cfg_p0, int C, int N, int IB, int ntypes, double* bi, double* bd,
+                        double* bv, double* as, hipStream_t st);
+// out[0 .. K(K+1)/2) = upper triangle of sum w^2 a a^T (row-major), out[K(K+1)/2 + k] = sum w^2 a_k b;
+// ekl[e] = (k, l) of output e (l = -1 for h); part: G * (K(K+1)/2 + K) doubles
+size_t lsq_gram_lds(int K);
+void launch_lsq_gram(const LsqChunk* chunks, int nchunks, const double* A, const double* T, const double* W, int K,
+                     const int2* ekl, double* part, int G, double* out, hipStream_t st);
+
 }  // namespace mdp

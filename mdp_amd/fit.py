@@ -39,6 +39,9 @@ def _bind():
     c.mdp_fit_pairs.argtypes = [vp, vp, vp, vp]
     c.mdp_fit_tuning.argtypes = [vp, ctypes.c_char_p, ctypes.c_int]
     c.mdp_fit_profile.argtypes = [vp, vp]
+    c.mdp_fit_descriptors.argtypes = [vp, vp, vp, vp]
+    c.mdp_fit_normal_equations.argtypes = [vp] * 7 + [ctypes.c_int, vp, vp]
+    c.mdp_lsqsolve.argtypes = [ctypes.c_int] + [vp] * 11
     _bound = True
     return c
 
@@ -152,3 +155,61 @@ class SnapFit:
         ms = np.zeros(4)
         self._check(self._c.mdp_fit_profile(self._h, ms.ctypes.data), "mdp_fit_profile")
         return dict(zip(("pairs", "utot", "zb", "dbdr"), (float(v) for v in ms)))
+
+    def descriptors(self):
+        """Per-configuration tallies of the last compute, as the upstream descriptor driver returns them
+        (snapcpp.jl:645-664): a list of (bi (Kt,), bd (3, N_c, Kt), bv (6, Kt) in the row order xx, yy, zz,
+        yz, xz, xy), Kt = ntypes * ncoeff."""
+        kt = len(self._keep[0]) * self.ncoeff
+        N = int(self.natoms.sum())
+        bi = np.zeros(self.C * kt)
+        bd = np.zeros(3 * N * kt)
+        bv = np.zeros(6 * self.C * kt)
+        self._check(self._c.mdp_fit_descriptors(self._h, bi.ctypes.data, bd.ctypes.data, bv.ctypes.data),
+                    "mdp_fit_descriptors")
+        out, a0 = [], 0
+        for c in range(self.C):
+            n = int(self.natoms[c])
+            d = bd[3 * a0 * kt:3 * (a0 + n) * kt].reshape(kt, n, 3).transpose(2, 1, 0)
+            v = bv[6 * c * kt:6 * (c + 1) * kt].reshape(kt, 6).T[[0, 1, 2, 5, 4, 3]]
+            out.append((bi[c * kt:(c + 1) * kt].copy(), np.ascontiguousarray(d), np.ascontiguousarray(v)))
+            a0 += n
+        return out
+
+    def normal_equations(self, e=None, f=None, s=None, we=None, wf=None, ws=None, normalizeenergy=0):
+        """Weighted normal equations (L (3, Kt, Kt), h (3, Kt)) of the energy, force and stress rows
+        (linearfit.jl:168-200): e[C] energies, f[N, 3] forces, s[C, 6] stresses in the order xx, yy, zz,
+        yz, xz, xy; we, wf, ws per-configuration weights (default 1).  A missing target leaves its group
+        zero."""
+        kt = len(self._keep[0]) * self.ncoeff
+        C, N = self.C, int(self.natoms.sum())
+
+        def arr(a, n):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, np.float64).ravel()
+            if a.size != n:
+                raise ValueError(f"expected {n} values, got {a.size}")
+            return a
+        e, f, s = arr(e, C), arr(f, 3 * N), arr(s, 6 * C)
+        we, wf, ws = arr(we, C), arr(wf, C), arr(ws, C)
+        ptr = lambda a: None if a is None else a.ctypes.data
+        L = np.zeros((3, kt, kt))
+        h = np.zeros((3, kt))
+        self._check(self._c.mdp_fit_normal_equations(self._h, ptr(e), ptr(f), ptr(s), ptr(we), ptr(wf), ptr(ws),
+                                                     int(normalizeenergy), L.ctypes.data, h.ctypes.data),
+                    "mdp_fit_normal_equations")
+        return L, h
+
+
+def lsqsolve(Le, Lf, Ls, he, hf, hs):
+    """lsqsolve of linearfit.jl:29-75: (ce, cf, cef, cefs), None for a system that is not solved."""
+    c = _bind()
+    K = len(he)
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (Le, Lf, Ls, he, hf, hs)]
+    outs = [np.zeros(K) for _ in range(4)]
+    solved = np.zeros(4, np.int32)
+    rc = c.mdp_lsqsolve(K, *[a.ctypes.data for a in arrs], *[o.ctypes.data for o in outs], solved.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"mdp_lsqsolve: {c.mdp_snap_last_error().decode()}")
+    return tuple(o if ok else None for o, ok in zip(outs, solved))
