@@ -136,6 +136,20 @@ def bench_fit(a):
     out_bytes = 8 * 3 * 55 * npairs + 8 * 55 * natoms
     log(f"{a.steps} replays: {ms:.3f} ms per batch, {value:.4e} atoms/s, {npairs} pairs "
         f"({npairs / natoms:.2f}/atom); output {out_bytes / 1e9:.2f} GB per batch; {dom}: {achieved:.2f} TFLOP/s")
+    # the linear-fit consumer of the same descriptors (not part of the timed replays): tallies and
+    # normal equations of energy, force and stress rows on the GPU, then the host solve
+    from mdp_amd.fit import lsqsolve
+    rng = np.random.default_rng(0)
+    e, f, s = rng.normal(size=fit.C), rng.normal(size=(natoms, 3)), rng.normal(size=(fit.C, 6))
+    fit.normal_equations(e, f, s)
+    t0 = time.perf_counter()
+    L, h = fit.normal_equations(e, f, s)
+    t_ne = 1e3 * (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    lsqsolve(L[0], L[1], L[2], h[0], h[1], h[2])
+    t_sol = 1e3 * (time.perf_counter() - t0)
+    log(f"linear fit on the batch: tallies + normal equations (3 groups, {3 * natoms + 7 * fit.C} rows) "
+        f"{t_ne:.2f} ms, lsqsolve (4 systems of {L.shape[1]}) {t_sol:.2f} ms")
     out = {
         "metric": "atoms/s, batched SNAP B + dB/dr, 2J=8, 512 configurations", "value": value, "unit": "atoms/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True,
