@@ -370,6 +370,12 @@ struct mdp_fit {
         MDP_HIP(hipGraphLaunch(gexec, st));
     }
 
+    int max_natoms() const {
+        int m = 0;
+        for (int c = 0; c < C; ++c) m = std::max(m, a0_h[c + 1] - a0_h[c]);
+        return m;
+    }
+
     // Tallies of the last compute for the linear fit (upstream cpuSnapTallyBispectrum{,Deriv,Virial}
     // layouts per configuration) and the stress rows of the fit.
     void lsq_tallies() {
@@ -387,8 +393,8 @@ struct mdp_fit {
                             scan_tmp.as<int>(), st);
         launch_lsq_tallies(blist.as<double>(), dblist.as<double>(), rij.as<double>(), poff.as<int>(), l_off.as<int>(),
                            l_idx.as<int>(), pti.as<int>(), type.as<int>(), cfg_of.as<int>(), cfg_a0.as<int>(),
-                           cfg_p0.as<int>(), C, N, IB, ntypes, l_bi.as<double>(), l_bd.as<double>(), l_bv.as<double>(),
-                           l_as.as<double>(), st);
+                           cfg_p0.as<int>(), C, N, IB, ntypes, max_natoms(), l_bi.as<double>(), l_bd.as<double>(),
+                           l_bv.as<double>(), l_as.as<double>(), st);
         MDP_HIP(hipGetLastError());
     }
 

@@ -81,8 +81,8 @@ void launch_lsq_incoming(const int* pj, long P, int N, int* in_cnt, int* in_off,
 // (upstream tally layouts, Kt = ntypes*IB, col = (type-1)*IB + k) and the stress rows as[(6c + m')*Kt + col]
 void launch_lsq_tallies(const double* blist, const double* dblist, const double* rij, const int* poff,
                         const int* in_off, const int* in_idx, const int* pti, const int* type, const int* cfg_of,
-                        const int* cfg_a0, const int* cfg_p0, int C, int N, int IB, int ntypes, double* bi, double* bd,
-                        double* bv, double* as, hipStream_t st);
+                        const int* cfg_a0, const int* cfg_p0, int C, int N, int IB, int ntypes, int max_natoms,
+                        double* bi, double* bd, double* bv, double* as, hipStream_t st);
 // out[0 .. K(K+1)/2) = upper triangle of sum w^2 a a^T (row-major), out[K(K+1)/2 + k] = sum w^2 a_k b;
 // ekl[e] = (k, l) of output e (l = -1 for h); part: G * (K(K+1)/2 + K) doubles
 size_t lsq_gram_lds(int K);

@@ -108,6 +108,54 @@ __global__ void k_lsq_bd(const double* __restrict__ dblist, const int* __restric
     }
 }
 
+// The same tally for one element type, one block per (configuration, comp + 3 k): the block streams
+// that plane of dB/dr (pair-contiguous) through LDS in tiles of LSQ_TILE pairs with coalesced loads, and
+// each atom adds its own pairs and subtracts its incoming ones (both ascending) into an LDS accumulator
+// only it touches.  Every dB/dr value is read from HBM once.  LDS: LSQ_TILE + N_c doubles, N_c ints.
+constexpr int LSQ_TILE = 4096;
+__global__ __launch_bounds__(256) void k_lsq_bd_tiled(const double* __restrict__ dblist, const int* __restrict__ poff,
+                                                      const int* __restrict__ in_off, const int* __restrict__ in_idx,
+                                                      const int* __restrict__ cfg_a0, const int* __restrict__ cfg_p0,
+                                                      int IB, double* __restrict__ bd) {
+    extern __shared__ double sm[];
+    const int c = blockIdx.x / (3 * IB), plane = blockIdx.x % (3 * IB);
+    const int comp = plane % 3, k = plane / 3;
+    const int a0 = cfg_a0[c], nc = cfg_a0[c + 1] - a0, p0 = cfg_p0[c];
+    const int pc = cfg_p0[c + 1] - p0;
+    double* tile = sm;
+    double* acc = sm + LSQ_TILE;
+    int* cur = reinterpret_cast<int*>(acc + nc);
+    const double* __restrict__ db = dblist + static_cast<size_t>(p0) * 3 * IB + static_cast<size_t>(pc) * (comp + 3 * k);
+    for (int a = threadIdx.x; a < nc; a += 256) {
+        acc[a] = 0.0;
+        cur[a] = in_off[a0 + a];
+    }
+    for (int t0 = 0; t0 < pc; t0 += LSQ_TILE) {
+        const int t1 = min(pc, t0 + LSQ_TILE);
+        __syncthreads();
+        for (int i = threadIdx.x; i < t1 - t0; i += 256) tile[i] = db[t0 + i];
+        __syncthreads();
+        for (int a = threadIdx.x; a < nc; a += 256) {
+            const int g = a0 + a;
+            double s = acc[a];
+            const int q0 = max(poff[g] - p0, t0), q1 = min(poff[g + 1] - p0, t1);
+            for (int q = q0; q < q1; ++q) s += tile[q - t0];
+            int m = cur[a];
+            const int mend = in_off[g + 1];
+            for (; m < mend; ++m) {
+                const int q = in_idx[m] - p0;
+                if (q >= t1) break;
+                s -= tile[q - t0];
+            }
+            cur[a] = m;
+            acc[a] = s;
+        }
+    }
+    __syncthreads();
+    for (int a = threadIdx.x; a < nc; a += 256)
+        bd[3 * static_cast<size_t>(a0) * IB + comp + 3 * a + 3 * static_cast<size_t>(nc) * k] = acc[a];
+}
+
 // bi[c*Kt + col] = sum of B_k over the atoms of type t of configuration c (atoms in order)
 __global__ void k_lsq_bi(const double* __restrict__ blist, const int* __restrict__ type,
                          const int* __restrict__ cfg_a0, int C, int IB, int ntypes, double* __restrict__ bi) {
@@ -170,11 +218,16 @@ __global__ __launch_bounds__(256) void k_lsq_bv(const double* __restrict__ dblis
 
 void launch_lsq_tallies(const double* blist, const double* dblist, const double* rij, const int* poff,
                         const int* in_off, const int* in_idx, const int* pti, const int* type, const int* cfg_of,
-                        const int* cfg_a0, const int* cfg_p0, int C, int N, int IB, int ntypes, double* bi, double* bd,
-                        double* bv, double* as, hipStream_t st) {
+                        const int* cfg_a0, const int* cfg_p0, int C, int N, int IB, int ntypes, int max_natoms,
+                        double* bi, double* bd, double* bv, double* as, hipStream_t st) {
     const int kt = ntypes * IB;
-    hipLaunchKernelGGL(k_lsq_bd, dim3(nblocks(static_cast<long>(N) * IB, 256)), dim3(256), 0, st, dblist, poff, in_off,
-                       in_idx, pti, type, cfg_of, cfg_a0, cfg_p0, N, IB, ntypes, bd);
+    const size_t lds = sizeof(double) * (LSQ_TILE + static_cast<size_t>(max_natoms)) + sizeof(int) * max_natoms;
+    if (ntypes == 1 && lds <= 65536)
+        hipLaunchKernelGGL(k_lsq_bd_tiled, dim3(static_cast<unsigned>(C) * 3 * IB), dim3(256), lds, st, dblist, poff,
+                           in_off, in_idx, cfg_a0, cfg_p0, IB, bd);
+    else
+        hipLaunchKernelGGL(k_lsq_bd, dim3(nblocks(static_cast<long>(N) * IB, 256)), dim3(256), 0, st, dblist, poff,
+                           in_off, in_idx, pti, type, cfg_of, cfg_a0, cfg_p0, N, IB, ntypes, bd);
     hipLaunchKernelGGL(k_lsq_bi, dim3(nblocks(static_cast<long>(C) * kt, 256)), dim3(256), 0, st, blist, type, cfg_a0,
                        C, IB, ntypes, bi);
     hipLaunchKernelGGL(k_lsq_bv, dim3(static_cast<unsigned>(C) * kt), dim3(256), 0, st, dblist, rij, pti, cfg_p0, IB,

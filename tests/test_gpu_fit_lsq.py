@@ -23,8 +23,9 @@ TOL = 1e-10
 
 def _batch(seed=3):
     cfg = []
+    # the 250-atom cell has more pairs than one LDS tile of the derivative tally (4096)
     for k, (kind, n, a) in enumerate([("bcc", 2, 3.3), ("fcc", (2, 2, 3), 4.1), ("bcc", 3, 3.25),
-                                      ("bcc", (1, 2, 2), 3.4), ("bcc", 3, 3.12)]):
+                                      ("bcc", (1, 2, 2), 3.4), ("bcc", 3, 3.12), ("bcc", 5, 3.3)]):
         x, box = lattice.crystal(kind, n, a, 0.1, seed + k)
         cfg.append((x, box))
     x, box = lattice.random_packing(40, 0.0554, 2.2, seed=seed + 10)
