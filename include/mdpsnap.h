@@ -12,7 +12,8 @@
  *
  * Part 2 is this library's own device-resident MD engine (no upstream counterpart in this ABI; it
  * plays the role of the upstream C++ velocity-Verlet driver, src/C++/Integration/Integration.cpp,
- * around the same SNAP kernels): neighbour lists, SNAP forces and NVE integration stay on the GPU.
+ * around the same SNAP kernels): neighbour lists, SNAP forces and NVE or Nose-Hoover NVT integration
+ * stay on the GPU.
  *
  * Part 3 computes fitting descriptors (B and dB/dr) of many configurations in one batch.
  * Part 4 holds the kernels of the learned surrogate (per-atom MLP over bispectrum descriptors).
@@ -203,6 +204,22 @@ int mdp_md_mean_pairs(mdp_md *md, double *mean);
 
 /* Total potential and kinetic energy (eV) of the current state. */
 int mdp_md_energy(mdp_md *md, double *epot, double *ekin);
+
+/* Nose-Hoover chain NVT instead of NVE for the following mdp_md_run steps (cpuNVTInitialIntegrate /
+ * cpuNVTFinalIntegrate / cpuNoseHooverThermostat, src/C++/Core/cpuVelocityIntegration.cpp:505-648):
+ * a thermostat half-step before the first velocity half-step and after the second, the chain masses
+ * recomputed from the target every half-step (eta_mass_flag = 1).  The target ramps linearly,
+ * t_start + (step - beginstep) / (endstep - beginstep) (t_stop - t_start), where step counts from
+ * currentstep and is advanced at the start of every step.  t_period is the damping time (ps;
+ * t_freq = 1 / t_period, readdata.cpp:432); tdof <= 0 selects 3 (N - 1) (Calculation.cpp:450);
+ * mtchain 1..9 (upstream default 3), nc_tchain >= 1 (default 1).  The chain starts at rest. */
+int mdp_md_set_nvt(mdp_md *md, double t_start, double t_stop, double t_period, double drag, double tdof, int mtchain,
+                   int nc_tchain, long beginstep, long endstep, long currentstep);
+/* back to NVE */
+int mdp_md_clear_nvt(mdp_md *md);
+/* out[43]: eta_mass[10], eta[10], eta_dot[10], eta_dotdot[10], the temperature after the last
+ * thermostat half-step, its target, and the step counter. */
+int mdp_md_nvt_state(mdp_md *md, double *out);
 
 /* Learned surrogate (no upstream counterpart): from the next force evaluation on, forces and
  * energies come from the per-atom MLP of part 4 (params: its FP32 parameters; aff: mu[16] |

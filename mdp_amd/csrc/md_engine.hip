@@ -38,6 +38,7 @@ constexpr int CELLCAP = 48;    // atoms per cell accepted (bounds the per-cell i
 constexpr int SHK = 8;         // image shifts are coded in [-SHK, SHK] per axis
 constexpr double FTM2V = 1.0 / 1.0364269e-4;   // metal units, eV/A/amu -> A/ps^2
 constexpr double MVV2E = 1.0364269e-4;
+constexpr double BOLTZ = 8.617343e-5;                // metal units, eV/K
 
 enum Timer { T_VV1, T_REBUILD, T_PAIRS, T_UI, T_YI, T_DEDR, T_TALLY, T_SUR, T_AUDIT, T_COUNT };
 const char* const kTimerNames[T_COUNT] = {"integrate", "neighbor_build", "pair_list", "compute_ui",
@@ -79,6 +80,111 @@ __global__ void k_vv1(double* __restrict__ x, double* __restrict__ v, const doub
         v[3 * i + d] = vi;
         x[3 * i + d] += dt * vi;
     }
+}
+
+// ---- Nose-Hoover chain thermostat (cpuVelocityIntegration.cpp:505-648) ----------------------------
+// Chain state in one device array: eta_mass, eta, eta_dot (one extra zero entry), eta_dotdot, then the
+// temperature, the velocity factor of the last thermostat half-step and the target temperature.
+constexpr int NH_MAX = 10;
+constexpr int NH_MASS = 0, NH_ETA = NH_MAX, NH_DOT = 2 * NH_MAX, NH_DDOT = 3 * NH_MAX + 1, NH_T = 4 * NH_MAX + 1,
+              NH_FACTOR = NH_T + 1, NH_TTARGET = NH_T + 2, NH_SIZE = NH_T + 3;
+
+// block sums of m v^2 (cpuComputeTempScalar, cpuComputes.cpp:276-296), fixed-order tree per block
+__global__ __launch_bounds__(256) void k_ke_part(const double* __restrict__ v, const int* __restrict__ type,
+                                                 const double* __restrict__ mass_t, int N, double* __restrict__ part) {
+    __shared__ double red[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double s = 0.0;
+    if (i < N) s = (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]) * mass_t[type[i]];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// One thermostat half-step of the chain (one thread): the temperature from the block sums, then the
+// chain update of cpuNoseHooverThermostat with its nc_tchain sub-steps.  The velocities are scaled by
+// the product of the sub-step factors, written to nh[NH_FACTOR] for the kernel that applies it.
+__global__ void k_nh_chain(const double* __restrict__ part, int nparts, double* __restrict__ nh, double t_target,
+                           double t_freq, double tdof, double boltz, double drag, double dt, double tfactor,
+                           int mtchain, int nc_tchain, int eta_mass_flag) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double ke = 0.0;
+    for (int b = 0; b < nparts; ++b) ke += part[b];
+    double* eta_mass = nh + NH_MASS;
+    double* eta = nh + NH_ETA;
+    double* eta_dot = nh + NH_DOT;
+    double* eta_dotdot = nh + NH_DDOT;
+    double t_current = ke * tfactor;
+    const double dthalf = 0.5 * dt, dt4 = 0.25 * dt, dt8 = 0.125 * dt;
+    const double tdrag_factor = 1.0 - (dt * t_freq * drag / nc_tchain);
+    double kecurrent = tdof * boltz * t_current;
+    const double ke_target = tdof * boltz * t_target;
+    if (eta_mass_flag) {
+        eta_mass[0] = tdof * boltz * t_target / (t_freq * t_freq);
+        for (int ich = 1; ich < mtchain; ++ich) eta_mass[ich] = boltz * t_target / (t_freq * t_freq);
+    }
+    eta_dotdot[0] = eta_mass[0] > 0.0 ? (kecurrent - ke_target) / eta_mass[0] : 0.0;
+    const double ncfac = 1.0 / nc_tchain;
+    double factor = 1.0;
+    for (int iloop = 0; iloop < nc_tchain; ++iloop) {
+        for (int ich = mtchain - 1; ich > 0; --ich) {
+            const double expfac = exp(-ncfac * dt8 * eta_dot[ich + 1]);
+            eta_dot[ich] *= expfac;
+            eta_dot[ich] += eta_dotdot[ich] * ncfac * dt4;
+            eta_dot[ich] *= tdrag_factor;
+            eta_dot[ich] *= expfac;
+        }
+        double expfac = exp(-ncfac * dt8 * eta_dot[1]);
+        eta_dot[0] *= expfac;
+        eta_dot[0] += eta_dotdot[0] * ncfac * dt4;
+        eta_dot[0] *= tdrag_factor;
+        eta_dot[0] *= expfac;
+        const double factor_eta = exp(-ncfac * dthalf * eta_dot[0]);
+        factor *= factor_eta;
+        t_current *= factor_eta * factor_eta;
+        kecurrent = tdof * boltz * t_current;
+        eta_dotdot[0] = eta_mass[0] > 0.0 ? (kecurrent - ke_target) / eta_mass[0] : 0.0;
+        for (int ich = 0; ich < mtchain; ++ich) eta[ich] += ncfac * dthalf * eta_dot[ich];
+        eta_dot[0] *= expfac;
+        eta_dot[0] += eta_dotdot[0] * ncfac * dt4;
+        eta_dot[0] *= expfac;
+        for (int ich = 1; ich < mtchain; ++ich) {
+            expfac = exp(-ncfac * dt8 * eta_dot[ich + 1]);
+            eta_dot[ich] *= expfac;
+            eta_dotdot[ich] = (eta_mass[ich - 1] * eta_dot[ich - 1] * eta_dot[ich - 1] - boltz * t_target) / eta_mass[ich];
+            eta_dot[ich] += eta_dotdot[ich] * ncfac * dt4;
+            eta_dot[ich] *= expfac;
+        }
+    }
+    nh[NH_T] = t_current;
+    nh[NH_FACTOR] = factor;
+    nh[NH_TTARGET] = t_target;
+}
+
+// first half-step after the thermostat: v = v * factor + dtf/m f, x += dt v
+__global__ void k_vv1_nh(double* __restrict__ x, double* __restrict__ v, const double* __restrict__ f,
+                         const int* __restrict__ type, const double* __restrict__ massinv_t,
+                         const double* __restrict__ nh, double dtf, double dt, int N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double fac = nh[NH_FACTOR];
+    const double dtfm = dtf * massinv_t[type[i]];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double vi = v[3 * i + d] * fac + dtfm * f[3 * i + d];
+        v[3 * i + d] = vi;
+        x[3 * i + d] += dt * vi;
+    }
+}
+
+__global__ void k_vscale_nh(double* __restrict__ v, const double* __restrict__ nh, int N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * N) return;
+    v[i] *= nh[NH_FACTOR];
 }
 
 // wrap into [0, L), record the maximum squared displacement since the last build, bin into cells
@@ -492,6 +598,12 @@ struct mdp_md {
     DevBuf x, v, f, type, massinv, xref, cell_of, cell_cnt, cell_start, cell_fill, cell_atoms, scan_tmp, nb_cnt, nb_j,
         nb_s, in_cnt, in_e, pcnt, poff, pi, pj, pti, ptj, prij, entry_pair, Ur, Ui, Yr, Yi, epart, dedr, eatom, vir,
         beta_t, rc2, flags, maxd2, row_order, yt, pair_sum, pgeo;
+    // Nose-Hoover NVT (mdp_md_set_nvt); off: NVE
+    bool nvt = false;
+    double nvt_tstart = 0, nvt_tstop = 0, nvt_tfreq = 0, nvt_drag = 0, nvt_tdof = 0;
+    int nvt_mtchain = 3, nvt_nc = 1, nvt_massflag = 1;
+    long nvt_begin = 0, nvt_end = 1, nvt_step = 0;
+    DevBuf nh, kepart, mass_t;
     long pair_steps0 = 0;   // force evaluations counted in pair_sum since the last reset
     long stats[6] = {0, 0, 0, 0, 0, 0};
     bool forces_current = false;
@@ -636,6 +748,9 @@ struct mdp_md {
         upload(v, v0 ? v0 : vz.data(), sizeof(double) * 3 * n);
         upload(type, types, sizeof(int) * n);
         upload(massinv, massinv_h.data(), sizeof(double) * massinv_h.size());
+        std::vector<double> mass_h(ntypes + 1, 0.0);
+        for (int t = 1; t <= ntypes; ++t) mass_h[t] = p->mass[t - 1];
+        upload(mass_t, mass_h.data(), sizeof(double) * mass_h.size());
         upload(beta_t, bt.data(), sizeof(double) * bt.size());
         upload(rc2, rc2_h.data(), sizeof(double) * rc2_h.size());
         std::vector<int> order(nrows);
@@ -999,12 +1114,61 @@ struct mdp_md {
         forces_current = true;
     }
 
+    // One thermostat half-step on the device: block sums of m v^2, then the chain (one thread).
+    void nh_half(double t_target) {
+        const int nparts = nb(N, 256);
+        hipLaunchKernelGGL(k_ke_part, dim3(nparts), dim3(256), 0, st, v.as<double>(), type.as<int>(), mass_t.as<double>(),
+                           N, kepart.as<double>());
+        const double tfactor = MVV2E / (nvt_tdof * BOLTZ);
+        hipLaunchKernelGGL(k_nh_chain, dim3(1), dim3(64), 0, st, kepart.as<double>(), nparts, nh.as<double>(), t_target,
+                           nvt_tfreq, nvt_tdof, BOLTZ, nvt_drag, dt, tfactor, nvt_mtchain, nvt_nc, nvt_massflag);
+    }
+
+    void set_nvt(double t_start, double t_stop, double t_period, double drag, double tdof, int mtchain, int nc_tchain,
+                 long beginstep, long endstep, long currentstep) {
+        if (!(t_period > 0.0)) throw Error("mdp_md_set_nvt: the thermostat period must be positive");
+        if (mtchain < 1 || mtchain > NH_MAX - 1) throw Error("mdp_md_set_nvt: mtchain must be 1..9");
+        if (nc_tchain < 1) throw Error("mdp_md_set_nvt: nc_tchain must be positive");
+        if (endstep <= beginstep) throw Error("mdp_md_set_nvt: endstep must be above beginstep");
+        if (!(t_start > 0.0) || !(t_stop > 0.0)) throw Error("mdp_md_set_nvt: temperatures must be positive");
+        nvt = true;
+        nvt_tstart = t_start;
+        nvt_tstop = t_stop;
+        nvt_tfreq = 1.0 / t_period;   // readdata.cpp:432
+        nvt_drag = drag;
+        nvt_tdof = tdof > 0.0 ? tdof : 3.0 * (N - 1);   // Calculation.cpp:450
+        nvt_mtchain = mtchain;
+        nvt_nc = nc_tchain;
+        nvt_begin = beginstep;
+        nvt_end = endstep;
+        nvt_step = currentstep;
+        nh.ensure(sizeof(double) * NH_SIZE);
+        kepart.ensure(sizeof(double) * nb(N, 256));
+        std::vector<double> z(NH_SIZE, 0.0);
+        z[NH_FACTOR] = 1.0;
+        MDP_HIP(hipMemcpyAsync(nh.p, z.data(), sizeof(double) * NH_SIZE, hipMemcpyHostToDevice, st));
+        MDP_HIP(hipStreamSynchronize(st));
+    }
+
     void run(int nsteps) {
         if (!forces_current) compute();
         for (int s = 0; s < nsteps; ++s) {
+            double t_target = 0.0;
+            if (nvt) {   // target ramp of cpuNVTInitialIntegrate (cpuVelocityIntegration.cpp:611-612)
+                ++nvt_step;
+                const double delta = static_cast<double>(nvt_step - nvt_begin) / static_cast<double>(nvt_end - nvt_begin);
+                t_target = nvt_tstart + delta * (nvt_tstop - nvt_tstart);
+            }
             timed(T_VV1, [&] {
-                hipLaunchKernelGGL(k_vv1, dim3(nb(N, 256)), dim3(256), 0, st, x.as<double>(), v.as<double>(),
-                                   f.as<double>(), type.as<int>(), massinv.as<double>(), dtf, dt, N);
+                if (nvt) {
+                    nh_half(t_target);
+                    hipLaunchKernelGGL(k_vv1_nh, dim3(nb(N, 256)), dim3(256), 0, st, x.as<double>(), v.as<double>(),
+                                       f.as<double>(), type.as<int>(), massinv.as<double>(), nh.as<double>(), dtf, dt,
+                                       N);
+                } else {
+                    hipLaunchKernelGGL(k_vv1, dim3(nb(N, 256)), dim3(256), 0, st, x.as<double>(), v.as<double>(),
+                                       f.as<double>(), type.as<int>(), massinv.as<double>(), dtf, dt, N);
+                }
             });
             stats[4] += 1;
             if (stats[4] % rebuild_every == 0) timed(T_REBUILD, [&] { rebuild(true); });
@@ -1016,6 +1180,12 @@ struct mdp_md {
             } else {
                 snap_and_tally(true);
             }
+            if (nvt)   // second thermostat half-step after the final velocity update (cpuNVTFinalIntegrate)
+                timed(T_VV1, [&] {
+                    nh_half(t_target);
+                    hipLaunchKernelGGL(k_vscale_nh, dim3(nb(3L * N, 256)), dim3(256), 0, st, v.as<double>(),
+                                       nh.as<double>(), N);
+                });
         }
         MDP_HIP(hipGetLastError());
         MDP_HIP(hipStreamSynchronize(st));
@@ -1040,6 +1210,40 @@ int api(F&& f) {
 }  // namespace
 
 extern "C" {
+
+int mdp_md_set_nvt(mdp_md* md, double t_start, double t_stop, double t_period, double drag, double tdof, int mtchain,
+                   int nc_tchain, long beginstep, long endstep, long currentstep) {
+    return api([&] {
+        if (!md) throw Error("mdp_md_set_nvt: null handle");
+        md->set_nvt(t_start, t_stop, t_period, drag, tdof, mtchain, nc_tchain, beginstep, endstep, currentstep);
+    });
+}
+
+int mdp_md_clear_nvt(mdp_md* md) {
+    return api([&] {
+        if (!md) throw Error("mdp_md_clear_nvt: null handle");
+        md->nvt = false;
+    });
+}
+
+int mdp_md_nvt_state(mdp_md* md, double* out) {
+    return api([&] {
+        if (!md || !out) throw Error("mdp_md_nvt_state: null argument");
+        if (!md->nvt) throw Error("mdp_md_nvt_state: NVT is not set");
+        std::vector<double> h(NH_SIZE);
+        MDP_HIP(hipStreamSynchronize(md->st));
+        MDP_HIP(hipMemcpy(h.data(), md->nh.p, sizeof(double) * NH_SIZE, hipMemcpyDeviceToHost));
+        for (int k = 0; k < NH_MAX; ++k) {
+            out[k] = h[NH_MASS + k];
+            out[NH_MAX + k] = h[NH_ETA + k];
+            out[2 * NH_MAX + k] = h[NH_DOT + k];
+            out[3 * NH_MAX + k] = h[NH_DDOT + k];
+        }
+        out[4 * NH_MAX] = h[NH_T];
+        out[4 * NH_MAX + 1] = h[NH_TTARGET];
+        out[4 * NH_MAX + 2] = static_cast<double>(md->nvt_step);
+    });
+}
 
 int mdp_md_create(mdp_md** md, const mdp_md_params* p, int natoms, const double* box, const double* x, const double* v,
                   const int* types) {

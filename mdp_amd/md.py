@@ -58,6 +58,9 @@ def _bind():
     c.mdp_md_clear_surrogate.argtypes = [vp]
     c.mdp_md_audits.argtypes = [vp, vp]
     c.mdp_md_mean_pairs.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    c.mdp_md_set_nvt.argtypes = [vp] + [ctypes.c_double] * 5 + [ctypes.c_int] * 2 + [ctypes.c_long] * 3
+    c.mdp_md_clear_nvt.argtypes = [vp]
+    c.mdp_md_nvt_state.argtypes = [vp, ctypes.c_void_p]
     _bound = True
     return c
 
@@ -184,3 +187,20 @@ class SnapMD:
         ep, ek = ctypes.c_double(), ctypes.c_double()
         self._check(self._c.mdp_md_energy(self._h, ctypes.byref(ep), ctypes.byref(ek)), "mdp_md_energy")
         return ep.value, ek.value
+
+    def set_nvt(self, t_start, t_stop=None, t_period=0.1, drag=0.0, tdof=0.0, mtchain=3, nc_tchain=1,
+                beginstep=0, endstep=1, currentstep=0):
+        """Nose-Hoover chain NVT for the following steps (cpuVelocityIntegration.cpp:505-648)."""
+        t_stop = t_start if t_stop is None else t_stop
+        self._check(self._c.mdp_md_set_nvt(self._h, t_start, t_stop, t_period, drag, tdof, mtchain, nc_tchain,
+                                           beginstep, endstep, currentstep), "mdp_md_set_nvt")
+
+    def clear_nvt(self):
+        self._check(self._c.mdp_md_clear_nvt(self._h), "mdp_md_clear_nvt")
+
+    def nvt_state(self):
+        """dict of the chain: eta_mass, eta, eta_dot, eta_dotdot (10 each), t_current, t_target, step."""
+        o = np.zeros(43)
+        self._check(self._c.mdp_md_nvt_state(self._h, o.ctypes.data), "mdp_md_nvt_state")
+        return dict(eta_mass=o[0:10], eta=o[10:20], eta_dot=o[20:30], eta_dotdot=o[30:40], t_current=o[40],
+                    t_target=o[41], step=int(o[42]))

@@ -2,7 +2,7 @@
 // kernels of the upstream C++ route into one translation unit -- compiled unmodified from its sources
 // where they lie, with C entry points for the double instantiations of the SNAP force path and of the
 // fitting driver's descriptor chain that the backend-3 entry points of libmdpsnap (include/mdpsnap.h,
-// part 5) replace.  Built by the Makefile into
+// part 5) replace, and of the Nose-Hoover NVT integrator that the MD engine's thermostat follows.  Built by the Makefile into
 // _ref/ only when the upstream tree is present; never part of the product path.
 #include "cpuCore.cpp"
 
@@ -134,6 +134,23 @@ void r2_cpuSnapTallyBispectrumVirial(double *bv, double *bispectrum, double *dbd
                                      int quadraticflag) {
     cpuSnapTallyBispectrumVirial(bv, bispectrum, dbdr, rij, aii, ai, aj, ti, inum, ijnum, ncoeff, nperdim, ntype,
                                  quadraticflag);
+}
+
+// Nose-Hoover NVT integrator halves (src/C++/Core/cpuVelocityIntegration.cpp:602-660)
+void r2_cpuNVTInitialIntegrate(double *x, double *v, double *f, double *mass, double *dtarray, double *tarray,
+                               double *eta_mass, double *eta, double *eta_dot, double *eta_dotdot, int *type,
+                               int *ilist, int eta_mass_flag, int biasflag, int mtchain, int nc_tchain, int dim,
+                               int inum) {
+    cpuNVTInitialIntegrate(x, v, f, mass, dtarray, tarray, eta_mass, eta, eta_dot, eta_dotdot, type, ilist,
+                           eta_mass_flag, biasflag, mtchain, nc_tchain, dim, inum);
+}
+
+void r2_cpuNVTFinalIntegrate(double *x, double *v, double *f, double *mass, double *dtarray, double *tarray,
+                             double *eta_mass, double *eta, double *eta_dot, double *eta_dotdot, int *type,
+                             int *ilist, int eta_mass_flag, int biasflag, int mtchain, int nc_tchain, int dim,
+                             int inum) {
+    cpuNVTFinalIntegrate(x, v, f, mass, dtarray, tarray, eta_mass, eta, eta_dot, eta_dotdot, type, ilist,
+                         eta_mass_flag, biasflag, mtchain, nc_tchain, dim, inum);
 }
 
 }  // extern "C"
