@@ -217,6 +217,13 @@ int mdp_md_set_nvt(mdp_md *md, double t_start, double t_stop, double t_period, d
                    int nc_tchain, long beginstep, long endstep, long currentstep);
 /* back to NVE */
 int mdp_md_clear_nvt(mdp_md *md);
+/* Initial velocities as the upstream app creates them (readdata.cpp:609-640; host code): per-atom
+ * uniform (dist_flag 0) or normal (1) draws of cpuVelocityCreate in loop mode 2 ("geom", the mode of the
+ * upstream examples; the other modes are rejected), scaled by 1/sqrt(mass), then zero total momentum and
+ * rescaling to t_desired with tdof = 3 (natoms - 1).  mass[t] for type t (mass[0] unused); mvv2e and
+ * boltz as the application's units (metal: 1.0364269e-4, 8.617343e-5).  v[3*natoms] is overwritten. */
+int mdp_velocity_create(const double *x, const int *type, const double *mass, int natoms, double t_desired,
+                        int seed0, int dist_flag, int loop_flag, double mvv2e, double boltz, double *v);
 /* out[43]: eta_mass[10], eta[10], eta_dot[10], eta_dotdot[10], the temperature after the last
  * thermostat half-step, its target, and the step counter. */
 int mdp_md_nvt_state(mdp_md *md, double *out);

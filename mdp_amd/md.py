@@ -61,6 +61,8 @@ def _bind():
     c.mdp_md_set_nvt.argtypes = [vp] + [ctypes.c_double] * 5 + [ctypes.c_int] * 2 + [ctypes.c_long] * 3
     c.mdp_md_clear_nvt.argtypes = [vp]
     c.mdp_md_nvt_state.argtypes = [vp, ctypes.c_void_p]
+    c.mdp_velocity_create.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_double, ctypes.c_double, vp]
     _bound = True
     return c
 
@@ -204,3 +206,19 @@ class SnapMD:
         self._check(self._c.mdp_md_nvt_state(self._h, o.ctypes.data), "mdp_md_nvt_state")
         return dict(eta_mass=o[0:10], eta=o[10:20], eta_dot=o[20:30], eta_dotdot=o[30:40], t_current=o[40],
                     t_target=o[41], step=int(o[42]))
+
+
+def velocity_create(x, types, mass, t_desired, seed0, dist_flag=0, loop_flag=2, mvv2e=1.0364269e-4,
+                    boltz=8.617343e-5):
+    """Initial velocities (N, 3) as the upstream app creates them (readdata.cpp:609-640, loop mode "geom");
+    ``mass`` per type, 1-based (mass[0] unused)."""
+    c = _bind()
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    types = np.ascontiguousarray(types, np.int32).reshape(len(x))
+    mass = np.ascontiguousarray(mass, np.float64).ravel()
+    v = np.zeros_like(x)
+    rc = c.mdp_velocity_create(x.ctypes.data, types.ctypes.data, mass.ctypes.data, len(x), float(t_desired),
+                               int(seed0), int(dist_flag), int(loop_flag), mvv2e, boltz, v.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"mdp_velocity_create: {library().cdll.mdp_snap_last_error().decode()}")
+    return v

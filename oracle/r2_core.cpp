@@ -4,6 +4,8 @@
 // fitting driver's descriptor chain that the backend-3 entry points of libmdpsnap (include/mdpsnap.h,
 // part 5) replace, and of the Nose-Hoover NVT integrator that the MD engine's thermostat follows.  Built by the Makefile into
 // _ref/ only when the upstream tree is present; never part of the product path.
+#include <vector>
+
 #include "cpuCore.cpp"
 
 extern "C" {
@@ -151,6 +153,24 @@ void r2_cpuNVTFinalIntegrate(double *x, double *v, double *f, double *mass, doub
                              int inum) {
     cpuNVTFinalIntegrate(x, v, f, mass, dtarray, tarray, eta_mass, eta, eta_dot, eta_dotdot, type, ilist,
                          eta_mass_flag, biasflag, mtchain, nc_tchain, dim, inum);
+}
+
+// initial velocities: the call sequence of readdata.cpp:609-640 on the upstream functions
+void r2_velocity_create(double *x, double *v, double *mass, int *type, int natoms, double t_desired, int seed0,
+                        int dist_flag, int loop_flag, double mvv2e, double boltz) {
+    std::vector<int> ilist(natoms), seed(1, 0), save(1, 0);
+    std::vector<double> second(1, 0.0), amass(natoms);
+    for (int i = 0; i < natoms; ++i) ilist[i] = i;
+    cpuVelocityCreate(x, v, mass, second.data(), seed.data(), save.data(), ilist.data(), type, seed0, 0, dist_flag,
+                      loop_flag, 3, 0, natoms, natoms);
+    const double masstotal = cpuComputeMass(amass.data(), mass, type, ilist.data(), natoms);
+    double vcm[3];
+    cpuComputeVCM(vcm, v, mass, masstotal, ilist.data(), type, 3, natoms);
+    cpuVelocityZeroMomentum(v, vcm, 3, natoms);
+    const double tdof = (natoms - 1) * 3;
+    const double tfactor = mvv2e / (tdof * boltz);
+    const double temp = cpuComputeTempScalar(v, mass, tfactor, type, ilist.data(), 3, natoms);
+    cpuArrayMultiplyScalar(v, sqrt(t_desired / temp), natoms * 3);
 }
 
 }  // extern "C"
