@@ -204,6 +204,10 @@ int mdp_md_mean_pairs(mdp_md *md, double *mean);
 
 /* Total potential and kinetic energy (eV) of the current state. */
 int mdp_md_energy(mdp_md *md, double *epot, double *ekin);
+/* Thermo output of the upstream app (ThermoOutput, Calculation.cpp:436-465) for the current state:
+ * out4 = potential energy per atom (eV), temperature (K, tdof = 3 (N - 1)), kinetic energy per atom
+ * (eV, 0.5 tdof k T / N) and pressure (bar: (tdof k T + W_xx + W_yy + W_zz) / (3 V) nktv2p). */
+int mdp_md_thermo(mdp_md *md, double *out4);
 
 /* Nose-Hoover chain NVT instead of NVE for the following mdp_md_run steps (cpuNVTInitialIntegrate /
  * cpuNVTFinalIntegrate / cpuNoseHooverThermostat, src/C++/Core/cpuVelocityIntegration.cpp:505-648):

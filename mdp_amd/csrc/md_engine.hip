@@ -187,6 +187,33 @@ __global__ void k_vscale_nh(double* __restrict__ v, const double* __restrict__ n
     v[i] *= nh[NH_FACTOR];
 }
 
+// block sums for the thermo output: [0] energy, [1..3] virial xx, yy, zz, [4] m v^2 (fixed-order tree)
+__global__ __launch_bounds__(256) void k_thermo_part(const double* __restrict__ eatom, const double* __restrict__ vir,
+                                                     const double* __restrict__ v, const int* __restrict__ type,
+                                                     const double* __restrict__ mass_t, int N,
+                                                     double* __restrict__ part) {
+    __shared__ double red[5][256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double q[5] = {0, 0, 0, 0, 0};
+    if (i < N) {
+        q[0] = eatom[i];
+        q[1] = vir[6 * i];
+        q[2] = vir[6 * i + 1];
+        q[3] = vir[6 * i + 2];
+        q[4] = (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]) * mass_t[type[i]];
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) red[k][threadIdx.x] = q[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w)
+#pragma unroll
+            for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) part[5 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
 // wrap into [0, L), record the maximum squared displacement since the last build, bin into cells
 __global__ void k_wrap_bin(double* __restrict__ x, double* __restrict__ xref, int* __restrict__ cell_of,
                            int* __restrict__ cell_cnt, unsigned long long* __restrict__ maxd2_bits, double Lx,
@@ -1114,6 +1141,29 @@ struct mdp_md {
         forces_current = true;
     }
 
+    // ThermoOutput of the upstream app (Calculation.cpp:436-465) for the current state: potential energy
+    // per atom, temperature (tdof = 3 (N - 1)), kinetic energy per atom, pressure (bar, metal units).
+    void thermo(double* out) {
+        if (!forces_current) compute();
+        const int nparts = nb(N, 256);
+        kepart.ensure(sizeof(double) * 5 * nparts);
+        hipLaunchKernelGGL(k_thermo_part, dim3(nparts), dim3(256), 0, st, eatom.as<double>(), vir.as<double>(),
+                           v.as<double>(), type.as<int>(), mass_t.as<double>(), N, kepart.as<double>());
+        std::vector<double> h(5 * static_cast<size_t>(nparts));
+        MDP_HIP(hipMemcpyAsync(h.data(), kepart.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+        MDP_HIP(hipStreamSynchronize(st));
+        double q[5] = {0, 0, 0, 0, 0};
+        for (int b = 0; b < nparts; ++b)
+            for (int k = 0; k < 5; ++k) q[k] += h[5 * static_cast<size_t>(b) + k];
+        constexpr double NKTV2P = 1.6021765e6;
+        const double tdof = 3.0 * (N - 1), tfactor = MVV2E / (tdof * BOLTZ);
+        const double temp = tfactor * q[4];
+        out[0] = q[0] / N;
+        out[1] = temp;
+        out[2] = 0.5 * tdof * BOLTZ * temp / N;
+        out[3] = (tdof * BOLTZ * temp + q[1] + q[2] + q[3]) / 3.0 * (1.0 / (L[0] * L[1] * L[2])) * NKTV2P;
+    }
+
     // One thermostat half-step on the device: block sums of m v^2, then the chain (one thread).
     void nh_half(double t_target) {
         const int nparts = nb(N, 256);
@@ -1216,6 +1266,13 @@ int mdp_md_set_nvt(mdp_md* md, double t_start, double t_stop, double t_period, d
     return api([&] {
         if (!md) throw Error("mdp_md_set_nvt: null handle");
         md->set_nvt(t_start, t_stop, t_period, drag, tdof, mtchain, nc_tchain, beginstep, endstep, currentstep);
+    });
+}
+
+int mdp_md_thermo(mdp_md* md, double* out4) {
+    return api([&] {
+        if (!md || !out4) throw Error("mdp_md_thermo: null argument");
+        md->thermo(out4);
     });
 }
 

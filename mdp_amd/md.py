@@ -60,6 +60,7 @@ def _bind():
     c.mdp_md_mean_pairs.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     c.mdp_md_set_nvt.argtypes = [vp] + [ctypes.c_double] * 5 + [ctypes.c_int] * 2 + [ctypes.c_long] * 3
     c.mdp_md_clear_nvt.argtypes = [vp]
+    c.mdp_md_thermo.argtypes = [vp, ctypes.c_void_p]
     c.mdp_md_nvt_state.argtypes = [vp, ctypes.c_void_p]
     c.mdp_velocity_create.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_double, ctypes.c_double, vp]
@@ -196,6 +197,12 @@ class SnapMD:
         t_stop = t_start if t_stop is None else t_stop
         self._check(self._c.mdp_md_set_nvt(self._h, t_start, t_stop, t_period, drag, tdof, mtchain, nc_tchain,
                                            beginstep, endstep, currentstep), "mdp_md_set_nvt")
+
+    def thermo(self):
+        """ThermoOutput of the upstream app: dict pe (eV/atom), temp (K), ke (eV/atom), press (bar)."""
+        o = np.zeros(4)
+        self._check(self._c.mdp_md_thermo(self._h, o.ctypes.data), "mdp_md_thermo")
+        return dict(pe=o[0], temp=o[1], ke=o[2], press=o[3])
 
     def clear_nvt(self):
         self._check(self._c.mdp_md_clear_nvt(self._h), "mdp_md_clear_nvt")

@@ -3,7 +3,8 @@ cpuNVTInitialIntegrate / cpuNVTFinalIntegrate of the upstream C++ core (compiled
 oracle/_ref/libcpucore_r2.so), driven step by step with the oracle's forces.  Positions, velocities and
 the chain variables agree within 1e-9 after ten steps, for a constant target and for a ramp with a
 three-link chain and two sub-steps.  A longer run conserves the extended (Nose-Hoover) energy while the
-thermostat changes the kinetic energy.
+thermostat changes the kinetic energy.  The thermo output (per-atom PE and KE, temperature, pressure)
+matches the upstream formulas on the oracle's per-atom energies and virials.
 """
 import ctypes
 import os
@@ -125,3 +126,30 @@ def test_nvt_extended_energy_conserved(oracle):
     assert ek_max > 1.3 * ek0             # the thermostat heats the system towards 600 K
     assert drift < 1e-3 * ek0             # ... while the extended energy is conserved
     assert np.max(np.abs(g["v"].sum(axis=0))) < 1e-9 * np.max(np.abs(g["v"])) * n
+
+
+def test_thermo_output_vs_oracle(oracle):
+    """ThermoOutput (Calculation.cpp:436-465) of the engine against the same formulas on the oracle's
+    per-atom energies and virials at the same positions and velocities."""
+    from mdp_amd.md import SnapMD
+    J = 6
+    x, box = lattice.crystal("bcc", 4, 3.316, 0.08, 9)
+    n = len(x)
+    v = lattice.maxwell_boltzmann(n, MASS, 500.0, seed=9)
+    s0 = snapcpp.initsna(oracle, [1, J, RCUTFAC, RFAC0, 0, 1, 1, 0, 0, 0, 0], [0.5], [1.0])
+    coeff = lattice.snap_coefficients(s0.ncoeff, seed=9)
+    md = SnapMD(x, box, np.ones(n, np.int32), J, coeff, [0.5], [1.0], [MASS], v=v)
+    md.compute()
+    th = md.thermo()
+    s = snapcpp.initsna(oracle, [1, J, RCUTFAC, RFAC0, 0, 1, 1, 0, 0, 0, 0], [0.5], [1.0], coeff)
+    B = np.diag(box)
+    p = snapcpp.pairs_from_box(x, np.ones(n, np.int32), B[0], B[1], B[2], [1, 1, 1], s)
+    e, _, w = snapcpp.snappotential_pairs(oracle, s, p)
+    tdof = 3.0 * (n - 1)
+    temp = MVV2E / (tdof * BOLTZ) * (MASS * (v * v).sum())
+    press = (tdof * BOLTZ * temp + w[:, 0].sum() + w[:, 1].sum() + w[:, 2].sum()) / 3.0 / np.prod(box) * 1.6021765e6
+    print(f"thermo: {th}", flush=True)
+    assert abs(th["pe"] - e.sum() / n) < 1e-10 * abs(e.sum() / n)
+    assert abs(th["temp"] - temp) < 1e-10 * temp
+    assert abs(th["ke"] - 0.5 * tdof * BOLTZ * temp / n) < 1e-10 * th["ke"]
+    assert abs(th["press"] - press) < 1e-9 * max(abs(press), 1.0)
