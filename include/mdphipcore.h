@@ -218,4 +218,38 @@ inline void hipSnapTallyBispectrumVirial<double>(double *bv, double *bispectrum,
                                      quadraticflag);
 }
 
+template <typename T>
+void hipSnapTallyEnergyFull(T *eatom, T *bispectrum, T *coeffelem, int *ilist, int *map, int *type, int inum,
+                            int ncoeff, int ncoeffall, int quadraticflag);
+template <>
+inline void hipSnapTallyEnergyFull<double>(double *eatom, double *bispectrum, double *coeffelem, int *ilist, int *map,
+                                           int *type, int inum, int ncoeff, int ncoeffall, int quadraticflag) {
+    mdp_hipSnapTallyEnergyFull(eatom, bispectrum, coeffelem, ilist, map, type, inum, ncoeff, ncoeffall, quadraticflag);
+}
+
+template <typename T>
+void hipSnapTallyEnergyFull2(T *eatom, T *bispectrum, T *coeffelem, int *ilist, int *map, int *type, int inum,
+                             int ncoeff, int ncoeffall, int quadraticflag);
+template <>
+inline void hipSnapTallyEnergyFull2<double>(double *eatom, double *bispectrum, double *coeffelem, int *ilist, int *map,
+                                            int *type, int inum, int ncoeff, int ncoeffall, int quadraticflag) {
+    mdp_hipSnapTallyEnergyFull2(eatom, bispectrum, coeffelem, ilist, map, type, inum, ncoeff, ncoeffall,
+                                quadraticflag);
+}
+
+template <typename T>
+void hipSnapTallyForceFull(T *fatom, T *fij, int *ai, int *aj, int *alist, int ijnum);
+template <>
+inline void hipSnapTallyForceFull<double>(double *fatom, double *fij, int *ai, int *aj, int *alist, int ijnum) {
+    mdp_hipSnapTallyForceFull(fatom, fij, ai, aj, alist, ijnum);
+}
+
+template <typename T>
+void hipSnapTallyVirialFull(T *vatom, T *fij, T *rij, int *ai, int *aj, int inum, int ijnum);
+template <>
+inline void hipSnapTallyVirialFull<double>(double *vatom, double *fij, double *rij, int *ai, int *aj, int inum,
+                                           int ijnum) {
+    mdp_hipSnapTallyVirialFull(vatom, fij, rij, ai, aj, inum, ijnum);
+}
+
 #endif

@@ -462,6 +462,22 @@ void mdp_hipSnapTallyBispectrumVirial(double *bv, double *bispectrum, double *db
                                       int *aj, int *ti, int inum, int ijnum, int ncoeff, int nperdim, int ntype,
                                       int quadraticflag);
 
+/* Energy, force and virial tallies of the force path's split form (src/C++/Core/cpuSnap.cpp:1746-1831,
+ * cpuSnap2.cpp:1265-1292): eatom[ii] += beta0 + beta . B (+ quadratic terms) of element
+ * map[type[ilist[ii]]], fatom[ai] += f, fatom[aj] -= f, and the SoA virial vatom[c*inum + atom] +=
+ * -r (x) f / 2 for both atoms.  The force and virial sums use atomics, so their order (last bits) can
+ * vary from run to run. */
+/* replaces hipSnapTallyEnergyFull in SnapTallyEnergyFull (mdpcore.h:3630-3651), B at [ii*ncoeff + k] */
+void mdp_hipSnapTallyEnergyFull(double *eatom, double *bispectrum, double *coeffelem, int *ilist, int *map, int *type,
+                                int inum, int ncoeff, int ncoeffall, int quadraticflag);
+/* replaces hipSnapTallyEnergyFull2 in SnapTallyEnergyFull2 (mdpcore.h:3653-3674), B at [ii + inum*k] */
+void mdp_hipSnapTallyEnergyFull2(double *eatom, double *bispectrum, double *coeffelem, int *ilist, int *map,
+                                 int *type, int inum, int ncoeff, int ncoeffall, int quadraticflag);
+/* replaces hipSnapTallyForceFull in SnapTallyForceFull (mdpcore.h:3676-3691) */
+void mdp_hipSnapTallyForceFull(double *fatom, double *fij, int *ai, int *aj, int *alist, int ijnum);
+/* replaces hipSnapTallyVirialFull in SnapTallyVirialFull (mdpcore.h:3712-3730) */
+void mdp_hipSnapTallyVirialFull(double *vatom, double *fij, double *rij, int *ai, int *aj, int inum, int ijnum);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1076,4 +1076,47 @@ void mdp_hipSnapTallyBispectrumVirial(double* bv, double* bispectrum, double* db
                                  ntype, quadraticflag);
 }
 
+// ---- energy / force / virial tallies (the force path's split form, src/C++/Core/cpuSnap.cpp:1746-1831) --
+
+// SnapTallyEnergyFull backend 3 (mdpcore.h:3630-3651; bispectrum [ii*ncoeff + k])
+void mdp_hipSnapTallyEnergyFull(double* eatom, double* bispectrum, double* coeffelem, int* ilist, int* map, int* type,
+                                int inum, int ncoeff, int ncoeffall, int quadraticflag) {
+    guarded([&] {
+        hipStream_t st = library_stream();
+        launch_tally_energy_full(eatom, bispectrum, coeffelem, ilist, map, type, inum, ncoeff, ncoeffall,
+                                 quadraticflag, 0, st);
+        finish(st);
+    });
+}
+
+// SnapTallyEnergyFull2 backend 3 (mdpcore.h:3653-3674; bispectrum [ii + inum*k], cpuSnap2.cpp:1265-1291)
+void mdp_hipSnapTallyEnergyFull2(double* eatom, double* bispectrum, double* coeffelem, int* ilist, int* map,
+                                 int* type, int inum, int ncoeff, int ncoeffall, int quadraticflag) {
+    guarded([&] {
+        hipStream_t st = library_stream();
+        launch_tally_energy_full(eatom, bispectrum, coeffelem, ilist, map, type, inum, ncoeff, ncoeffall,
+                                 quadraticflag, 1, st);
+        finish(st);
+    });
+}
+
+// SnapTallyForceFull backend 3 (mdpcore.h:3676-3691; alist is unused upstream too)
+void mdp_hipSnapTallyForceFull(double* fatom, double* fij, int* ai, int* aj, int* alist, int ijnum) {
+    (void)alist;
+    guarded([&] {
+        hipStream_t st = library_stream();
+        launch_tally_force_full(fatom, fij, ai, aj, ijnum, st);
+        finish(st);
+    });
+}
+
+// SnapTallyVirialFull backend 3 (mdpcore.h:3712-3730; SoA vatom[c*inum + atom], half to each atom)
+void mdp_hipSnapTallyVirialFull(double* vatom, double* fij, double* rij, int* ai, int* aj, int inum, int ijnum) {
+    guarded([&] {
+        hipStream_t st = library_stream();
+        launch_tally_virial_full(vatom, fij, rij, ai, aj, inum, ijnum, st);
+        finish(st);
+    });
+}
+
 }  // extern "C"

@@ -793,4 +793,83 @@ void launch_add_utot_pairs(double* Stotr, double* Stoti, const double* Sr, const
                        Sr, Si, pairnum, pairnumsum, map, tj, idxu_max, inum, ijnum, chemflag);
 }
 
+// ================================================================================================
+// energy / force / virial tallies of the upstream C++ core (src/C++/Core/cpuSnap.cpp:1746-1831,
+// cpuSnap2.cpp:1265-1292)
+// ================================================================================================
+
+// eatom[ii] += beta0 + sum_k beta_k B_k (+ the quadratic terms), element map[type[ilist[ii]]];
+// B(ii, k) at b[ii*ncoeff + k] (soa = 0) or b[ii + inum*k] (soa = 1); terms in the upstream order
+__global__ void k_tally_energy_full(double* __restrict__ eatom, const double* __restrict__ b,
+                                    const double* __restrict__ coeff, const int* __restrict__ ilist,
+                                    const int* __restrict__ map, const int* __restrict__ type, int inum, int ncoeff,
+                                    int ncoeffall, int quadratic, int soa) {
+    const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ii >= inum) return;
+    const double* __restrict__ c = coeff + static_cast<size_t>(map[type[ilist[ii]]]) * ncoeffall;
+    auto B = [&](int k) { return soa ? b[ii + static_cast<size_t>(inum) * k] : b[static_cast<size_t>(ii) * ncoeff + k]; };
+    double e = c[0];
+    for (int k = 0; k < ncoeff; ++k) e += c[k + 1] * B(k);
+    if (quadratic) {
+        int q = ncoeff + 1;
+        for (int k = 0; k < ncoeff; ++k) {
+            const double bk = B(k);
+            e += 0.5 * c[q++] * bk * bk;
+            for (int l = k + 1; l < ncoeff; ++l) e += c[q++] * bk * B(l);
+        }
+    }
+    eatom[ii] += e;
+}
+
+// fatom[ai] += f_ij, fatom[aj] -= f_ij
+__global__ void k_tally_force_full(double* __restrict__ fatom, const double* __restrict__ fij,
+                                   const int* __restrict__ ai, const int* __restrict__ aj, int ijnum) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ijnum) return;
+    const int i = ai[k], j = aj[k];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        atomicAdd(fatom + 3 * i + d, fij[3 * k + d]);
+        atomicAdd(fatom + 3 * j + d, -fij[3 * k + d]);
+    }
+}
+
+// vatom[c*inum + atom] += 1/2 (-r (x) f) for both atoms of the pair, c = xx, yy, zz, xy, xz, yz
+__global__ void k_tally_virial_full(double* __restrict__ vatom, const double* __restrict__ fij,
+                                    const double* __restrict__ rij, const int* __restrict__ ai,
+                                    const int* __restrict__ aj, int inum, int ijnum) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ijnum) return;
+    const double dx = -rij[3 * k], dy = -rij[3 * k + 1], dz = -rij[3 * k + 2];
+    const double fx = fij[3 * k], fy = fij[3 * k + 1], fz = fij[3 * k + 2];
+    const double w[6] = {0.5 * dx * fx, 0.5 * dy * fy, 0.5 * dz * fz, 0.5 * dx * fy, 0.5 * dx * fz, 0.5 * dy * fz};
+    const int i = ai[k], j = aj[k];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        atomicAdd(vatom + static_cast<size_t>(c) * inum + i, w[c]);
+        atomicAdd(vatom + static_cast<size_t>(c) * inum + j, w[c]);
+    }
+}
+
+void launch_tally_energy_full(double* eatom, const double* b, const double* coeff, const int* ilist, const int* map,
+                              const int* type, int inum, int ncoeff, int ncoeffall, int quadratic, int soa,
+                              hipStream_t st) {
+    if (inum <= 0) return;
+    hipLaunchKernelGGL(k_tally_energy_full, dim3(nblk(inum, 128)), dim3(128), 0, st, eatom, b, coeff, ilist, map,
+                       type, inum, ncoeff, ncoeffall, quadratic, soa);
+}
+
+void launch_tally_force_full(double* fatom, const double* fij, const int* ai, const int* aj, int ijnum,
+                             hipStream_t st) {
+    if (ijnum <= 0) return;
+    hipLaunchKernelGGL(k_tally_force_full, dim3(nblk(ijnum, 256)), dim3(256), 0, st, fatom, fij, ai, aj, ijnum);
+}
+
+void launch_tally_virial_full(double* vatom, const double* fij, const double* rij, const int* ai, const int* aj,
+                              int inum, int ijnum, hipStream_t st) {
+    if (ijnum <= 0) return;
+    hipLaunchKernelGGL(k_tally_virial_full, dim3(nblk(ijnum, 256)), dim3(256), 0, st, vatom, fij, rij, ai, aj, inum,
+                       ijnum);
+}
+
 }  // namespace mdp

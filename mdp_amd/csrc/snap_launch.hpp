@@ -188,6 +188,16 @@ void launch_add_utot_pairs(double* Stotr, double* Stoti, const double* Sr, const
                            const int* pairnumsum, const int* map, const int* tj, int idxu_max, int inum, int ijnum,
                            int chemflag, hipStream_t st);
 
+// energy / force / virial tallies of the upstream C++ core: eatom[ii] += beta . B (AoS or SoA B),
+// fatom += f_ij on ai and -= on aj (atomics), SoA half/half virial of -r (x) f_ij (atomics)
+void launch_tally_energy_full(double* eatom, const double* b, const double* coeff, const int* ilist, const int* map,
+                              const int* type, int inum, int ncoeff, int ncoeffall, int quadratic, int soa,
+                              hipStream_t st);
+void launch_tally_force_full(double* fatom, const double* fij, const int* ai, const int* aj, int ijnum,
+                             hipStream_t st);
+void launch_tally_virial_full(double* vatom, const double* fij, const double* rij, const int* ai, const int* aj,
+                              int inum, int ijnum, hipStream_t st);
+
 // AddWself2Ui on device (wself on the diagonal, mirror of rows 2mb < j)
 void launch_add_wself(const SnapDev& s, double* Ur, double* Ui, double wself, const int* type, int wselfall,
                       int chemflag, int nelem, int inum, hipStream_t st);

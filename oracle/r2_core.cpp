@@ -173,4 +173,20 @@ void r2_velocity_create(double *x, double *v, double *mass, int *type, int natom
     cpuArrayMultiplyScalar(v, sqrt(t_desired / temp), natoms * 3);
 }
 
+// energy / force / virial tallies of the split force path (src/C++/Core/cpuSnap.cpp:1746-1831)
+void r2_cpuSnapTallyEnergyFull(double *eatom, double *bispectrum, double *coeffelem, int *ilist, int *map, int *type,
+                               int inum, int ncoeff, int ncoeffall, int quadraticflag) {
+    cpuSnapTallyEnergyFull(eatom, bispectrum, coeffelem, ilist, map, type, inum, ncoeff, ncoeffall, quadraticflag);
+}
+void r2_cpuSnapTallyEnergyFull2(double *eatom, double *bispectrum, double *coeffelem, int *ilist, int *map, int *type,
+                                int inum, int ncoeff, int ncoeffall, int quadraticflag) {
+    cpuSnapTallyEnergyFull2(eatom, bispectrum, coeffelem, ilist, map, type, inum, ncoeff, ncoeffall, quadraticflag);
+}
+void r2_cpuSnapTallyForceFull(double *fatom, double *fij, int *ai, int *aj, int *alist, int ijnum) {
+    cpuSnapTallyForceFull(fatom, fij, ai, aj, alist, ijnum);
+}
+void r2_cpuSnapTallyVirialFull(double *vatom, double *fij, double *rij, int *ai, int *aj, int inum, int ijnum) {
+    cpuSnapTallyVirialFull(vatom, fij, rij, ai, aj, inum, ijnum);
+}
+
 }  // extern "C"
