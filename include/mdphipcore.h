@@ -89,6 +89,15 @@ inline void hipNeighPairList<double>(int *pairnum, int *pairlist, double *x, dou
                          ntypes);
 }
 
+template <typename T>
+void hipNeighPairList(int *pairnum, int *pairlist, T *x, T rcutsq, int *ilist, int *neighlist, int *neighnum, int inum,
+                      int jnum, int dim);
+template <>
+inline void hipNeighPairList<double>(int *pairnum, int *pairlist, double *x, double rcutsq, int *ilist, int *neighlist,
+                                     int *neighnum, int inum, int jnum, int dim) {
+    mdp_hipNeighPairListCut(pairnum, pairlist, x, rcutsq, ilist, neighlist, neighnum, inum, jnum, dim);
+}
+
 inline void hipCumsum(int *d_out, int *d_in, int *d_sums, int *d_incr, int length) {
     mdp_hipCumsum(d_out, d_in, d_sums, d_incr, length);
 }

@@ -411,6 +411,10 @@ void mdp_hipSnapComputeFi(double *fatom, double *vatom, double *ylist_r, double 
  * 1e-20 < |x_j - x_i|^2 < rcutsq[atomtype[alist[j]] + atomtype[i]*(ntypes+1)] */
 void mdp_hipNeighPairList(int *pairnum, int *pairlist, double *x, double *rcutsq, int *ilist, int *neighlist,
                           int *neighnum, int *atomtype, int *alist, int inum, int jnum, int dim, int ntypes);
+/* replaces hipNeighPairList in the one-cutoff NeighborPairList (mdpcore.h:3807-3828): pair kept when
+ * |x_j - x_i|^2 <= rcutsq */
+void mdp_hipNeighPairListCut(int *pairnum, int *pairlist, double *x, double rcutsq, int *ilist, int *neighlist,
+                             int *neighnum, int inum, int jnum, int dim);
 /* replaces hipCumsum in Cumsum (mdpcore.h:1114-1131): d_out[0] = 0, d_out[j] = sum of d_in[0..j-1] for
  * j < length; d_sums and d_incr (caller scratch upstream) are not used */
 void mdp_hipCumsum(int *d_out, int *d_in, int *d_sums, int *d_incr, int length);

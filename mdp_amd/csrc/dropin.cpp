@@ -877,6 +877,18 @@ void mdp_hipNeighPairList(int* pairnum, int* pairlist, double* x, double* rcutsq
     });
 }
 
+// NeighborPairList backend 3, one cutoff (mdpcore.h:3807-3828; src/C++/Core/cpuSnap.cpp:1833-1853)
+void mdp_hipNeighPairListCut(int* pairnum, int* pairlist, double* x, double rcutsq, int* ilist, int* neighlist,
+                             int* neighnum, int inum, int jnum, int dim) {
+    guarded([&] {
+        if (inum <= 0) return;
+        if (jnum <= 0 || dim != 3) throw Error("NeighPairList: needs jnum > 0 and dim = 3");
+        hipStream_t st = library_stream();
+        launch_pair_list_strided_cut(pairnum, pairlist, x, rcutsq, ilist, neighlist, neighnum, inum, jnum, dim, st);
+        finish(st);
+    });
+}
+
 // Cumsum backend 3 (mdpcore.h:1114-1131; cpuCumsum src/C++/Core/cpuArrayOperations.cpp:22-27)
 void mdp_hipCumsum(int* d_out, int* d_in, int* d_sums, int* d_incr, int length) {
     (void)d_sums; (void)d_incr;

@@ -693,22 +693,35 @@ void launch_pair_arrays(double* xij, const double* x, int* aii, int* ai, int* aj
 // fixed-stride pair lists of the upstream C++ core (src/C++/Core/cpuSnap.cpp:1883-1935)
 // ================================================================================================
 
-// neighbours of atom i at neighlist[l + jnum*i] (l < neighnum[i]); kept ones to pairlist[c + jnum*ii]
+// neighbours of atom i at neighlist[l + jnum*i] (l < neighnum[i]); kept ones to pairlist[c + jnum*ii].
+// Typed cutoffs (atomtype != nullptr): 1e-20 < d^2 < rcutsq[tj + ti*(ntypes+1)] (cpuSnap.cpp:1883-1905);
+// one cutoff (atomtype == nullptr): d^2 <= cut2 (cpuSnap.cpp:1833-1853).
 __global__ void k_pair_list_strided(int* __restrict__ pairnum, int* __restrict__ pairlist, const double* __restrict__ x,
                                     const double* __restrict__ rcutsq, const int* __restrict__ ilist,
                                     const int* __restrict__ neighlist, const int* __restrict__ neighnum,
                                     const int* __restrict__ atomtype, const int* __restrict__ alist, int inum, int jnum,
-                                    int dim, int ntypes) {
+                                    int dim, int ntypes, double cut2) {
     const int ii = blockIdx.x * blockDim.x + threadIdx.x;
     if (ii >= inum) return;
-    const int i = ilist[ii], it = atomtype[i];
+    const int i = ilist[ii];
     const int* nl = neighlist + static_cast<size_t>(jnum) * i;
     int* pl = pairlist + static_cast<size_t>(jnum) * ii;
     const int m = neighnum[i];
     int c = 0;
-    for (int l = 0; l < m; ++l) {
-        const int j = nl[l];
-        if (in_pair(x, i, j, dim, rcutsq[atomtype[alist[j]] + it * (ntypes + 1)])) pl[c++] = j;
+    if (atomtype) {
+        const int it = atomtype[i];
+        for (int l = 0; l < m; ++l) {
+            const int j = nl[l];
+            if (in_pair(x, i, j, dim, rcutsq[atomtype[alist[j]] + it * (ntypes + 1)])) pl[c++] = j;
+        }
+    } else {
+        for (int l = 0; l < m; ++l) {
+#pragma clang fp contract(off)
+            const int j = nl[l];
+            const double dx = x[j * dim] - x[i * dim], dy = x[j * dim + 1] - x[i * dim + 1],
+                         dz = x[j * dim + 2] - x[i * dim + 2];
+            if (dx * dx + dy * dy + dz * dz <= cut2) pl[c++] = j;
+        }
     }
     pairnum[ii] = c;
 }
@@ -718,7 +731,15 @@ void launch_pair_list_strided(int* pairnum, int* pairlist, const double* x, cons
                               int inum, int jnum, int dim, int ntypes, hipStream_t st) {
     if (inum <= 0) return;
     hipLaunchKernelGGL(k_pair_list_strided, dim3(nblk(inum, 128)), dim3(128), 0, st, pairnum, pairlist, x, rcutsq,
-                       ilist, neighlist, neighnum, atomtype, alist, inum, jnum, dim, ntypes);
+                       ilist, neighlist, neighnum, atomtype, alist, inum, jnum, dim, ntypes, 0.0);
+}
+
+void launch_pair_list_strided_cut(int* pairnum, int* pairlist, const double* x, double cut2, const int* ilist,
+                                  const int* neighlist, const int* neighnum, int inum, int jnum, int dim,
+                                  hipStream_t st) {
+    if (inum <= 0) return;
+    hipLaunchKernelGGL(k_pair_list_strided, dim3(nblk(inum, 128)), dim3(128), 0, st, pairnum, pairlist, x, nullptr,
+                       ilist, neighlist, neighnum, nullptr, nullptr, inum, jnum, dim, 0, cut2);
 }
 
 // pair k = pairnumsum[ii] + l of the ii-th atom, neighbour pairlist[l + jnum*ii] (l < pairnum[ii])

@@ -180,6 +180,9 @@ void launch_pair_arrays(double* xij, const double* x, int* aii, int* ai, int* aj
 void launch_pair_list_strided(int* pairnum, int* pairlist, const double* x, const double* rcutsq, const int* ilist,
                               const int* neighlist, const int* neighnum, const int* atomtype, const int* alist,
                               int inum, int jnum, int dim, int ntypes, hipStream_t st);
+void launch_pair_list_strided_cut(int* pairnum, int* pairlist, const double* x, double cut2, const int* ilist,
+                                  const int* neighlist, const int* neighnum, int inum, int jnum, int dim,
+                                  hipStream_t st);
 void launch_pair_arrays_strided(double* xij, const double* x, int* aii, int* ai, int* aj, int* ti, int* tj,
                                 const int* pairnum, const int* pairlist, const int* pairnumsum, const int* ilist,
                                 const int* atomtype, const int* alist, int inum, int jnum, int dim, hipStream_t st);

@@ -58,6 +58,11 @@ void r2_cpuNeighPairList(int *pairnum, int *pairlist, double *x, double *rcutsq,
                      ntypes);
 }
 
+void r2_cpuNeighPairListCut(int *pairnum, int *pairlist, double *x, double rcutsq, int *ilist, int *neighlist,
+                            int *neighnum, int inum, int jnum, int dim) {
+    cpuNeighPairList(pairnum, pairlist, x, rcutsq, ilist, neighlist, neighnum, inum, jnum, dim);
+}
+
 void r2_cpuCumsum(int *d_out, int *d_in, int *d_sums, int *d_incr, int length) {
     (void)d_sums; (void)d_incr;
     cpuCumsum(d_out, d_in, length);

@@ -281,6 +281,20 @@ def test_backend3_descriptor_chain_vs_upstream_core(oracle, J, ncell, seed, ntyp
         assert relerr(a[:t], b[:t]) < TOL, k
     # both forms of AddUarraytot give the same totals
     assert relerr(host["Ur2"], host["Ur"]) < TOL and relerr(host["Ui2"], host["Ui"]) < TOL
+    # the one-cutoff NeighborPairList (d^2 <= rcutsq) on the same lists, bit for bit
+    cut2 = float(rcsq[-1])
+    outs = {}
+    for lib, pre, make, ptr in ((ctypes.CDLL(R2LIB), "r2_cpu", lambda a: a.copy(), lambda a: a.ctypes.data),
+                                (library().cdll, "mdp_hip", DevArray.from_numpy, lambda a: a.ptr)):
+        f = getattr(lib, pre + "NeighPairListCut")
+        f.argtypes = [P, P, P, D, P, P, P, I, I, I]
+        f.restype = None
+        pn, pl = make(np.full(n, -7, np.int32)), make(np.full(n * jnum, -7, np.int32))
+        args = {k: (host[k] if pre == "r2_cpu" else dev[k]) for k in ("y", "ilist", "neighlist", "neighnum")}
+        f(ptr(pn), ptr(pl), ptr(args["y"]), cut2, ptr(args["ilist"]), ptr(args["neighlist"]), ptr(args["neighnum"]),
+          n, jnum, 3)
+        outs[pre] = [a if isinstance(a, np.ndarray) else a.numpy() for a in (pn, pl)]
+    assert np.array_equal(outs["r2_cpu"][0], outs["mdp_hip"][0]) and np.array_equal(outs["r2_cpu"][1], outs["mdp_hip"][1])
 
 
 @pytest.mark.parametrize("quadratic", [0, 1])
