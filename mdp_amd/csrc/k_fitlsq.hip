@@ -111,9 +111,10 @@ __global__ void k_lsq_bd(const double* __restrict__ dblist, const int* __restric
 // The same tally for one element type, one block per (configuration, comp + 3 k): the block streams
 // that plane of dB/dr (pair-contiguous) through LDS in tiles of LSQ_TILE pairs with coalesced loads, and
 // each atom adds its own pairs and subtracts its incoming ones (both ascending) into an LDS accumulator
-// only it touches.  Every dB/dr value is read from HBM once.  LDS: LSQ_TILE + N_c doubles, N_c ints.
+// only it touches.  Every dB/dr value is read from HBM once.  LDS: LSQ_TILE + N_c doubles, N_c ints;
+// 1024 threads per block keep the CU busy although only two such blocks fit in its LDS.
 constexpr int LSQ_TILE = 4096;
-__global__ __launch_bounds__(256) void k_lsq_bd_tiled(const double* __restrict__ dblist, const int* __restrict__ poff,
+__global__ __launch_bounds__(1024) void k_lsq_bd_tiled(const double* __restrict__ dblist, const int* __restrict__ poff,
                                                       const int* __restrict__ in_off, const int* __restrict__ in_idx,
                                                       const int* __restrict__ cfg_a0, const int* __restrict__ cfg_p0,
                                                       int IB, double* __restrict__ bd) {
@@ -126,16 +127,16 @@ __global__ __launch_bounds__(256) void k_lsq_bd_tiled(const double* __restrict__
     double* acc = sm + LSQ_TILE;
     int* cur = reinterpret_cast<int*>(acc + nc);
     const double* __restrict__ db = dblist + static_cast<size_t>(p0) * 3 * IB + static_cast<size_t>(pc) * (comp + 3 * k);
-    for (int a = threadIdx.x; a < nc; a += 256) {
+    for (int a = threadIdx.x; a < nc; a += 1024) {
         acc[a] = 0.0;
         cur[a] = in_off[a0 + a];
     }
     for (int t0 = 0; t0 < pc; t0 += LSQ_TILE) {
         const int t1 = min(pc, t0 + LSQ_TILE);
         __syncthreads();
-        for (int i = threadIdx.x; i < t1 - t0; i += 256) tile[i] = db[t0 + i];
+        for (int i = threadIdx.x; i < t1 - t0; i += 1024) tile[i] = db[t0 + i];
         __syncthreads();
-        for (int a = threadIdx.x; a < nc; a += 256) {
+        for (int a = threadIdx.x; a < nc; a += 1024) {
             const int g = a0 + a;
             double s = acc[a];
             const int q0 = max(poff[g] - p0, t0), q1 = min(poff[g + 1] - p0, t1);
@@ -152,24 +153,31 @@ __global__ __launch_bounds__(256) void k_lsq_bd_tiled(const double* __restrict__
         }
     }
     __syncthreads();
-    for (int a = threadIdx.x; a < nc; a += 256)
+    for (int a = threadIdx.x; a < nc; a += 1024)
         bd[3 * static_cast<size_t>(a0) * IB + comp + 3 * a + 3 * static_cast<size_t>(nc) * k] = acc[a];
 }
 
-// bi[c*Kt + col] = sum of B_k over the atoms of type t of configuration c (atoms in order)
-__global__ void k_lsq_bi(const double* __restrict__ blist, const int* __restrict__ type,
-                         const int* __restrict__ cfg_a0, int C, int IB, int ntypes, double* __restrict__ bi) {
+// bi[c*Kt + col] = sum of B_k over the atoms of type t of configuration c: one block per (c, col), its
+// threads over the atoms (coalesced), fixed-order tree reduction
+__global__ __launch_bounds__(256) void k_lsq_bi(const double* __restrict__ blist, const int* __restrict__ type,
+                                                const int* __restrict__ cfg_a0, int IB, int ntypes,
+                                                double* __restrict__ bi) {
     const int kt = ntypes * IB;
-    const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= static_cast<long>(C) * kt) return;
-    const int c = static_cast<int>(t / kt), col = static_cast<int>(t % kt);
+    const int c = blockIdx.x / kt, col = blockIdx.x % kt;
     const int ty = col / IB + 1, k = col % IB;
     const int a0 = cfg_a0[c], nc = cfg_a0[c + 1] - a0;
     const double* __restrict__ b = blist + static_cast<size_t>(a0) * IB + static_cast<size_t>(nc) * k;
     double s = 0.0;
-    for (int ii = 0; ii < nc; ++ii)
+    for (int ii = threadIdx.x; ii < nc; ii += 256)
         if (type[a0 + ii] == ty) s += b[ii];
-    bi[t] = s;
+    __shared__ double red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bi[blockIdx.x] = red[0];
 }
 
 // virial tally of configuration c and column col (one block): v = -r (x) dB/dr summed over the pairs
@@ -223,13 +231,13 @@ void launch_lsq_tallies(const double* blist, const double* dblist, const double*
     const int kt = ntypes * IB;
     const size_t lds = sizeof(double) * (LSQ_TILE + static_cast<size_t>(max_natoms)) + sizeof(int) * max_natoms;
     if (ntypes == 1 && lds <= 65536)
-        hipLaunchKernelGGL(k_lsq_bd_tiled, dim3(static_cast<unsigned>(C) * 3 * IB), dim3(256), lds, st, dblist, poff,
+        hipLaunchKernelGGL(k_lsq_bd_tiled, dim3(static_cast<unsigned>(C) * 3 * IB), dim3(1024), lds, st, dblist, poff,
                            in_off, in_idx, cfg_a0, cfg_p0, IB, bd);
     else
         hipLaunchKernelGGL(k_lsq_bd, dim3(nblocks(static_cast<long>(N) * IB, 256)), dim3(256), 0, st, dblist, poff,
                            in_off, in_idx, pti, type, cfg_of, cfg_a0, cfg_p0, N, IB, ntypes, bd);
-    hipLaunchKernelGGL(k_lsq_bi, dim3(nblocks(static_cast<long>(C) * kt, 256)), dim3(256), 0, st, blist, type, cfg_a0,
-                       C, IB, ntypes, bi);
+    hipLaunchKernelGGL(k_lsq_bi, dim3(static_cast<unsigned>(C) * kt), dim3(256), 0, st, blist, type, cfg_a0, IB, ntypes,
+                       bi);
     hipLaunchKernelGGL(k_lsq_bv, dim3(static_cast<unsigned>(C) * kt), dim3(256), 0, st, dblist, rij, pti, cfg_p0, IB,
                        ntypes, bv, as);
 }
