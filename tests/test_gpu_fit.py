@@ -3,6 +3,7 @@
 * Pair lists: bit-exact against the oracle's brute-force periodic builder (same (j, image) order),
   r_ij bit-exact against x_j + s*L - x_i, for boxes below, at and above twice the cutoff (the GPU
   takes a single-image path above it).
+* Configurations without pairs: B - bzero of an isolated atom vanishes (closed form).
 * B and dB/dr of every configuration: against the oracle's per-configuration chain cpuComputeUij ->
   cpuZeroUarraytot2 -> cpuAddUarraytot -> cpuComputeZi2 -> cpuComputeBi2 -> cpuComputeDbidrj on the
   same pairs, within 1e-10 relative to the max norm of the oracle array.
@@ -40,6 +41,9 @@ def _small_batch(seed=7):
         x, box = lattice.random_packing(n, 0.0554, 2.2, seed=seed + 10 + k)
         x = np.mod(x + np.random.default_rng(seed + 20 + k).normal(0, 0.1, x.shape), box)
         cfg.append((x, box))
+    # configurations without pairs: one atom, and two atoms farther apart than the cutoff
+    cfg.append((np.array([[3.0, 4.0, 5.0]]), np.array([20.0, 20.0, 20.0])))
+    cfg.append((np.array([[1.0, 1.0, 1.0], [11.0, 11.0, 11.0]]), np.array([20.0, 20.0, 20.0])))
     return cfg
 
 
@@ -82,9 +86,15 @@ def _check_config(oracle, fit, c, cfgs, b, db, ai, aj, rij, J, bnorm=0, bzero=1)
     ref_rij = (x[nj] + ns * box) - x[ci]
     assert np.array_equal(rij[sl], ref_rij)
     ob, odb = _oracle_descriptors(oracle, J, x, ai[sl], aj[sl], rij[sl], bnorm, bzero)
-    assert relerr(bc, ob) < TOL, c
+    # B - bzero can vanish (an atom without pairs: U = identity, Z = identity, B = j + 1 = bzero), so
+    # its error is measured against the size of B before the subtraction (at least J + 1)
+    scale = max(np.max(np.abs(ob)), J + 1.0)
+    assert np.max(np.abs(bc - ob)) < TOL * scale, c
     if len(odb):
         assert relerr(dbc, odb) < TOL, c
+    else:   # no pairs: the closed form, B - bzero = 0 to rounding (with bzero subtracted)
+        if bzero:
+            assert np.max(np.abs(bc)) < 1e-13 * (J + 1), c
 
 
 @pytest.mark.parametrize("J,variant", [(4, "0"), (6, "0"), (8, "0"), (4, "1"), (6, "1"), (8, "1"), (8, "2"),
