@@ -64,6 +64,20 @@ inline void hipSnapComputeYi<double>(double *ylist_r, double *ylist_i, double *u
 }
 
 template <typename T>
+void hipSnapComputeYi(T *ylist_r, T *ylist_i, T *ulisttot_r, T *ulisttot_i, T *cglist, T *betalist, int *idxz,
+                      int *idxb_block, int *idxu_block, int *idxcg_block, int twojmax, int idxb_max, int idxu_max,
+                      int idxz_max, int nelements, int bnorm_flag, int inum);
+template <>
+inline void hipSnapComputeYi<double>(double *ylist_r, double *ylist_i, double *ulisttot_r, double *ulisttot_i,
+                                     double *cglist, double *betalist, int *idxz, int *idxb_block, int *idxu_block,
+                                     int *idxcg_block, int twojmax, int idxb_max, int idxu_max, int idxz_max,
+                                     int nelements, int bnorm_flag, int inum) {
+    mdp_hipSnapComputeYiPerAtom(ylist_r, ylist_i, ulisttot_r, ulisttot_i, cglist, betalist, idxz, idxb_block,
+                                idxu_block, idxcg_block, twojmax, idxb_max, idxu_max, idxz_max, nelements, bnorm_flag,
+                                inum);
+}
+
+template <typename T>
 void hipSnapComputeFi(T *fatom, T *vatom, T *ylist_r, T *ylist_i, T *rootpqarray, T *rij, T *wjelem, T *radelem,
                       T rmin0, T rfac0, T rcutfac, int *idxu_block, int *map, int *aii, int *ai, int *aj, int *ti,
                       int *tj, int twojmax, int idxu_max, int inum, int anum, int ijnum, int switchflag, int chemflag);

@@ -393,6 +393,12 @@ void mdp_hipSnapComputeYi(double *ylist_r, double *ylist_i, double *ulisttot_r, 
                           double *betalist, int *map, int *type, int *idxz, int *idxb_block, int *idxu_block,
                           int *idxcg_block, int twojmax, int idxb_max, int idxu_max, int idxz_max, int nelements,
                           int ncoeffall, int bnorm_flag, int inum);
+/* replaces hipSnapComputeYi in the per-atom-weight form of SnapComputeYi (mdpcore.h:3196-3219;
+   src/C++/Core/cpuSnap4.cpp:601-690): betalist[jjb * inum + atom], one element */
+void mdp_hipSnapComputeYiPerAtom(double *ylist_r, double *ylist_i, double *ulisttot_r, double *ulisttot_i,
+                                 double *cglist, double *betalist, int *idxz, int *idxb_block, int *idxu_block,
+                                 int *idxcg_block, int twojmax, int idxb_max, int idxu_max, int idxz_max,
+                                 int nelements, int bnorm_flag, int inum);
 /* replaces hipSnapComputeFi in SnapComputeFi (mdpcore.h:3246-3269) */
 void mdp_hipSnapComputeFi(double *fatom, double *vatom, double *ylist_r, double *ylist_i, double *rootpqarray,
                           double *rij, double *wjelem, double *radelem, double rmin0, double rfac0, double rcutfac,

@@ -831,6 +831,38 @@ void mdp_hipSnapComputeYi(double* ylist_r, double* ylist_i, double* ulisttot_r, 
                      ncoeffall, bnorm_flag, 1, inum);
 }
 
+// SnapComputeYi backend 3, per-atom form (mdpcore.h:3196-3219; src/C++/Core/cpuSnap4.cpp:601-690): every
+// atom carries its own bispectrum weights, betalist[jjb * inum + atom] (no constant term); one element
+void mdp_hipSnapComputeYiPerAtom(double* ylist_r, double* ylist_i, double* ulisttot_r, double* ulisttot_i,
+                                 double* cglist, double* betalist, int* idxz, int* idxb_block, int* idxu_block,
+                                 int* idxcg_block, int twojmax, int idxb_max, int idxu_max, int idxz_max,
+                                 int nelements, int bnorm_flag, int inum) {
+    (void)idxz; (void)idxb_block; (void)idxu_block; (void)idxcg_block; (void)idxz_max;
+    std::vector<double> h_cg;
+    if (!fetch_args([&] { h_cg = fetch(cglist, static_cast<size_t>(index_for(twojmax).idxcg_max)); })) return;
+    guarded([&] {
+        if (inum <= 0) return;
+        if (nelements != 1) throw Error("SnapComputeYi (per-atom weights): only one element is supported");
+        hipStream_t st = library_stream();
+        const SnapDev& s = tables(twojmax, nullptr, h_cg.data(), nullptr, nullptr, nullptr, 1, st);
+        if (idxu_max != s.idxu_max || idxb_max != s.idxb_max) throw Error("table sizes do not match twojmax");
+        std::vector<int> jjb;
+        std::vector<double> fac;
+        y_triple_weights(index_for(twojmax), bnorm_flag, jjb, fac);
+        const int ntrip = static_cast<int>(jjb.size());
+        int* d_jjb = put(0, jjb.data(), jjb.size(), st);
+        double* d_fac = put(1, fac.data(), fac.size(), st);
+        double* d_bt = dev<double>(2, static_cast<size_t>(ntrip) * inum);
+        const size_t nu = static_cast<size_t>(inum) * idxu_max;
+        MDP_HIP(hipMemsetAsync(ylist_r, 0, sizeof(double) * nu, st));
+        MDP_HIP(hipMemsetAsync(ylist_i, 0, sizeof(double) * nu, st));
+        launch_beta_peratom(betalist, d_jjb, d_fac, ntrip, inum, d_bt, st);
+        if (!launch_yi_peratom(s, ulisttot_r, ulisttot_i, d_bt, ylist_r, ylist_i, inum, st))
+            throw Error("SnapComputeYi (per-atom weights): twojmax not compiled");
+        finish(st);
+    });
+}
+
 // SnapComputeFi backend 3 (mdpcore.h:3246-3269; src/C++/Core/cpuSnap4.cpp:798-1210): forces to ai/aj,
 // virial in the SoA layout vatom[k * anum + atom] with -r(x)dE/2 to both atoms
 void mdp_hipSnapComputeFi(double* fatom, double* vatom, double* ylist_r, double* ylist_i, double* rootpqarray,

@@ -41,20 +41,39 @@ __global__ __launch_bounds__(64) void k_yi_pa(SnapDev s, const double* __restric
     });
 }
 
+__global__ __launch_bounds__(256) void k_beta_peratom(const double* __restrict__ g, const int* __restrict__ jjb,
+                                                      const double* __restrict__ fac, int ntrip, int N,
+                                                      double* __restrict__ beta) {
+    const long k = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= static_cast<long>(ntrip) * N) return;
+    const int t = static_cast<int>(k / N), a = static_cast<int>(k % N);
+    beta[k] = fac[t] * g[static_cast<size_t>(jjb[t]) * N + a];
+}
+
+void launch_beta_peratom(const double* g, const int* jjb, const double* fac, int ntrip, int N, double* beta_t,
+                         hipStream_t st) {
+    const long n = static_cast<long>(ntrip) * N;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_beta_peratom, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, g, jjb, fac,
+                       ntrip, N, beta_t);
+}
+
 bool launch_yi_peratom(const SnapDev& s, const double* Ur, const double* Ui, const double* beta_t, double* Yr,
                        double* Yi, int N, hipStream_t st) {
     if (N <= 0) return true;
+#define YI_PA_CASE(JV)                                                                                            \
+    case JV:                                                                                                      \
+        hipLaunchKernelGGL((k_yi_pa<JV>), dim3((N + 63) / 64, n_half_rows(JV)), dim3(64), 0, st, s, Ur, Ui, beta_t, \
+                           Yr, Yi, N);                                                                            \
+        return true;
     switch (s.J) {
-        case 2:
-            hipLaunchKernelGGL((k_yi_pa<2>), dim3((N + 63) / 64, n_half_rows(2)), dim3(64), 0, st, s, Ur, Ui, beta_t, Yr,
-                               Yi, N);
-            return true;
-        case 4:
-            hipLaunchKernelGGL((k_yi_pa<4>), dim3((N + 63) / 64, n_half_rows(4)), dim3(64), 0, st, s, Ur, Ui, beta_t, Yr,
-                               Yi, N);
-            return true;
+        YI_PA_CASE(2)
+        YI_PA_CASE(4)
+        YI_PA_CASE(6)
+        YI_PA_CASE(8)
         default: return false;
     }
+#undef YI_PA_CASE
 }
 
 void launch_yi(const SnapDev& s, const double* Ur, const double* Ui, const double* beta_t, double* Yr, double* Yi,

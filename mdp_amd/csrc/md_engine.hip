@@ -576,15 +576,6 @@ __global__ void k_vv2(double* __restrict__ v, const double* __restrict__ f, cons
     v[3 * i + 2] += dtfm * f[3 * i + 2];
 }
 
-// per-atom, per-coupling-triple weights of the surrogate's Y: beta[t][a] = fac[t] * g[jjb[t]][a]
-__global__ void k_beta_peratom(const double* __restrict__ g, const int* __restrict__ jjb, const double* __restrict__ fac,
-                               int ntrip, int N, double* __restrict__ beta) {
-    const long k = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= static_cast<long>(ntrip) * N) return;
-    const int t = static_cast<int>(k / N), a = static_cast<int>(k % N);
-    beta[k] = fac[t] * g[static_cast<size_t>(jjb[t]) * N + a];
-}
-
 // per-block partial sums of |a - b|^2
 __global__ __launch_bounds__(256) void k_diff_sumsq(const double* __restrict__ a, const double* __restrict__ b, long n,
                                                     double* __restrict__ part) {
@@ -946,28 +937,10 @@ struct mdp_md {
         tab_s.view.rfac0 = tab.view.rfac0;
         tab_s.view.rmin0 = tab.view.rmin0;
         tab_s.view.switchflag = tab.view.switchflag;
-        // Y weights of each coupling triple as (bispectrum index, multiplicity factor), read off the
-        // uniform-beta table with one-hot coefficient vectors
-        const int nc = ix.idxb_max + 1;
+        // Y weights of each coupling triple as (bispectrum index, multiplicity factor)
         std::vector<int> jjb;
         std::vector<double> fac;
-        for (int k = 0; k < ix.idxb_max; ++k) {
-            std::vector<double> beta(nc, 0.0);
-            beta[1 + k] = 1.0;
-            const std::vector<double> t = y_beta_table(ix, beta.data(), 0, 0, 0, 1, 0, nc, 0);
-            if (jjb.empty()) {
-                jjb.assign(t.size(), -1);
-                fac.assign(t.size(), 0.0);
-            }
-            for (size_t q = 0; q < t.size(); ++q)
-                if (t[q] != 0.0) {
-                    if (jjb[q] >= 0) throw Error("mdp_md_set_surrogate: coupling triple with two coefficients");
-                    jjb[q] = k;
-                    fac[q] = t[q];
-                }
-        }
-        for (int q : jjb)
-            if (q < 0) throw Error("mdp_md_set_surrogate: coupling triple without a coefficient");
+        y_triple_weights(ix, 0, jjb, fac);
         sur_ntrip = static_cast<int>(jjb.size());
         // the model's descriptors are B - bzero_j (wself = 1): fold bzero into the centring
         std::vector<double> a(aff, aff + 34);
@@ -1018,8 +991,8 @@ struct mdp_md {
             launch_b_half(s4, sUr.as<double>(), sUi.as<double>(), sB.as<double>(), N, 0, st);
             launch_mlp_eval(sPacked.as<unsigned short>(), sParams.as<float>(), sAff.as<double>(), sur_nin,
                             sB.as<double>(), N, sE.as<double>(), sG.as<double>(), st);
-            hipLaunchKernelGGL(k_beta_peratom, dim3(nb(static_cast<long>(sur_ntrip) * N, 256)), dim3(256), 0, st,
-                               sG.as<double>(), sJjb.as<int>(), sFac.as<double>(), sur_ntrip, N, sBeta.as<double>());
+            launch_beta_peratom(sG.as<double>(), sJjb.as<int>(), sFac.as<double>(), sur_ntrip, N, sBeta.as<double>(),
+                                st);
             launch_yi_half_pa(s4, sUr.as<double>(), sUi.as<double>(), sBeta.as<double>(), sYr.as<double>(),
                               sYi.as<double>(), sRowOrder.as<int>(), N, st);
             launch_dedr(s4, sYr.as<double>(), sYi.as<double>(), prij.as<double>(), pi.as<int>(), pti.as<int>(),

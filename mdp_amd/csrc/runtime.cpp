@@ -96,4 +96,28 @@ std::vector<double> y_beta_table(const SnapIndex& ix, const double* beta, int e1
     return out;
 }
 
+void y_triple_weights(const SnapIndex& ix, int bnorm, std::vector<int>& jjb, std::vector<double>& fac) {
+    // read off the weight table with one-hot coefficient vectors
+    const int nc = ix.idxb_max + 1;
+    jjb.clear();
+    fac.clear();
+    for (int k = 0; k < ix.idxb_max; ++k) {
+        std::vector<double> beta(nc, 0.0);
+        beta[1 + k] = 1.0;
+        const std::vector<double> t = y_beta_table(ix, beta.data(), 0, 0, 0, 1, 0, nc, bnorm);
+        if (jjb.empty()) {
+            jjb.assign(t.size(), -1);
+            fac.assign(t.size(), 0.0);
+        }
+        for (size_t q = 0; q < t.size(); ++q)
+            if (t[q] != 0.0) {
+                if (jjb[q] >= 0) throw Error("coupling triple with two bispectrum coefficients");
+                jjb[q] = k;
+                fac[q] = t[q];
+            }
+    }
+    for (int q : jjb)
+        if (q < 0) throw Error("coupling triple without a bispectrum coefficient");
+}
+
 }  // namespace mdp

@@ -73,4 +73,8 @@ std::mutex& library_mutex();
 std::vector<double> y_beta_table(const SnapIndex& ix, const double* beta, int e1, int e2, int e3, int nelem,
                                  int ielem, int ncoeffall, int bnorm);
 
+// The same weights for one element as (bispectrum index, factor) per coupling triple: the weight of
+// triple t is fac[t] * beta[jjb[t]] (multiplicity, the (j1+1)/(j+1) of unnormalised B, or bnorm's 1/(j+1)).
+void y_triple_weights(const SnapIndex& ix, int bnorm, std::vector<int>& jjb, std::vector<double>& fac);
+
 }  // namespace mdp

@@ -53,7 +53,10 @@ void launch_yi(const SnapDev& s, const double* Ur, const double* Ui, const doubl
                double* Yi, double* epart, int natoms, bool full_mid, hipStream_t st);
 
 // Y with one weight per (coupling triple, atom): beta_t[triple*N + atom]; full layout, every idxz entry.
-// Compiled for twojmax 2 and 4 (the surrogate's descriptors); returns false for other J.
+// Returns false for a twojmax without a compiled kernel.
+// Per-atom weights of Y from per-atom bispectrum weights: beta_t[t*N + a] = fac[t] * g[jjb[t]*N + a].
+void launch_beta_peratom(const double* g, const int* jjb, const double* fac, int ntrip, int N, double* beta_t,
+                         hipStream_t st);
 bool launch_yi_peratom(const SnapDev& s, const double* Ur, const double* Ui, const double* beta_t, double* Yr,
                        double* Yi, int natoms, hipStream_t st);
 

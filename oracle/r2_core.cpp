@@ -42,6 +42,14 @@ void r2_cpuSnapComputeYi(double *ylist_r, double *ylist_i, double *ulisttot_r, d
                      inum);
 }
 
+void r2_cpuSnapComputeYiPerAtom(double *ylist_r, double *ylist_i, double *ulisttot_r, double *ulisttot_i,
+                                double *cglist, double *betalist, int *idxz, int *idxb_block, int *idxu_block,
+                                int *idxcg_block, int twojmax, int idxb_max, int idxu_max, int idxz_max,
+                                int nelements, int bnorm_flag, int inum) {
+    cpuSnapComputeYi(ylist_r, ylist_i, ulisttot_r, ulisttot_i, cglist, betalist, idxz, idxb_block, idxu_block,
+                     idxcg_block, twojmax, idxb_max, idxu_max, idxz_max, nelements, bnorm_flag, inum);
+}
+
 void r2_cpuSnapComputeFi(double *fatom, double *vatom, double *ylist_r, double *ylist_i, double *rootpqarray,
                          double *rij, double *wjelem, double *radelem, double rmin0, double rfac0, double rcutfac,
                          int *idxu_block, int *map, int *aii, int *ai, int *aj, int *ti, int *tj, int twojmax,

@@ -123,6 +123,68 @@ def test_backend3_force_chain_vs_upstream_core(oracle, J, ncell, seed):
     assert relerr(vt, v1.sum(axis=0)) < TOL
 
 
+@pytest.mark.parametrize("J,bnorm,seed", [(4, 0, 4), (8, 0, 5), (6, 1, 6)])
+def test_backend3_yi_peratom(oracle, J, bnorm, seed):
+    """The per-atom-weight form of SnapComputeYi (betalist[jjb * inum + atom]) on the U of the GPU chain:
+    against the upstream C++ core's cpuSnapComputeYi (cpuSnap4.cpp:601-690) on the same U and weights, and,
+    with every atom given the same weights, against the coefficient form (one weight table for all)."""
+    from mdp_amd import library
+    from mdp_amd.devarray import DevArray
+    gpu = _bind(library().cdll, "mdp_hip")
+    pa = library().cdll.mdp_hipSnapComputeYiPerAtom
+    pa.argtypes, pa.restype = [P] * 10 + [I] * 7, None
+    x, box = lattice.crystal("bcc", 3, 3.316, 0.1, seed)
+    param = [1, J, RCUTFAC, RFAC0, 0, 1, 1, 0, 0, bnorm, 0]
+    s0 = snapcpp.initsna(oracle, param, [0.5], [1.0])
+    coeff = lattice.snap_coefficients(s0.ncoeff, seed=seed)
+    s = snapcpp.initsna(oracle, param, [0.5], [1.0], coeff)
+    B = np.diag(box)
+    p = snapcpp.pairs_from_box(x, np.ones(len(x), np.int32), B[0], B[1], B[2], [1, 1, 1], s)
+    N, iu, ib = p["n"], s.idxu_max, s.idxb_max
+    rng = np.random.default_rng(seed)
+    beta = np.ascontiguousarray(rng.normal(0.0, 1.0, (ib, N)))
+    tables = dict(rootpq=s.rootpqarray, rij=np.ascontiguousarray(p["rij"], np.float64), wj=s.wjelem, rad=s.radelem,
+                  cg=s.cglist, coeff=s.coeffelem, idxu_block=s.idxu_block, idxz=s.idxz, idxb_block=s.idxb_block,
+                  idxcg_block=s.idxcg_block, map=s.map, ai=np.ascontiguousarray(p["ai"], np.int32),
+                  ti=np.ascontiguousarray(p["ti"], np.int32), tj=np.ascontiguousarray(p["tj"], np.int32),
+                  type=p["atomtype"], beta=beta,
+                  beta_u=np.ascontiguousarray(np.repeat(s.coeffelem[1:ib + 1, None], N, axis=1)))
+    dev = {k: DevArray.from_numpy(np.ascontiguousarray(v), np.asarray(v).dtype) for k, v in tables.items()}
+    for k in ("Ur", "Ui", "Yr", "Yi", "Yr1", "Yi1"):
+        dev[k] = DevArray.zeros(N * iu)
+    d = lambda k: dev[k].ptr
+    gpu["SnapComputeUi"](d("Ur"), d("Ui"), d("rootpq"), d("rij"), d("wj"), d("rad"), s.rmin0, s.rfac0, s.rcutfac,
+                         d("idxu_block"), d("map"), d("ai"), d("ti"), d("tj"), J, iu, N, p["ijnum"], s.switchflag, 0)
+    gpu["AddWself2Ui"](d("Ur"), d("Ui"), s.wself, d("idxu_block"), d("type"), d("map"), d("ai"), s.wselfallflag, 0,
+                       iu, 1, J, N)
+    idx = lambda: [d("idxz"), d("idxb_block"), d("idxu_block"), d("idxcg_block"), J, ib, iu, s.idxz_max, 1, bnorm, N]
+    pa(d("Yr"), d("Yi"), d("Ur"), d("Ui"), d("cg"), d("beta"), *idx())
+    pa(d("Yr1"), d("Yi1"), d("Ur"), d("Ui"), d("cg"), d("beta_u"), *idx())
+    assert library().cdll.mdp_snap_last_error().decode() == ""
+    got = (dev["Yr"].numpy(), dev["Yi"].numpy())
+    print(f"per-atom Y J={J} bnorm={bnorm}: {N} atoms", flush=True)
+    # uniform weights: the coefficient form on the same U
+    for k in ("Yr", "Yi"):
+        dev[k] = DevArray.zeros(N * iu)
+    gpu["SnapComputeYi"](d("Yr"), d("Yi"), d("Ur"), d("Ui"), d("cg"), d("coeff"), d("map"), d("type"), d("idxz"),
+                         d("idxb_block"), d("idxu_block"), d("idxcg_block"), J, ib, iu, s.idxz_max, 1, s.ncoeffall,
+                         bnorm, N)
+    assert relerr(dev["Yr1"].numpy(), dev["Yr"].numpy()) < TOL
+    assert relerr(dev["Yi1"].numpy(), dev["Yi"].numpy()) < TOL
+    if not os.path.exists(R2LIB):
+        pytest.skip("upstream C++ core checker not built (oracle/_ref)")
+    r2 = ctypes.CDLL(R2LIB).r2_cpuSnapComputeYiPerAtom
+    r2.argtypes, r2.restype = [P] * 10 + [I] * 7, None
+    h = {k: np.ascontiguousarray(v) for k, v in tables.items()}
+    Ur, Ui = dev["Ur"].numpy(), dev["Ui"].numpy()
+    Yr, Yi = np.zeros(N * iu), np.zeros(N * iu)
+    hp = lambda a: a.ctypes.data
+    r2(hp(Yr), hp(Yi), hp(Ur), hp(Ui), hp(h["cg"]), hp(h["beta"]), hp(h["idxz"]), hp(h["idxb_block"]),
+       hp(h["idxu_block"]), hp(h["idxcg_block"]), J, ib, iu, s.idxz_max, 1, bnorm, N)
+    assert relerr(got[0], Yr) < TOL
+    assert relerr(got[1], Yi) < TOL
+
+
 # ---- the descriptor chain of the upstream fitting driver ComputeSnap (snappotential.cpp:762-910) ----
 
 DSIGS = {
