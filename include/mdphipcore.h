@@ -211,6 +211,20 @@ inline void hipComputeDbidrj<double>(double *dblist, double *zlist_r, double *zl
 }
 
 template <typename T>
+void hipComputeDbidrj(T *dblist, T *zlist_r, T *zlist_i, T *dulist_r, T *dulist_i, int *idxb, int *idxu_block,
+                      int *idxz_block, int *map, int *ai, int *aj, int *ti, int *tj, int twojmax, int idxb_max,
+                      int idxu_max, int idxz_max, int nelements, int bnorm_flag, int chemflag, int ijnum);
+template <>
+inline void hipComputeDbidrj<double>(double *dblist, double *zlist_r, double *zlist_i, double *dulist_r,
+                                     double *dulist_i, int *idxb, int *idxu_block, int *idxz_block, int *map, int *ai,
+                                     int *aj, int *ti, int *tj, int twojmax, int idxb_max, int idxu_max, int idxz_max,
+                                     int nelements, int bnorm_flag, int chemflag, int ijnum) {
+    mdp_hipComputeDbidrjAtomMajor(dblist, zlist_r, zlist_i, dulist_r, dulist_i, idxb, idxu_block, idxz_block, map, ai,
+                                  aj, ti, tj, twojmax, idxb_max, idxu_max, idxz_max, nelements, bnorm_flag, chemflag,
+                                  ijnum);
+}
+
+template <typename T>
 void hipSnapTallyBispectrum(T *bi, T *bispectrum, int *ilist, int *type, int inum, int ncoeff, int nperdim, int ntype,
                             int quadraticflag);
 template <>

@@ -460,6 +460,12 @@ void mdp_hipComputeDbidrj(double *dblist, double *zlist_r, double *zlist_i, doub
                           int *idxb, int *idxu_block, int *idxz_block, int *map, int *ai, int *tj, int twojmax,
                           int idxb_max, int idxu_max, int idxz_max, int nelements, int bnorm_flag, int chemflag,
                           int inum, int ijnum);
+/* replaces hipComputeDbidrj in the (ai, aj, ti, tj) form of ComputeDbidrj (mdpcore.h:3366-3393;
+   src/C++/Core/cpuSnap.cpp:1050-1266): Z atom-major, dU and dB pair-major; aj and ti are not read */
+void mdp_hipComputeDbidrjAtomMajor(double *dblist, double *zlist_r, double *zlist_i, double *dulist_r,
+                                   double *dulist_i, int *idxb, int *idxu_block, int *idxz_block, int *map, int *ai,
+                                   int *aj, int *ti, int *tj, int twojmax, int idxb_max, int idxu_max, int idxz_max,
+                                   int nelements, int bnorm_flag, int chemflag, int ijnum);
 /* replaces hipSnapTallyBispectrum in SnapTallyBispectrum (mdpcore.h:3750-3767) */
 void mdp_hipSnapTallyBispectrum(double *bi, double *bispectrum, int *ilist, int *type, int inum, int ncoeff,
                                 int nperdim, int ntype, int quadraticflag);

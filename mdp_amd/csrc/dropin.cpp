@@ -1071,6 +1071,41 @@ void mdp_hipComputeDbidrj(double* dblist, double* zlist_r, double* zlist_i, doub
                      inum, ijnum);
 }
 
+// ComputeDbidrj backend 3, (ai, aj, ti, tj) form (mdpcore.h:3366-3393; src/C++/Core/cpuSnap.cpp:1050-1266):
+// Z atom-major (zlist[(iT + ne^2 i) idxz_max + jjz]), dU and dB pair-major with x, y, z innermost
+void mdp_hipComputeDbidrjAtomMajor(double* dblist, double* zlist_r, double* zlist_i, double* dulist_r,
+                                   double* dulist_i, int* idxb, int* idxu_block, int* idxz_block, int* map, int* ai,
+                                   int* aj, int* ti, int* tj, int twojmax, int idxb_max, int idxu_max, int idxz_max,
+                                   int nelements, int bnorm_flag, int chemflag, int ijnum) {
+    (void)idxb; (void)idxu_block; (void)idxz_block; (void)aj; (void)ti;
+    std::vector<int> h_ai, h_tj, h_map;
+    if (!fetch_args([&] {
+            const size_t np = static_cast<size_t>(std::max(ijnum, 0));
+            h_ai = fetch(ai, np);
+            h_tj = fetch(tj, np);
+            h_map = fetch(map, (chemflag ? max_type(h_tj, np) : 1) + 1);
+        }))
+        return;
+    guarded([&] {
+        if (ijnum <= 0) return;
+        if (nelements < 1) throw Error("ComputeDbidrj: nelements must be positive");
+        for (int p = 0; p < ijnum; ++p)
+            if (h_ai[p] < 0 || (chemflag && h_tj[p] < 0)) throw Error("ComputeDbidrj: negative pair index");
+        if (chemflag)
+            for (int t : h_tj)
+                if (h_map[t] < 0 || h_map[t] >= nelements) throw Error("ComputeDbidrj: element map out of range");
+        hipStream_t st = library_stream();
+        const int ntypes = chemflag ? max_type(h_tj, h_tj.size()) : 1;
+        const SnapDev& s = tables(twojmax, nullptr, nullptr, nullptr, nullptr, chemflag ? h_map.data() : nullptr,
+                                  ntypes, st);
+        if (idxu_max != s.idxu_max || idxz_max != s.idxz_max || idxb_max != s.idxb_max)
+            throw Error("table sizes do not match twojmax");
+        launch_dbidrj_atom_major(s, dblist, zlist_r, zlist_i, dulist_r, dulist_i, ai, tj, nelements, bnorm_flag,
+                                 chemflag, ijnum, st);
+        finish(st);
+    });
+}
+
 // SnapTallyBispectrum backend 3 (mdpcore.h:3750-3767; src/C++/Core/cpuSnap2.cpp:1297-1322)
 void mdp_hipSnapTallyBispectrum(double* bi, double* bispectrum, int* ilist, int* type, int inum, int ncoeff,
                                 int nperdim, int ntype, int quadraticflag) {

@@ -893,4 +893,70 @@ void launch_tally_virial_full(double* vatom, const double* fij, const double* ri
                        ijnum);
 }
 
+// dB/dr of the per-pair route (cpuSnap.cpp:1050-1266): Z atom-major zr[(iT + ne^2 i) idxz_max + jjz], dU and
+// dB pair-major du[(ij idxu_max + jju) 3 + k], db[((ij ne^3 + itriple) idxb_max + jjb) 3 + k].  One thread per
+// (pair, bispectrum triple) owns every element triple of its dB entry, so the three symmetric terms add up
+// without atomics.
+__device__ __forceinline__ void dbidrj_term(int jj, int jjz, int jju, const double* __restrict__ zr,
+                                            const double* __restrict__ zi, const double* __restrict__ dur,
+                                            const double* __restrict__ dui, double s[3]) {
+    s[0] = s[1] = s[2] = 0.0;
+    for (int mb = 0; 2 * mb < jj; ++mb)
+        for (int ma = 0; ma <= jj; ++ma, ++jjz, ++jju)
+            for (int k = 0; k < 3; ++k) s[k] += dur[3 * jju + k] * zr[jjz] + dui[3 * jju + k] * zi[jjz];
+    if (jj % 2 == 0) {
+        const int mb = jj / 2;
+        for (int ma = 0; ma < mb; ++ma, ++jjz, ++jju)
+            for (int k = 0; k < 3; ++k) s[k] += dur[3 * jju + k] * zr[jjz] + dui[3 * jju + k] * zi[jjz];
+        for (int k = 0; k < 3; ++k) s[k] += (dur[3 * jju + k] * zr[jjz] + dui[3 * jju + k] * zi[jjz]) * 0.5;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dbidrj_atom_major(SnapDev s, double* __restrict__ db,
+                                                           const double* __restrict__ zr,
+                                                           const double* __restrict__ zi,
+                                                           const double* __restrict__ dur,
+                                                           const double* __restrict__ dui, const int* __restrict__ ai,
+                                                           const int* __restrict__ tj, int ne, int bnorm, int chem,
+                                                           int ijnum) {
+    const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int nb = s.idxb_max, jd = s.J + 1, ne2 = ne * ne, ne3 = ne2 * ne;
+    if (t >= static_cast<long>(ijnum) * nb) return;
+    const int ij = static_cast<int>(t / nb), jjb = static_cast<int>(t % nb);
+    double* __restrict__ d = db + static_cast<size_t>(ij) * ne3 * nb * 3;
+    for (int it = 0; it < ne3; ++it)
+        for (int k = 0; k < 3; ++k) d[(it * nb + jjb) * 3 + k] = 0.0;
+    const int e3 = chem ? s.map[tj[ij]] : 0, i = ai[ij];
+    const int j1 = s.idxb[3 * jjb], j2 = s.idxb[3 * jjb + 1], j = s.idxb[3 * jjb + 2];
+    const double* __restrict__ pr = dur + static_cast<size_t>(ij) * s.idxu_max * 3;
+    const double* __restrict__ pi = dui + static_cast<size_t>(ij) * s.idxu_max * 3;
+    const double j1fac = (j + 1) / (j1 + 1.0), j2fac = (j + 1) / (j2 + 1.0);
+    double sm[3];
+    for (int e1 = 0; e1 < ne; ++e1)
+        for (int e2 = 0; e2 < ne; ++e2) {
+            const size_t z12 = (static_cast<size_t>(i) * ne2 + e1 * ne + e2) * s.idxz_max;
+            const size_t z21 = (static_cast<size_t>(i) * ne2 + e2 * ne + e1) * s.idxz_max;
+            double* o = d + (((e1 * ne + e2) * ne + e3) * nb + jjb) * 3;
+            dbidrj_term(j, s.idxz_block[j + j2 * jd + j1 * jd * jd], s.idxu_block[j], zr + z12, zi + z12, pr, pi, sm);
+            for (int k = 0; k < 3; ++k) o[k] += 2.0 * sm[k];
+            o = d + (((e3 * ne + e2) * ne + e1) * nb + jjb) * 3;
+            dbidrj_term(j1, s.idxz_block[j1 + j2 * jd + j * jd * jd], s.idxu_block[j1], zr + z12, zi + z12, pr, pi,
+                        sm);
+            for (int k = 0; k < 3; ++k) o[k] += bnorm ? 2.0 * sm[k] : 2.0 * sm[k] * j1fac;
+            o = d + (((e1 * ne + e3) * ne + e2) * nb + jjb) * 3;
+            dbidrj_term(j2, s.idxz_block[j2 + j1 * jd + j * jd * jd], s.idxu_block[j2], zr + z21, zi + z21, pr, pi,
+                        sm);
+            for (int k = 0; k < 3; ++k) o[k] += bnorm ? 2.0 * sm[k] : 2.0 * sm[k] * j2fac;
+        }
+}
+
+void launch_dbidrj_atom_major(const SnapDev& s, double* db, const double* zr, const double* zi, const double* dur,
+                              const double* dui, const int* ai, const int* tj, int ne, int bnorm, int chem, int ijnum,
+                              hipStream_t st) {
+    const long n = static_cast<long>(ijnum) * s.idxb_max;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_dbidrj_atom_major, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, s, db, zr,
+                       zi, dur, dui, ai, tj, ne, bnorm, chem, ijnum);
+}
+
 }  // namespace mdp

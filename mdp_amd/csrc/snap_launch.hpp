@@ -147,6 +147,11 @@ void launch_ei_from_b(const SnapDev& s, double* eatom, const double* blist, cons
                       hipStream_t st);
 
 // dblist of cpuComputeDbidrj (zeroed by the caller)
+// dB/dr of the per-pair route (cpuSnap.cpp:1050-1266): Z atom-major, dU and dB pair-major (AoS in k);
+// writes every dB entry of the ijnum pairs (no prior clear needed).
+void launch_dbidrj_atom_major(const SnapDev& s, double* db, const double* zr, const double* zi, const double* dur,
+                              const double* dui, const int* ai, const int* tj, int ne, int bnorm, int chem, int ijnum,
+                              hipStream_t st);
 void launch_dbidrj(const SnapDev& s, double* dblist, const double* zr, const double* zi, const double* dur,
                    const double* dui, const int* ai, const int* tj, int bnorm, int chemflag, int nelem, int inum,
                    int ijnum, hipStream_t st);

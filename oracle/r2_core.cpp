@@ -133,6 +133,15 @@ void r2_cpuComputeDbidrj(double *dblist, double *zlist_r, double *zlist_i, doubl
                      idxb_max, idxu_max, idxz_max, nelements, bnorm_flag, chemflag, inum, ijnum);
 }
 
+// the per-pair route's form (cpuSnap.cpp:1050-1266)
+void r2_cpuComputeDbidrjAtomMajor(double *dblist, double *zlist_r, double *zlist_i, double *dulist_r,
+                                  double *dulist_i, int *idxb, int *idxu_block, int *idxz_block, int *map, int *ai,
+                                  int *aj, int *ti, int *tj, int twojmax, int idxb_max, int idxu_max, int idxz_max,
+                                  int nelements, int bnorm_flag, int chemflag, int ijnum) {
+    cpuComputeDbidrj(dblist, zlist_r, zlist_i, dulist_r, dulist_i, idxb, idxu_block, idxz_block, map, ai, aj, ti, tj,
+                     twojmax, idxb_max, idxu_max, idxz_max, nelements, bnorm_flag, chemflag, ijnum);
+}
+
 void r2_cpuSnapTallyBispectrum(double *bi, double *bispectrum, int *ilist, int *type, int inum, int ncoeff,
                                int nperdim, int ntype, int quadraticflag) {
     cpuSnapTallyBispectrum(bi, bispectrum, ilist, type, inum, ncoeff, nperdim, ntype, quadraticflag);

@@ -11,7 +11,8 @@ NeighborPairList -> Cumsum -> NeighborPairs -> ComputeSij -> ZeroUarraytot2 -> A
 (ai, tj) and the (pairnum, pairnumsum) forms) -> ComputeZi2 -> ComputeBi2 -> ComputeDbidrj ->
 SnapTally{Bispectrum, BispectrumDeriv, BispectrumVirial} -- runs the same way on a ghost-extended,
 fixed-stride neighbour list: integer arrays bit for bit, the rest within 1e-10, with quadratic tallies
-and a two-type chemflag case.
+and a two-type chemflag case.  The per-atom-weight SnapComputeYi, the (ai, aj, ti, tj) ComputeDbidrj of
+the per-pair route and the full tallies are compared one function at a time.
 """
 import ctypes
 import os
@@ -412,3 +413,42 @@ def test_backend3_full_tallies_vs_upstream_core(quadratic):
     assert gpu.mdp_snap_last_error().decode() == ""
     assert relerr(d["fatom"].numpy(), fref) < 1e-12
     assert relerr(d["vatom"].numpy(), vref) < 1e-12
+
+
+@pytest.mark.parametrize("J,ne,chem,bnorm", [(4, 1, 0, 0), (8, 1, 0, 1), (6, 2, 1, 0)])
+def test_backend3_dbidrj_atom_major_vs_upstream_core(oracle, J, ne, chem, bnorm):
+    """The (ai, aj, ti, tj) form of ComputeDbidrj (Z atom-major, dU and dB pair-major) against the upstream
+    C++ core's cpuComputeDbidrj of cpuSnap.cpp:1050-1266 on the same seeded Z, dU and pairs."""
+    if not os.path.exists(R2LIB):
+        pytest.skip("upstream C++ core checker not built (oracle/_ref)")
+    from mdp_amd import library
+    from mdp_amd.devarray import DevArray
+    sig = [P] * 13 + [I] * 8
+    r2 = ctypes.CDLL(R2LIB).r2_cpuComputeDbidrjAtomMajor
+    gpu = library().cdll.mdp_hipComputeDbidrjAtomMajor
+    for f in (r2, gpu):
+        f.argtypes, f.restype = sig, None
+    s = snapcpp.initsna(oracle, [ne, J, RCUTFAC, RFAC0, 0, 1, 1, 0, chem, bnorm, 0], [0.5] * ne, [1.0] * ne)
+    nel = s.nelements
+    inum, ijnum = 37, 411
+    rng = np.random.default_rng(J + 10 * ne)
+    ai = np.sort(rng.integers(0, inum, ijnum)).astype(np.int32)
+    tj = rng.integers(1, ne + 1, ijnum).astype(np.int32)
+    host = dict(zr=rng.normal(0, 1, inum * s.idxz_max * nel * nel), zi=rng.normal(0, 1, inum * s.idxz_max * nel * nel),
+                dur=rng.normal(0, 1, 3 * ijnum * s.idxu_max), dui=rng.normal(0, 1, 3 * ijnum * s.idxu_max),
+                idxb=s.idxb, idxu_block=s.idxu_block, idxz_block=s.idxz_block, map=s.map, ai=ai,
+                aj=(ai + 1) % inum, ti=np.ones(ijnum, np.int32), tj=tj)
+    host = {k: np.ascontiguousarray(v) for k, v in host.items()}
+    ndb = ijnum * s.idxb_max * 3 * nel ** 3
+    ref = np.full(ndb, np.nan)
+    dev = {k: DevArray.from_numpy(v, v.dtype) for k, v in host.items()}
+    dev["db"] = DevArray.from_numpy(np.full(ndb, np.nan), np.float64)
+    order = ("zr", "zi", "dur", "dui", "idxb", "idxu_block", "idxz_block", "map", "ai", "aj", "ti", "tj")
+    ints = [J, s.idxb_max, s.idxu_max, s.idxz_max, nel, bnorm, chem, ijnum]
+    r2(ref.ctypes.data, *[host[k].ctypes.data for k in order], *ints)
+    gpu(dev["db"].ptr, *[dev[k].ptr for k in order], *ints)
+    assert library().cdll.mdp_snap_last_error().decode() == ""
+    got = dev["db"].numpy()
+    print(f"dB/dr atom-major J={J} ne={nel} chem={chem} bnorm={bnorm}: {ijnum} pairs", flush=True)
+    assert np.isfinite(ref).all() and np.isfinite(got).all()
+    assert relerr(got, ref) < TOL
