@@ -225,6 +225,43 @@ inline void hipComputeDbidrj<double>(double *dblist, double *zlist_r, double *zl
 }
 
 template <typename T>
+void hipCellList3D(int *clist, T *x, T *eta1, T *eta2, T *eta3, T *s2rmap, int *nc, int inum, int natom, int dim);
+template <>
+inline void hipCellList3D<double>(int *clist, double *x, double *eta1, double *eta2, double *eta3, double *s2rmap,
+                                  int *nc, int inum, int natom, int dim) {
+    mdp_hipCellList3D(clist, x, eta1, eta2, eta3, s2rmap, nc, inum, natom, dim);
+}
+
+inline void hipCell2AtomList(int *c2alist, int *c2anumsum, int *c2anum, int *clist, int *dtemp, int natom, int ncell) {
+    mdp_hipCell2AtomList(c2alist, c2anumsum, c2anum, clist, dtemp, natom, ncell);
+}
+
+template <typename T>
+void hipFullNeighborList3D(int *neighlist, int *neighnum, T *x, T *rcutsq, int anum, int inum, int jnum, int dim);
+template <>
+inline void hipFullNeighborList3D<double>(int *neighlist, int *neighnum, double *x, double *rcutsq, int anum, int inum,
+                                          int jnum, int dim) {
+    mdp_hipFullNeighborList3D(neighlist, neighnum, x, rcutsq, anum, inum, jnum, dim);
+}
+
+/* the cell-list form; the dispatch's backend-3 branch calls it hipFullNeighborList23 */
+template <typename T>
+void hipFullNeighborList3D(int *neighlist, int *neighnum, T *x, T *rcutsq, int *alist, int *clist, int *c2alist,
+                           int *c2anumsum, int *nc, int inum, int jnum, int dim);
+template <>
+inline void hipFullNeighborList3D<double>(int *neighlist, int *neighnum, double *x, double *rcutsq, int *alist,
+                                          int *clist, int *c2alist, int *c2anumsum, int *nc, int inum, int jnum,
+                                          int dim) {
+    mdp_hipFullNeighborList3DCell(neighlist, neighnum, x, rcutsq, alist, clist, c2alist, c2anumsum, nc, inum, jnum,
+                                  dim);
+}
+template <typename T>
+inline void hipFullNeighborList23(int *neighlist, int *neighnum, T *x, T *rcutsq, int *alist, int *clist,
+                                  int *c2alist, int *c2anumsum, int *nc, int inum, int jnum, int dim) {
+    hipFullNeighborList3D<T>(neighlist, neighnum, x, rcutsq, alist, clist, c2alist, c2anumsum, nc, inum, jnum, dim);
+}
+
+template <typename T>
 void hipSnapTallyBispectrum(T *bi, T *bispectrum, int *ilist, int *type, int inum, int ncoeff, int nperdim, int ntype,
                             int quadraticflag);
 template <>

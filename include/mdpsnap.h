@@ -460,6 +460,20 @@ void mdp_hipComputeDbidrj(double *dblist, double *zlist_r, double *zlist_i, doub
                           int *idxb, int *idxu_block, int *idxz_block, int *map, int *ai, int *tj, int twojmax,
                           int idxb_max, int idxu_max, int idxz_max, int nelements, int bnorm_flag, int chemflag,
                           int inum, int ijnum);
+/* replaces hipCellList3D in CellList3D (mdpcore.h:1332-1349; src/C++/Core/cpuNeighborList3D.cpp:334-356) */
+void mdp_hipCellList3D(int *clist, double *x, double *eta1, double *eta2, double *eta3, double *s2rmap, int *nc,
+                       int inum, int natom, int dim);
+/* replaces hipCell2AtomList in Cell2AtomList (mdpcore.h:1351-1368; src/C++/Core/cpuNeighborList2D.cpp:298-308);
+   dtemp is not used */
+void mdp_hipCell2AtomList(int *c2alist, int *c2anumsum, int *c2anum, int *clist, int *dtemp, int natom, int ncell);
+/* replaces hipFullNeighborList23 in the cell-list form of FullNeighborList3D (mdpcore.h:1432-1453;
+   src/C++/Core/cpuNeighborList3D.cpp:361-412) */
+void mdp_hipFullNeighborList3DCell(int *neighlist, int *neighnum, double *x, double *rcutsq, int *alist, int *clist,
+                                   int *c2alist, int *c2anumsum, int *nc, int inum, int jnum, int dim);
+/* replaces hipFullNeighborList3D in the all-pairs form of FullNeighborList3D (mdpcore.h:1390-1408;
+   src/C++/Core/cpuNeighborList3D.cpp:417-441) */
+void mdp_hipFullNeighborList3D(int *neighlist, int *neighnum, double *x, double *rcutsq, int anum, int inum, int jnum,
+                               int dim);
 /* replaces hipComputeDbidrj in the (ai, aj, ti, tj) form of ComputeDbidrj (mdpcore.h:3366-3393;
    src/C++/Core/cpuSnap.cpp:1050-1266): Z atom-major, dU and dB pair-major; aj and ti are not read */
 void mdp_hipComputeDbidrjAtomMajor(double *dblist, double *zlist_r, double *zlist_i, double *dulist_r,

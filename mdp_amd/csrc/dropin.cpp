@@ -934,6 +934,77 @@ void mdp_hipCumsum(int* d_out, int* d_in, int* d_sums, int* d_incr, int length) 
     });
 }
 
+// CellList3D backend 3 (mdpcore.h:1332-1349; src/C++/Core/cpuNeighborList3D.cpp:334-356): cell of every atom
+void mdp_hipCellList3D(int* clist, double* x, double* eta1, double* eta2, double* eta3, double* s2rmap, int* nc,
+                       int inum, int natom, int dim) {
+    std::vector<int> h_nc;
+    if (!fetch_args([&] { h_nc = fetch(nc, 3); })) return;
+    guarded([&] {
+        if (dim != 3) throw Error("CellList3D: dim must be 3");
+        if (natom <= 0) return;
+        if (h_nc[0] < 1 || h_nc[1] < 1 || h_nc[2] < 1) throw Error("CellList3D: cell counts must be positive");
+        hipStream_t st = library_stream();
+        launch_cell_list3d(clist, x, eta1, eta2, eta3, s2rmap, h_nc.data(), inum, natom, st);
+        finish(st);
+    });
+}
+
+// Cell2AtomList backend 3 (mdpcore.h:1351-1368; src/C++/Core/cpuNeighborList2D.cpp:298-308): atoms sorted by
+// cell (stable), the first entry of each cell and the sorted cell ids.  dtemp is not used (library scratch).
+void mdp_hipCell2AtomList(int* c2alist, int* c2anumsum, int* c2anum, int* clist, int* dtemp, int natom, int ncell) {
+    (void)dtemp;
+    std::vector<int> h_cl;
+    if (!fetch_args([&] { h_cl = fetch(clist, static_cast<size_t>(std::max(natom, 0))); })) return;
+    guarded([&] {
+        if (ncell <= 0 || natom <= 0) return;
+        for (int c : h_cl)
+            if (c < 0 || c >= ncell) throw Error("Cell2AtomList: a cell id is outside [0, ncell)");
+        hipStream_t st = library_stream();
+        int* cursor = dev<int>(0, ncell);
+        int* tmp = dev<int>(1, scan_tmp_ints(ncell));
+        launch_cell2atom(c2alist, c2anumsum, c2anum, clist, cursor, tmp, natom, ncell, st);
+        finish(st);
+    });
+}
+
+// FullNeighborList3D backend 3, cell-list form (mdpcore.h:1432-1453; cpuNeighborList3D.cpp:361-412):
+// neighbours of atom i = alist[ii] in the 27 cells around its own, neighlist[l + jnum * i], count in neighnum[ii]
+void mdp_hipFullNeighborList3DCell(int* neighlist, int* neighnum, double* x, double* rcutsq, int* alist, int* clist,
+                                   int* c2alist, int* c2anumsum, int* nc, int inum, int jnum, int dim) {
+    std::vector<int> h_nc;
+    std::vector<double> h_rc;
+    if (!fetch_args([&] {
+            h_nc = fetch(nc, 3);
+            h_rc = fetch(rcutsq, 1);
+        }))
+        return;
+    guarded([&] {
+        if (dim != 3) throw Error("FullNeighborList3D: dim must be 3");
+        if (inum <= 0) return;
+        if (jnum < 1) throw Error("FullNeighborList3D: jnum must be positive");
+        hipStream_t st = library_stream();
+        launch_full_neigh_cell(neighlist, neighnum, x, h_rc[0], alist, clist, c2alist, c2anumsum, h_nc.data(), inum,
+                               jnum, st);
+        finish(st);
+    });
+}
+
+// FullNeighborList3D backend 3, all-pairs form (mdpcore.h:1390-1408; cpuNeighborList3D.cpp:417-441)
+void mdp_hipFullNeighborList3D(int* neighlist, int* neighnum, double* x, double* rcutsq, int anum, int inum, int jnum,
+                               int dim) {
+    std::vector<double> h_rc;
+    if (!fetch_args([&] { h_rc = fetch(rcutsq, 1); })) return;
+    guarded([&] {
+        if (dim != 3) throw Error("FullNeighborList3D: dim must be 3");
+        if (inum <= 0) return;
+        if (jnum < 1) throw Error("FullNeighborList3D: jnum must be positive");
+        if (anum < inum) throw Error("FullNeighborList3D: anum must be at least inum");
+        hipStream_t st = library_stream();
+        launch_full_neigh_all(neighlist, neighnum, x, h_rc[0], anum, inum, jnum, st);
+        finish(st);
+    });
+}
+
 // NeighborPairs backend 3 (mdpcore.h:3853-3875; src/C++/Core/cpuSnap.cpp:1912-1935)
 void mdp_hipNeighPairs(double* xij, double* x, int* aii, int* ai, int* aj, int* ti, int* tj, int* pairnum,
                        int* pairlist, int* pairnumsum, int* ilist, int* atomtype, int* alist, int inum, int jnum,

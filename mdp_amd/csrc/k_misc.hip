@@ -959,4 +959,153 @@ void launch_dbidrj_atom_major(const SnapDev& s, double* db, const double* zr, co
                        zi, dur, dui, ai, tj, ne, bnorm, chem, ijnum);
 }
 
+// ---- cell-list neighbour search of the upstream core (cpuNeighborList3D.cpp:334-443) -------------------
+
+// Cell of every atom: its position mapped into the unit cube by s2rmap (column-major 3x3) and located on the
+// cell edges eta1..3; atoms i < inum search only the inner cells [1, nc - 2], ghosts every cell.  Products
+// and sums are rounded one by one, as upstream, so that the cell of an atom on an edge is the same.
+__global__ __launch_bounds__(256) void k_cell_list3d(int* __restrict__ clist, const double* __restrict__ x,
+                                                     const double* __restrict__ eta1, const double* __restrict__ eta2,
+                                                     const double* __restrict__ eta3, const double* __restrict__ m9,
+                                                     int nc0, int nc1, int nc2, int inum, int natom) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= natom) return;
+    const int m = i < inum ? 1 : 0;
+    const double x0 = x[3 * i], x1 = x[3 * i + 1], x2 = x[3 * i + 2];
+    const double t0 = m9[0] * x0 + m9[3] * x1 + m9[6] * x2;
+    const double t1 = m9[1] * x0 + m9[4] * x1 + m9[7] * x2;
+    const double t2 = m9[2] * x0 + m9[5] * x1 + m9[8] * x2;
+    int j1 = m, j2 = m, j3 = m;
+    for (; j1 < nc0 - m; ++j1)
+        if (eta1[j1] <= t0 && t0 <= eta1[j1 + 1]) break;
+    for (; j2 < nc1 - m; ++j2)
+        if (eta2[j2] <= t1 && t1 <= eta2[j2 + 1]) break;
+    for (; j3 < nc2 - m; ++j3)
+        if (eta3[j3] <= t2 && t2 <= eta3[j3 + 1]) break;
+    clist[i] = j1 + nc0 * j2 + nc0 * nc1 * j3;
+}
+
+__global__ __launch_bounds__(256) void k_fill_int(int* __restrict__ a, int v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_copy_int(int* __restrict__ a, const int* __restrict__ b, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = b[i];
+}
+
+__global__ __launch_bounds__(256) void k_cell_count(const int* __restrict__ clist, int natom, int* __restrict__ cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < natom) atomicAdd(&cnt[clist[i]], 1);
+}
+
+__global__ __launch_bounds__(256) void k_cell_place(const int* __restrict__ clist, int natom, int* __restrict__ cur,
+                                                    int* __restrict__ c2alist) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < natom) c2alist[atomicAdd(&cur[clist[i]], 1)] = i;
+}
+
+// atoms of each cell in ascending order (the order of a stable sort by cell), and the sorted cell ids
+__global__ __launch_bounds__(256) void k_cell_order(const int* __restrict__ start, int ncell, int* __restrict__ c2alist,
+                                                    int* __restrict__ c2anum) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int a = start[c], b = start[c + 1];
+    for (int p = a + 1; p < b; ++p) {
+        const int v = c2alist[p];
+        int q = p - 1;
+        while (q >= a && c2alist[q] > v) {
+            c2alist[q + 1] = c2alist[q];
+            --q;
+        }
+        c2alist[q + 1] = v;
+    }
+    for (int p = a; p < b; ++p) c2anum[p] = c;
+}
+
+// Neighbours of atom i = alist[ii] among the 27 cells around its cell, in upstream's cell and list order,
+// at most jnum of them; cells outside the grid are skipped.
+__global__ __launch_bounds__(256) void k_full_neigh_cell(int* __restrict__ nl, int* __restrict__ nn,
+                                                         const double* __restrict__ x, double rc2,
+                                                         const int* __restrict__ alist, const int* __restrict__ clist,
+                                                         const int* __restrict__ c2alist,
+                                                         const int* __restrict__ c2anumsum, int nc0, int nc1, int nc2,
+                                                         int inum, int jnum) {
+#pragma clang fp contract(off)
+    const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ii >= inum) return;
+    const int i = alist[ii];
+    const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
+    const int ca = clist[i], cb = ca % (nc0 * nc1);
+    const int c3 = (ca - cb) / (nc0 * nc1), c1 = cb % nc0, c2 = (cb - c1) / nc0;
+    int inc = 0;
+    for (int d1 = -1; d1 <= 1 && inc < jnum; ++d1)
+        for (int d2 = -1; d2 <= 1 && inc < jnum; ++d2)
+            for (int d3 = -1; d3 <= 1 && inc < jnum; ++d3) {
+                const int k1 = c1 + d1, k2 = c2 + d2, k3 = c3 + d3;
+                if (k1 < 0 || k1 >= nc0 || k2 < 0 || k2 >= nc1 || k3 < 0 || k3 >= nc2) continue;
+                const int k = k1 + nc0 * k2 + nc0 * nc1 * k3;
+                for (int l = c2anumsum[k]; l < c2anumsum[k + 1] && inc < jnum; ++l) {
+                    const int j = c2alist[l];
+                    if (j == i) continue;
+                    const double dx = x[3 * j] - xi0, dy = x[3 * j + 1] - xi1, dz = x[3 * j + 2] - xi2;
+                    if (dx * dx + dy * dy + dz * dz <= rc2) nl[inc++ + static_cast<size_t>(jnum) * i] = j;
+                }
+            }
+    nn[ii] = inc;
+}
+
+// Neighbours of atom i < inum among all anum atoms, in atom order, at most jnum of them.
+__global__ __launch_bounds__(256) void k_full_neigh_all(int* __restrict__ nl, int* __restrict__ nn,
+                                                        const double* __restrict__ x, double rc2, int anum, int inum,
+                                                        int jnum) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= inum) return;
+    const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
+    int inc = 0;
+    for (int j = 0; j < anum && inc < jnum; ++j) {
+        if (j == i) continue;
+        const double dx = x[3 * j] - xi0, dy = x[3 * j + 1] - xi1, dz = x[3 * j + 2] - xi2;
+        if (dx * dx + dy * dy + dz * dz <= rc2) nl[inc++ + static_cast<size_t>(jnum) * i] = j;
+    }
+    nn[i] = inc;
+}
+
+void launch_cell_list3d(int* clist, const double* x, const double* eta1, const double* eta2, const double* eta3,
+                        const double* s2rmap, const int nc[3], int inum, int natom, hipStream_t st) {
+    if (natom <= 0) return;
+    hipLaunchKernelGGL(k_cell_list3d, dim3((natom + 255) / 256), dim3(256), 0, st, clist, x, eta1, eta2, eta3, s2rmap,
+                       nc[0], nc[1], nc[2], inum, natom);
+}
+
+void launch_cell2atom(int* c2alist, int* c2anumsum, int* c2anum, const int* clist, int* cursor, int* scan_tmp,
+                      int natom, int ncell, hipStream_t st) {
+    if (ncell <= 0) return;
+    hipLaunchKernelGGL(k_fill_int, dim3((ncell + 255) / 256), dim3(256), 0, st, cursor, 0, ncell);
+    if (natom > 0)
+        hipLaunchKernelGGL(k_cell_count, dim3((natom + 255) / 256), dim3(256), 0, st, clist, natom, cursor);
+    exclusive_scan(cursor, c2anumsum, ncell, scan_tmp, st);
+    if (natom <= 0) return;
+    hipLaunchKernelGGL(k_copy_int, dim3((ncell + 255) / 256), dim3(256), 0, st, cursor, c2anumsum, ncell);
+    hipLaunchKernelGGL(k_cell_place, dim3((natom + 255) / 256), dim3(256), 0, st, clist, natom, cursor, c2alist);
+    hipLaunchKernelGGL(k_cell_order, dim3((ncell + 255) / 256), dim3(256), 0, st, c2anumsum, ncell, c2alist, c2anum);
+}
+
+void launch_full_neigh_cell(int* nl, int* nn, const double* x, double rc2, const int* alist, const int* clist,
+                            const int* c2alist, const int* c2anumsum, const int nc[3], int inum, int jnum,
+                            hipStream_t st) {
+    if (inum <= 0) return;
+    hipLaunchKernelGGL(k_full_neigh_cell, dim3((inum + 255) / 256), dim3(256), 0, st, nl, nn, x, rc2, alist, clist,
+                       c2alist, c2anumsum, nc[0], nc[1], nc[2], inum, jnum);
+}
+
+void launch_full_neigh_all(int* nl, int* nn, const double* x, double rc2, int anum, int inum, int jnum,
+                           hipStream_t st) {
+    if (inum <= 0) return;
+    hipLaunchKernelGGL(k_full_neigh_all, dim3((inum + 255) / 256), dim3(256), 0, st, nl, nn, x, rc2, anum, inum, jnum);
+}
+
 }  // namespace mdp

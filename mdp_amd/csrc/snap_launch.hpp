@@ -113,6 +113,19 @@ int dedr_variants(int* ids);
 size_t scan_tmp_ints(int n);
 void exclusive_scan(const int* in, int* out, int n, int* tmp, hipStream_t st);
 
+// cell-list neighbour search of the upstream core (cpuNeighborList3D.cpp:334-443, cpuNeighborList2D.cpp:298-308)
+void launch_cell_list3d(int* clist, const double* x, const double* eta1, const double* eta2, const double* eta3,
+                        const double* s2rmap, const int nc[3], int inum, int natom, hipStream_t st);
+// c2alist: atoms by cell (stable), c2anumsum[ncell+1]: first entry of each cell, c2anum: sorted cell ids;
+// cursor[ncell] and scan_tmp (scan_tmp_ints(ncell)) are scratch
+void launch_cell2atom(int* c2alist, int* c2anumsum, int* c2anum, const int* clist, int* cursor, int* scan_tmp,
+                      int natom, int ncell, hipStream_t st);
+void launch_full_neigh_cell(int* nl, int* nn, const double* x, double rc2, const int* alist, const int* clist,
+                            const int* c2alist, const int* c2anumsum, const int nc[3], int inum, int jnum,
+                            hipStream_t st);
+void launch_full_neigh_all(int* nl, int* nn, const double* x, double rc2, int anum, int inum, int jnum,
+                           hipStream_t st);
+
 }  // namespace mdp
 
 namespace mdp {

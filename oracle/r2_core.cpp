@@ -142,6 +142,23 @@ void r2_cpuComputeDbidrjAtomMajor(double *dblist, double *zlist_r, double *zlist
                      twojmax, idxb_max, idxu_max, idxz_max, nelements, bnorm_flag, chemflag, ijnum);
 }
 
+// cell-list neighbour search (cpuNeighborList3D.cpp:334-441, cpuNeighborList2D.cpp:298-308)
+void r2_cpuCellList3D(int *clist, double *x, double *eta1, double *eta2, double *eta3, double *s2rmap, int *nc,
+                      int inum, int natom, int dim) {
+    cpuCellList3D(clist, x, eta1, eta2, eta3, s2rmap, nc, inum, natom, dim);
+}
+void r2_cpuCell2AtomList(int *c2alist, int *c2anumsum, int *c2anum, int *clist, int natom, int ncell) {
+    cpuCell2AtomList(c2alist, c2anumsum, c2anum, clist, natom, ncell);
+}
+void r2_cpuFullNeighborList3DCell(int *neighlist, int *neighnum, double *x, double *rcutsq, int *alist, int *clist,
+                                  int *c2alist, int *c2anumsum, int *nc, int inum, int jnum, int dim) {
+    cpuFullNeighborList3D(neighlist, neighnum, x, rcutsq, alist, clist, c2alist, c2anumsum, nc, inum, jnum, dim);
+}
+void r2_cpuFullNeighborList3D(int *neighlist, int *neighnum, double *x, double *rcutsq, int anum, int inum, int jnum,
+                              int dim) {
+    cpuFullNeighborList3D(neighlist, neighnum, x, rcutsq, anum, inum, jnum, dim);
+}
+
 void r2_cpuSnapTallyBispectrum(double *bi, double *bispectrum, int *ilist, int *type, int inum, int ncoeff,
                                int nperdim, int ntype, int quadraticflag) {
     cpuSnapTallyBispectrum(bi, bispectrum, ilist, type, inum, ncoeff, nperdim, ntype, quadraticflag);

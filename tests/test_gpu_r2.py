@@ -452,3 +452,79 @@ def test_backend3_dbidrj_atom_major_vs_upstream_core(oracle, J, ne, chem, bnorm)
     print(f"dB/dr atom-major J={J} ne={nel} chem={chem} bnorm={bnorm}: {ijnum} pairs", flush=True)
     assert np.isfinite(ref).all() and np.isfinite(got).all()
     assert relerr(got, ref) < TOL
+
+
+@pytest.mark.parametrize("inum,ncell,jnum,seed", [(60, 4, 200, 1), (300, 5, 6, 2)])
+def test_backend3_cell_lists_vs_upstream_core(inum, ncell, jnum, seed):
+    """CellList3D -> Cell2AtomList -> FullNeighborList3D (cell-list form) and the all-pairs
+    FullNeighborList3D against the upstream C++ core (cpuNeighborList3D.cpp:334-441) on a periodic box with
+    its ghost images in a one-cell-deep ghost layer: every output bit for bit, with a neighbour cap jnum
+    that truncates in the second case."""
+    if not os.path.exists(R2LIB):
+        pytest.skip("upstream C++ core checker not built (oracle/_ref)")
+    from mdp_amd import library
+    from mdp_amd.devarray import DevArray
+    lib, ref = library().cdll, ctypes.CDLL(R2LIB)
+    sigs = {"CellList3D": [P] * 7 + [I] * 3, "Cell2AtomList": [P] * 5 + [I] * 2,
+            "FullNeighborList3DCell": [P] * 9 + [I] * 3, "FullNeighborList3D": [P] * 4 + [I] * 4}
+    g, r = {}, {}
+    for name, sig in sigs.items():
+        for d, l, pre in ((g, lib, "mdp_hip"), (r, ref, "r2_cpu")):
+            f = getattr(l, pre + name)
+            f.argtypes, f.restype = sig, None
+            d[name] = f
+    r["Cell2AtomList"].argtypes = [P] * 4 + [I] * 2
+    rng = np.random.default_rng(seed)
+    L, rc = 10.0, 10.0 / ncell
+    xin = rng.uniform(0, L, (inum, 3))
+    ghosts = []
+    for sh in np.array(np.meshgrid([-1, 0, 1], [-1, 0, 1], [-1, 0, 1])).T.reshape(-1, 3):
+        if not sh.any():
+            continue
+        y = xin + sh * L
+        ghosts.append(y[np.all((y >= -rc) & (y <= L + rc), axis=1)])
+    x = np.ascontiguousarray(np.concatenate([xin] + ghosts))
+    natom = len(x)
+    d = rc / L
+    eta = np.ascontiguousarray(np.concatenate([[-d], np.linspace(0.0, 1.0, ncell + 1), [1.0 + d]]))
+    nc = np.array([ncell + 2] * 3, np.int32)
+    nct = int(np.prod(nc))
+    s2r = np.ascontiguousarray((np.eye(3) / L).ravel(order="F"))
+    rcsq = np.array([rc * rc])
+    alist = np.arange(natom, dtype=np.int32)
+    h = dict(x=x.ravel(), eta=eta, s2r=s2r, nc=nc, rcsq=rcsq, alist=alist)
+    dv = {k: DevArray.from_numpy(np.ascontiguousarray(v), np.asarray(v).dtype) for k, v in h.items()}
+    print(f"cell lists: {inum} atoms + {natom - inum} ghosts, {nct} cells, jnum {jnum}", flush=True)
+    hp = lambda a: a.ctypes.data
+    # CellList3D
+    cl_ref = np.zeros(natom, np.int32)
+    r["CellList3D"](hp(cl_ref), hp(h["x"]), hp(eta), hp(eta), hp(eta), hp(s2r), hp(nc), inum, natom, 3)
+    dcl = DevArray.zeros(natom, np.int32)
+    g["CellList3D"](dcl.ptr, dv["x"].ptr, dv["eta"].ptr, dv["eta"].ptr, dv["eta"].ptr, dv["s2r"].ptr, dv["nc"].ptr,
+                    inum, natom, 3)
+    assert lib.mdp_snap_last_error().decode() == ""
+    assert np.array_equal(dcl.numpy(), cl_ref)
+    # Cell2AtomList
+    c2a, c2s, c2n = np.zeros(natom, np.int32), np.zeros(nct + 1, np.int32), np.zeros(natom, np.int32)
+    r["Cell2AtomList"](hp(c2a), hp(c2s), hp(c2n), hp(cl_ref), natom, nct)
+    dc2a, dc2s, dc2n = DevArray.zeros(natom, np.int32), DevArray.zeros(nct + 1, np.int32), DevArray.zeros(natom, np.int32)
+    g["Cell2AtomList"](dc2a.ptr, dc2s.ptr, dc2n.ptr, dcl.ptr, None, natom, nct)
+    assert lib.mdp_snap_last_error().decode() == ""
+    assert np.array_equal(dc2a.numpy(), c2a) and np.array_equal(dc2s.numpy(), c2s)
+    assert np.array_equal(dc2n.numpy(), c2n)
+    # FullNeighborList3D, cell-list form and all-pairs form
+    for name in ("FullNeighborList3DCell", "FullNeighborList3D"):
+        nl, nn = np.full(jnum * inum, -1, np.int32), np.zeros(inum, np.int32)
+        dnl = DevArray.from_numpy(np.full(jnum * inum, -1, np.int32), np.int32)
+        dnn = DevArray.zeros(inum, np.int32)
+        if name == "FullNeighborList3DCell":
+            r[name](hp(nl), hp(nn), hp(h["x"]), hp(rcsq), hp(alist), hp(cl_ref), hp(c2a), hp(c2s), hp(nc), inum, jnum, 3)
+            g[name](dnl.ptr, dnn.ptr, dv["x"].ptr, dv["rcsq"].ptr, dv["alist"].ptr, dcl.ptr, dc2a.ptr, dc2s.ptr,
+                    dv["nc"].ptr, inum, jnum, 3)
+        else:
+            r[name](hp(nl), hp(nn), hp(h["x"]), hp(rcsq), natom, inum, jnum, 3)
+            g[name](dnl.ptr, dnn.ptr, dv["x"].ptr, dv["rcsq"].ptr, natom, inum, jnum, 3)
+        assert lib.mdp_snap_last_error().decode() == ""
+        assert np.array_equal(dnn.numpy(), nn), name
+        assert np.array_equal(dnl.numpy(), nl), name
+        assert nn.max() <= jnum and (jnum > 50 or nn.max() == jnum)
