@@ -225,6 +225,21 @@ inline void hipComputeDbidrj<double>(double *dblist, double *zlist_r, double *zl
 }
 
 template <typename T>
+void hipAddUarraytot(T *ulisttot_r, T *ulisttot_i, T *ulist_r, T *ulist_i, T *rij, T *wjelem, T *radelem, T rmin0,
+                     T rcutfac, int *idxu_block, int *ilist, int *type, int *pairnum, int *pairnumsum, int *map,
+                     int *tj, int twojmax, int idxu_max, int nelements, int inum, int switch_flag, int chemflag);
+template <>
+inline void hipAddUarraytot<double>(double *ulisttot_r, double *ulisttot_i, double *ulist_r, double *ulist_i,
+                                    double *rij, double *wjelem, double *radelem, double rmin0, double rcutfac,
+                                    int *idxu_block, int *ilist, int *type, int *pairnum, int *pairnumsum, int *map,
+                                    int *tj, int twojmax, int idxu_max, int nelements, int inum, int switch_flag,
+                                    int chemflag) {
+    mdp_hipAddUarraytotRij(ulisttot_r, ulisttot_i, ulist_r, ulist_i, rij, wjelem, radelem, rmin0, rcutfac, idxu_block,
+                           ilist, type, pairnum, pairnumsum, map, tj, twojmax, idxu_max, nelements, inum, switch_flag,
+                           chemflag);
+}
+
+template <typename T>
 void hipCellList3D(int *clist, T *x, T *eta1, T *eta2, T *eta3, T *s2rmap, int *nc, int inum, int natom, int dim);
 template <>
 inline void hipCellList3D<double>(int *clist, double *x, double *eta1, double *eta2, double *eta3, double *s2rmap,

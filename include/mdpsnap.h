@@ -460,6 +460,12 @@ void mdp_hipComputeDbidrj(double *dblist, double *zlist_r, double *zlist_i, doub
                           int *idxb, int *idxu_block, int *idxz_block, int *map, int *ai, int *tj, int twojmax,
                           int idxb_max, int idxu_max, int idxz_max, int nelements, int bnorm_flag, int chemflag,
                           int inum, int ijnum);
+/* replaces hipAddUarraytot in the rij form of AddUarraytot (mdpcore.h:2911-2938; src/C++/Core/cpuSnap.cpp:358-395):
+   U pair-major, Utot atom-major, switching and element weights from rij */
+void mdp_hipAddUarraytotRij(double *ulisttot_r, double *ulisttot_i, double *ulist_r, double *ulist_i, double *rij,
+                            double *wjelem, double *radelem, double rmin0, double rcutfac, int *idxu_block, int *ilist,
+                            int *type, int *pairnum, int *pairnumsum, int *map, int *tj, int twojmax, int idxu_max,
+                            int nelements, int inum, int switch_flag, int chemflag);
 /* replaces hipCellList3D in CellList3D (mdpcore.h:1332-1349; src/C++/Core/cpuNeighborList3D.cpp:334-356) */
 void mdp_hipCellList3D(int *clist, double *x, double *eta1, double *eta2, double *eta3, double *s2rmap, int *nc,
                        int inum, int natom, int dim);

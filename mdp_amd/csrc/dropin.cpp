@@ -934,6 +934,43 @@ void mdp_hipCumsum(int* d_out, int* d_in, int* d_sums, int* d_incr, int length) 
     });
 }
 
+// AddUarraytot backend 3, rij form of the per-pair route (mdpcore.h:2911-2938; src/C++/Core/cpuSnap.cpp:358-395):
+// Utot[(ii ne + jelem) idxu_max + u] += sfac(r) w_j U[k idxu_max + u] over the pairs k of each listed atom
+void mdp_hipAddUarraytotRij(double* ulisttot_r, double* ulisttot_i, double* ulist_r, double* ulist_i, double* rij,
+                            double* wjelem, double* radelem, double rmin0, double rcutfac, int* idxu_block, int* ilist,
+                            int* type, int* pairnum, int* pairnumsum, int* map, int* tj, int twojmax, int idxu_max,
+                            int nelements, int inum, int switch_flag, int chemflag) {
+    (void)idxu_block;
+    std::vector<int> h_il, h_type, h_pn, h_ps, h_tj, h_map;
+    if (!fetch_args([&] {
+            const size_t n = static_cast<size_t>(std::max(inum, 0));
+            h_il = fetch(ilist, n);
+            h_type = fetch(type, static_cast<size_t>(max_of(h_il.data(), static_cast<long>(n)) + 1));
+            h_pn = fetch(pairnum, n);
+            h_ps = fetch(pairnumsum, n);
+            long np = 0;
+            for (size_t ii = 0; ii < n; ++ii) np = std::max<long>(np, static_cast<long>(h_ps[ii]) + h_pn[ii]);
+            h_tj = fetch(tj, static_cast<size_t>(np));
+            h_map = fetch(map, std::max(max_type(h_tj, h_tj.size()), max_type(h_type, h_type.size())) + 1);
+        }))
+        return;
+    guarded([&] {
+        if (inum <= 0) return;
+        if (idxu_max != index_for(twojmax).idxu_max) throw Error("AddUarraytot: idxu_max does not match twojmax");
+        for (int ii = 0; ii < inum; ++ii)
+            if (h_il[ii] < 0 || h_pn[ii] < 0 || h_ps[ii] < 0) throw Error("AddUarraytot: negative list entry");
+        for (int t : h_tj)
+            if (t < 0 || (chemflag && (h_map[t] < 0 || h_map[t] >= nelements)))
+                throw Error("AddUarraytot: pair type or element out of range");
+        for (int t : h_type)
+            if (t < 0) throw Error("AddUarraytot: negative atom type");
+        hipStream_t st = library_stream();
+        launch_add_utot_rij(ulisttot_r, ulisttot_i, ulist_r, ulist_i, rij, wjelem, radelem, rmin0, rcutfac, ilist, type,
+                            pairnum, pairnumsum, map, tj, idxu_max, nelements, inum, switch_flag, chemflag, st);
+        finish(st);
+    });
+}
+
 // CellList3D backend 3 (mdpcore.h:1332-1349; src/C++/Core/cpuNeighborList3D.cpp:334-356): cell of every atom
 void mdp_hipCellList3D(int* clist, double* x, double* eta1, double* eta2, double* eta3, double* s2rmap, int* nc,
                        int inum, int natom, int dim) {

@@ -142,6 +142,15 @@ void r2_cpuComputeDbidrjAtomMajor(double *dblist, double *zlist_r, double *zlist
                      twojmax, idxb_max, idxu_max, idxz_max, nelements, bnorm_flag, chemflag, ijnum);
 }
 
+// the per-pair route's rij form (cpuSnap.cpp:358-395)
+void r2_cpuAddUarraytotRij(double *ulisttot_r, double *ulisttot_i, double *ulist_r, double *ulist_i, double *rij,
+                           double *wjelem, double *radelem, double rmin0, double rcutfac, int *idxu_block, int *ilist,
+                           int *type, int *pairnum, int *pairnumsum, int *map, int *tj, int twojmax, int idxu_max,
+                           int nelements, int inum, int switch_flag, int chemflag) {
+    cpuAddUarraytot(ulisttot_r, ulisttot_i, ulist_r, ulist_i, rij, wjelem, radelem, rmin0, rcutfac, idxu_block, ilist,
+                    type, pairnum, pairnumsum, map, tj, twojmax, idxu_max, nelements, inum, switch_flag, chemflag);
+}
+
 // cell-list neighbour search (cpuNeighborList3D.cpp:334-441, cpuNeighborList2D.cpp:298-308)
 void r2_cpuCellList3D(int *clist, double *x, double *eta1, double *eta2, double *eta3, double *s2rmap, int *nc,
                       int inum, int natom, int dim) {

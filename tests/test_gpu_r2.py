@@ -12,7 +12,8 @@ NeighborPairList -> Cumsum -> NeighborPairs -> ComputeSij -> ZeroUarraytot2 -> A
 SnapTally{Bispectrum, BispectrumDeriv, BispectrumVirial} -- runs the same way on a ghost-extended,
 fixed-stride neighbour list: integer arrays bit for bit, the rest within 1e-10, with quadratic tallies
 and a two-type chemflag case.  The per-atom-weight SnapComputeYi, the (ai, aj, ti, tj) ComputeDbidrj of
-the per-pair route and the full tallies are compared one function at a time.
+the per-pair route with its rij-form AddUarraytot, the cell-list neighbour builders and the full tallies
+are compared one function at a time.
 """
 import ctypes
 import os
@@ -528,3 +529,51 @@ def test_backend3_cell_lists_vs_upstream_core(inum, ncell, jnum, seed):
         assert np.array_equal(dnn.numpy(), nn), name
         assert np.array_equal(dnl.numpy(), nl), name
         assert nn.max() <= jnum and (jnum > 50 or nn.max() == jnum)
+
+
+@pytest.mark.parametrize("J,chem,sw", [(4, 0, 1), (8, 1, 1), (6, 0, 0)])
+def test_backend3_add_utot_rij_vs_upstream_core(oracle, J, chem, sw):
+    """The rij form of AddUarraytot (the per-pair route: U pair-major, Utot atom-major, switching and element
+    weights from rij) against the upstream C++ core's cpuAddUarraytot of cpuSnap.cpp:358-395, accumulating
+    into a nonzero Utot, with a permuted ilist, ragged pair counts (some atoms without pairs) and pairs on
+    both sides of rmin0 and of the cutoff."""
+    if not os.path.exists(R2LIB):
+        pytest.skip("upstream C++ core checker not built (oracle/_ref)")
+    from mdp_amd import library
+    from mdp_amd.devarray import DevArray
+    sig = [P] * 7 + [D, D] + [P] * 7 + [I] * 6
+    r2 = ctypes.CDLL(R2LIB).r2_cpuAddUarraytotRij
+    gpu = library().cdll.mdp_hipAddUarraytotRij
+    for f in (r2, gpu):
+        f.argtypes, f.restype = sig, None
+    s = snapcpp.initsna(oracle, [2, J, RCUTFAC, RFAC0, 0.3, 1, sw, 0, chem, 0, 0], [0.5, 0.45], [1.0, 0.8])
+    ne, iu = s.nelements, s.idxu_max
+    rng = np.random.default_rng(J + chem)
+    inum = 30
+    ilist = rng.permutation(inum).astype(np.int32)
+    atype = rng.integers(1, 3, inum).astype(np.int32)
+    pairnum = rng.integers(0, 13, inum).astype(np.int32)
+    pairnum[:3] = 0
+    pairnumsum = np.concatenate([[0], np.cumsum(pairnum)[:-1]]).astype(np.int32)
+    npair = int(pairnum.sum())
+    d = rng.normal(size=(npair, 3))
+    r = rng.uniform(0.1, 1.15 * s.rcutmax, npair)
+    rij = d / np.linalg.norm(d, axis=1)[:, None] * r[:, None]
+    host = dict(utr=rng.normal(size=inum * ne * iu), uti=rng.normal(size=inum * ne * iu),
+                ur=rng.normal(size=npair * iu), ui=rng.normal(size=npair * iu), rij=rij.ravel(), wj=s.wjelem,
+                rad=s.radelem, idxu_block=s.idxu_block, ilist=ilist, type=atype, pairnum=pairnum,
+                pairnumsum=pairnumsum, map=s.map, tj=rng.integers(1, 3, npair).astype(np.int32))
+    host = {k: np.ascontiguousarray(v) for k, v in host.items()}
+    dev = {k: DevArray.from_numpy(v, v.dtype) for k, v in host.items()}
+    order = ("utr", "uti", "ur", "ui", "rij", "wj", "rad")
+    iorder = ("idxu_block", "ilist", "type", "pairnum", "pairnumsum", "map", "tj")
+    ints = [J, iu, ne, inum, sw, chem]
+    ref = {k: host[k].copy() for k in ("utr", "uti")}
+    hp = lambda k: (ref[k] if k in ref else host[k]).ctypes.data
+    r2(*[hp(k) for k in order], s.rmin0, s.rcutfac, *[hp(k) for k in iorder], *ints)
+    gpu(*[dev[k].ptr for k in order], s.rmin0, s.rcutfac, *[dev[k].ptr for k in iorder], *ints)
+    assert library().cdll.mdp_snap_last_error().decode() == ""
+    print(f"AddUarraytot rij form J={J} chem={chem} switch={sw}: {inum} atoms, {npair} pairs", flush=True)
+    assert not np.array_equal(ref["utr"], host["utr"])
+    assert relerr(dev["utr"].numpy(), ref["utr"]) < TOL
+    assert relerr(dev["uti"].numpy(), ref["uti"]) < TOL
