@@ -394,7 +394,7 @@ void mdp_hipSnapComputeYi(double *ylist_r, double *ylist_i, double *ulisttot_r, 
                           int *idxcg_block, int twojmax, int idxb_max, int idxu_max, int idxz_max, int nelements,
                           int ncoeffall, int bnorm_flag, int inum);
 /* replaces hipSnapComputeYi in the per-atom-weight form of SnapComputeYi (mdpcore.h:3196-3219;
-   src/C++/Core/cpuSnap4.cpp:601-690): betalist[jjb * inum + atom], one element */
+   src/C++/Core/cpuSnap4.cpp:601-690): betalist[(elemblock * idxb_max + jjb) * inum + atom] */
 void mdp_hipSnapComputeYiPerAtom(double *ylist_r, double *ylist_i, double *ulisttot_r, double *ulisttot_i,
                                  double *cglist, double *betalist, int *idxz, int *idxb_block, int *idxu_block,
                                  int *idxcg_block, int twojmax, int idxb_max, int idxu_max, int idxz_max,

@@ -832,7 +832,7 @@ void mdp_hipSnapComputeYi(double* ylist_r, double* ylist_i, double* ulisttot_r, 
 }
 
 // SnapComputeYi backend 3, per-atom form (mdpcore.h:3196-3219; src/C++/Core/cpuSnap4.cpp:601-690): every
-// atom carries its own bispectrum weights, betalist[jjb * inum + atom] (no constant term); one element
+// atom carries its own bispectrum weights, betalist[(elemblock * idxb_max + jjb) * inum + atom] (no constant term)
 void mdp_hipSnapComputeYiPerAtom(double* ylist_r, double* ylist_i, double* ulisttot_r, double* ulisttot_i,
                                  double* cglist, double* betalist, int* idxz, int* idxb_block, int* idxu_block,
                                  int* idxcg_block, int twojmax, int idxb_max, int idxu_max, int idxz_max,
@@ -842,7 +842,7 @@ void mdp_hipSnapComputeYiPerAtom(double* ylist_r, double* ylist_i, double* ulist
     if (!fetch_args([&] { h_cg = fetch(cglist, static_cast<size_t>(index_for(twojmax).idxcg_max)); })) return;
     guarded([&] {
         if (inum <= 0) return;
-        if (nelements != 1) throw Error("SnapComputeYi (per-atom weights): only one element is supported");
+        if (nelements < 1) throw Error("SnapComputeYi (per-atom weights): nelements must be positive");
         hipStream_t st = library_stream();
         const SnapDev& s = tables(twojmax, nullptr, h_cg.data(), nullptr, nullptr, nullptr, 1, st);
         if (idxu_max != s.idxu_max || idxb_max != s.idxb_max) throw Error("table sizes do not match twojmax");
@@ -856,9 +856,44 @@ void mdp_hipSnapComputeYiPerAtom(double* ylist_r, double* ylist_i, double* ulist
         const size_t nu = static_cast<size_t>(inum) * idxu_max;
         MDP_HIP(hipMemsetAsync(ylist_r, 0, sizeof(double) * nu, st));
         MDP_HIP(hipMemsetAsync(ylist_i, 0, sizeof(double) * nu, st));
-        launch_beta_peratom(betalist, d_jjb, d_fac, ntrip, inum, d_bt, st);
-        if (!launch_yi_peratom(s, ulisttot_r, ulisttot_i, d_bt, ylist_r, ylist_i, inum, st))
-            throw Error("SnapComputeYi (per-atom weights): twojmax not compiled");
+        if (nelements == 1) {
+            launch_beta_peratom(betalist, d_jjb, d_fac, ntrip, inum, d_bt, st);
+            if (!launch_yi_peratom(s, ulisttot_r, ulisttot_i, d_bt, ylist_r, ylist_i, inum, st))
+                throw Error("SnapComputeYi (per-atom weights): twojmax not compiled");
+            finish(st);
+            return;
+        }
+        // several elements: combination (e1, e2, e3) of coupling triple (j1, j2, j) reads the weights of element
+        // block (e1, e2, e3) when j >= j1, (e3, e2, e1) when j1 > j >= j2, (e2, e3, e1) when j < j2
+        const int ne = nelements, J = twojmax;
+        std::vector<int> branch;
+        for (int j1 = 0; j1 <= J; ++j1)
+            for (int j2 = 0; j2 <= j1; ++j2)
+                for (int j = j1 - j2; j <= std::min(J, j1 + j2); j += 2) branch.push_back(j >= j1 ? 0 : (j >= j2 ? 1 : 2));
+        if (static_cast<int>(branch.size()) != ntrip) throw Error("coupling triple count mismatch");
+        std::vector<int> idx(static_cast<size_t>(ne) * ne * ne * ntrip);
+        for (int e1 = 0; e1 < ne; ++e1)
+            for (int e2 = 0; e2 < ne; ++e2)
+                for (int e3 = 0; e3 < ne; ++e3)
+                    for (int t = 0; t < ntrip; ++t) {
+                        const int blk = branch[t] == 0 ? (e1 * ne + e2) * ne + e3
+                                        : branch[t] == 1 ? (e3 * ne + e2) * ne + e1
+                                                         : (e2 * ne + e3) * ne + e1;
+                        idx[((static_cast<size_t>(e1) * ne + e2) * ne + e3) * ntrip + t] = blk * idxb_max + jjb[t];
+                    }
+        int* d_idx = put(3, idx.data(), idx.size(), st);
+        const long nu1 = static_cast<long>(inum) * idxu_max;
+        const size_t nall = nu * ne;
+        MDP_HIP(hipMemsetAsync(ylist_r, 0, sizeof(double) * nall, st));
+        MDP_HIP(hipMemsetAsync(ylist_i, 0, sizeof(double) * nall, st));
+        for (int e1 = 0; e1 < ne; ++e1)
+            for (int e2 = 0; e2 < ne; ++e2)
+                for (int e3 = 0; e3 < ne; ++e3) {
+                    const int* di = d_idx + ((static_cast<size_t>(e1) * ne + e2) * ne + e3) * ntrip;
+                    launch_beta_peratom(betalist, di, d_fac, ntrip, inum, d_bt, st);
+                    launch_yi_acc(s, ulisttot_r, ulisttot_i, nu1 * e1, nu1 * e2, nu1 * e3, d_bt, ylist_r, ylist_i,
+                                  nullptr, -1, inum, st, true);
+                }
         finish(st);
     });
 }

@@ -10,8 +10,9 @@
 
 namespace mdp {
 
-// element-block variant: Y[off3] += sum beta Z(U[off1], U[off2]) for atoms of element isel
-template <int J>
+// element-block variant: Y[off3] += sum beta Z(U[off1], U[off2]) for atoms of element isel; PA: one weight
+// per (coupling triple, atom), beta_t[triple*N + atom]
+template <int J, bool PA>
 __global__ __launch_bounds__(64) void k_yi_acc(SnapDev s, const double* __restrict__ Ur, const double* __restrict__ Ui,
                                                long off1, long off2, long off3, const double* __restrict__ beta_t,
                                                double* __restrict__ Yr, double* __restrict__ Yi,
@@ -24,18 +25,23 @@ __global__ __launch_bounds__(64) void k_yi_acc(SnapDev s, const double* __restri
     static_for<0, J + 1>([&](auto JOc) {
         constexpr int JO = decltype(JOc)::value;
         if (jo == JO)
-            y_row<J, JO, true, false, true>(s, row, mb, atom, N, Ur + off1, Ui + off1, Ur + off2, Ui + off2, beta_t,
+            y_row<J, JO, true, false, true, PA>(s, row, mb, atom, N, Ur + off1, Ui + off1, Ur + off2, Ui + off2, beta_t,
                                            Yr + off3, Yi + off3, nullptr);
     });
 }
 
 void launch_yi_acc(const SnapDev& s, const double* Ur, const double* Ui, long off1, long off2, long off3,
-                   const double* beta_t, double* Yr, double* Yi, const int* atom_elem, int isel, int N, hipStream_t st) {
+                   const double* beta_t, double* Yr, double* Yi, const int* atom_elem, int isel, int N, hipStream_t st,
+                   bool per_atom) {
     if (N <= 0) return;
 #define YA_CASE(JV)                                                                                           \
     case JV:                                                                                                  \
-        hipLaunchKernelGGL((k_yi_acc<JV>), dim3((N + 63) / 64, n_half_rows(JV)), dim3(64), 0, st, s, Ur, Ui, off1, \
-                           off2, off3, beta_t, Yr, Yi, atom_elem, isel, N);                                   \
+        if (per_atom)                                                                                         \
+            hipLaunchKernelGGL((k_yi_acc<JV, true>), dim3((N + 63) / 64, n_half_rows(JV)), dim3(64), 0, st, s, Ur, \
+                               Ui, off1, off2, off3, beta_t, Yr, Yi, atom_elem, isel, N);                     \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_yi_acc<JV, false>), dim3((N + 63) / 64, n_half_rows(JV)), dim3(64), 0, st, s, Ur, \
+                               Ui, off1, off2, off3, beta_t, Yr, Yi, atom_elem, isel, N);                     \
         break;
     switch (s.J) {
         YA_CASE(2)

@@ -233,9 +233,9 @@ void launch_add_wself(const SnapDev& s, double* Ur, double* Ui, double wself, co
                       int chemflag, int nelem, int inum, hipStream_t st);
 
 // generic compute_yi accumulate for element blocks: Y[e3] += sum beta_t(e1,e2,e3,ielem) Z(e1,e2) over
-// the atoms whose element (map[type[ii]]) equals isel (all atoms if isel < 0)
+// the atoms whose element (map[type[ii]]) equals isel (all atoms if isel < 0); per_atom: beta_t[triple*N + atom]
 void launch_yi_acc(const SnapDev& s, const double* Ur, const double* Ui, long off1, long off2, long off3,
                    const double* beta_t, double* Yr, double* Yi, const int* atom_elem, int isel, int N,
-                   hipStream_t st);
+                   hipStream_t st, bool per_atom = false);
 
 }  // namespace mdp

@@ -577,3 +577,38 @@ def test_backend3_add_utot_rij_vs_upstream_core(oracle, J, chem, sw):
     assert not np.array_equal(ref["utr"], host["utr"])
     assert relerr(dev["utr"].numpy(), ref["utr"]) < TOL
     assert relerr(dev["uti"].numpy(), ref["uti"]) < TOL
+
+
+@pytest.mark.parametrize("J,bnorm", [(4, 0), (6, 1)])
+def test_backend3_yi_peratom_two_elements_vs_upstream_core(oracle, J, bnorm):
+    """The per-atom-weight SnapComputeYi with two element blocks (each coupling triple and element
+    combination reads its own block of betalist) against the upstream core on seeded U and weights."""
+    if not os.path.exists(R2LIB):
+        pytest.skip("upstream C++ core checker not built (oracle/_ref)")
+    from mdp_amd import library
+    from mdp_amd.devarray import DevArray
+    sig = [P] * 10 + [I] * 7
+    r2 = ctypes.CDLL(R2LIB).r2_cpuSnapComputeYiPerAtom
+    gpu = library().cdll.mdp_hipSnapComputeYiPerAtom
+    for f in (r2, gpu):
+        f.argtypes, f.restype = sig, None
+    ne, N = 2, 77
+    s = snapcpp.initsna(oracle, [2, J, RCUTFAC, RFAC0, 0, 1, 1, 0, 1, bnorm, 0], [0.5, 0.45], [1.0, 0.8])
+    assert s.nelements == ne
+    iu, ib = s.idxu_max, s.idxb_max
+    rng = np.random.default_rng(J + bnorm)
+    host = dict(Ur=rng.normal(size=N * iu * ne), Ui=rng.normal(size=N * iu * ne), cg=s.cglist,
+                beta=rng.normal(size=ne ** 3 * ib * N), idxz=s.idxz, idxb_block=s.idxb_block, idxu_block=s.idxu_block,
+                idxcg_block=s.idxcg_block)
+    host = {k: np.ascontiguousarray(v) for k, v in host.items()}
+    dev = {k: DevArray.from_numpy(v, v.dtype) for k, v in host.items()}
+    dev["Yr"], dev["Yi"] = DevArray.zeros(N * iu * ne), DevArray.zeros(N * iu * ne)
+    Yr, Yi = np.zeros(N * iu * ne), np.zeros(N * iu * ne)
+    order = ("Ur", "Ui", "cg", "beta", "idxz", "idxb_block", "idxu_block", "idxcg_block")
+    ints = [J, ib, iu, s.idxz_max, ne, bnorm, N]
+    r2(Yr.ctypes.data, Yi.ctypes.data, *[host[k].ctypes.data for k in order], *ints)
+    gpu(dev["Yr"].ptr, dev["Yi"].ptr, *[dev[k].ptr for k in order], *ints)
+    assert library().cdll.mdp_snap_last_error().decode() == ""
+    print(f"per-atom Y, two elements, J={J} bnorm={bnorm}: {N} atoms", flush=True)
+    assert relerr(dev["Yr"].numpy(), Yr) < TOL
+    assert relerr(dev["Yi"].numpy(), Yi) < TOL
