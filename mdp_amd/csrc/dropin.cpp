@@ -854,9 +854,9 @@ void mdp_hipSnapComputeYiPerAtom(double* ylist_r, double* ylist_i, double* ulist
         double* d_fac = put(1, fac.data(), fac.size(), st);
         double* d_bt = dev<double>(2, static_cast<size_t>(ntrip) * inum);
         const size_t nu = static_cast<size_t>(inum) * idxu_max;
-        MDP_HIP(hipMemsetAsync(ylist_r, 0, sizeof(double) * nu, st));
-        MDP_HIP(hipMemsetAsync(ylist_i, 0, sizeof(double) * nu, st));
         if (nelements == 1) {
+            MDP_HIP(hipMemsetAsync(ylist_r, 0, sizeof(double) * nu, st));
+            MDP_HIP(hipMemsetAsync(ylist_i, 0, sizeof(double) * nu, st));
             launch_beta_peratom(betalist, d_jjb, d_fac, ntrip, inum, d_bt, st);
             if (!launch_yi_peratom(s, ulisttot_r, ulisttot_i, d_bt, ylist_r, ylist_i, inum, st))
                 throw Error("SnapComputeYi (per-atom weights): twojmax not compiled");
