@@ -86,3 +86,59 @@ def sampled_eval(lib, sna, x, box, sample, rc):
 def init(lib, J, coeff, rcutfac=4.67637, rfac0=0.99363, rmin0=0.0):
     param = [1, J, rcutfac, rfac0, rmin0, 1, 1, 0, 0, 0, 0]
     return snapcpp.initsna(lib, param, [0.5], [1.0], coeff)
+
+
+def sampled_surrogate_forces(lib, sna, x, box, sample, rc, forward):
+    """float64 descriptor-chain-rule surrogate forces on the atoms in ``sample`` of a large periodic
+    system, from the oracle's descriptors (``sna``: the descriptor set-up, e.g. ``init(lib, 4, None)``).
+
+    ``forward(B) -> (E_atoms, dE/dB)`` is the per-atom model on the (n, K) descriptors.  The force on a
+    sampled atom s is F_s = sum_{pairs (s,j)} g_s . dB_s/dr_sj - sum_{pairs (a,s)} g_a . dB_a/dr_as, so it
+    needs B (hence every pair) of s and of each of its neighbours a: the local problem holds those
+    centres and all their pairs.  Returns (F (S,3), E of the sampled atoms (S,)).
+    """
+    x = np.ascontiguousarray(x, np.float64)
+    box = np.asarray(box, np.float64)
+    xw = np.mod(x, box)
+    tree = cKDTree(xw, boxsize=box)
+    sample = np.asarray(sample, np.int64)
+    lists = tree.query_ball_point(xw[sample], r=rc * (1.0 + 1e-9))
+    nbr = set()
+    for l in lists:
+        nbr.update(l)
+    centers = np.array(list(sample) + sorted(nbr - set(sample.tolist())), np.int64)
+    ai, rij, ajg = _pairs_for(xw, box, centers, rc, tree)
+    n, P = len(centers), len(ai)
+    p = dict(rij=np.ascontiguousarray(rij.ravel()), ai=ai.astype(np.int32), aj=np.zeros(P, np.int32),
+             ti=np.ones(P, np.int32), tj=np.ones(P, np.int32), atomtype=np.ones(n, np.int32),
+             ilist=np.arange(n, dtype=np.int32), n=n, ijnum=P)
+    iu, iz, ib = sna.idxu_max, sna.idxz_max, sna.idxb_max
+    ur, ui = np.zeros(iu * P), np.zeros(iu * P)
+    dur, dui = np.zeros(3 * iu * P), np.zeros(3 * iu * P)
+    lib.cpuComputeUij(ur, ui, dur, dui, sna.rootpqarray, p["rij"], sna.wjelem, sna.radelem, sna.rmin0, sna.rfac0,
+                      sna.rcutfac, sna.idxu_block, p["ti"], p["tj"], sna.twojmax, iu, P, sna.switchflag)
+    Ur, Ui = np.zeros(n * iu), np.zeros(n * iu)
+    lib.cpuZeroUarraytot2(Ur, Ui, sna.wself, sna.idxu_block, p["atomtype"], sna.map, p["ai"], sna.wselfallflag,
+                          0, iu, 1, sna.twojmax, n)
+    lib.cpuAddUarraytot(Ur, Ui, ur, ui, sna.map, p["ai"], p["tj"], iu, n, P, 0)
+    zr, zi = np.zeros(iz * n), np.zeros(iz * n)
+    lib.cpuComputeZi2(zr, zi, Ur, Ui, sna.cglist, sna.idxz, sna.idxu_block, sna.idxcg_block, sna.twojmax, iu, iz,
+                      1, sna.bnormflag, n)
+    blist = np.zeros(ib * n)
+    lib.cpuComputeBi2(blist, zr, zi, Ur, Ui, sna.bzero, p["ilist"], p["atomtype"], sna.map, sna.idxb,
+                      sna.idxu_block, sna.idxz_block, sna.twojmax, ib, iu, iz, 1, sna.bzeroflag, sna.wselfallflag, 0, n)
+    db = np.zeros(ib * 3 * P)
+    lib.cpuComputeDbidrj(db, zr, zi, dur, dui, sna.idxb, sna.idxu_block, sna.idxz_block, sna.map, p["ai"], p["tj"],
+                         sna.twojmax, ib, iu, iz, 1, sna.bnormflag, 0, n, P)
+    B = blist.reshape(ib, n).T
+    e, g = forward(B)
+    s = np.einsum("pk,kcp->pc", g[ai], db.reshape(ib, 3, P))
+    S = len(sample)
+    F = np.zeros((S, 3))
+    own = ai < S
+    np.add.at(F, ai[own], s[own])
+    pos = {int(gi): k for k, gi in enumerate(sample)}
+    for q in np.nonzero(np.isin(ajg, sample))[0]:
+        F[pos[int(ajg[q])]] -= s[q]
+    return F, e[:S]
+
