@@ -341,4 +341,33 @@ inline void hipSnapTallyVirialFull<double>(double *vatom, double *fij, double *r
     mdp_hipSnapTallyVirialFull(vatom, fij, rij, ai, aj, inum, ijnum);
 }
 
+/* reference pair potential (part 6): the HIP counterparts of the generated pair gradient and the
+ * full-list pair tallies of the upstream empirical-potential route */
+template <typename T>
+void hipPairaGradient(T *u, T *u_xij, T *xij, T *qi, T *qj, int *ti, int *tj, int *ai, int *aj, T *mu, T *eta,
+                      int *kappa, int dim, int ncq, int nmu, int neta, int nkappa, int ng, int potnum);
+template <>
+inline void hipPairaGradient<double>(double *u, double *u_xij, double *xij, double *qi, double *qj, int *ti, int *tj,
+                                     int *ai, int *aj, double *mu, double *eta, int *kappa, int dim, int ncq, int nmu,
+                                     int neta, int nkappa, int ng, int potnum) {
+    mdp_hipPairaGradient(u, u_xij, xij, qi, qj, ti, tj, ai, aj, mu, eta, kappa, dim, ncq, nmu, neta, nkappa, ng,
+                         potnum);
+}
+
+template <typename T>
+void hipFullNeighPairDecomposition(T *e, T *f, T *eij, T *fij, int *ai, int ijnum, int dim);
+template <>
+inline void hipFullNeighPairDecomposition<double>(double *e, double *f, double *eij, double *fij, int *ai, int ijnum,
+                                                  int dim) {
+    mdp_hipFullNeighPairDecomposition(e, f, eij, fij, ai, ijnum, dim);
+}
+
+template <typename T>
+void hipVirialPairTally(T *v, T *fij, T *rij, T factor, int *ai, int dim, int inum, int ijnum);
+template <>
+inline void hipVirialPairTally<double>(double *v, double *fij, double *rij, double factor, int *ai, int dim, int inum,
+                                       int ijnum) {
+    mdp_hipVirialPairTally(v, fij, rij, factor, ai, dim, inum, ijnum);
+}
+
 #endif

@@ -20,6 +20,8 @@
  * Part 5 is the backend-3 (HIP) branch of the upstream C++ core dispatch (src/C++/Common/mdpcore.h) for
  * the SNAP force path and the descriptor path of the upstream fitting driver: device arrays in and out;
  * include/mdphipcore.h wraps it as the hip* templates.
+ * Part 6 is the reference pair potential (ZBL) the SNAP application adds to SNAP, as backend-3 entry
+ * points of the upstream empirical-potential route.
  *
  * Errors: the drop-in functions return void like upstream; a failure leaves a message readable
  * with mdp_snap_last_error() (empty string when the last call succeeded).  Part 2 returns 0 on
@@ -221,6 +223,13 @@ int mdp_md_set_nvt(mdp_md *md, double t_start, double t_stop, double t_period, d
                    int nc_tchain, long beginstep, long endstep, long currentstep);
 /* back to NVE */
 int mdp_md_clear_nvt(mdp_md *md);
+/* Adds the ZBL reference pair potential of the SNAP Ta application (simulating/snap/ZBLpotential.jl with
+ * mu = [cut_inner, cut_outer, z], snapTA06A.jl:62-66: 4, 4.8, 73) to SNAP from the next force evaluation
+ * on, with the upstream full-neighbour-list conventions (src/C++/Potentials/cpuPotentials.cpp:1587-1649):
+ * e_i += sum_j u(r_ij)/2, F_i += sum_j du/dxij, per-atom virial -1/2 sum_j xij (x) du/dxij, pairs with
+ * r_ij^2 <= cut_outer^2.  The neighbour list is then built at max(SNAP cutoff, cut_outer) + skin.
+ * Needs 0 < cut_inner < cut_outer and z > 0. */
+int mdp_md_set_zbl(mdp_md *md, double cut_inner, double cut_outer, double z);
 /* Initial velocities as the upstream app creates them (readdata.cpp:609-640; host code): per-atom
  * uniform (dist_flag 0) or normal (1) draws of cpuVelocityCreate in loop mode 2 ("geom", the mode of the
  * upstream examples; the other modes are rejected), scaled by 1/sqrt(mass), then zero total momentum and
@@ -272,6 +281,14 @@ typedef struct {
  * |x_j + s*L - x_i|^2 <= ((r_ti + r_tj) rcutfac)^2, ordered by j then (sz, sy, sx). */
 int mdp_fit_create(mdp_fit **h, const mdp_fit_params *p, int nconfig, const int *natoms, const double *boxes,
                    const double *x, const int *types);
+/* The same for periodic cells that need not be orthogonal: cells[9c..9c+8] holds the cell vectors a, b, c
+ * of configuration c in the form a = (Lx, 0, 0), b = (bx, Ly, 0), c = (cx, cy, Lz) (the form
+ * src/Julia/Preprocessing/rotatelattice.jl:1-21 rotates every lattice into), tilts smaller than the box
+ * lengths.  Pairs of atom i are all images x_j + s1 a + s2 b + s3 c within the cutoff, ordered by j then
+ * (s3, s2, s1); orthogonal cells give exactly the pairs of mdp_fit_create.  Positions must be finite and
+ * within [-2L, 3L) per component when a cell is tilted. */
+int mdp_fit_create_cells(mdp_fit **h, const mdp_fit_params *p, int nconfig, const int *natoms, const double *cells,
+                         const double *x, const int *types);
 int mdp_fit_destroy(mdp_fit *h);
 /* Replace all positions (same configurations and atom counts). */
 int mdp_fit_set_positions(mdp_fit *h, const double *x);
@@ -513,6 +530,27 @@ void mdp_hipSnapTallyEnergyFull2(double *eatom, double *bispectrum, double *coef
 void mdp_hipSnapTallyForceFull(double *fatom, double *fij, int *ai, int *aj, int *alist, int ijnum);
 /* replaces hipSnapTallyVirialFull in SnapTallyVirialFull (mdpcore.h:3712-3730) */
 void mdp_hipSnapTallyVirialFull(double *vatom, double *fij, double *rij, int *ai, int *aj, int inum, int ijnum);
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Part 6: reference pair potential (ZBL) of the SNAP application, backend-3 entry points          */
+/* ---------------------------------------------------------------------------------------------- */
+/* The empirical pair route of the upstream application (src/C++/Potentials/cpuPotentials.cpp:
+ * 1587-1649 for a full neighbour list, then the force sign flip of :1837), device arrays in and out.
+ * replaces gpuPairaGradient (src/C++/Common/mdpgpucore.h:373; the generated
+ * src/C++/Potentials/cpuPairaGradient.cpp:1-52 on the CPU): u[p] and u_xij[3p..3p+2] = du/dxij of each
+ * pair vector xij = x_j - x_i.  potnum must be 1, the ZBL potential of simulating/snap/ZBLpotential.jl,
+ * with mu = [cut_inner, cut_outer, z] (snapTA06A.jl:62-66); dim must be 3; qi, qj, ti, tj, ai, aj, eta
+ * and kappa are not read (the potential does not use them). */
+void mdp_hipPairaGradient(double *u, double *u_xij, double *xij, double *qi, double *qj, int *ti, int *tj, int *ai,
+                          int *aj, double *mu, double *eta, int *kappa, int dim, int ncq, int nmu, int neta,
+                          int nkappa, int ng, int potnum);
+/* replaces cpuFullNeighPairDecomposition (src/C++/Core/cpuEnergyForceTally.cpp:216-227) on the device:
+ * e[ai] += eij/2, f[ai*dim + d] -= fij[p*dim + d] (accumulates) */
+void mdp_hipFullNeighPairDecomposition(double *e, double *f, double *eij, double *fij, int *ai, int ijnum, int dim);
+/* replaces cpuVirialPairTally (src/C++/Core/cpuVirialTally.cpp:34-57) on the device:
+ * v[k*inum + ai] += factor * (rij (x) fij)_k, k = xx, yy, zz, xy, xz, yz (accumulates) */
+void mdp_hipVirialPairTally(double *v, double *fij, double *rij, double factor, int *ai, int dim, int inum,
+                            int ijnum);
 
 #ifdef __cplusplus
 }

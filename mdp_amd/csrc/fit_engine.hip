@@ -37,8 +37,8 @@ struct mdp_fit {
     hipStream_t st = nullptr;
     DevTables tab;
     std::vector<int> a0_h;   // [C+1]
-    std::vector<double> boxes_h;
-    DevBuf x, type, cfg_of, cfg_a0, cfg_L, cfg_p0, rc2, pcnt, poff, scan_tmp, flags, counter;
+    std::vector<double> boxes_h, tilts_h;   // tilts_h empty: every cell orthogonal
+    DevBuf x, type, cfg_of, cfg_a0, cfg_L, cfg_T, cfg_p0, rc2, pcnt, poff, scan_tmp, flags, counter;
     DevBuf pi, pj, pti, ptj, rij, Ur, Ui, zt, blist, dblist, bzero, t_lo, t_b, t_z, t_w, zorder, t_wtab, t_hlo, t_hb, t_hz,
         t_hw;
     int dbdr_variant = 0;
@@ -61,8 +61,10 @@ struct mdp_fit {
         if (st) (void)hipStreamDestroy(st);
     }
 
+    // cells (may be null): per configuration a, b, c row vectors, a = (Lx, 0, 0), b = (bx, Ly, 0),
+    // c = (cx, cy, Lz); boxes then unused
     void init(const mdp_fit_params* p, int nconfig, const int* natoms, const double* boxes, const double* x0,
-              const int* types) {
+              const int* types, const double* cells = nullptr) {
         if (!p) throw Error("mdp_fit_create: null parameters");
         if (!templated_j(p->twojmax)) throw Error("mdp_fit_create: twojmax must be 2, 4, 6 or 8");
         if (nconfig <= 0) throw Error("mdp_fit_create: nconfig must be positive");
@@ -73,6 +75,31 @@ struct mdp_fit {
         bnorm = p->bnormflag;
         bzeroflag = p->bzeroflag;
         a0_h.assign(C + 1, 0);
+        std::vector<double> bx_h;
+        if (cells) {
+            bx_h.resize(3 * C);
+            tilts_h.resize(3 * C);
+            bool any = false;
+            for (int c = 0; c < C; ++c) {
+                const double* m = cells + 9 * static_cast<size_t>(c);
+                for (int k = 0; k < 9; ++k)
+                    if (!std::isfinite(m[k])) throw Error("mdp_fit_create_cells: non-finite cell vector");
+                if (m[1] != 0.0 || m[2] != 0.0 || m[5] != 0.0)
+                    throw Error("mdp_fit_create_cells: cells must have a along x and b in the xy plane "
+                                "(a = (Lx,0,0), b = (bx,Ly,0), c = (cx,cy,Lz))");
+                bx_h[3 * c] = m[0];
+                bx_h[3 * c + 1] = m[4];
+                bx_h[3 * c + 2] = m[8];
+                tilts_h[3 * c] = m[3];
+                tilts_h[3 * c + 1] = m[6];
+                tilts_h[3 * c + 2] = m[7];
+                if (!(std::fabs(m[3]) < m[0]) || !(std::fabs(m[6]) < m[0]) || !(std::fabs(m[7]) < m[4]))
+                    throw Error("mdp_fit_create_cells: tilts must be smaller than the box lengths");
+                any = any || m[3] != 0.0 || m[6] != 0.0 || m[7] != 0.0;
+            }
+            if (!any) tilts_h.clear();
+            boxes = bx_h.data();
+        }
         for (int c = 0; c < C; ++c) {
             if (natoms[c] <= 0) throw Error("mdp_fit_create: every configuration needs at least one atom");
             for (int d = 0; d < 3; ++d)
@@ -126,6 +153,7 @@ struct mdp_fit {
         put(cfg_of, cof.data(), sizeof(int) * n);
         put(cfg_a0, a0_h.data(), sizeof(int) * (C + 1));
         put(cfg_L, boxes, sizeof(double) * 3 * C);
+        if (!tilts_h.empty()) put(cfg_T, tilts_h.data(), sizeof(double) * 3 * C);
         put(rc2, rc2h.data(), sizeof(double) * rc2h.size());
         put(bzero, bz.data(), sizeof(double) * bz.size());
         put(t_lo, lo.data(), sizeof(int) * lo.size());
@@ -189,8 +217,10 @@ struct mdp_fit {
             for (long g = a0_h[c]; g < a0_h[c + 1]; ++g)
                 for (int d = 0; d < 3; ++d) {
                     const double v = xh[3 * g + d], L = boxes_h[3 * c + d];
+                    // a tilted cell reaches up to one more box length along x and y
+                    const double lo = tilts_h.empty() ? -L : -2.0 * L, hi = tilts_h.empty() ? 2.0 * L : 3.0 * L;
                     if (!std::isfinite(v)) throw Error(std::string(where) + ": non-finite position");
-                    if (v < -L || v >= 2.0 * L) throw Error(std::string(where) + ": position outside [-L, 2L)");
+                    if (v < lo || v >= hi) throw Error(std::string(where) + ": position outside the box bounds");
                 }
     }
 
@@ -225,12 +255,12 @@ struct mdp_fit {
             case 0:
                 launch_fill_zero_int(flags.as<int>(), 4, st);
                 launch_fit_pairs(false, x.as<double>(), type.as<int>(), cfg_of.as<int>(), cfg_a0.as<int>(),
-                                 cfg_L.as<double>(), rc2.as<double>(), ntypes, N, pcnt.as<int>(), nullptr, 0, nullptr,
+                                 cfg_L.as<double>(), tilts_h.empty() ? nullptr : cfg_T.as<double>(), rc2.as<double>(), ntypes, N, pcnt.as<int>(), nullptr, 0, nullptr,
                                  nullptr, nullptr, nullptr, nullptr, flags.as<int>(), st);
                 exclusive_scan(pcnt.as<int>(), poff.as<int>(), N, scan_tmp.as<int>(), st);
                 launch_fit_cfg_p0(poff.as<int>(), cfg_a0.as<int>(), cfg_p0.as<int>(), C, st);
                 launch_fit_pairs(true, x.as<double>(), type.as<int>(), cfg_of.as<int>(), cfg_a0.as<int>(),
-                                 cfg_L.as<double>(), rc2.as<double>(), ntypes, N, pcnt.as<int>(), poff.as<int>(), cap,
+                                 cfg_L.as<double>(), tilts_h.empty() ? nullptr : cfg_T.as<double>(), rc2.as<double>(), ntypes, N, pcnt.as<int>(), poff.as<int>(), cap,
                                  rij.as<double>(), pi.as<int>(), pj.as<int>(), pti.as<int>(), ptj.as<int>(),
                                  flags.as<int>(), st);
                 launch_fit_guard(flags.as<int>(), poff.as<int>(), N, st);
@@ -259,7 +289,7 @@ struct mdp_fit {
 
     long count_pairs() {
         launch_fit_pairs(false, x.as<double>(), type.as<int>(), cfg_of.as<int>(), cfg_a0.as<int>(), cfg_L.as<double>(),
-                         rc2.as<double>(), ntypes, N, pcnt.as<int>(), nullptr, 0, nullptr, nullptr, nullptr, nullptr,
+                         tilts_h.empty() ? nullptr : cfg_T.as<double>(), rc2.as<double>(), ntypes, N, pcnt.as<int>(), nullptr, 0, nullptr, nullptr, nullptr, nullptr,
                          nullptr, flags.as<int>(), st);
         exclusive_scan(pcnt.as<int>(), poff.as<int>(), N, scan_tmp.as<int>(), st);
         int tot = 0;
@@ -504,6 +534,18 @@ int mdp_fit_create(mdp_fit** h, const mdp_fit_params* p, int nconfig, const int*
         *h = nullptr;
         auto e = std::make_unique<mdp_fit>();
         e->init(p, nconfig, natoms, boxes, x, types);
+        *h = e.release();
+    });
+}
+
+int mdp_fit_create_cells(mdp_fit** h, const mdp_fit_params* p, int nconfig, const int* natoms, const double* cells,
+                         const double* x, const int* types) {
+    return api([&] {
+        if (!h) throw Error("mdp_fit_create_cells: null handle pointer");
+        if (!cells) throw Error("mdp_fit_create_cells: null cells");
+        *h = nullptr;
+        auto e = std::make_unique<mdp_fit>();
+        e->init(p, nconfig, natoms, nullptr, x, types, cells);
         *h = e.release();
     });
 }

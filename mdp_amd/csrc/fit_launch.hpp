@@ -1,6 +1,7 @@
 // Launchers of the batched fitting-descriptor kernels (k_fit.hip).  All pointers are device pointers.
 // The batch concatenates the atoms of C configurations (config c owns atoms cfg_a0[c]..cfg_a0[c+1]-1,
-// orthogonal periodic box cfg_L[3c..3c+2]); pairs are one CSR list over all atoms.
+// periodic cell a = (Lx, 0, 0), b = (bx, Ly, 0), c = (cx, cy, Lz) with cfg_L[3c..3c+2] = (Lx, Ly, Lz) and
+// cfg_T[3c..3c+2] = (bx, cx, cy), cfg_T null for orthogonal boxes); pairs are one CSR list over all atoms.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -15,8 +16,9 @@ namespace mdp {
 // entries); when poff[N] exceeds cap it leaves the outputs unwritten, sets bit 0 of flags[0] and
 // stores poff[N] in flags[1].
 void launch_fit_pairs(bool fill, const double* x, const int* type, const int* cfg_of, const int* cfg_a0,
-                      const double* cfg_L, const double* rc2, int ntypes, int N, int* pcnt, const int* poff, long cap,
-                      double* rij, int* pi, int* pj, int* pti, int* ptj, int* flags, hipStream_t st);
+                      const double* cfg_L, const double* cfg_T, const double* rc2, int ntypes, int N, int* pcnt,
+                      const int* poff, long cap, double* rij, int* pi, int* pj, int* pti, int* ptj, int* flags,
+                      hipStream_t st);
 // after an overflow flagged by the fill pass: poff[0..N] = 0 (no pairs for the remaining stages)
 void launch_fit_guard(const int* flags, int* poff, int N, hipStream_t st);
 // cfg_p0[c] = poff[cfg_a0[c]] for c = 0..C (first pair of each configuration, total at C)

@@ -89,8 +89,9 @@ __device__ __forceinline__ double dist2(double dx, double dy, double dz) {
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_fit_pairs(const double* __restrict__ x, const int* __restrict__ type,
                                                    const int* __restrict__ cfg_of, const int* __restrict__ cfg_a0,
-                                                   const double* __restrict__ cfg_L, const double* __restrict__ rc2,
-                                                   int ntypes, int N, int* __restrict__ pcnt,
+                                                   const double* __restrict__ cfg_L, const double* __restrict__ cfg_T,
+                                                   const double* __restrict__ rc2, int ntypes, int N,
+                                                   int* __restrict__ pcnt,
                                                    const int* __restrict__ poff, long cap, double* __restrict__ rij,
                                                    int* __restrict__ pi, int* __restrict__ pj,
                                                    int* __restrict__ pti, int* __restrict__ ptj,
@@ -100,6 +101,10 @@ __global__ __launch_bounds__(256) void k_fit_pairs(const double* __restrict__ x,
     const int c = cfg_of[g];
     const int j0 = cfg_a0[c], j1 = cfg_a0[c + 1];
     const double Lx = cfg_L[3 * c], Ly = cfg_L[3 * c + 1], Lz = cfg_L[3 * c + 2];
+    // tilt of a triclinic cell a = (Lx, 0, 0), b = (bx, Ly, 0), c = (cx, cy, Lz); all zero when orthogonal
+    const double bx = cfg_T ? cfg_T[3 * c] : 0.0, cx = cfg_T ? cfg_T[3 * c + 1] : 0.0,
+                 cy = cfg_T ? cfg_T[3 * c + 2] : 0.0;
+    const bool tri = bx != 0.0 || cx != 0.0 || cy != 0.0;
     const double xi = x[3 * g], yi = x[3 * g + 1], zi = x[3 * g + 2];
     const int ti = type[g];
     long q = 0;
@@ -126,7 +131,7 @@ __global__ __launch_bounds__(256) void k_fit_pairs(const double* __restrict__ x,
     // single-image test below evaluates exactly the r_ij and |r_ij|^2 the full enumeration would for
     // that image: same pairs, same order, same bits.
     const double wide2 = 4.0 * rcmax2 * (1.0 + 1e-9);
-    const bool single = wide2 < Lx * Lx && wide2 < Ly * Ly && wide2 < Lz * Lz;
+    const bool single = !tri && wide2 < Lx * Lx && wide2 < Ly * Ly && wide2 < Lz * Lz;
     int cnt = 0;
     auto emit = [&](int j, int tj, double dx, double dy, double dz) __attribute__((always_inline)) {
         if constexpr (FILL) {
@@ -173,7 +178,33 @@ __global__ __launch_bounds__(256) void k_fit_pairs(const double* __restrict__ x,
         if (rconst) scan(ja, jb, std::true_type{});
         else scan(ja, jb, std::false_type{});
     };
-    if (single) {
+    if (tri) {
+        // every image x_j + s1 a + s2 b + s3 c within the cutoff, ordered by j then (s3, s2, s1): the z
+        // component fixes the s3 range, then y (given s3) the s2 range, then x (given s2, s3) the s1
+        // range, each from the real-number bound rounded outwards (the exact test decides)
+        for (int j = j0; j < j1; ++j) {
+            const double xj = x[3 * j], yj = x[3 * j + 1], zj = x[3 * j + 2];
+            const int tj = type[j];
+            const double r2 = rc2[ti * (ntypes + 1) + tj];
+            const double rc = sqrt(r2);
+            const int s30 = static_cast<int>(floor((zi - zj - rc) * iLz)), s31 = static_cast<int>(ceil((zi - zj + rc) * iLz));
+            for (int s3 = s30; s3 <= s31; ++s3) {
+                const double dz = image_delta(zj, s3, Lz, zi);
+                const double yo = yj + s3 * cy, xo3 = s3 * cx;
+                const int s20 = static_cast<int>(floor((yi - yo - rc) * iLy)), s21 = static_cast<int>(ceil((yi - yo + rc) * iLy));
+                for (int s2 = s20; s2 <= s21; ++s2) {
+                    const double dy = image_delta(yo, s2, Ly, yi);
+                    const double xo = xj + (s2 * bx + xo3);
+                    const int s10 = static_cast<int>(floor((xi - xo - rc) * iLx)), s11 = static_cast<int>(ceil((xi - xo + rc) * iLx));
+                    for (int s1 = s10; s1 <= s11; ++s1) {
+                        if (j == g && s1 == 0 && s2 == 0 && s3 == 0) continue;
+                        const double dx = image_delta(xo, s1, Lx, xi);
+                        if (dist2(dx, dy, dz) <= r2) emit(j, tj, dx, dy, dz);
+                    }
+                }
+            }
+        }
+    } else if (single) {
         const int cu = __builtin_amdgcn_readfirstlane(c);
         if (__all(c == cu))   // the wave's atoms share one configuration: j, x_j and t_j are wave-uniform
             scan_rc(__builtin_amdgcn_readfirstlane(j0), __builtin_amdgcn_readfirstlane(j1));
@@ -832,16 +863,17 @@ int fit_term_table_halves(int J, int bnorm, std::vector<int>& lo, std::vector<in
 // ---- launchers ----------------------------------------------------------------------------------
 
 void launch_fit_pairs(bool fill, const double* x, const int* type, const int* cfg_of, const int* cfg_a0,
-                      const double* cfg_L, const double* rc2, int ntypes, int N, int* pcnt, const int* poff, long cap,
-                      double* rij, int* pi, int* pj, int* pti, int* ptj, int* flags, hipStream_t st) {
+                      const double* cfg_L, const double* cfg_T, const double* rc2, int ntypes, int N, int* pcnt,
+                      const int* poff, long cap, double* rij, int* pi, int* pj, int* pti, int* ptj, int* flags,
+                      hipStream_t st) {
     if (N <= 0) return;
     const dim3 grid((N + 255) / 256), block(256);
     if (fill)
-        hipLaunchKernelGGL((k_fit_pairs<true>), grid, block, 0, st, x, type, cfg_of, cfg_a0, cfg_L, rc2, ntypes, N, pcnt,
-                           poff, cap, rij, pi, pj, pti, ptj, flags);
-    else
-        hipLaunchKernelGGL((k_fit_pairs<false>), grid, block, 0, st, x, type, cfg_of, cfg_a0, cfg_L, rc2, ntypes, N,
+        hipLaunchKernelGGL((k_fit_pairs<true>), grid, block, 0, st, x, type, cfg_of, cfg_a0, cfg_L, cfg_T, rc2, ntypes, N,
                            pcnt, poff, cap, rij, pi, pj, pti, ptj, flags);
+    else
+        hipLaunchKernelGGL((k_fit_pairs<false>), grid, block, 0, st, x, type, cfg_of, cfg_a0, cfg_L, cfg_T, rc2, ntypes,
+                           N, pcnt, poff, cap, rij, pi, pj, pti, ptj, flags);
 }
 
 void launch_fit_guard(const int* flags, int* poff, int N, hipStream_t st) {

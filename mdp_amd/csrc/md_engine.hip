@@ -28,6 +28,7 @@
 #include "mlp_launch.hpp"
 #include "runtime.hpp"
 #include "snap_device.hpp"
+#include "zbl.hpp"
 
 using namespace mdp;
 
@@ -40,10 +41,10 @@ constexpr double FTM2V = 1.0 / 1.0364269e-4;   // metal units, eV/A/amu -> A/ps^
 constexpr double MVV2E = 1.0364269e-4;
 constexpr double BOLTZ = 8.617343e-5;                // metal units, eV/K
 
-enum Timer { T_VV1, T_REBUILD, T_PAIRS, T_UI, T_YI, T_DEDR, T_TALLY, T_SUR, T_AUDIT, T_COUNT };
+enum Timer { T_VV1, T_REBUILD, T_PAIRS, T_UI, T_YI, T_DEDR, T_TALLY, T_SUR, T_AUDIT, T_PAIRPOT, T_COUNT };
 const char* const kTimerNames[T_COUNT] = {"integrate", "neighbor_build", "pair_list", "compute_ui",
                                            "compute_yi", "compute_dedr", "tally_integrate", "surrogate_force",
-                                           "audit"};
+                                           "audit", "pair_potential"};
 
 __host__ __device__ inline int shift_code(int sx, int sy, int sz) {
     const int w = 2 * SHK + 1;
@@ -512,7 +513,8 @@ __global__ __launch_bounds__(256) void k_tally(const double* __restrict__ dedr, 
                                                double* __restrict__ vir, double* __restrict__ eatom,
                                                double* __restrict__ v, const int* __restrict__ type,
                                                const double* __restrict__ massinv_t, double dtf, int integrate, int N,
-                                               unsigned long long* __restrict__ pair_sum) {
+                                               unsigned long long* __restrict__ pair_sum,
+                                               const double* __restrict__ zacc) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = t / PSUB, sub = t % PSUB;
     const bool live = i < N;
@@ -551,12 +553,18 @@ __global__ __launch_bounds__(256) void k_tally(const double* __restrict__ dedr, 
 #pragma unroll
         for (int c = 0; c < 10; ++c) a[c] += __shfl_xor(a[c], w);
     if (!live || sub != 0) return;
+    double ez = 0.0;
+    if (zacc) {   // reference pair potential of the same atom (k_zbl_md)
+#pragma unroll
+        for (int c = 0; c < 9; ++c) a[c] += zacc[10 * static_cast<size_t>(i) + c];
+        ez = zacc[10 * static_cast<size_t>(i) + 9];
+    }
     f[3 * i] = a[0];
     f[3 * i + 1] = a[1];
     f[3 * i + 2] = a[2];
 #pragma unroll
     for (int c = 0; c < 6; ++c) vir[6 * i + c] = a[3 + c];
-    eatom[i] = eext ? eext[i] : e0 + (2.0 / 3.0) * a[9];
+    eatom[i] = (eext ? eext[i] : e0 + (2.0 / 3.0) * a[9]) + ez;
     if (integrate) {
         const double dtfm = dtf * massinv_t[type[i]];
         v[3 * i] += dtfm * a[0];
@@ -574,6 +582,54 @@ __global__ void k_vv2(double* __restrict__ v, const double* __restrict__ f, cons
     v[3 * i] += dtfm * f[3 * i];
     v[3 * i + 1] += dtfm * f[3 * i + 1];
     v[3 * i + 2] += dtfm * f[3 * i + 2];
+}
+
+// Reference pair potential (ZBL) of every atom over its full neighbour list, with the upstream
+// full-list conventions (cpuFullNeighPairList's d^2 <= rc^2 test, src/C++/Core/cpuNeighCount.cpp:60-81;
+// cpuFullNeighPairDecomposition and cpuVirialPairTally with factor -1/2, cpuPotentials.cpp:1587-1649 and
+// the sign flip of :1837): e_i = sum_j u(r_ij)/2, F_i = sum_j du/dxij, W_i = -1/2 sum_j xij (x) du/dxij,
+// xij = x_j - x_i.  Each atom owns its sums, so there are no atomics; 16 lanes per atom take every 16th
+// entry and combine by a butterfly.  zacc[10 i + c]: force x, y, z, virial xx, yy, zz, xy, xz, yz, energy.
+__global__ __launch_bounds__(256) void k_zbl_md(const double* __restrict__ x, const int* __restrict__ nb_cnt,
+                                                const int* __restrict__ nb_j, const int* __restrict__ nb_s,
+                                                double Lx, double Ly, double Lz, int N, ZblParams zp,
+                                                double* __restrict__ zacc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / PSUB, sub = t % PSUB;
+    const bool live = i < N;
+    double a[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (live) {
+        const int n = nb_cnt[i];
+        for (int k = sub; k < n; k += PSUB) {
+            const int e = k * N + i;
+            double dx, dy, dz;
+            entry_disp(x, i, nb_j[e], nb_s[e], Lx, Ly, Lz, dx, dy, dz);
+            const double d2 = dist2_nc(dx, dy, dz);
+            if (d2 <= zp.rout2) {
+                const double r = sqrt(d2);
+                double u, dudr;
+                zbl_eval(zp, r, u, dudr);
+                const double g = dudr / r, gx = g * dx, gy = g * dy, gz = g * dz;
+                a[0] += gx;
+                a[1] += gy;
+                a[2] += gz;
+                a[3] -= 0.5 * dx * gx;
+                a[4] -= 0.5 * dy * gy;
+                a[5] -= 0.5 * dz * gz;
+                a[6] -= 0.5 * dx * gy;
+                a[7] -= 0.5 * dx * gz;
+                a[8] -= 0.5 * dy * gz;
+                a[9] += 0.5 * u;
+            }
+        }
+    }
+#pragma unroll
+    for (int w = PSUB / 2; w >= 1; w /= 2)
+#pragma unroll
+        for (int c = 0; c < 10; ++c) a[c] += __shfl_xor(a[c], w);
+    if (!live || sub != 0) return;
+#pragma unroll
+    for (int c = 0; c < 10; ++c) zacc[10 * static_cast<size_t>(i) + c] = a[c];
 }
 
 // per-block partial sums of |a - b|^2
@@ -603,7 +659,11 @@ inline int nb(long n, int b) { return static_cast<int>((n + b - 1) / b); }
 struct mdp_md {
     int N = 0, ntypes = 0, J = 0;
     double L[3] = {0, 0, 0};
-    double dt = 0, dtf = 0, skin = 0, rcmax = 0;
+    double dt = 0, dtf = 0, skin = 0, rcmax = 0, rcnb = 0;   // rcnb: neighbour-list cutoff (skin included)
+    // reference pair potential (mdp_md_set_zbl); off: SNAP alone
+    bool zbl = false;
+    ZblParams zp;
+    double zbl_rout = 0;
     int rebuild_every = 10;
     int ncell[3] = {0, 0, 0};
     bool use_cells = false;
@@ -615,7 +675,7 @@ struct mdp_md {
     // device state
     DevBuf x, v, f, type, massinv, xref, cell_of, cell_cnt, cell_start, cell_fill, cell_atoms, scan_tmp, nb_cnt, nb_j,
         nb_s, in_cnt, in_e, pcnt, poff, pi, pj, pti, ptj, prij, entry_pair, Ur, Ui, Yr, Yi, epart, dedr, eatom, vir,
-        beta_t, rc2, flags, maxd2, row_order, yt, pair_sum, pgeo;
+        beta_t, rc2, flags, maxd2, row_order, yt, pair_sum, pgeo, zacc;
     // Nose-Hoover NVT (mdp_md_set_nvt); off: NVE
     bool nvt = false;
     double nvt_tstart = 0, nvt_tstop = 0, nvt_tfreq = 0, nvt_drag = 0, nvt_tdof = 0;
@@ -750,8 +810,8 @@ struct mdp_md {
                 const double rc = (radelem[a] + radelem[b]) * p->rcutfac;
                 rc2_h[a * (ntypes + 1) + b] = rc * rc;
             }
-        const double rcs = rcmax + skin;
-        for (int d = 0; d < 3; ++d) ncell[d] = std::max(1, static_cast<int>(std::floor(L[d] / rcs)));
+        rcnb = rcmax + skin;
+        for (int d = 0; d < 3; ++d) ncell[d] = std::max(1, static_cast<int>(std::floor(L[d] / rcnb)));
         use_cells = ncell[0] >= 3 && ncell[1] >= 3 && ncell[2] >= 3;
         if (!use_cells && N > 200000) throw Error("mdp_md_create: box too small for cell lists at this atom count");
         // U and Y of the MD path live in the compact half layout (rows 2mb <= j)
@@ -820,7 +880,7 @@ struct mdp_md {
     }
 
     void rebuild(bool check_disp) {
-        const double rcs = rcmax + skin;
+        const double rcs = rcnb;
         const size_t ncells = static_cast<size_t>(ncell[0]) * ncell[1] * ncell[2];
         MDP_HIP(hipMemsetAsync(cell_cnt.p, 0, sizeof(int) * (ncells + 1), st));
         MDP_HIP(hipMemsetAsync(cell_fill.p, 0, sizeof(int) * (ncells + 1), st));
@@ -897,6 +957,42 @@ struct mdp_md {
         }
     }
 
+    // reference pair potential of every atom into zacc, added by the tally that follows
+    void pair_potential() {
+        if (!zbl) return;
+        timed(T_PAIRPOT, [&] {
+            hipLaunchKernelGGL(k_zbl_md, dim3(nb(static_cast<long>(N) * PSUB, 256)), dim3(256), 0, st, x.as<double>(),
+                               nb_cnt.as<int>(), nb_j.as<int>(), nb_s.as<int>(), L[0], L[1], L[2], N, zp,
+                               zacc.as<double>());
+        });
+    }
+
+    // Adds the ZBL pair potential (mu = [cut_inner, cut_outer, z]) to the forces from the next force
+    // evaluation on.  The neighbour list is rebuilt at max(SNAP cutoff, cut_outer) + skin, so that the
+    // pair potential's pairs stay in the list between rebuilds exactly as SNAP's do.
+    void set_zbl(double cut_inner, double cut_outer, double z) {
+        if (!(cut_outer > cut_inner) || !(cut_inner > 0.0) || !(z > 0.0))
+            throw Error("mdp_md_set_zbl: need 0 < cut_inner < cut_outer and z > 0");
+        const double rcs = std::max(rcmax, cut_outer) + skin;
+        int nc[3];
+        for (int d = 0; d < 3; ++d) nc[d] = std::max(1, static_cast<int>(std::floor(L[d] / rcs)));
+        const bool cells = nc[0] >= 3 && nc[1] >= 3 && nc[2] >= 3;
+        if (!cells && N > 200000) throw Error("mdp_md_set_zbl: box too small for cell lists at this atom count");
+        const size_t ncells = static_cast<size_t>(nc[0]) * nc[1] * nc[2];
+        cell_cnt.ensure(sizeof(int) * (ncells + 1));
+        cell_start.ensure(sizeof(int) * (ncells + 1));
+        cell_fill.ensure(sizeof(int) * (ncells + 1));
+        scan_tmp.ensure(sizeof(int) * (scan_tmp_ints(static_cast<int>(std::max(static_cast<size_t>(N), ncells))) + 8));
+        zacc.ensure(sizeof(double) * 10 * static_cast<size_t>(N));
+        for (int d = 0; d < 3; ++d) ncell[d] = nc[d];
+        use_cells = cells;
+        rcnb = rcs;
+        zp = zbl_params(cut_inner, cut_outer, z);
+        zbl_rout = cut_outer;
+        zbl = true;
+        forces_current = false;
+    }
+
     // plain SNAP forces; outputs to (fo, vo, eo), by default the engine's f, vir and eatom
     void snap_and_tally(bool integrate, double* fo = nullptr, double* vo = nullptr, double* eo = nullptr) {
         const SnapDev& s = tab.view;
@@ -915,11 +1011,13 @@ struct mdp_md {
         timed(T_DEDR, [&] {
             dedr_launch(dedr_variant);
         });
+        pair_potential();
         timed(T_TALLY, [&] {
             hipLaunchKernelGGL(k_tally, dim3(nb(static_cast<long>(N) * PSUB, 256)), dim3(256), 0, st, dedr.as<double>(), prij.as<double>(),
                                poff.as<int>(), in_cnt.as<int>(), in_e.as<int>(), entry_pair.as<int>(),
                                epart.as<double>(), nrows, e0, nullptr, fo, vo, eo, v.as<double>(), type.as<int>(),
-                               massinv.as<double>(), dtf, integrate ? 1 : 0, N, count ? pair_sum.as<unsigned long long>() : nullptr);
+                               massinv.as<double>(), dtf, integrate ? 1 : 0, N, count ? pair_sum.as<unsigned long long>() : nullptr,
+                               zbl ? zacc.as<double>() : nullptr);
         });
     }
 
@@ -999,12 +1097,13 @@ struct mdp_md {
                         ptj.as<int>(), nullptr, dedr.as<double>(), N, static_cast<int>(stats[1]), poff.as<int>() + N, st,
                         true, 10);
         });
+        pair_potential();
         timed(T_TALLY, [&] {
             hipLaunchKernelGGL(k_tally, dim3(nb(static_cast<long>(N) * PSUB, 256)), dim3(256), 0, st, dedr.as<double>(), prij.as<double>(),
                                poff.as<int>(), in_cnt.as<int>(), in_e.as<int>(), entry_pair.as<int>(),
                                epart.as<double>(), 0, 0.0, sE.as<double>(), f.as<double>(), vir.as<double>(),
                                eatom.as<double>(), v.as<double>(), type.as<int>(), massinv.as<double>(), dtf,
-                               integrate ? 1 : 0, N, pair_sum.as<unsigned long long>());
+                               integrate ? 1 : 0, N, pair_sum.as<unsigned long long>(), zbl ? zacc.as<double>() : nullptr);
         });
     }
 
@@ -1233,6 +1332,13 @@ int api(F&& f) {
 }  // namespace
 
 extern "C" {
+
+int mdp_md_set_zbl(mdp_md* md, double cut_inner, double cut_outer, double z) {
+    return api([&] {
+        if (!md) throw Error("mdp_md_set_zbl: null handle");
+        md->set_zbl(cut_inner, cut_outer, z);
+    });
+}
 
 int mdp_md_set_nvt(mdp_md* md, double t_start, double t_stop, double t_period, double drag, double tdof, int mtchain,
                    int nc_tchain, long beginstep, long endstep, long currentstep) {

@@ -30,6 +30,7 @@ def _bind():
         return c
     vp = ctypes.c_void_p
     c.mdp_fit_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(_Params), ctypes.c_int, vp, vp, vp, vp]
+    c.mdp_fit_create_cells.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(_Params), ctypes.c_int, vp, vp, vp, vp]
     for fn in ("mdp_fit_destroy", "mdp_fit_compute", "mdp_fit_sync"):
         getattr(c, fn).argtypes = [vp]
     c.mdp_fit_set_positions.argtypes = [vp, vp]
@@ -54,7 +55,9 @@ def ncoeff_of(twojmax):
 class SnapFit:
     """B and dB/dr of a batch of periodic configurations.
 
-    ``configs`` is a list of (x (N_c,3), box (3,)) or (x, box, types); types are 1-based (default 1).
+    ``configs`` is a list of (x (N_c,3), box, [types]) with box either the three lengths of an orthogonal
+    box or a (3, 3) cell whose rows are the cell vectors a = (Lx, 0, 0), b = (bx, Ly, 0), c = (cx, cy, Lz)
+    (the form rotatelattice.jl produces); types are 1-based (default 1).
     """
 
     def __init__(self, configs, twojmax, elemradius=(0.5,), elemweight=(1.0,), rcutfac=4.67637,
@@ -64,8 +67,15 @@ class SnapFit:
         self.natoms = np.array([len(cfg[0]) for cfg in configs], np.int32)
         self.x = np.ascontiguousarray(np.concatenate([np.asarray(cfg[0], np.float64).reshape(-1, 3)
                                                       for cfg in configs]))
-        self.boxes = np.ascontiguousarray(np.stack([np.asarray(cfg[1], np.float64).reshape(3)
-                                                    for cfg in configs]))
+        self.cells = None
+        if any(np.asarray(cfg[1]).size == 9 for cfg in configs):
+            self.cells = np.ascontiguousarray(np.stack([
+                np.asarray(cfg[1], np.float64).reshape(3, 3) if np.asarray(cfg[1]).size == 9
+                else np.diag(np.asarray(cfg[1], np.float64).reshape(3)) for cfg in configs]))
+            self.boxes = np.ascontiguousarray(self.cells[:, [0, 1, 2], [0, 1, 2]])
+        else:
+            self.boxes = np.ascontiguousarray(np.stack([np.asarray(cfg[1], np.float64).reshape(3)
+                                                        for cfg in configs]))
         self.types = np.ascontiguousarray(np.concatenate([
             (np.asarray(cfg[2], np.int32).ravel() if len(cfg) > 2 and not isinstance(cfg[2], str)
              else np.ones(len(cfg[0]), np.int32)) for cfg in configs]))
@@ -77,9 +87,15 @@ class SnapFit:
                     self._keep[0].ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                     self._keep[1].ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         h = ctypes.c_void_p()
-        self._check(self._c.mdp_fit_create(ctypes.byref(h), ctypes.byref(p), self.C, self.natoms.ctypes.data,
-                                           self.boxes.ctypes.data, self.x.ctypes.data, self.types.ctypes.data),
-                    "mdp_fit_create")
+        if self.cells is None:
+            self._check(self._c.mdp_fit_create(ctypes.byref(h), ctypes.byref(p), self.C, self.natoms.ctypes.data,
+                                               self.boxes.ctypes.data, self.x.ctypes.data, self.types.ctypes.data),
+                        "mdp_fit_create")
+        else:
+            self._check(self._c.mdp_fit_create_cells(ctypes.byref(h), ctypes.byref(p), self.C,
+                                                     self.natoms.ctypes.data, self.cells.ctypes.data,
+                                                     self.x.ctypes.data, self.types.ctypes.data),
+                        "mdp_fit_create_cells")
         self._h = h
 
     def _check(self, rc, what):

@@ -60,6 +60,7 @@ def _bind():
     c.mdp_md_mean_pairs.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     c.mdp_md_set_nvt.argtypes = [vp] + [ctypes.c_double] * 5 + [ctypes.c_int] * 2 + [ctypes.c_long] * 3
     c.mdp_md_clear_nvt.argtypes = [vp]
+    c.mdp_md_set_zbl.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
     c.mdp_md_thermo.argtypes = [vp, ctypes.c_void_p]
     c.mdp_md_nvt_state.argtypes = [vp, ctypes.c_void_p]
     c.mdp_velocity_create.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -203,6 +204,11 @@ class SnapMD:
         o = np.zeros(4)
         self._check(self._c.mdp_md_thermo(self._h, o.ctypes.data), "mdp_md_thermo")
         return dict(pe=o[0], temp=o[1], ke=o[2], press=o[3])
+
+    def set_zbl(self, cut_inner=4.0, cut_outer=4.8, z=73.0):
+        """Add the ZBL reference pair potential (simulating/snap/ZBLpotential.jl; the Ta application's
+        parameters by default, snapTA06A.jl:62-66) to the forces from the next evaluation on."""
+        self._check(self._c.mdp_md_set_zbl(self._h, float(cut_inner), float(cut_outer), float(z)), "mdp_md_set_zbl")
 
     def clear_nvt(self):
         self._check(self._c.mdp_md_clear_nvt(self._h), "mdp_md_clear_nvt")

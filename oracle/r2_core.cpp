@@ -7,6 +7,9 @@
 #include <vector>
 
 #include "cpuCore.cpp"
+// the generated ZBL pair gradient of the Ta application (src/C++/Potentials, generated from
+// simulating/snap/ZBLpotential.jl with cut_inner 4, cut_outer 4.8, z 73)
+#include "cpuPairaGradient.cpp"
 
 extern "C" {
 
@@ -235,6 +238,21 @@ void r2_cpuSnapTallyForceFull(double *fatom, double *fij, int *ai, int *aj, int 
 }
 void r2_cpuSnapTallyVirialFull(double *vatom, double *fij, double *rij, int *ai, int *aj, int inum, int ijnum) {
     cpuSnapTallyVirialFull(vatom, fij, rij, ai, aj, inum, ijnum);
+}
+
+// the empirical pair route of the upstream app (src/C++/Potentials/cpuPotentials.cpp:1587-1649): the
+// generated pair gradient (potnum 1), the full-list energy/force decomposition and the pair virial
+void r2_cpuPairaGradient(double *u, double *u_xij, double *xij, int ng) {
+    cpuPairaGradient(u, u_xij, xij, (double *)nullptr, (double *)nullptr, (int *)nullptr, (int *)nullptr,
+                     (int *)nullptr, (int *)nullptr, (double *)nullptr, (double *)nullptr, (int *)nullptr, 3, 0, 0, 0,
+                     0, ng, 1);
+}
+void r2_cpuFullNeighPairDecomposition(double *e, double *f, double *eij, double *fij, int *ai, int ijnum, int dim) {
+    cpuFullNeighPairDecomposition(e, f, eij, fij, ai, ijnum, dim);
+}
+void r2_cpuVirialPairTally(double *v, double *fij, double *rij, double factor, int *ai, int dim, int inum,
+                           int ijnum) {
+    cpuVirialPairTally(v, fij, rij, factor, ai, dim, inum, ijnum);
 }
 
 }  // extern "C"

@@ -1091,3 +1091,90 @@ long oracle_neighbor_bruteforce(const double *x, int n, const double *box, doubl
     offsets[n] = (int)(cnt < maxnb ? cnt : maxnb);
     return cnt <= maxnb ? cnt : -cnt;
 }
+
+/* ------------------------------------------------------------------------------------------------
+ * ZBL reference pair potential (the empirical pair term used beside SNAP in the Ta application).
+ *
+ * Restated from simulating/snap/ZBLpotential.jl:13-59 (ezbl) and :61-108 (Paira) with the
+ * parameters mu = [cut_inner, cut_outer, z] (the mu-driven form of ZBLpotential.jl:68-71;
+ * snapTA06A.jl:62-66 passes [4, 4.8, 73]).  u(r) = E(r) + S(r) with
+ *   E(r) = zze/r * sum_k c_k exp(-d_k a r),   a = (z^0.23 + z^0.23)/0.4685,  zze = z z 14.399645,
+ *   S(r) = sw5 + A(t) t^3 (sw3 + sw4 t),      t = r - cut_inner,  A = 1/2 + tanh(100 t)/2,
+ * the switching constants sw3..sw5 from E, E', E'' at cut_outer (ZBLpotential.jl:79-89).  The
+ * generated code the upstream application compiles (src/C++/Potentials/cpuPairaGradient.cpp:1-43)
+ * differentiates the same expression symbolically, tanh step included; here
+ *   du/dr = E'(r) + 50 (1 - tanh^2(100 t)) t^3 (sw3 + sw4 t) + A(t) (3 sw3 t^2 + 4 sw4 t^3),
+ * and u_xij = du/dr * xij / r, with xij = x_j - x_i.
+ * ---------------------------------------------------------------------------------------------- */
+static void zbl_e(double r, double ainv, double zze, double *e, double *ep, double *epp)
+{
+    static const double c[4] = {0.02817, 0.28022, 0.50986, 0.18175};
+    static const double d[4] = {0.20162, 0.40290, 0.94229, 3.19980};
+    double s = 0.0, sp = 0.0, spp = 0.0;
+    for (int k = 0; k < 4; k++) {
+        const double da = d[k] * ainv, ek = exp(-da * r);
+        s += c[k] * ek;
+        sp -= c[k] * da * ek;
+        spp += c[k] * ek * da * da;
+    }
+    const double rinv = 1.0 / r;
+    *e = zze * s * rinv;
+    *ep = zze * (sp - s * rinv) * rinv;
+    *epp = zze * (spp - 2.0 * sp * rinv + 2.0 * s * rinv * rinv) * rinv;
+}
+
+void oracle_PairaGradient(double *u, double *u_xij, const double *xij, const double *mu, int ng)
+{
+    const double rin = mu[0], rout = mu[1], z = mu[2];
+    const double ainv = (pow(z, 0.23) + pow(z, 0.23)) / 0.46850;
+    const double zze = z * z * 14.399645;
+    double fc, fcp, fcpp;
+    zbl_e(rout, ainv, zze, &fc, &fcp, &fcpp);
+    const double tc = rout - rin;
+    const double sw3 = (-3.0 * fcp + tc * fcpp) / (tc * tc) / 3.0;
+    const double sw4 = (2.0 * fcp - tc * fcpp) / (tc * tc * tc) / 4.0;
+    const double sw5 = -fc + (tc / 2.0) * fcp - (tc * tc / 12.0) * fcpp;
+    for (int p = 0; p < ng; p++) {
+        const double x = xij[3 * p], y = xij[3 * p + 1], zz = xij[3 * p + 2];
+        const double r = sqrt(x * x + y * y + zz * zz);
+        double e, ep, epp;
+        zbl_e(r, ainv, zze, &e, &ep, &epp);
+        const double t = r - rin, th = tanh(100.0 * t), a = 0.5 + 0.5 * th;
+        const double poly = t * t * t * (sw3 + sw4 * t);
+        u[p] = e + sw5 + a * poly;
+        const double dudr = ep + 50.0 * (1.0 - th * th) * poly + a * (3.0 * sw3 * t * t + 4.0 * sw4 * t * t * t);
+        u_xij[3 * p] = dudr * x / r;
+        u_xij[3 * p + 1] = dudr * y / r;
+        u_xij[3 * p + 2] = dudr * zz / r;
+    }
+}
+
+/* cpuFullNeighPairDecomposition (src/C++/Core/cpuEnergyForceTally.cpp:216-227): e[ai] += eij / 2,
+ * f[ai] -= fij, over a full pair list (each pair seen from both atoms). */
+void oracle_FullNeighPairDecomposition(double *e, double *f, const double *eij, const double *fij, const int *ai,
+                                       int ijnum, int dim)
+{
+    for (int p = 0; p < ijnum; p++) {
+        const int i = ai[p];
+        e[i] += 0.5 * eij[p];
+        for (int d = 0; d < dim; d++) f[dim * i + d] += -fij[dim * p + d];
+    }
+}
+
+/* cpuVirialPairTally (src/C++/Core/cpuVirialTally.cpp:34-57): v[k*inum + ai] += factor * (rij (x) fij)_k
+ * for k = xx, yy, zz, xy, xz, yz (SoA). */
+void oracle_VirialPairTally(double *v, const double *fij, const double *rij, double factor, const int *ai, int dim,
+                            int inum, int ijnum)
+{
+    for (int p = 0; p < ijnum; p++) {
+        const int i = ai[p];
+        const double dx = rij[dim * p], dy = rij[dim * p + 1], dz = dim == 3 ? rij[dim * p + 2] : 0.0;
+        const double fx = fij[dim * p], fy = fij[dim * p + 1], fz = dim == 3 ? fij[dim * p + 2] : 0.0;
+        v[0 * inum + i] += factor * dx * fx;
+        v[1 * inum + i] += factor * dy * fy;
+        v[2 * inum + i] += factor * dz * fz;
+        v[3 * inum + i] += factor * dx * fy;
+        v[4 * inum + i] += factor * dx * fz;
+        v[5 * inum + i] += factor * dy * fz;
+    }
+}

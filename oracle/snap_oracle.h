@@ -106,6 +106,14 @@ void cpuSnapTallyBispectrumVirial(double *bv, double *bispectrum, double *dbdr, 
 long oracle_neighbor_bruteforce(const double *x, int n, const double *box, double rc,
         int *offsets, int *nbr_j, int *nbr_shift, long maxnb);
 
+/* ZBL reference pair potential and the pair tallies of the upstream empirical-potential route
+ * (src/C++/Potentials/cpuPotentials.cpp:1587-1649, :1837).  mu = [cut_inner, cut_outer, z]. */
+void oracle_PairaGradient(double *u, double *u_xij, const double *xij, const double *mu, int ng);
+void oracle_FullNeighPairDecomposition(double *e, double *f, const double *eij, const double *fij, const int *ai,
+        int ijnum, int dim);
+void oracle_VirialPairTally(double *v, const double *fij, const double *rij, double factor, const int *ai, int dim,
+        int inum, int ijnum);
+
 #ifdef __cplusplus
 }
 #endif

@@ -161,3 +161,25 @@ def test_short_trajectory_vs_oracle_integration(oracle):
     d = np.mod(g["x"] - xh + 0.5 * box, box) - 0.5 * box
     assert np.max(np.abs(d)) < 1e-9
     assert relerr(g["v"], vh) < 1e-9
+
+
+def test_twojmax8_energy_error_scales_with_dt2(oracle):
+    """twojmax = 8 on the headline lattice (sigma = 0.1 A, 300 K): velocity Verlet's largest total-energy
+    excursion over 25 fs falls about fourfold when dt is halved, so the forces are the gradient of the
+    energy the engine reports over many steps (an inconsistent force term would not scale this way)."""
+    x, box = lattice.crystal("bcc", 5, 3.316, 0.1, 91)
+    v = lattice.maxwell_boltzmann(len(x), MASS, 300.0, seed=91)
+    coeff = _coeff(oracle, 8, 91)
+    dev = []
+    for dt, steps in ((0.001, 25), (0.0005, 50)):
+        md = _engine(x, box, 8, coeff, v=v, dt=dt, every=1)
+        md.compute()
+        e0 = sum(md.energy())
+        d = 0.0
+        for _ in range(steps):
+            md.run(1)
+            d = max(d, abs(sum(md.energy()) - e0))
+        dev.append(d)
+        md.close()
+    ratio = dev[0] / dev[1]
+    assert 3.0 < ratio < 5.0, (dev, ratio)
