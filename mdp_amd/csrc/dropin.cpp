@@ -1032,9 +1032,8 @@ void mdp_hipCell2AtomList(int* c2alist, int* c2anumsum, int* c2anum, int* clist,
         for (int c : h_cl)
             if (c < 0 || c >= ncell) throw Error("Cell2AtomList: a cell id is outside [0, ncell)");
         hipStream_t st = library_stream();
-        int* cursor = dev<int>(0, ncell);
-        int* tmp = dev<int>(1, scan_tmp_ints(ncell));
-        launch_cell2atom(c2alist, c2anumsum, c2anum, clist, cursor, tmp, natom, ncell, st);
+        launch_cell2atom_sorted(c2alist, c2anumsum, c2anum, clist, natom, ncell,
+                                [](size_t bytes) -> void* { return dev<unsigned char>(0, bytes); }, st);
         finish(st);
     });
 }

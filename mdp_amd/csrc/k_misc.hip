@@ -1034,40 +1034,6 @@ __global__ __launch_bounds__(256) void k_fill_int(int* __restrict__ a, int v, in
     if (i < n) a[i] = v;
 }
 
-__global__ __launch_bounds__(256) void k_copy_int(int* __restrict__ a, const int* __restrict__ b, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = b[i];
-}
-
-__global__ __launch_bounds__(256) void k_cell_count(const int* __restrict__ clist, int natom, int* __restrict__ cnt) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < natom) atomicAdd(&cnt[clist[i]], 1);
-}
-
-__global__ __launch_bounds__(256) void k_cell_place(const int* __restrict__ clist, int natom, int* __restrict__ cur,
-                                                    int* __restrict__ c2alist) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < natom) c2alist[atomicAdd(&cur[clist[i]], 1)] = i;
-}
-
-// atoms of each cell in ascending order (the order of a stable sort by cell), and the sorted cell ids
-__global__ __launch_bounds__(256) void k_cell_order(const int* __restrict__ start, int ncell, int* __restrict__ c2alist,
-                                                    int* __restrict__ c2anum) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    const int a = start[c], b = start[c + 1];
-    for (int p = a + 1; p < b; ++p) {
-        const int v = c2alist[p];
-        int q = p - 1;
-        while (q >= a && c2alist[q] > v) {
-            c2alist[q + 1] = c2alist[q];
-            --q;
-        }
-        c2alist[q + 1] = v;
-    }
-    for (int p = a; p < b; ++p) c2anum[p] = c;
-}
-
 // Neighbours of atom i = alist[ii] among the 27 cells around its cell, in upstream's cell and list order,
 // at most jnum of them; cells outside the grid are skipped.
 __global__ __launch_bounds__(256) void k_full_neigh_cell(int* __restrict__ nl, int* __restrict__ nn,
@@ -1122,19 +1088,6 @@ void launch_cell_list3d(int* clist, const double* x, const double* eta1, const d
     if (natom <= 0) return;
     hipLaunchKernelGGL(k_cell_list3d, dim3((natom + 255) / 256), dim3(256), 0, st, clist, x, eta1, eta2, eta3, s2rmap,
                        nc[0], nc[1], nc[2], inum, natom);
-}
-
-void launch_cell2atom(int* c2alist, int* c2anumsum, int* c2anum, const int* clist, int* cursor, int* scan_tmp,
-                      int natom, int ncell, hipStream_t st) {
-    if (ncell <= 0) return;
-    hipLaunchKernelGGL(k_fill_int, dim3((ncell + 255) / 256), dim3(256), 0, st, cursor, 0, ncell);
-    if (natom > 0)
-        hipLaunchKernelGGL(k_cell_count, dim3((natom + 255) / 256), dim3(256), 0, st, clist, natom, cursor);
-    exclusive_scan(cursor, c2anumsum, ncell, scan_tmp, st);
-    if (natom <= 0) return;
-    hipLaunchKernelGGL(k_copy_int, dim3((ncell + 255) / 256), dim3(256), 0, st, cursor, c2anumsum, ncell);
-    hipLaunchKernelGGL(k_cell_place, dim3((natom + 255) / 256), dim3(256), 0, st, clist, natom, cursor, c2alist);
-    hipLaunchKernelGGL(k_cell_order, dim3((ncell + 255) / 256), dim3(256), 0, st, c2anumsum, ncell, c2alist, c2anum);
 }
 
 void launch_full_neigh_cell(int* nl, int* nn, const double* x, double rc2, const int* alist, const int* clist,

@@ -123,9 +123,9 @@ void launch_add_utot_rij(double* utr, double* uti, const double* ur, const doubl
 void launch_cell_list3d(int* clist, const double* x, const double* eta1, const double* eta2, const double* eta3,
                         const double* s2rmap, const int nc[3], int inum, int natom, hipStream_t st);
 // c2alist: atoms by cell (stable), c2anumsum[ncell+1]: first entry of each cell, c2anum: sorted cell ids;
-// cursor[ncell] and scan_tmp (scan_tmp_ints(ncell)) are scratch
-void launch_cell2atom(int* c2alist, int* c2anumsum, int* c2anum, const int* clist, int* cursor, int* scan_tmp,
-                      int natom, int ncell, hipStream_t st);
+// a stable radix sort (cellsort.hip) whose temporary storage comes from temp_alloc(bytes)
+void launch_cell2atom_sorted(int* c2alist, int* c2anumsum, int* c2anum, const int* clist, int natom, int ncell,
+                             void* (*temp_alloc)(size_t), hipStream_t st);
 void launch_full_neigh_cell(int* nl, int* nn, const double* x, double rc2, const int* alist, const int* clist,
                             const int* c2alist, const int* c2anumsum, const int nc[3], int inum, int jnum,
                             hipStream_t st);

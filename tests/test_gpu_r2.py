@@ -531,6 +531,31 @@ def test_backend3_cell_lists_vs_upstream_core(inum, ncell, jnum, seed):
         assert nn.max() <= jnum and (jnum > 50 or nn.max() == jnum)
 
 
+@pytest.mark.parametrize("natom,ncell", [(300000, 2), (100000, 1), (50000, 70000), (1, 5)])
+def test_backend3_cell2atom_crowded_cells(natom, ncell):
+    """Cell2AtomList when a few cells hold most atoms (one cell holds them all in one case, most cells are
+    empty in another): the atoms of each cell in ascending order (numpy's stable argsort), the first entry
+    of every cell and the sorted ids, in time linear in natom."""
+    from mdp_amd import library
+    from mdp_amd.devarray import DevArray
+    lib = library().cdll
+    f = lib.mdp_hipCell2AtomList
+    f.argtypes, f.restype = [P] * 5 + [I] * 2, None
+    rng = np.random.default_rng(natom + ncell)
+    cl = (rng.integers(0, ncell, natom) if ncell > 2 else (rng.random(natom) < 0.999).astype(np.int64) % ncell)
+    cl = cl.astype(np.int32)
+    order = np.argsort(cl, kind="stable").astype(np.int32)
+    start = np.searchsorted(cl[order], np.arange(ncell + 1)).astype(np.int32)
+    dcl = DevArray.from_numpy(cl, np.int32)
+    da, ds, dn = DevArray.zeros(natom, np.int32), DevArray.zeros(ncell + 1, np.int32), DevArray.zeros(natom, np.int32)
+    print(f"Cell2AtomList: {natom} atoms in {ncell} cells", flush=True)
+    f(da.ptr, ds.ptr, dn.ptr, dcl.ptr, None, natom, ncell)
+    assert lib.mdp_snap_last_error().decode() == ""
+    assert np.array_equal(da.numpy(), order)
+    assert np.array_equal(ds.numpy(), start)
+    assert np.array_equal(dn.numpy(), cl[order])
+
+
 @pytest.mark.parametrize("J,chem,sw", [(4, 0, 1), (8, 1, 1), (6, 0, 0)])
 def test_backend3_add_utot_rij_vs_upstream_core(oracle, J, chem, sw):
     """The rij form of AddUarraytot (the per-pair route: U pair-major, Utot atom-major, switching and element
