@@ -489,6 +489,15 @@ void mdp_hipCellList3D(int *clist, double *x, double *eta1, double *eta2, double
 /* replaces hipCell2AtomList in Cell2AtomList (mdpcore.h:1351-1368; src/C++/Core/cpuNeighborList2D.cpp:298-308);
    dtemp is not used */
 void mdp_hipCell2AtomList(int *c2alist, int *c2anumsum, int *c2anum, int *clist, int *dtemp, int natom, int ncell);
+/* No upstream counterpart: mdp_hipCell2AtomList for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same arguments and outputs;
+   it calls no hipMalloc/hipFree: the sort's temporary storage is a fixed 64 MiB device array of the
+   library (one per GPU, held until the process ends), enough for natom up to about 4 million; larger
+   natom, ncell <= 0 or natom < 0 are rejected with an error before any work.  Cell ids outside
+   [0, ncell) set an error after the call (the outputs are then unspecified).  Calls that overlap on one
+   GPU share that array and race; serialize them.  dtemp is not used. */
+void mdp_hipCell2AtomListNoAlloc(int *c2alist, int *c2anumsum, int *c2anum, int *clist, int *dtemp, int natom,
+                                 int ncell);
 /* replaces hipFullNeighborList23 in the cell-list form of FullNeighborList3D (mdpcore.h:1432-1453;
    src/C++/Core/cpuNeighborList3D.cpp:361-412) */
 void mdp_hipFullNeighborList3DCell(int *neighlist, int *neighnum, double *x, double *rcutsq, int *alist, int *clist,
