@@ -22,6 +22,8 @@
  * include/mdphipcore.h wraps it as the hip* templates.
  * Part 6 is the reference pair potential (ZBL) the SNAP application adds to SNAP, as backend-3 entry
  * points of the upstream empirical-potential route.
+ * Part 7 reads the upstream application's binary inputs (app.bin, config.bin, lattice.bin, ...) and sets
+ * up the MD engine from them.
  *
  * Errors: the drop-in functions return void like upstream; a failure leaves a message readable
  * with mdp_snap_last_error() (empty string when the last call succeeded).  Part 2 returns 0 on
@@ -223,6 +225,11 @@ int mdp_md_set_nvt(mdp_md *md, double t_start, double t_stop, double t_period, d
                    int nc_tchain, long beginstep, long endstep, long currentstep);
 /* back to NVE */
 int mdp_md_clear_nvt(mdp_md *md);
+/* New positions x[3N] and, when v is not NULL, velocities v[3N] from the host (a restart, or a state
+ * prepared elsewhere).  Non-finite values are rejected and leave the engine unchanged; the next compute
+ * or step rebuilds the neighbour lists and recomputes the forces.  (During a run, a position that turns
+ * non-finite or runs away is reported by the next neighbour rebuild as an error.) */
+int mdp_md_set_state(mdp_md *md, const double *x, const double *v);
 /* Adds the ZBL reference pair potential of the SNAP Ta application (simulating/snap/ZBLpotential.jl with
  * mu = [cut_inner, cut_outer, z], snapTA06A.jl:62-66: 4, 4.8, 73) to SNAP from the next force evaluation
  * on, with the upstream full-neighbour-list conventions (src/C++/Potentials/cpuPotentials.cpp:1587-1649):
@@ -560,6 +567,37 @@ void mdp_hipFullNeighPairDecomposition(double *e, double *f, double *eij, double
  * v[k*inum + ai] += factor * (rij (x) fij)_k, k = xx, yy, zz, xy, xz, yz (accumulates) */
 void mdp_hipVirialPairTally(double *v, double *fij, double *rij, double factor, int *ai, int dim, int inum,
                             int ijnum);
+
+/* ---------------------------------------------------------------------------------------------- */
+/* Part 7: the upstream application's binary inputs                                                */
+/* ---------------------------------------------------------------------------------------------- */
+/* The files the Julia preprocessing writes for the C++ application (mdp(app), Preprocessing.jl:60-321:
+ * <prefix>app.bin, and when present config.bin, lattice.bin, region.bin, domain.bin), read as the upstream
+ * reader does (src/C++/Configuration/readdata.cpp:11-290).  Host code; no GPU is touched until
+ * mdp_md_create_app.  Errors: return value (nonzero / -1) and mdp_snap_last_error(). */
+typedef struct mdp_app mdp_app;   /* opaque */
+int mdp_app_read(mdp_app **h, const char *prefix);
+int mdp_app_destroy(mdp_app *h);
+/* Values of one field, by the upstream name: app.bin's ndims, flag, bcs, pbc, boxoffset, atomnumbers,
+ * atommasses, atomcharges, simparam, solparam, eta, kappa, muml, mu1a..mu4b, pot1a..pot4b, rcutsqml,
+ * rcutsq2a..rcutsq4b, rcutsqmax, atom1b..atom4b, traininglist, validatelist, nveparam, nvtparam,
+ * snaparam, snapelemradius, snapelemweight, snapcoeff, createvelocity, setatomtypefraction; config.bin's
+ * cfg_ndims, natoms, a, b, c, e, t, x, q, v, f, we, wf, ws, stress, cfg_pbc, move, group.  Copies at most
+ * cap values to out (may be NULL) and returns the field's length, -1 for an unknown name. */
+long mdp_app_field(const mdp_app *h, const char *name, double *out, long cap);
+/* flags4 = which of config.bin, lattice.bin, region.bin, domain.bin were present */
+int mdp_app_has(const mdp_app *h, int *flags4);
+/* The atoms the application starts from: the first configuration of config.bin, or else the lattice
+ * points inside the region (readdata.cpp:490-551, 647-676; cpuLattice, cpuAtomLattice, cpuAtomAdd), the
+ * same positions and 1-based types bit for bit, relative to the box origin; box3 = the orthogonal box
+ * lengths.  Returns the atom count (x, type may be NULL to query it; cap = their capacity in atoms), or
+ * -1 (tilted cells, units other than metal, setatomtypefraction). */
+long mdp_app_atoms(const mdp_app *h, double *x, int *type, double *box3, long cap);
+/* An MD engine (part 2) for the application: SNAP from snaparam/snapelemradius/snapelemweight/snapcoeff,
+ * masses, dt (simparam[1]), neighbour skin (solparam[1]), rebuilds every neighevery steps, the ZBL pair
+ * potential when pot2a = [1] (mu2a), NVE or Nose-Hoover NVT (nvtparam), and the initial velocities of
+ * config.bin and createvelocity.  Run it with mdp_md_run for the application's ntimesteps (ndims[3]). */
+int mdp_md_create_app(mdp_md **md, const mdp_app *h);
 
 #ifdef __cplusplus
 }

@@ -240,6 +240,10 @@ __global__ void k_wrap_bin(double* __restrict__ x, double* __restrict__ xref, in
         xi -= floor(xi / L[d]) * L[d];
         if (xi >= L[d]) xi -= L[d];
         if (xi < 0.0) xi = 0.0;
+        if (!(xi < L[d])) {   // a runaway coordinate (|x| beyond 2^53 box lengths) does not wrap: report it
+            atomicOr(flags, 8);
+            xi = 0.0;
+        }
         x[3 * i + d] = xi;
         xref[3 * i + d] = xi;
         c[d] = min(static_cast<int>(xi * nc[d] / L[d]), nc[d] - 1);
@@ -918,7 +922,7 @@ struct mdp_md {
             stats[5] += 1;
             throw Error(std::string("neighbour build failed:") + ((fl[0] & 1) ? " >96 neighbours per atom" : "") +
                         ((fl[0] & 2) ? " >48 atoms per cell" : "") + ((fl[0] & 4) ? " >96 incoming entries" : "") +
-                        ((fl[0] & 8) ? " non-finite positions" : ""));
+                        ((fl[0] & 8) ? " non-finite or runaway positions" : ""));
         }
         double d2;
         std::memcpy(&d2, &md2, sizeof(d2));
@@ -1246,6 +1250,18 @@ struct mdp_md {
                            nvt_tfreq, nvt_tdof, BOLTZ, nvt_drag, dt, tfactor, nvt_mtchain, nvt_nc, nvt_massflag);
     }
 
+    // new positions (and velocities, when given) from the host; the next step rebuilds and recomputes
+    void set_state(const double* xh, const double* vh) {
+        if (!xh) throw Error("mdp_md_set_state: null positions");
+        for (long k = 0; k < 3L * N; ++k)
+            if (!std::isfinite(xh[k]) || (vh && !std::isfinite(vh[k])))
+                throw Error("mdp_md_set_state: non-finite position or velocity");
+        MDP_HIP(hipMemcpyAsync(x.p, xh, sizeof(double) * 3 * N, hipMemcpyHostToDevice, st));
+        if (vh) MDP_HIP(hipMemcpyAsync(v.p, vh, sizeof(double) * 3 * N, hipMemcpyHostToDevice, st));
+        MDP_HIP(hipStreamSynchronize(st));
+        forces_current = false;
+    }
+
     void set_nvt(double t_start, double t_stop, double t_period, double drag, double tdof, int mtchain, int nc_tchain,
                  long beginstep, long endstep, long currentstep) {
         if (!(t_period > 0.0)) throw Error("mdp_md_set_nvt: the thermostat period must be positive");
@@ -1332,6 +1348,13 @@ int api(F&& f) {
 }  // namespace
 
 extern "C" {
+
+int mdp_md_set_state(mdp_md* md, const double* x, const double* v) {
+    return api([&] {
+        if (!md) throw Error("mdp_md_set_state: null handle");
+        md->set_state(x, v);
+    });
+}
 
 int mdp_md_set_zbl(mdp_md* md, double cut_inner, double cut_outer, double z) {
     return api([&] {

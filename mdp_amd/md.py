@@ -61,6 +61,7 @@ def _bind():
     c.mdp_md_set_nvt.argtypes = [vp] + [ctypes.c_double] * 5 + [ctypes.c_int] * 2 + [ctypes.c_long] * 3
     c.mdp_md_clear_nvt.argtypes = [vp]
     c.mdp_md_set_zbl.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    c.mdp_md_set_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_void_p]
     c.mdp_md_thermo.argtypes = [vp, ctypes.c_void_p]
     c.mdp_md_nvt_state.argtypes = [vp, ctypes.c_void_p]
     c.mdp_velocity_create.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -98,6 +99,19 @@ class SnapMD:
                                    self.x0.ctypes.data, v.ctypes.data, self.types.ctypes.data)
         self._check(rc, "mdp_md_create")
         self._h = h
+
+    @classmethod
+    def _from_handle(cls, h, x, box, types):
+        """Wrap an engine handle made by another entry point (mdp_md_create_app)."""
+        self = cls.__new__(cls)
+        self._c = _bind()
+        self.x0 = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        self.n = len(self.x0)
+        self.box = np.ascontiguousarray(box, np.float64).reshape(3)
+        self.types = np.ascontiguousarray(types, np.int32).reshape(self.n)
+        self._keep = []
+        self._h = h
+        return self
 
     def _check(self, rc, what):
         if rc != 0:
@@ -204,6 +218,13 @@ class SnapMD:
         o = np.zeros(4)
         self._check(self._c.mdp_md_thermo(self._h, o.ctypes.data), "mdp_md_thermo")
         return dict(pe=o[0], temp=o[1], ke=o[2], press=o[3])
+
+    def set_state(self, x, v=None):
+        """New positions (and velocities) from the host; forces are recomputed at the next step."""
+        x = np.ascontiguousarray(x, np.float64).reshape(self.n, 3)
+        v = None if v is None else np.ascontiguousarray(v, np.float64).reshape(self.n, 3)
+        self._check(self._c.mdp_md_set_state(self._h, x.ctypes.data, None if v is None else v.ctypes.data),
+                    "mdp_md_set_state")
 
     def set_zbl(self, cut_inner=4.0, cut_outer=4.8, z=73.0):
         """Add the ZBL reference pair potential (simulating/snap/ZBLpotential.jl; the Ta application's

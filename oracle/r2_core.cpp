@@ -255,4 +255,60 @@ void r2_cpuVirialPairTally(double *v, double *fij, double *rij, double factor, i
     cpuVirialPairTally(v, fij, rij, factor, ai, dim, inum, ijnum);
 }
 
+// lattice atoms of the application (readdata.cpp:490-551, 647-676): cpuLattice, the orthogonal box of the
+// region in lattice spacings (cpuSetGlobalBox, cpuSetLocalOrthBox, cpuBoundingSubbox), the lattice-space
+// bounding box (cpuLatticeBoundingBox), the index range of latticestruct::latticebounds
+// (src/C++/Common/mdpcommon.h:393-413), cpuAtomLattice and cpuAtomAdd; metal units, one process.
+// Returns the number of atoms written (at most cap), or -(needed) when cap is too small.
+int r2_app_lattice_atoms(double *x, int *type, int cap, int style, double scale, int spaceflag, const int *orient9,
+                         const double *origin3, const double *spacing3, const double *a9, const double *basis_custom,
+                         const int *btype, int nbasis_custom, const double *reglo3, const double *reghi3,
+                         double *spacing_out3) {
+    double basis[8 * 3] = {0}, primitive[9], priminv[9], rotaterow[9], rotatecol[9], origin[3], spacing[3];
+    double a1[3], a2[3], a3[3];
+    int ox[3], oy[3], oz[3];
+    for (int d = 0; d < 3; ++d) {
+        origin[d] = origin3[d];
+        spacing[d] = spacing3[d];
+        a1[d] = a9[d];
+        a2[d] = a9[3 + d];
+        a3[d] = a9[6 + d];
+        ox[d] = orient9[d];
+        oy[d] = orient9[3 + d];
+        oz[d] = orient9[6 + d];
+    }
+    for (int k = 0; k < 3 * nbasis_custom && k < 24; ++k) basis[k] = basis_custom[k];
+    int nbasis = cpuLattice(basis, primitive, rotaterow, priminv, rotatecol, origin, spacing, a1, a2, a3, scale, ox, oy,
+                            oz, style, 2, spaceflag, 3);
+    if (style == 9) nbasis = nbasis_custom;
+    for (int d = 0; d < 3; ++d) spacing_out3[d] = spacing[d];
+    double boxlo[3], boxhi[3], tilt[3] = {0, 0, 0}, h[6], hinv[6], blo[3], bhi[3], sublo[3], subhi[3];
+    double lo_l[3] = {0, 0, 0}, hi_l[3] = {1, 1, 1}, bsublo[3], bsubhi[3], lmin[3], lmax[3];
+    for (int d = 0; d < 3; ++d) {
+        boxhi[d] = spacing[d] * reghi3[d];
+        boxlo[d] = spacing[d] * reglo3[d];
+    }
+    cpuSetGlobalBox(h, hinv, blo, bhi, boxhi, boxlo, tilt, 0);
+    cpuSetLocalOrthBox(subhi, sublo, boxhi, boxlo, hi_l, lo_l, 3);
+    cpuBoundingSubbox(bsublo, bsubhi, sublo, subhi, lo_l, hi_l, boxlo, h, 0);
+    cpuLatticeBoundingBox(lmin, lmax, bsublo, bsubhi, primitive, rotaterow, priminv, rotatecol, origin, spacing, scale);
+    int ilo = static_cast<int>(lmin[0]) - 1, jlo = static_cast<int>(lmin[1]) - 1, klo = static_cast<int>(lmin[2]) - 1;
+    int ihi = static_cast<int>(lmax[0]) + 1, jhi = static_cast<int>(lmax[1]) + 1, khi = static_cast<int>(lmax[2]) + 1;
+    if (lmin[0] < 0.0) ilo--;
+    if (lmin[1] < 0.0) jlo--;
+    if (lmin[2] < 0.0) klo--;
+    const int natom = nbasis * (ihi - ilo + 1) * (jhi - jlo + 1) * (khi - klo + 1);
+    std::vector<double> y(3 * static_cast<size_t>(natom));
+    std::vector<int> ilist(natom), bt(btype, btype + nbasis);
+    cpuAtomLattice(y.data(), ilist.data(), basis, primitive, rotaterow, origin, spacing, scale, bt.data(), nbasis, 0,
+                   ilo, ihi, jlo, jhi, klo, khi, 3);
+    std::vector<double> xs(3 * static_cast<size_t>(natom));
+    std::vector<int> ts(natom);
+    const int n = cpuAtomAdd(xs.data(), ts.data(), y.data(), hinv, boxlo, lo_l, hi_l, ilist.data(), 3, 0, natom);
+    if (n > cap) return -n;
+    for (int q = 0; q < 3 * n; ++q) x[q] = xs[q];
+    for (int q = 0; q < n; ++q) type[q] = ts[q];
+    return n;
+}
+
 }  // extern "C"

@@ -115,14 +115,18 @@ def test_backend3_force_chain_vs_upstream_core(oracle, J, ncell, seed):
     for key in ("U_ui", "U", "Y"):
         for a, b in zip(got[key], ref[key]):
             assert relerr(a, b) < TOL, key
-    for key in ("eatom", "f", "v"):
-        assert relerr(got[key], ref[key]) < TOL, key
-    # the upstream C++ core against the oracle (C route): energies and forces; total virial
     e1, f1, v1 = snapcpp.snappotential_pairs(oracle, s, p)
-    assert relerr(ref["eatom"], e1) < TOL
-    assert relerr(ref["f"].reshape(N, 3), f1) < TOL
-    vt = ref["v"].reshape(6, N).sum(axis=1)
-    assert relerr(vt, v1.sum(axis=0)) < TOL
+    assert relerr(got["eatom"], ref["eatom"]) < TOL and relerr(ref["eatom"], e1) < TOL
+    # upstream cpuSnapComputeFi writes past its 11-element stack arrays at twojmax >= 7 (cpuSnap4.cpp's
+    # mirror loop), so its forces there depend on the stack layout: at twojmax 8 the GPU forces and
+    # virials are checked against the oracle (the C route) only
+    if J < 7:
+        for key in ("f", "v"):
+            assert relerr(got[key], ref[key]) < TOL, key
+        assert relerr(ref["f"].reshape(N, 3), f1) < TOL
+        assert relerr(ref["v"].reshape(6, N).sum(axis=1), v1.sum(axis=0)) < TOL
+    assert relerr(got["f"].reshape(N, 3), f1) < TOL
+    assert relerr(got["v"].reshape(6, N).sum(axis=1), v1.sum(axis=0)) < TOL
 
 
 @pytest.mark.parametrize("J,bnorm,seed", [(4, 0, 4), (8, 0, 5), (6, 1, 6)])
@@ -604,7 +608,7 @@ def test_backend3_add_utot_rij_vs_upstream_core(oracle, J, chem, sw):
     assert relerr(dev["uti"].numpy(), ref["uti"]) < TOL
 
 
-@pytest.mark.parametrize("J,bnorm", [(4, 0), (6, 1)])
+@pytest.mark.parametrize("J,bnorm", [(4, 0), (4, 1), (6, 0), (6, 1), (8, 0), (8, 1)])
 def test_backend3_yi_peratom_two_elements_vs_upstream_core(oracle, J, bnorm):
     """The per-atom-weight SnapComputeYi with two element blocks (each coupling triple and element
     combination reads its own block of betalist) against the upstream core on seeded U and weights."""
