@@ -77,26 +77,178 @@ def periodic_pairs(x, box, rc):
                 ilist=np.arange(n, dtype=np.int32), n=n, ijnum=len(ij))
 
 
-def cpu_baseline(ncell):
-    """The upstream SNAP C library (oracle/_ref, compiled from the upstream source) -- or, where the
-    upstream tree was absent at build time, this project's CPU restatement -- timed on one host core
-    for the first force call (Ui -> AddWself2Ui -> Ei -> Yi -> Fi, snapcpp.jl:386-423 order) of an
-    ncell^3-cell cut of the same lattice."""
-    from mdp_amd import _abi, snapcpp
+def cpu_library():
+    """(kind, path) of the CPU SNAP library timed as the baseline: the upstream C library compiled from
+    the upstream source (oracle/_ref), or this project's CPU restatement where that is absent."""
     ref = os.path.join(ROOT, "oracle", "_ref", "libcpusnap_ref.so")
     port = os.path.join(ROOT, "oracle", "liboracle.so")
-    kind, path = ("reference", ref) if os.path.exists(ref) else ("port", port)
+    return ("reference", ref) if os.path.exists(ref) else ("port", port)
+
+
+def host_cores():
+    """Host threads for the all-cores baseline: this process's CPU affinity, at most 16 (one GPU's share
+    of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def force_chain(lib, s, rij, ai, aj, lo, hi, n):
+    """Ui -> AddWself2Ui -> Ei -> Yi -> Fi (snapcpp.jl:386-423 order) of the CPU library for the centre
+    atoms [lo, hi) of centre-sorted pair arrays (ai ascending): U, E and Y are held for those atoms only,
+    forces and virials land in the full n-atom arrays (cpuSnapComputeFi's aii/ai/aj and inum/anum
+    arguments, cpuSnap.cpp:990-993).  Returns (eatom of the slice, fatom (n,3), vatom (n,6))."""
+    p0, p1 = np.searchsorted(ai, [lo, hi])
+    rij = np.ascontiguousarray(rij[3 * p0:3 * p1])
+    aig = np.ascontiguousarray(ai[p0:p1])
+    ajg = np.ascontiguousarray(aj[p0:p1])
+    ail = np.ascontiguousarray(aig - lo, np.int32)
+    m, P = hi - lo, p1 - p0
+    ones_p, ones_m = np.ones(P, np.int32), np.ones(m, np.int32)
+    iu = s.idxu_max
+    Ur, Ui = np.zeros(m * iu), np.zeros(m * iu)
+    lib.cpuSnapComputeUi(Ur, Ui, s.rootpqarray, rij, s.wjelem, s.radelem, s.rmin0, s.rfac0, s.rcutfac, s.map,
+                         ail, ones_p, ones_p, s.twojmax, iu, m, P, s.switchflag, s.chemflag)
+    lib.cpuAddWself2Ui(Ur, Ui, s.wself, s.idxu_block, ones_m, s.map, s.wselfallflag, s.chemflag, iu,
+                       s.nelements, s.twojmax, m)
+    e = np.zeros(m)
+    lib.cpuSnapComputeEi(e, Ur, Ui, s.cglist, s.bzero, s.coeffelem, np.arange(m, dtype=np.int32), s.map, ones_m,
+                         s.idxb, s.idxcg_block, s.idxu_block, s.twojmax, s.idxb_max, iu, s.nelements,
+                         s.ncoeffall, s.bnormflag, s.bzeroflag, s.wselfallflag, s.chemflag, m)
+    Yr, Yi = np.zeros(m * iu), np.zeros(m * iu)
+    lib.cpuSnapComputeYi(Yr, Yi, Ur, Ui, s.cglist, s.coeffelem, s.map, ones_m, s.idxz, s.idxb_block,
+                         s.idxu_block, s.idxcg_block, s.twojmax, s.idxb_max, iu, s.idxz_max, s.nelements,
+                         s.ncoeffall, s.bnormflag, s.chemflag, m)
+    f, v = np.zeros(3 * n), np.zeros(6 * n)
+    lib.cpuSnapComputeFi(f, v, Yr, Yi, s.rootpqarray, rij, s.wjelem, s.radelem, s.rmin0, s.rfac0, s.rcutfac,
+                         s.map, ail, aig, ajg, ones_p, ones_p, s.twojmax, iu, m, n, P, s.switchflag, s.chemflag)
+    return e, f.reshape(n, 3), v.reshape(n, 6)
+
+
+def _sna(lib, twojmax, coeff):
+    from mdp_amd import snapcpp
+    return snapcpp.initsna(lib, [1, twojmax, RCUTFAC, RFAC0, 0.0, 1, 1, 0, 0, 0, 0], [0.5], [1.0], coeff)
+
+
+def cpu_worker(task_path, rank, nrank):
+    """One process of the all-cores CPU baseline (started by all_cores): loads the task, prepares its
+    share, prints "ready", waits for "go" on stdin, times its share and prints {"elapsed", "units"}."""
+    from mdp_amd import _abi, snapcpp
+    rank, nrank = int(rank), int(nrank)
+    d = np.load(task_path)
+    lib = _abi.bind(str(d["lib"]))
+    task = str(d["task"])
+    if task == "force":
+        # a contiguous slice of centre atoms of one force call
+        n = int(d["n"])
+        s = _sna(lib, int(d["twojmax"]), d["coeff"])
+        lo, hi = rank * n // nrank, (rank + 1) * n // nrank
+        rij, ai, aj = d["rij"], d["ai"], d["aj"]
+        work = lambda: force_chain(lib, s, rij, ai, aj, lo, hi, n)   # noqa: E731
+        units = hi - lo
+    else:
+        # every nrank-th configuration of the fitting batch: the descriptor chain of snapcpp.jl:599-664
+        s = snapcpp.initsna(lib, [1, int(d["twojmax"]), RCUTFAC, RFAC0, 0.0, 1, 1, 0, 0, 0, 0], [0.5], [1.0])
+        off, xs, boxes = d["off"], d["x"], d["box"]
+        mine = range(rank, len(off) - 1, nrank)
+        pairs = [snapcpp.pairs_from_box(xs[off[c]:off[c + 1]], np.ones(off[c + 1] - off[c], np.int32),
+                                        [boxes[c, 0], 0, 0], [0, boxes[c, 1], 0], [0, 0, boxes[c, 2]], [1, 1, 1], s)
+                 for c in mine]
+
+        def work():
+            for p in pairs:
+                snapcpp.snapdescriptors_pairs(lib, s, p)
+        units = int(sum(off[c + 1] - off[c] for c in mine))
+    print("ready", flush=True)
+    if sys.stdin.readline().strip() != "go":
+        sys.exit(3)
+    t0 = time.perf_counter()
+    work()
+    print(json.dumps({"elapsed": time.perf_counter() - t0, "units": units}), flush=True)
+
+
+def fit_cpu_baseline(cfg, every, cores=None):
+    """The CPU library's descriptor chain (snapcpp.jl:599-664: Uij, Utot, Zi, Bi, dB/dr and the three
+    tallies -- the per-configuration body of the loop in Regression.cpp:68-95, neighbour lists built
+    beforehand) over every `every`-th configuration of the batch, one process per host core taking
+    configurations in turn."""
+    kind, path = cpu_library()
+    sample = cfg[::every]
+    off = np.concatenate([[0], np.cumsum([len(x) for x, _ in sample])]).astype(np.int64)
+    nw = cores or host_cores()
+    rate, slowest = all_cores(dict(task="fit", lib=path, twojmax=J, off=off,
+                                   x=np.concatenate([x for x, _ in sample]), box=np.array([b for _, b in sample])), nw)
+    return dict(value=rate, unit="atoms/s", cores=nw, kind=kind,
+                sample=f"{len(sample)} configurations (every {every}th of the batch, {int(off[-1])} atoms): the "
+                       f"descriptor chain per configuration, {nw} processes, slowest {slowest:.2f} s",
+                seconds_per_configuration=slowest * nw / len(sample))
+
+
+def all_cores(arrays, nwork):
+    """Runs cpu_worker in nwork child processes on the same task, starts their timed parts together and
+    returns (units done per second over the slowest process, slowest process's seconds)."""
+    import subprocess
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "task.npz")
+        np.savez(path, **arrays)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", path, str(r),
+                                   str(nwork)], stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True, env=env)
+                 for r in range(nwork)]
+        try:
+            for p in procs:
+                if p.stdout.readline().strip() != "ready":
+                    raise RuntimeError("CPU baseline worker failed during set-up")
+            for p in procs:
+                p.stdin.write("go\n")
+                p.stdin.flush()
+            res = [json.loads(p.stdout.readline()) for p in procs]
+            for p in procs:
+                if p.wait() != 0:
+                    raise RuntimeError("CPU baseline worker failed")
+        finally:
+            for p in procs:
+                if p.poll() is None:
+                    p.kill()
+                    p.wait()
+    slowest = max(r["elapsed"] for r in res)
+    return sum(r["units"] for r in res) / slowest, slowest
+
+
+def cpu_baseline(ncell, full_x=None, full_box=None, cores=None):
+    """The CPU SNAP library (cpu_library) timed for the force call (Ui -> AddWself2Ui -> Ei -> Yi -> Fi,
+    snapcpp.jl:386-423 order) of the workload's lattice: on one core for an ncell^3-cell cut, and on all
+    host cores (one process per core over disjoint slices of centre atoms) for the full system
+    (full_x, full_box) when given."""
+    from mdp_amd import _abi
+    kind, path = cpu_library()
     lib = _abi.bind(path)
     x, box, _, coeff = workload(ncell)
-    param = [1, J, RCUTFAC, RFAC0, 0.0, 1, 1, 0, 0, 0, 0]
-    s = snapcpp.initsna(lib, param, [0.5], [1.0], coeff)
+    s = _sna(lib, J, coeff)
     p = periodic_pairs(x, box, RCUTFAC)
     t0 = time.perf_counter()
-    snapcpp.snappotential_pairs(lib, s, p)
+    force_chain(lib, s, p["rij"], p["ai"], p["aj"], 0, len(x), len(x))
     dt = time.perf_counter() - t0
-    return dict(value=len(x) / dt, unit="atom-steps/s", cores=1, kind=kind,
-                sample=f"first E/F/V force call of {len(x)} atoms ({ncell}^3 BCC cells, same a/sigma/twojmax/"
-                       f"coefficients, {p['ijnum']} pairs), single thread, {dt:.2f} s")
+    one = len(x) / dt
+    out = dict(value=one, unit="atom-steps/s", cores=1, kind=kind,
+               sample=f"first E/F/V force call of {len(x)} atoms ({ncell}^3 BCC cells, same a/sigma/twojmax/"
+                      f"coefficients, {p['ijnum']} pairs), single thread, {dt:.2f} s",
+               single_core_value=one,
+               build=("upstream cpuSnap.cpp" if kind == "reference" else "oracle/snap_oracle.c") +
+                     " built by oracle/Makefile at -O2 -fno-fast-math -ffp-contract=off")
+    if full_x is not None:
+        nw = cores or host_cores()
+        q = periodic_pairs(full_x, full_box, RCUTFAC)
+        rate, slowest = all_cores(dict(task="force", lib=path, n=len(full_x), twojmax=J, coeff=coeff,
+                                       rij=q["rij"], ai=q["ai"], aj=q["aj"]), nw)
+        out.update(value=rate, cores=nw, all_cores_value=rate,
+                   sample=f"first E/F/V force call of the full {len(full_x)}-atom system ({q['ijnum']} pairs) "
+                          f"split over {nw} processes by centre atom, slowest {slowest:.2f} s; single thread on "
+                          f"{len(x)} atoms ({ncell}^3 cells): {one:.1f} atom-steps/s in {dt:.2f} s")
+    return out
 
 
 def bench_fit(a):
@@ -165,6 +317,10 @@ def bench_fit(a):
         "cpu_baseline": None,
     }
     fit.close()
+    if not a.no_cpu_baseline:
+        t0 = time.perf_counter()
+        out["cpu_baseline"] = fit_cpu_baseline(cfg, 4, a.cpu_cores or None)
+        log(f"cpu baseline: {out['cpu_baseline']['value']:.1f} atoms/s ({time.perf_counter() - t0:.1f} s)")
     print(json.dumps(out), flush=True)
 
 
@@ -259,9 +415,102 @@ def bench_surrogate(a):
     print(json.dumps(out), flush=True)
 
 
+def cpu_md(lib, s, x, box, v, steps, dt=DT, skin=SKIN, every=EVERY):
+    """Velocity-Verlet NVE on the host with the CPU library's force call (force_chain) at every step and a
+    minimum-image pair list of radius rcut + skin rebuilt every `every` steps (the upstream step loop,
+    Integration.cpp:17-156, on one core).  Returns (final x, final v, seconds)."""
+    from scipy.spatial import cKDTree
+    from mdp_amd.lattice import FTM2V_METAL
+    n = len(x)
+    rc = s.rcutmax
+    x, v = x.copy(), v.copy()
+    dtf = 0.5 * dt * FTM2V_METAL / MASS
+    ij = None
+
+    def forces():
+        d = x[ij[:, 1]] - x[ij[:, 0]]
+        d -= box * np.round(d / box)
+        keep = np.einsum("ij,ij->i", d, d) < rc * rc
+        return force_chain(lib, s, np.ascontiguousarray(d[keep].ravel()), ij[keep, 0], ij[keep, 1], 0, n, n)[1]
+
+    t0 = time.perf_counter()
+    for step in range(steps + 1):
+        if step % every == 0:
+            xw = np.mod(x, box)
+            q = cKDTree(xw, boxsize=box).query_pairs(rc + skin, output_type="ndarray")
+            q = np.concatenate([q, q[:, ::-1]]).astype(np.int32)
+            ij = q[np.lexsort((q[:, 1], q[:, 0]))]
+        f = forces()
+        if step:
+            v += dtf * f
+        if step < steps:
+            v += dtf * f
+            x += dt * v
+    return x, v, time.perf_counter() - t0
+
+
+def bench_config0(a):
+    """The 2,000-atom plain-SNAP MD case (BCC Ta 10^3 cells, sigma = 0.05 A, twojmax = 6, 100 NVE steps)
+    on the GPU engine, beside the same 100 steps driven by the CPU library on one host core."""
+    import mdp_amd
+    from mdp_amd import _abi, lattice
+    from mdp_amd.md import SnapMD
+
+    mdp_amd.library()
+    J0, steps_cpu = 6, 100
+    x, box = lattice.crystal("bcc", 10, A0, 0.05, 20241200)
+    v = lattice.maxwell_boltzmann(len(x), MASS, TEMP, seed=20241201)
+    coeff = lattice.snap_coefficients(30, beta0=-11.7, sigma=0.05, seed=20241202)
+    n = len(x)
+
+    def engine():
+        return SnapMD(x, box, np.ones(n, np.int32), J0, coeff, [0.5], [1.0], [MASS], rcutfac=RCUTFAC,
+                      rfac0=RFAC0, v=v, dt=DT, skin=SKIN, rebuild_every=EVERY)
+    md = engine()
+    md.compute()
+    md.run(a.warmup)
+    md.close()
+    md = engine()
+    t0 = time.perf_counter()
+    md.compute()
+    md.run(a.steps)
+    elapsed = time.perf_counter() - t0
+    st = md.state()
+    md.close()
+    log(f"GPU: {n} atoms, {a.steps} steps (incl. the first force call) in {1e3 * elapsed:.2f} ms")
+    out = {
+        "metric": "atom-steps/s, plain SNAP MD, 2J=6, 2,000 atoms", "value": n * a.steps / elapsed,
+        "unit": "atom-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * elapsed / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded BCC lattice, displacements, velocities and SNAP coefficients)",
+        "config": {"workload": "plain FP64 SNAP NVE MD, BCC Ta 10^3 cells (2000 atoms), sigma 0.05 A, "
+                               "twojmax=6, dt=0.5 fs, skin 0.3 A rebuilt every 10 steps", "atoms": n,
+                   "twojmax": J0, "parallelism": "1 GPU"},
+        "roofline": None, "cpu_baseline": None,
+    }
+    if not a.no_cpu_baseline:
+        kind, path = cpu_library()
+        lib = _abi.bind(path)
+        s = _sna(lib, J0, coeff)
+        xc, vc, t_cpu = cpu_md(lib, s, x, box, v, steps_cpu)
+        log(f"CPU ({kind}, 1 core): {steps_cpu} steps in {t_cpu:.2f} s")
+        if a.steps == steps_cpu:
+            d = np.mod(st["x"] - xc + 0.5 * box, box) - 0.5 * box
+            log(f"GPU vs CPU after {steps_cpu} steps: max |dx| {np.max(np.abs(d)):.3e} A, "
+                f"max |dv| {np.max(np.abs(st['v'] - vc)):.3e} A/ps")
+        out["cpu_baseline"] = dict(value=n * steps_cpu / t_cpu, unit="atom-steps/s", cores=1, kind=kind,
+                                   sample=f"the whole {steps_cpu}-step trajectory ({n} atoms), host velocity "
+                                          f"Verlet with the library's force call at every step, single thread, "
+                                          f"{t_cpu:.2f} s", seconds_for_100_steps=t_cpu * 100 / steps_cpu)
+    print(json.dumps(out), flush=True)
+
+
 def main():
+    if len(sys.argv) == 5 and sys.argv[1] == "--cpu-worker":
+        cpu_worker(*sys.argv[2:])
+        return
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--workload", choices=("md", "fit", "surrogate"), default="md")
+    ap.add_argument("--workload", choices=("md", "fit", "surrogate", "config0"), default="md")
     ap.add_argument("--train-configs", type=int, default=4000)
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--sur-ncell", type=int, default=80)
@@ -272,6 +521,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--ncell", type=int, default=NCELL)
     ap.add_argument("--cpu-ncell", type=int, default=24, help="cells per side of the CPU baseline sample")
+    ap.add_argument("--cpu-cores", type=int, default=0, help="processes of the all-cores CPU baseline (0: host_cores())")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -285,6 +535,9 @@ def main():
         return
     if a.workload == "surrogate":
         bench_surrogate(a)
+        return
+    if a.workload == "config0":
+        bench_config0(a)
         return
 
     import mdp_amd
@@ -352,7 +605,7 @@ def main():
     }
     if not a.no_cpu_baseline:
         t0 = time.perf_counter()
-        out["cpu_baseline"] = cpu_baseline(a.cpu_ncell)
+        out["cpu_baseline"] = cpu_baseline(a.cpu_ncell, x, box, a.cpu_cores or None)
         log(f"cpu baseline: {out['cpu_baseline']['value']:.1f} atom-steps/s ({time.perf_counter() - t0:.1f} s)")
     md.close()
     print(json.dumps(out), flush=True)

@@ -244,6 +244,17 @@ int mdp_md_set_zbl(mdp_md *md, double cut_inner, double cut_outer, double z);
  * boltz as the application's units (metal: 1.0364269e-4, 8.617343e-5).  v[3*natoms] is overwritten. */
 int mdp_velocity_create(const double *x, const int *type, const double *mass, int natoms, double t_desired,
                         int seed0, int dist_flag, int loop_flag, double mvv2e, double boltz, double *v);
+/* cpuVelocityCreate (src/C++/Core/cpuVelocityIntegration.cpp:54-135; host code) with the generator
+ * state held by the caller: seed[0] (Park-Miller state), save[0] and second[0] (a waiting normal
+ * deviate), read at entry and written back.  loop_flag 0 ("all"): every atom of the system draws in
+ * global-ID order, ID k = 1..natoms, and map[k] (natoms + 1 entries) names its local index, which takes
+ * the draw when 0 <= map[k] < nlocal; 1 ("local"): seed[0] += mpi_rank, 100 discarded draws, then the
+ * nlocal atoms in order; 2 ("geom", dim = 3): each atom's stream re-seeded from seed0 and its
+ * coordinates.  Three uniform - 0.5 (dist_flag 0) or normal (1) draws per atom, the first dim of them
+ * added as v += draw / sqrt(mass[type[i]]); v[dim*nlocal] is zeroed first unless sum_flag. */
+int mdp_velocity_create_loop(const double *x, double *v, const double *mass, double *second, int *seed, int *save,
+                             const int *map, const int *type, int seed0, int sum_flag, int dist_flag, int loop_flag,
+                             int dim, int mpi_rank, int nlocal, int natoms);
 /* out[43]: eta_mass[10], eta[10], eta_dot[10], eta_dotdot[10], the temperature after the last
  * thermostat half-step, its target, and the step counter. */
 int mdp_md_nvt_state(mdp_md *md, double *out);

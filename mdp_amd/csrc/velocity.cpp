@@ -70,7 +70,73 @@ struct Stream {
     }
 };
 
+// cpuVelocityCreate's draws (cpuVelocityIntegration.cpp:54-135) on a caller-held generator state:
+// g.s = seed[0], the waiting normal deviate = (save[0], second[0]); written back by the caller
+void draw(Stream& g, int dist_flag, double r[3]) {
+    for (int d = 0; d < 3; ++d) r[d] = dist_flag == 0 ? g.uniform() - 0.5 : g.normal();
+}
+
 }  // namespace
+
+extern "C" int mdp_velocity_create_loop(const double* x, double* v, const double* mass, double* second, int* seed,
+                                        int* save, const int* map, const int* type, int seed0, int sum_flag,
+                                        int dist_flag, int loop_flag, int dim, int mpi_rank, int nlocal, int natoms) {
+    mdp::clear_error();
+    try {
+        if (!v || !mass || !second || !seed || !save || !type) throw mdp::Error("mdp_velocity_create_loop: null argument");
+        if (dim != 2 && dim != 3) throw mdp::Error("mdp_velocity_create_loop: dim must be 2 or 3");
+        if (nlocal < 0 || natoms < 0) throw mdp::Error("mdp_velocity_create_loop: negative atom count");
+        if (loop_flag < 0 || loop_flag > 2) throw mdp::Error("mdp_velocity_create_loop: loop_flag must be 0, 1 or 2");
+        if (loop_flag == 0 && !map) throw mdp::Error("mdp_velocity_create_loop: loop mode 0 needs map");
+        if (loop_flag == 2 && (!x || dim != 3))
+            throw mdp::Error("mdp_velocity_create_loop: loop mode 2 needs x and dim = 3 (each atom's stream is seeded "
+                             "from three coordinates)");
+        if (sum_flag == 0)
+            for (long i = 0; i < long(dim) * nlocal; ++i) v[i] = 0.0;
+        Stream g;
+        g.s = seed[0];
+        g.have = false;            // every loop mode starts with save = 0
+        g.second = second[0];
+        auto add = [&](int i, const double r[3]) {
+            const double factor = 1.0 / std::sqrt(mass[type[i]]);
+            v[i * dim + 0] += r[0] * factor;
+            v[i * dim + 1] += r[1] * factor;
+            if (dim == 3) v[i * dim + 2] += r[2] * factor;
+        };
+        double r[3];
+        if (loop_flag == 0) {
+            // every atom of the system draws in global-ID order (IDs 1..natoms); map[id] is its local
+            // index, and only the atoms held here (0 <= map[id] < nlocal) take their draw
+            for (int id = 1; id <= natoms; ++id) {
+                draw(g, dist_flag, r);
+                const int m = map[id];
+                if (m >= 0 && m < nlocal) add(m, r);
+            }
+        } else if (loop_flag == 1) {
+            // each rank's own stream: seed + rank, 100 discarded draws, then its atoms in local order
+            g.s = g.s + mpi_rank;
+            for (int i = 0; i < 100; ++i) g.uniform();
+            for (int i = 0; i < nlocal; ++i) {
+                draw(g, dist_flag, r);
+                add(i, r);
+            }
+        } else {
+            g.s = 1;
+            for (int i = 0; i < nlocal; ++i) {
+                g.reseed(seed0, x + 3 * i);
+                draw(g, dist_flag, r);
+                add(i, r);
+            }
+        }
+        seed[0] = g.s;
+        save[0] = g.have ? 1 : 0;
+        second[0] = g.second;
+        return 0;
+    } catch (const std::exception& e) {
+        mdp::set_error(e.what());
+        return 1;
+    }
+}
 
 extern "C" int mdp_velocity_create(const double* x, const int* type, const double* mass, int natoms, double t_desired,
                                    int seed0, int dist_flag, int loop_flag, double mvv2e, double boltz, double* v) {
@@ -79,8 +145,9 @@ extern "C" int mdp_velocity_create(const double* x, const int* type, const doubl
         if (!x || !type || !mass || !v) throw mdp::Error("mdp_velocity_create: null argument");
         if (natoms < 2) throw mdp::Error("mdp_velocity_create: needs at least two atoms");
         if (loop_flag != 2)
-            throw mdp::Error("mdp_velocity_create: only loop mode 2 (geom) is supported; the other modes draw from "
-                             "the application's running generator state");
+            throw mdp::Error("mdp_velocity_create: only loop mode 2 (geom) is supported here: loop modes 0 (all) and "
+                             "1 (local) draw from the application's generator state, which the upstream app never "
+                             "sets; mdp_velocity_create_loop takes that state explicitly");
         if (!(t_desired > 0.0)) throw mdp::Error("mdp_velocity_create: the temperature must be positive");
         Stream g;
         for (int i = 0; i < natoms; ++i) {

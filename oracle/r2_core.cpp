@@ -206,6 +206,14 @@ void r2_cpuNVTFinalIntegrate(double *x, double *v, double *f, double *mass, doub
                          eta_mass_flag, biasflag, mtchain, nc_tchain, dim, inum);
 }
 
+// the upstream velocity draws themselves, all loop modes (cpuVelocityIntegration.cpp:54-135)
+void r2_cpuVelocityCreate(double *x, double *v, double *mass, double *second, int *seed, int *save, int *map,
+                          int *type, int seed0, int sum_flag, int dist_flag, int loop_flag, int dim, int mpiRank,
+                          int nlocal, int natoms) {
+    cpuVelocityCreate(x, v, mass, second, seed, save, map, type, seed0, sum_flag, dist_flag, loop_flag, dim, mpiRank,
+                      nlocal, natoms);
+}
+
 // initial velocities: the call sequence of readdata.cpp:609-640 on the upstream functions
 void r2_velocity_create(double *x, double *v, double *mass, int *type, int natoms, double t_desired, int seed0,
                         int dist_flag, int loop_flag, double mvv2e, double boltz) {

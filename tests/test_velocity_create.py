@@ -58,3 +58,50 @@ def test_properties():
     assert not np.allclose(velocity_create(x, t, m, 450.0, 12346), v)
     with pytest.raises(RuntimeError, match="geom"):
         velocity_create(x, t, m, 450.0, 12345, loop_flag=0)
+
+
+def _draw_args(natoms, nlocal, seed, loop_flag, map_kind):
+    x, t, m = _system(11, True)
+    x, t = x[:nlocal], t[:nlocal]
+    if map_kind == "tags":        # global ID k -> local index k - 1
+        mp = np.concatenate([[-1], np.arange(natoms)]).astype(np.int32)
+    else:                         # the app's ilist padded by one entry that names no local atom
+        mp = np.concatenate([np.arange(nlocal), [-1] * (natoms + 1 - nlocal)]).astype(np.int32)
+    return np.ascontiguousarray(x), t, m, mp
+
+
+@pytest.mark.parametrize("loop_flag,dist,sum_flag,map_kind,rank", [
+    (0, 0, 0, "tags", 0), (0, 1, 0, "tags", 0), (0, 1, 1, "ilist", 0),
+    (1, 0, 0, "tags", 0), (1, 1, 0, "tags", 3), (1, 1, 1, "tags", 1),
+    (2, 0, 0, "tags", 0), (2, 1, 1, "tags", 0)])
+def test_every_loop_mode_bit_exact_with_upstream(loop_flag, dist, sum_flag, map_kind, rank):
+    """cpuVelocityCreate's three loop modes (cpuVelocityIntegration.cpp:54-135) with a caller-held
+    generator state, against the compiled upstream function: velocities and the final state agree bit for
+    bit.  "all" runs with the LAMMPS tag map and with the app's index list."""
+    if not os.path.exists(R2LIB):
+        pytest.skip("upstream C++ core checker not built (oracle/_ref)")
+    import mdp_amd
+    from mdp_amd import _abi  # noqa: F401
+    lib = ctypes.CDLL(R2LIB)
+    ours = mdp_amd.library().cdll.mdp_velocity_create_loop
+    P, I = ctypes.c_void_p, ctypes.c_int
+    args_t = [P] * 8 + [I] * 8
+    lib.r2_cpuVelocityCreate.argtypes = args_t
+    lib.r2_cpuVelocityCreate.restype = None
+    ours.argtypes = args_t
+    ours.restype = I
+    natoms = 432
+    x, t, m, mp = _draw_args(natoms, natoms, 0, loop_flag, map_kind)
+    v0 = np.random.default_rng(2).normal(size=x.shape)
+    res = []
+    for f in (lib.r2_cpuVelocityCreate, ours):
+        v = v0.copy()
+        second, seed, save = np.array([0.25]), np.array([987654], np.int32), np.array([0], np.int32)
+        rc = f(x.ctypes.data, v.ctypes.data, m.ctypes.data, second.ctypes.data, seed.ctypes.data, save.ctypes.data,
+               mp.ctypes.data, t.ctypes.data, 4928459, sum_flag, dist, loop_flag, 3, rank, natoms, natoms)
+        assert rc in (None, 0)
+        res.append((v, second.copy(), seed.copy(), save.copy()))
+    for a, b in zip(res[0], res[1]):
+        assert np.array_equal(a, b)
+    if map_kind == "ilist" and loop_flag == 0:
+        assert np.array_equal(res[1][0][0], v0[0])   # atom 0 takes no draw through the app's index list
