@@ -332,8 +332,10 @@ def bench_surrogate(a):
     from mdp_amd.md import SnapMD
 
     mdp_amd.library()
+    pot = S.tungsten(os.path.join(ROOT, "tests", "golden", "snapW.coeff")) if a.potential == "W" else S.seeded_tantalum()
+    log(f"potential: {pot.name} (a0 {pot.a0} A, rcutfac {pot.rcutfac}, mass {pot.mass})")
     t0 = time.perf_counter()
-    data, dropped = S.make_training_set(nconfig=a.train_configs, log=log)
+    data, dropped = S.make_training_set(nconfig=a.train_configs, log=log, pot=pot)
     log(f"plain-SNAP training data: {len(data)} configurations in {time.perf_counter() - t0:.1f} s "
         f"({dropped} samples dropped: a pair under 2 A or a force above 200 eV/A)")
     ntest = max(1, len(data) // 10)
@@ -342,7 +344,8 @@ def bench_surrogate(a):
     train = [data[i] for i in perm[ntest:]]
     test = [data[i] for i in perm[:ntest]]
     t0 = time.perf_counter()
-    ds_tr, ds_te = S.DeviceSet(train, log=log), S.DeviceSet(test)
+    ds_tr = S.DeviceSet(train, log=log, rcutfac=pot.rcutfac, rfac0=pot.rfac0)
+    ds_te = S.DeviceSet(test, rcutfac=pot.rcutfac, rfac0=pot.rfac0)
     log(f"twojmax=4 descriptors: {ds_tr.N + ds_te.N} atoms, {ds_tr.P + ds_te.P} pairs in {time.perf_counter() - t0:.1f} s")
     mu, sg, es, esh = S.affine_from(ds_tr)
     m0 = S.Model.init(mu, sg, es, esh, seed=1)
@@ -355,38 +358,63 @@ def bench_surrogate(a):
         f"(linear twojmax=4 SNAP {f_lin:.4f}), energy RMSE {e_te:.4f} eV/atom (linear {e_lin:.4f})")
 
     # production system: 80^3 BCC cells, 0.5% vacancies, 600 K, dt = 1 fs
-    x, box = lattice.crystal("bcc", a.sur_ncell, A0, SIGMA, SEED + 10)
+    x, box = lattice.crystal("bcc", a.sur_ncell, pot.a0, SIGMA, SEED + 10)
     keep = np.ones(len(x), bool)
     keep[np.random.default_rng(SEED + 11).choice(len(x), int(round(0.005 * len(x))), replace=False)] = False
     x = np.ascontiguousarray(x[keep])
     n = len(x)
-    v = lattice.maxwell_boltzmann(n, MASS, 600.0, seed=SEED + 12)
-    coeff = S.plain_coefficients()
-    md = SnapMD(x, box, np.ones(n, np.int32), J, coeff, [0.5], [1.0], [MASS], rcutfac=RCUTFAC, rfac0=RFAC0, v=v,
-                dt=0.001, skin=SKIN, rebuild_every=EVERY)
+    v = lattice.maxwell_boltzmann(n, pot.mass, 600.0, seed=SEED + 12)
+
+    def engine():
+        return SnapMD(x, box, np.ones(n, np.int32), J, pot.coeff, [0.5], [1.0], [pot.mass], rcutfac=pot.rcutfac,
+                      rfac0=pot.rfac0, v=v, dt=0.001, skin=SKIN, rebuild_every=EVERY)
+
+    def run_chunks(md, steps, chunk=50):
+        """Steps in chunks; a run that fails (e.g. the neighbour build reports a collapsed system) stops
+        there.  Returns (steps done, seconds, error or None)."""
+        done, t0 = 0, time.perf_counter()
+        try:
+            while done < steps:
+                k = min(chunk, steps - done)
+                md.run(k)
+                done += k
+        except RuntimeError as e:
+            return done, time.perf_counter() - t0, str(e)
+        return done, time.perf_counter() - t0, None
+
+    # each phase starts from the same state
+    md = engine()
     md.compute()
     md.run(a.warmup)
     md.set_timing(True)
     t0 = time.perf_counter()
     md.run(a.plain_steps)
     t_plain = (time.perf_counter() - t0) / a.plain_steps
-    md.set_timing(False)
+    md.close()
+    md = engine()
     md.set_surrogate(model, audit_every=0)
     md.compute()
-    md.run(a.warmup)
-    t0 = time.perf_counter()
-    md.run(a.noaudit_steps)
-    t_noaudit = (time.perf_counter() - t0) / a.noaudit_steps
+    w_done, _, na_err = run_chunks(md, a.warmup)
+    na_done, na_s = 0, 0.0
+    if not na_err:
+        na_done, na_s, na_err = run_chunks(md, a.noaudit_steps)
+    t_noaudit = na_s / max(na_done, 1)
+    md.close()
+    if na_err:
+        log(f"surrogate without audits stopped after {w_done + na_done} steps: {na_err}")
+    md = engine()
     md.set_surrogate(model, audit_every=50, audit_tol=0.05)
     md.compute()
     md.set_timing(True)
-    t0 = time.perf_counter()
-    md.run(a.steps)
-    elapsed = time.perf_counter() - t0
+    au_done, elapsed, au_err = run_chunks(md, a.steps)
+    if au_err:
+        log(f"surrogate with audits stopped after {au_done} steps: {au_err}")
     timers = md.timers()
     aud = md.audits()
     st = md.stats()
     md.close()
+    steps_timed = max(au_done, 1)
+    elapsed = elapsed * a.steps / steps_timed if au_done else elapsed
     value = n * a.steps / elapsed
     log(f"{n} atoms: plain {1e3 * t_plain:.2f} ms/step, surrogate without audits {1e3 * t_noaudit:.2f} ms/step, "
         f"with audits {1e3 * elapsed / a.steps:.2f} ms/step; audits {aud}")
@@ -399,12 +427,16 @@ def bench_surrogate(a):
         "ms_per_step": 1e3 * elapsed / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "bf16 MLP / f64 descriptors, forces and integration",
         "data": "synthetic (plain-SNAP MD samples for training; seeded BCC lattice with vacancies for MD)",
-        "config": {"workload": f"surrogate NVE MD, BCC Ta {a.sur_ncell}^3 cells with 0.5% vacancies ({n} atoms), "
-                               "600 K, dt=1 fs, audit every 50 steps", "atoms": n, "parallelism": "1 GPU",
+        "config": {"workload": f"surrogate NVE MD, BCC {a.sur_ncell}^3 cells with 0.5% vacancies ({n} atoms), "
+                               "600 K, dt=1 fs, audit every 50 steps", "potential": pot.name, "atoms": n,
+                   "parallelism": "1 GPU",
                    "train_configs": len(train), "train_steps": a.train_steps, "samples_dropped": dropped},
-        "surrogate": {"atom_steps_per_s_without_audits": n / t_noaudit, "plain_gpu_atom_steps_per_s": n / t_plain,
+        "surrogate": {"atom_steps_per_s_without_audits": n / t_noaudit if na_done else None,
+                      "plain_gpu_atom_steps_per_s": n / t_plain,
+                      "steps_without_audits": w_done + na_done, "error_without_audits": na_err,
+                      "steps_with_audits": au_done, "error_with_audits": au_err,
                       "speedup_vs_plain_with_audits": (n * a.steps / elapsed) / (n / t_plain),
-                      "speedup_vs_plain_without_audits": t_plain / t_noaudit,
+                      "speedup_vs_plain_without_audits": t_plain / t_noaudit if na_done else None,
                       "heldout_force_rmse": f_te, "heldout_energy_rmse_per_atom": e_te,
                       "linear_j4_force_rmse": f_lin, "linear_j4_energy_rmse_per_atom": e_lin,
                       "audit_force_rmse_mean": aud["mean_rmse"], "audit_force_rmse_max": aud["max_rmse"],
@@ -514,6 +546,8 @@ def main():
     ap.add_argument("--train-configs", type=int, default=4000)
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--sur-ncell", type=int, default=80)
+    ap.add_argument("--potential", choices=("seeded", "W"), default="seeded",
+                    help="surrogate workload: the seeded Ta coefficients or the upstream tungsten potential")
     ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--noaudit-steps", type=int, default=500)
     ap.add_argument("--gpus", type=int, default=1)

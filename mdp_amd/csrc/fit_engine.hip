@@ -40,7 +40,7 @@ struct mdp_fit {
     std::vector<double> boxes_h, tilts_h;   // tilts_h empty: every cell orthogonal
     DevBuf x, type, cfg_of, cfg_a0, cfg_L, cfg_T, cfg_p0, rc2, pcnt, poff, scan_tmp, flags, counter;
     DevBuf pi, pj, pti, ptj, rij, Ur, Ui, zt, blist, dblist, bzero, t_lo, t_b, t_z, t_w, zorder, t_wtab, t_hlo, t_hb, t_hz,
-        t_hw;
+        t_hw, t_mz, t_mw, t_mcomb;
     int dbdr_variant = 0;
     bool zb_lds = false;     // Z + B from the LDS-staged half U (twojmax 4, 6, 8)
     long cap = 0;            // pair capacity of the pair and dB/dr buffers
@@ -191,6 +191,17 @@ struct mdp_fit {
             terms.h_z = t_hz.as<int>();
             terms.h_w = t_hw.as<double>();
         }
+        if (fit_mfma_supported(J)) {   // packed term rows of the matrix-core dB/dr (variant 6)
+            std::vector<int> mz, mcomb;
+            std::vector<double> mw;
+            if (!fit_mfma_tables(J, bnorm, mz, mw, mcomb)) throw Error("fit: matrix-core dB/dr term packing failed");
+            put(t_mz, mz.data(), sizeof(int) * mz.size());
+            put(t_mw, mw.data(), sizeof(double) * mw.size());
+            put(t_mcomb, mcomb.data(), sizeof(int) * mcomb.size());
+            terms.m_z = t_mz.as<int>();
+            terms.m_w = t_mw.as<double>();
+            terms.m_comb = t_mcomb.as<int>();
+        }
         terms.wtab = t_wtab.as<double>();
         terms.lo = t_lo.as<int>();
         terms.b = t_b.as<int>();
@@ -280,9 +291,14 @@ struct mdp_fit {
                 break;
             default:
                 launch_fill_zero_int(counter.as<int>(), 2, st);
-                launch_fit_dbdr(s, zt.as<double>(), terms, rij.as<double>(), pti.as<int>(), ptj.as<int>(),
-                                poff.as<int>(), cfg_of.as<int>(), cfg_p0.as<int>(), dblist.as<double>(),
-                                counter.as<int>(), N, dbdr_groups, st, dbdr_variant);
+                if (dbdr_variant == 6 && terms.m_z)
+                    launch_fit_dbdr_mfma(s, zt.as<double>(), terms.m_z, terms.m_w, terms.m_comb, rij.as<double>(),
+                                         pti.as<int>(), ptj.as<int>(), poff.as<int>(), cfg_of.as<int>(),
+                                         cfg_p0.as<int>(), dblist.as<double>(), counter.as<int>(), N, dbdr_groups, st);
+                else
+                    launch_fit_dbdr(s, zt.as<double>(), terms, rij.as<double>(), pti.as<int>(), ptj.as<int>(),
+                                    poff.as<int>(), cfg_of.as<int>(), cfg_p0.as<int>(), dblist.as<double>(),
+                                    counter.as<int>(), N, dbdr_groups, st, dbdr_variant);
                 break;
         }
     }
@@ -332,6 +348,7 @@ struct mdp_fit {
             dbdr_variant = (v == "1" && j468) ? 1 : ((v == "2" && (J == 6 || J == 8)) ? 2 : ((v == "3" && j468) ? 3 : 0));
             if (v == "4" && j468) dbdr_variant = 4;
             if (v == "5" && j468) dbdr_variant = 5;
+            if (v == "6" && j468) dbdr_variant = 6;
         }
         if ((env && std::string(env) == "0") || fv) return;
         int props_cu = 256;
@@ -342,10 +359,10 @@ struct mdp_fit {
         const int cand[] = {props_cu * 2, props_cu * 4, props_cu * 8, props_cu * 12};
         // dB/dr variants: 0 table walk, 1 unrolled terms, 2 table walk at two waves per SIMD, 3 variant 0
         // with the term loop pipelined, 4 generator form with the bispectrum indices split over two waves,
-        // 5 variant 4 with the atom's Z block staged in LDS
+        // 5 variant 4 with the atom's Z block staged in LDS, 6 the contraction on the FP64 matrix cores
         std::vector<int> vars = {0};
-        if (J == 4) vars = {0, 1, 3, 4, 5};
-        if (J == 6 || J == 8) vars = {0, 1, 2, 3, 4, 5};
+        if (J == 4) vars = {0, 1, 3, 4, 5, 6};
+        if (J == 6 || J == 8) vars = {0, 1, 2, 3, 4, 5, 6};
         double best = 1e30;
         int pick = dbdr_groups, pickv = 0;
         std::string js = "{\"fit_dbdr\": {";

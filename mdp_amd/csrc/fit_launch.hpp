@@ -50,6 +50,10 @@ struct FitTerms {
     const int* h_z = nullptr;
     const double* h_w = nullptr;
     int hb = 0;
+    // the rows of the matrix-core variant (fit_mfma_tables), null where twojmax has no such kernel
+    const int* m_z = nullptr;
+    const double* m_w = nullptr;
+    const int* m_comb = nullptr;
 };
 // Host term table split into two halves of the bispectrum index range (FitTerms::h_*): returns hb.
 int fit_term_table_halves(int J, int bnorm, std::vector<int>& lo, std::vector<int>& b, std::vector<int>& z,
@@ -67,6 +71,17 @@ void fit_term_table(int J, int bnorm, std::vector<int>& lo, std::vector<int>& b,
 void launch_fit_dbdr(const SnapDev& s, const double* zt, const FitTerms& tt, const double* rij, const int* pti,
                      const int* ptj, const int* poff, const int* cfg_of, const int* cfg_p0, double* dblist,
                      int* counter, int N, int groups, hipStream_t st, int variant = 0);
+
+// dB/dr on the FP64 matrix cores (k_fit_mfma.hip): twojmax 4, 6 and 8.  fit_mfma_tables writes the
+// packed term rows (per tile T, layer l and tile row m: Z block offset mz and weight mw at
+// (T*(J+1) + l)*16 + m) and, for every bispectrum index t, its tile rows comb[3t..3t+2] (-1 padded);
+// launch_fit_dbdr_mfma runs the persistent kernel (counter[0] must be 0) with the same output layout as
+// launch_fit_dbdr.
+bool fit_mfma_supported(int J);
+bool fit_mfma_tables(int J, int bnorm, std::vector<int>& mz, std::vector<double>& mw, std::vector<int>& comb);
+bool launch_fit_dbdr_mfma(const SnapDev& s, const double* zt, const int* mz, const double* mw, const int* comb,
+                          const double* rij, const int* pti, const int* ptj, const int* poff, const int* cfg_of,
+                          const int* cfg_p0, double* dblist, int* counter, int N, int groups, hipStream_t st);
 
 // ---- linear fit over the batched descriptors (k_fitlsq.hip) ----
 // one chunk of rows of a weighted least-squares group: a(r, k) = A[abase + r*rs + k*cs], target

@@ -66,6 +66,7 @@ def _bind():
     c.mdp_md_nvt_state.argtypes = [vp, ctypes.c_void_p]
     c.mdp_velocity_create.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_double, ctypes.c_double, vp]
+    c.mdp_velocity_create_loop.argtypes = [vp] * 8 + [ctypes.c_int] * 8
     _bound = True
     return c
 
@@ -255,4 +256,43 @@ def velocity_create(x, types, mass, t_desired, seed0, dist_flag=0, loop_flag=2, 
                                int(seed0), int(dist_flag), int(loop_flag), mvv2e, boltz, v.ctypes.data)
     if rc != 0:
         raise RuntimeError(f"mdp_velocity_create: {library().cdll.mdp_snap_last_error().decode()}")
+    return v
+
+
+class RandomState:
+    """The upstream app's generator state (commonstruct seed/save/second): Park-Miller state and a
+    waiting normal deviate."""
+
+    def __init__(self, seed, save=0, second=0.0):
+        self.seed = np.array([seed], np.int32)
+        self.save = np.array([save], np.int32)
+        self.second = np.array([second], np.float64)
+
+
+def velocity_draws(x, v, types, mass, state, seed0=0, sum_flag=0, dist_flag=0, loop_flag=0, mpi_rank=0, natoms=None,
+                   idmap=None):
+    """cpuVelocityCreate (cpuVelocityIntegration.cpp:54-135) in any loop mode, on ``v`` (N, 3) in place:
+    loop_flag 0 ("all": every atom of the system draws in global-ID order; idmap[k] for k = 1..natoms
+    names the local index, default k - 1), 1 ("local": seed + mpi_rank, 100 discarded draws, then local
+    order) or 2 ("geom": each atom's stream seeded from seed0 and its coordinates).  ``state`` (a
+    RandomState) is advanced as the upstream generator would be."""
+    c = _bind()
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    if not (isinstance(v, np.ndarray) and v.dtype == np.float64 and v.flags.c_contiguous and v.shape == x.shape):
+        raise ValueError("v must be a C-contiguous float64 array shaped like x")
+    n = len(x)
+    natoms = n if natoms is None else int(natoms)
+    types = np.ascontiguousarray(types, np.int32).reshape(n)
+    mass = np.ascontiguousarray(mass, np.float64).ravel()
+    if idmap is None:
+        idmap = np.concatenate([[-1], np.arange(natoms)])
+    idmap = np.ascontiguousarray(idmap, np.int32)
+    if len(idmap) < natoms + 1:
+        raise ValueError("idmap needs natoms + 1 entries")
+    rc = c.mdp_velocity_create_loop(x.ctypes.data, v.ctypes.data, mass.ctypes.data, state.second.ctypes.data,
+                                    state.seed.ctypes.data, state.save.ctypes.data, idmap.ctypes.data,
+                                    types.ctypes.data, int(seed0), int(sum_flag), int(dist_flag), int(loop_flag), 3,
+                                    int(mpi_rank), n, natoms)
+    if rc != 0:
+        raise RuntimeError(f"mdp_velocity_create_loop: {library().cdll.mdp_snap_last_error().decode()}")
     return v

@@ -67,6 +67,35 @@ def plain_coefficients(seed=20241202 + 2):
     return lattice.snap_coefficients(55, beta0=-11.7, sigma=0.05, seed=seed)
 
 
+class Potential:
+    """A single-element plain twojmax=8 SNAP potential and the BCC lattice the surrogate samples around:
+    coeff = [beta0, beta_1..beta_55], lattice constant a0 (A), rcutfac, rfac0, mass (amu)."""
+
+    def __init__(self, name, coeff, a0, rcutfac, rfac0, mass):
+        self.name, self.coeff, self.a0 = name, np.ascontiguousarray(coeff, np.float64), float(a0)
+        self.rcutfac, self.rfac0, self.mass = float(rcutfac), float(rfac0), float(mass)
+
+
+def seeded_tantalum():
+    """The headline workload's seeded coefficients on BCC Ta (a = 3.316 A)."""
+    return Potential("seeded coefficients, BCC Ta", plain_coefficients(), 3.316, RCUTFAC, RFAC0, MASS)
+
+
+def load_coefficients(path):
+    """A SNAP coefficient file as the upstream examples ship it: whitespace-separated numbers."""
+    with open(path) as fh:
+        return np.array([float(v) for v in fh.read().split()])
+
+
+def tungsten(path):
+    """The upstream tungsten potential (simulating/snap/snapW.m:34-53: BCC a = 3.1803 A, rcutfac 4.8123,
+    rfac0 0.99363, twojmax 8, mass 183.84) with the coefficients of snapW.coeff at ``path``."""
+    c = load_coefficients(path)
+    if len(c) != 56:
+        raise ValueError(f"{path}: expected 56 coefficients, found {len(c)}")
+    return Potential("snapW.coeff, BCC W", c, 3.1803, 4.8123, 0.99363, 183.84)
+
+
 class Model:
     """Parameters (FP32, the kernels' flat layout) and the descriptor/energy affine map."""
 
@@ -100,8 +129,9 @@ class Model:
 
 
 def make_training_set(nconfig=4000, ncell=5, tmin=300.0, tmax=3000.0, every=10, equil=10, per_run=10, seed=777,
-                      fmax=200.0, rmin=2.0, log=None):
-    """Configurations of 250 atoms (BCC 5^3 cells, a = 3.316 A, and random packings at the BCC density)
+                      fmax=200.0, rmin=2.0, log=None, pot=None):
+    """Configurations of 250 atoms (BCC 5^3 cells at the potential's lattice constant -- the seeded Ta
+    workload by default, a = 3.316 A -- and random packings at the BCC density)
     sampled from short plain twojmax=8 SNAP NVE runs started at temperatures spread over [tmin, tmax];
     labels are the plain SNAP total energy and forces.  Short runs keep the samples near their
     starting structures: with untrained (random) coefficients long hot runs drift into collapsed
@@ -110,7 +140,8 @@ def make_training_set(nconfig=4000, ncell=5, tmin=300.0, tmax=3000.0, every=10, 
     import os
     from scipy.spatial import cKDTree
     from .md import SnapMD
-    coeff = plain_coefficients()
+    pot = pot or seeded_tantalum()
+    coeff, a0, mass = pot.coeff, pot.a0, pot.mass
     out = []
     dropped = 0
     old = os.environ.get("MDP_AUTOTUNE")
@@ -122,12 +153,12 @@ def make_training_set(nconfig=4000, ncell=5, tmin=300.0, tmax=3000.0, every=10, 
             kind = "bcc" if run % 2 == 0 else "liquid"
             T = tmin + (tmax - tmin) * ((run // 2) % max(1, nruns_est // 2)) / max(1, nruns_est // 2 - 1)
             if kind == "bcc":
-                x, box = lattice.crystal("bcc", ncell, 3.316, 0.05, seed + run)
+                x, box = lattice.crystal("bcc", ncell, a0, 0.05, seed + run)
             else:
-                x, box = lattice.random_packing(2 * ncell ** 3, 2 / 3.316 ** 3, 2.2, seed=seed + run)
-            v = lattice.maxwell_boltzmann(len(x), MASS, T, seed=seed + 100000 + run)
-            md = SnapMD(x, box, np.ones(len(x), np.int32), J_PLAIN, coeff, [0.5], [1.0], [MASS], v=v, dt=0.0005,
-                        skin=0.3, rebuild_every=10)
+                x, box = lattice.random_packing(2 * ncell ** 3, 2 / a0 ** 3, 2.2, seed=seed + run)
+            v = lattice.maxwell_boltzmann(len(x), mass, T, seed=seed + 100000 + run)
+            md = SnapMD(x, box, np.ones(len(x), np.int32), J_PLAIN, coeff, [0.5], [1.0], [mass], rcutfac=pot.rcutfac,
+                        rfac0=pot.rfac0, v=v, dt=0.0005, skin=0.3, rebuild_every=10)
             md.compute()
             md.run(equil)
             for k in range(per_run):
@@ -156,7 +187,7 @@ def make_training_set(nconfig=4000, ncell=5, tmin=300.0, tmax=3000.0, every=10, 
 class DeviceSet:
     """Descriptors, pair chain-rule arrays and labels of a list of configurations, on the GPU."""
 
-    def __init__(self, configs, chunk=500, log=None):
+    def __init__(self, configs, chunk=500, log=None, rcutfac=RCUTFAC, rfac0=RFAC0):
         from .fit import SnapFit
         C = len(configs)
         nat = np.array([len(c["x"]) for c in configs])
@@ -167,7 +198,7 @@ class DeviceSet:
         p_tot = 0
         for c0 in range(0, C, chunk):
             sub = configs[c0:c0 + chunk]
-            fit = SnapFit([(c["x"], c["box"]) for c in sub], J_SUR, rcutfac=RCUTFAC, rfac0=RFAC0)
+            fit = SnapFit([(c["x"], c["box"]) for c in sub], J_SUR, rcutfac=rcutfac, rfac0=rfac0)
             fit.compute()
             b, db = fit.raw()
             ai, aj, _ = fit.pairs()

@@ -72,3 +72,10 @@ def test_fit_cpu_baseline_small(every):
     cfg = [(x, box) for x, box, _ in lattice.fitting_batch(nconfig=6, nmin=16, nmax=40)]
     out = bench.fit_cpu_baseline(cfg, every, cores=2)
     assert out["value"] > 0 and out["cores"] == 2
+
+
+def test_tungsten_coefficients_load():
+    from mdp_amd import surrogate as S
+    pot = S.tungsten(os.path.join(os.path.dirname(__file__), "golden", "snapW.coeff"))
+    assert pot.coeff.shape == (56,) and np.all(np.isfinite(pot.coeff))
+    assert (pot.a0, pot.rcutfac, pot.mass) == (3.1803, 4.8123, 183.84)

@@ -99,7 +99,7 @@ def _check_config(oracle, fit, c, cfgs, b, db, ai, aj, rij, J, bnorm=0, bzero=1)
 
 @pytest.mark.parametrize("J,variant", [(4, "0"), (6, "0"), (8, "0"), (4, "1"), (6, "1"), (8, "1"), (8, "2"),
                                        (4, "3"), (8, "3"), (4, "4"), (6, "4"), (8, "4"), (4, "5"), (6, "5"),
-                                       (8, "5")])
+                                       (8, "5"), (4, "6"), (6, "6"), (8, "6")])
 def test_small_batch_vs_oracle(oracle, monkeypatch, J, variant):
     from mdp_amd.fit import SnapFit
     monkeypatch.setenv("MDP_FIT_DBDR_VARIANT", variant)   # each dB/dr kernel variant in turn
@@ -120,18 +120,21 @@ def test_dbdr_variants_agree(monkeypatch, J):
     from mdp_amd.fit import SnapFit
     cfg = _small_batch(9)
     out = {}
-    for v in ("0", "3", "4", "5"):
+    for v in ("0", "3", "4", "5", "6"):
         monkeypatch.setenv("MDP_FIT_DBDR_VARIANT", v)
         fit = SnapFit(cfg, J)
         fit.compute()
         out[v] = fit.raw()
-    for v in ("3", "4", "5"):
+    for v in ("3", "4", "5", "6"):
         assert np.array_equal(out[v][0], out["0"][0]), v
         assert relerr(out[v][1], out["0"][1]) < 1e-13, v
 
 
-def test_bnorm_and_no_bzero(oracle):
+@pytest.mark.parametrize("variant", [None, "6"])
+def test_bnorm_and_no_bzero(oracle, monkeypatch, variant):
     from mdp_amd.fit import SnapFit
+    if variant:
+        monkeypatch.setenv("MDP_FIT_DBDR_VARIANT", variant)
     cfg = _small_batch(11)[:3]
     fit = SnapFit(cfg, 6, bnormflag=1, bzeroflag=0)
     fit.compute()

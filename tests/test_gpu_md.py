@@ -183,3 +183,26 @@ def test_twojmax8_energy_error_scales_with_dt2(oracle):
         md.close()
     ratio = dev[0] / dev[1]
     assert 3.0 < ratio < 5.0, (dev, ratio)
+
+
+def test_tungsten_potential_first_step(oracle):
+    """The upstream tungsten potential (snapW.coeff with snapW.m:34-53: BCC a = 3.1803 A, rcutfac 4.8123,
+    twojmax 8): first-step energies, forces and virials against the oracle; the per-atom energy of the
+    SNAP part alone is about -15.7 eV on this lattice (oracle)."""
+    import os
+    from mdp_amd import surrogate as S
+    from mdp_amd.md import SnapMD
+    pot = S.tungsten(os.path.join(os.path.dirname(__file__), "golden", "snapW.coeff"))
+    x, box = lattice.crystal("bcc", 5, pot.a0, 0.05, 77)
+    md = SnapMD(x, box, np.ones(len(x), np.int32), 8, pot.coeff, [0.5], [1.0], [pot.mass], rcutfac=pot.rcutfac,
+                rfac0=pot.rfac0)
+    md.compute()
+    st = md.state()
+    s = snapcpp.initsna(oracle, [1, 8, pot.rcutfac, pot.rfac0, 0, 1, 1, 0, 0, 0, 0], [0.5], [1.0], pot.coeff)
+    B = np.diag(box)
+    p = snapcpp.pairs_from_box(x, np.ones(len(x), np.int32), B[0], B[1], B[2], [1, 1, 1], s)
+    e, f, v = snapcpp.snappotential_pairs(oracle, s, p)
+    assert relerr(st["eatom"], e) < TOL
+    assert relerr(st["f"], f) < TOL
+    assert relerr(st["vatom"], v) < TOL
+    assert -17.0 < np.mean(e) < -14.0

@@ -105,3 +105,27 @@ def test_every_loop_mode_bit_exact_with_upstream(loop_flag, dist, sum_flag, map_
         assert np.array_equal(a, b)
     if map_kind == "ilist" and loop_flag == 0:
         assert np.array_equal(res[1][0][0], v0[0])   # atom 0 takes no draw through the app's index list
+
+
+def test_loop_modes_python_wrapper_and_state():
+    """mdp_amd.md.velocity_draws: "all" over the whole system equals "local" on rank 0 only when the local
+    stream's 100 discarded draws are taken out, the generator state advances by three draws per atom
+    (uniform), and a bad loop mode is an error."""
+    from mdp_amd.md import RandomState, velocity_draws
+    x, t, m = _system(13)
+    n = len(x)
+    st_all = RandomState(12345)
+    v_all = velocity_draws(x, np.zeros_like(x), t, m, st_all, loop_flag=0)
+    st_ref = RandomState(12345)
+    # "local" discards 100 draws first: start it 100 draws earlier by hand
+    st_loc = RandomState(12345)
+    velocity_draws(x[:1], np.zeros((1, 3)), t[:1], m, st_loc, loop_flag=1)   # 100 + 3 draws
+    assert st_loc.seed[0] != st_ref.seed[0]
+    assert np.all(np.abs(v_all * np.sqrt(m[t][:, None])) <= 0.5)
+    # three uniform draws per atom: running "all" on 2n atoms continues the same stream
+    st2 = RandomState(12345)
+    x2 = np.concatenate([x, x + 100.0])
+    v2 = velocity_draws(x2, np.zeros_like(x2), np.concatenate([t, t]), m, st2, loop_flag=0)
+    assert np.array_equal(v2[:n], v_all)
+    with pytest.raises(RuntimeError, match="loop_flag"):
+        velocity_draws(x, np.zeros_like(x), t, m, RandomState(1), loop_flag=3)
