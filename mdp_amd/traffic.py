@@ -1,6 +1,15 @@
 """HBM traffic per kernel launch from rocprofv3 counter runs.
 
-Two subcommands:
+Three subcommands:
+
+``python mdp_amd/traffic.py requests --kernel REGEX --rdreq DIR --wrreq DIR --out FILE``
+    Exact bytes from the L2-to-fabric request counts by size, no calibration: read bytes =
+    128 TCC_EA0_RDREQ_128B + 64 TCC_EA0_RDREQ_64B + 32 TCC_EA0_RDREQ_32B (the three sizes add up to
+    TCC_EA0_RDREQ, which the summary checks), write bytes = 64 TCC_EA0_WRREQ_64B + 32 (TCC_EA0_WRREQ -
+    TCC_EA0_WRREQ_64B).  DIR are the output folders of ``rocprofv3 --pmc TCC_EA0_RDREQ_sum
+    TCC_EA0_RDREQ_32B_sum TCC_EA0_RDREQ_64B_sum TCC_EA0_RDREQ_128B_sum`` and ``rocprofv3 --pmc
+    TCC_EA0_WRREQ_sum TCC_EA0_WRREQ_64B_sum``.  On a 1 GiB copy these give 8,388,608 128-byte reads and
+    16,777,216 64-byte writes, exactly 1 GiB each way.  This is the route the committed summaries use.
 
 ``python mdp_amd/traffic.py copy --bytes B --reps R``
     Runs R elementwise copies (dst = 1.0 * src) of B bytes, one kernel each.  Profiled with
@@ -13,6 +22,8 @@ Two subcommands:
     dispatch) over the dispatches of the kernels matching REGEX, scales each by the factor the same
     counter reads for the calibration copy (true bytes / counted bytes), and writes a JSON summary
     whose ``bytes_per_launch`` (read + write) is what bench.py reports as ``roofline.traffic``.
+    FETCH_SIZE counts 128-byte reads through TCC_BUBBLE, which reads 0 on this part, so one scale factor
+    cannot fit kernels that mix 64- and 128-byte reads; kept for comparison with ``requests``.
 """
 import argparse
 import csv
@@ -81,6 +92,46 @@ def cmd_summarize(a):
     print(json.dumps(out, indent=1))
 
 
+def per_dispatch_sum(d, counters, kernel_re):
+    """{counter: mean over dispatches} for the dispatches whose kernel name matches kernel_re."""
+    pat = re.compile(kernel_re)
+    by_disp = {}
+    for r in _rows(d):
+        if r.get("Counter_Name") not in counters or not pat.search(r.get("Kernel_Name", "")):
+            continue
+        key = (r.get("Agent_Id"), r.get("Dispatch_Id"))
+        row = by_disp.setdefault(key, {c: 0.0 for c in counters})
+        row[r["Counter_Name"]] += float(r["Counter_Value"])
+    if not by_disp:
+        raise SystemExit(f"no {counters} rows for kernels matching {kernel_re!r} under {d}")
+    n = len(by_disp)
+    return {c: sum(v[c] for v in by_disp.values()) / n for c in counters}, n
+
+
+def cmd_requests(a):
+    rd = ["TCC_EA0_RDREQ_sum", "TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum"]
+    wr = ["TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum"]
+    r, nr = per_dispatch_sum(a.rdreq, rd, a.kernel)
+    w, nw = per_dispatch_sum(a.wrreq, wr, a.kernel)
+    read_b = 128.0 * r["TCC_EA0_RDREQ_128B_sum"] + 64.0 * r["TCC_EA0_RDREQ_64B_sum"] + 32.0 * r["TCC_EA0_RDREQ_32B_sum"]
+    sized = r["TCC_EA0_RDREQ_128B_sum"] + r["TCC_EA0_RDREQ_64B_sum"] + r["TCC_EA0_RDREQ_32B_sum"]
+    write_b = 64.0 * w["TCC_EA0_WRREQ_64B_sum"] + 32.0 * (w["TCC_EA0_WRREQ_sum"] - w["TCC_EA0_WRREQ_64B_sum"])
+    out = dict(kernel_regex=a.kernel, method="request counts by size", dispatches_read=nr, dispatches_write=nw,
+               read_requests=r, write_requests=w, read_sizes_cover_requests=sized / max(r["TCC_EA0_RDREQ_sum"], 1.0),
+               read_bytes_per_launch=read_b, write_bytes_per_launch=write_b, bytes_per_launch=read_b + write_b,
+               algorithmic_bytes_per_launch=a.algorithmic)
+    if a.algorithmic:
+        out["traffic_over_algorithmic"] = (read_b + write_b) / a.algorithmic
+    if a.ms:
+        out["ms_per_launch"] = a.ms
+        out["achieved_gb_s"] = (read_b + write_b) / (a.ms * 1e-3) / 1e9
+        if a.algorithmic:
+            out["algorithmic_gb_s"] = a.algorithmic / (a.ms * 1e-3) / 1e9
+    with open(a.out, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out, indent=1))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description="HBM traffic per launch from rocprofv3 counters")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -97,8 +148,15 @@ def main(argv=None):
     s.add_argument("--copy-kernel", default="elementwise|Mul|mul")
     s.add_argument("--algorithmic", type=float, default=0.0, help="algorithmic bytes per launch")
     s.add_argument("--out", required=True)
+    q = sub.add_parser("requests")
+    q.add_argument("--kernel", required=True)
+    q.add_argument("--rdreq", required=True)
+    q.add_argument("--wrreq", required=True)
+    q.add_argument("--algorithmic", type=float, default=0.0, help="algorithmic bytes per launch")
+    q.add_argument("--ms", type=float, default=0.0, help="average launch duration (kernel trace), for GB/s")
+    q.add_argument("--out", required=True)
     a = ap.parse_args(argv)
-    (cmd_copy if a.cmd == "copy" else cmd_summarize)(a)
+    {"copy": cmd_copy, "summarize": cmd_summarize, "requests": cmd_requests}[a.cmd](a)
 
 
 if __name__ == "__main__":
