@@ -74,7 +74,8 @@ void launch_fit_dbdr(const SnapDev& s, const double* zt, const FitTerms& tt, con
 
 // dB/dr on the FP64 matrix cores (k_fit_mfma.hip): twojmax 4, 6 and 8.  fit_mfma_tables writes the
 // packed term rows (per tile T, layer l and tile row m: Z block offset mz and weight mw at
-// (T*(J+1) + l)*16 + m) and, for every bispectrum index t, its tile rows comb[3t..3t+2] (-1 padded);
+// (T*(J+1) + l)*16 + m; mw has one more entry, 0.0) and, for every bispectrum index t, its tile rows
+// comb[3t..3t+2] (-1 padded);
 // launch_fit_dbdr_mfma runs the persistent kernel (counter[0] must be 0) with the same output layout as
 // launch_fit_dbdr.
 bool fit_mfma_supported(int J);
