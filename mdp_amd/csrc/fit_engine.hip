@@ -39,7 +39,7 @@ struct mdp_fit {
     std::vector<int> a0_h;   // [C+1]
     std::vector<double> boxes_h, tilts_h;   // tilts_h empty: every cell orthogonal
     DevBuf x, type, cfg_of, cfg_a0, cfg_L, cfg_T, cfg_p0, rc2, pcnt, poff, scan_tmp, flags, counter;
-    DevBuf pi, pj, pti, ptj, rij, Ur, Ui, zt, blist, dblist, bzero, t_lo, t_b, t_z, t_w, zorder, t_wtab, t_hlo, t_hb, t_hz,
+    DevBuf pi, pj, pti, ptj, rij, Ur, Ui, zt, blist, bpart, dblist, bzero, t_lo, t_b, t_z, t_w, zorder, t_wtab, t_hlo, t_hb, t_hz,
         t_hw, t_mz, t_mw, t_mcomb;
     int dbdr_variant = 0;
     bool zb_lds = false;     // Z + B from the LDS-staged half U (twojmax 4, 6, 8)
@@ -218,6 +218,7 @@ struct mdp_fit {
         Ui.ensure(sizeof(double) * n * I.idxu_max);
         zt.ensure(sizeof(double) * 2 * n * I.idxz_max);
         blist.ensure(sizeof(double) * n * I.idxb_max);
+        if (zb_lds) bpart.ensure(sizeof(double) * n * static_cast<size_t>(fit_zb_partials(J)));
         MDP_HIP(hipStreamSynchronize(st));   // host staging vectors go out of scope
     }
 
@@ -283,8 +284,8 @@ struct mdp_fit {
             case 2:
                 if (zb_lds)
                     launch_fit_zb_lds(s, Ur.as<double>(), Ui.as<double>(), zt.as<double>(), blist.as<double>(),
-                                      bzero.as<double>(), cfg_of.as<int>(), cfg_a0.as<int>(), zorder.as<int>(), N, bnorm,
-                                      bzeroflag, st);
+                                      bpart.as<double>(), bzero.as<double>(), cfg_of.as<int>(), cfg_a0.as<int>(),
+                                      zorder.as<int>(), N, bnorm, bzeroflag, st);
                 else
                     launch_fit_zb(s, Ur.as<double>(), Ui.as<double>(), zt.as<double>(), blist.as<double>(),
                                   bzero.as<double>(), cfg_of.as<int>(), cfg_a0.as<int>(), N, bnorm, bzeroflag, st);

@@ -3,7 +3,8 @@
 // twojmax = 8) from HBM once, then its waves take whole coupling triples, heaviest first, from a shared
 // counter.  A triple's rows mb <= j/2 are produced with every U row read from LDS (mirrored rows through
 // the symmetry of U), written atom-major to zt, and for bispectrum triples (j >= j1) contracted with
-// U(j) into B (cpuSnap.cpp:1899-2072 semantics, as k_fit_zb).
+// U(j) into that row's share of B, which a second kernel sums per atom in a fixed order
+// (cpuSnap.cpp:1899-2072 semantics, as k_fit_zb).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -51,16 +52,39 @@ __host__ __device__ constexpr int zb_n_b(int J) {
             for (int c = a - b; c <= cx_min(J, a + b); c += 2) n += (c >= a);
     return n;
 }
+// partial sums of B, one per half row (mb = 0..j/2) of each bispectrum triple in idxb order: the first
+// slot of (j1, j2, j), and the number of slots
+__host__ __device__ constexpr int zb_bp_first(int J, int j1, int j2, int j) {
+    int n = 0;
+    for (int a = 0; a <= J; ++a)
+        for (int b = 0; b <= a; ++b)
+            for (int c = a - b; c <= cx_min(J, a + b); c += 2) {
+                if (c < a) continue;
+                if (a == j1 && b == j2 && c == j) return n;
+                n += c / 2 + 1;
+            }
+    return -1;
+}
+__host__ __device__ constexpr int zb_n_bp(int J) {
+    int n = 0;
+    for (int a = 0; a <= J; ++a)
+        for (int b = 0; b <= a; ++b)
+            for (int c = a - b; c <= cx_min(J, a + b); c += 2) n += (c >= a) ? c / 2 + 1 : 0;
+    return n;
+}
 
 }  // namespace
 
 // Z rows (JO, mb) of every coupling (J1, J2) -> JO for this lane's atom, from the LDS half U, stored
 // atom-major.  The middle row of an even layer stops at ma = mb: the dB/dr contraction and B read no
-// entry beyond it.
+// entry beyond it.  For bispectrum triples (JO >= J1) the row's share of B, sum' U*(JO).Z over its
+// entries (the middle entry of an even layer halved; cpuSnap.cpp:1971-2072), goes to its own slot of
+// bpart ([slot][N]), summed per atom in fixed order by k_fit_bsum.
 template <int J, int JO>
 __device__ __forceinline__ void z_rows_lds(int mb, int atom, bool live, const double* __restrict__ lr,
                                            const double* __restrict__ li, int lane, const double* __restrict__ cg,
-                                           double scale, double* __restrict__ zt) {
+                                           double scale, double* __restrict__ zt, double* __restrict__ bpart,
+                                           size_t n) {
     constexpr int IZ = zb_n_z(J);
     const int malim = (2 * mb == JO) ? mb : JO;
     static_for<0, J + 1>([&](auto J1c) {
@@ -82,6 +106,18 @@ __device__ __forceinline__ void z_rows_lds(int mb, int atom, bool live, const do
                             zo[2 * ma + 1] = zi[ma];
                         }
                 }
+                if constexpr (JO >= J1) {
+                    constexpr int HB = h_block(JO), BP = zb_bp_first(J, J1, J2, JO);
+                    double sb = 0.0;
+#pragma unroll
+                    for (int ma = 0; ma <= JO; ++ma)
+                        if (ma <= malim) {
+                            const double w = (2 * mb < JO || ma < mb) ? 1.0 : 0.5;
+                            const int e = HB + (JO + 1) * mb + ma;
+                            sb += w * (lr[e * 64 + lane] * zr[ma] + li[e * 64 + lane] * zi[ma]);
+                        }
+                    if (live) bpart[static_cast<size_t>(BP + mb) * n + atom] = sb;
+                }
                 // couplings are independent: keep the scheduler from interleaving them (register pressure)
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -92,7 +128,8 @@ __device__ __forceinline__ void z_rows_lds(int mb, int atom, bool live, const do
 template <int J, int W>
 __global__ __launch_bounds__(64 * W) void k_fit_z_lds(const double* __restrict__ cg, const double* __restrict__ Ur,
                                                       const double* __restrict__ Ui, double* __restrict__ zt,
-                                                      const int* __restrict__ order, int N, int bnorm) {
+                                                      double* __restrict__ bpart, const int* __restrict__ order, int N,
+                                                      int bnorm) {
     constexpr int HE = n_half_elems(J), NR = n_half_rows(J);
     __shared__ double lr[HE * 64], li[HE * 64];
     __shared__ int next_row;
@@ -115,24 +152,23 @@ __global__ __launch_bounds__(64 * W) void k_fit_z_lds(const double* __restrict__
         const int jo = half_row_j(row), mb = half_row_mb(row);
         static_for<0, J + 1>([&](auto JOc) {
             constexpr int JO = decltype(JOc)::value;
-            if (jo == JO) z_rows_lds<J, JO>(mb, atom, live, lr, li, lane, cg, bnorm ? 1.0 / (JO + 1) : 1.0, zt);
+            if (jo == JO)
+                z_rows_lds<J, JO>(mb, atom, live, lr, li, lane, cg, bnorm ? 1.0 / (JO + 1) : 1.0, zt, bpart, n);
         });
     }
 }
 
-// B = 2 sum' U*(j).Z(j1,j2,j) - bzero[j] from the half U ([e][N]) and the atom-major Z; lane = atom
+// B = 2 sum_mb (row shares in bpart) - bzero[j], rows in ascending mb; lane = atom
 template <int J>
-__global__ __launch_bounds__(256) void k_fit_b(const double* __restrict__ Ur, const double* __restrict__ Ui,
-                                               const double* __restrict__ zt, double* __restrict__ blist,
-                                               const double* __restrict__ bzero, const int* __restrict__ cfg_of,
-                                               const int* __restrict__ cfg_a0, int N, int bzeroflag) {
-    constexpr int IZ = zb_n_z(J), IB = zb_n_b(J);
+__global__ __launch_bounds__(256) void k_fit_bsum(const double* __restrict__ bpart, double* __restrict__ blist,
+                                                  const double* __restrict__ bzero, const int* __restrict__ cfg_of,
+                                                  const int* __restrict__ cfg_a0, int N, int bzeroflag) {
+    constexpr int IB = zb_n_b(J);
     const int atom = blockIdx.x * blockDim.x + threadIdx.x;
     if (atom >= N) return;
     const size_t n = static_cast<size_t>(N);
     const int c = cfg_of[atom];
     const int a0 = cfg_a0[c], nc = cfg_a0[c + 1] - a0;
-    const double* __restrict__ z = zt + static_cast<size_t>(atom) * IZ * 2;
     double* __restrict__ bo = blist + static_cast<size_t>(a0) * IB + (atom - a0);
     static_for<0, J + 1>([&](auto J1c) {
         constexpr int J1 = decltype(J1c)::value;
@@ -141,19 +177,10 @@ __global__ __launch_bounds__(256) void k_fit_b(const double* __restrict__ Ur, co
             static_for<J1, J + 1>([&](auto JOc) {
                 constexpr int JO = decltype(JOc)::value;
                 if constexpr (coupled(J1, J2, JO)) {
-                    constexpr int ZB = zb_z_block(J, J1, J2, JO), jjb = zb_b_index(J, J1, J2, JO);
+                    constexpr int BP = zb_bp_first(J, J1, J2, JO), jjb = zb_b_index(J, J1, J2, JO);
                     double sum = 0.0;
 #pragma unroll
-                    for (int mb = 0; 2 * mb <= JO; ++mb) {
-                        const int malim = (2 * mb == JO) ? mb : JO;
-#pragma unroll
-                        for (int ma = 0; ma <= malim; ++ma) {
-                            const double w = (2 * mb < JO || ma < mb) ? 1.0 : 0.5;
-                            const size_t u = static_cast<size_t>(h_block(JO) + (JO + 1) * mb + ma) * n + atom;
-                            const int zo = 2 * (ZB + (JO + 1) * mb + ma);
-                            sum += w * (Ur[u] * z[zo] + Ui[u] * z[zo + 1]);
-                        }
-                    }
+                    for (int mb = 0; 2 * mb <= JO; ++mb) sum += bpart[static_cast<size_t>(BP + mb) * n + atom];
                     double b = 2.0 * sum;
                     if (bzeroflag) b -= bzero[JO];
                     bo[static_cast<size_t>(nc) * jjb] = b;
@@ -163,16 +190,19 @@ __global__ __launch_bounds__(256) void k_fit_b(const double* __restrict__ Ur, co
     });
 }
 
+int fit_zb_partials(int J) { return (J == 4 || J == 6 || J == 8) ? zb_n_bp(J) : 0; }
+
 bool launch_fit_zb_lds(const SnapDev& s, const double* Uhr, const double* Uhi, double* zt, double* blist,
-                       const double* bzero, const int* cfg_of, const int* cfg_a0, const int* row_order, int N,
-                       int bnorm, int bzeroflag, hipStream_t st) {
+                       double* bpart, const double* bzero, const int* cfg_of, const int* cfg_a0,
+                       const int* row_order, int N, int bnorm, int bzeroflag, hipStream_t st) {
     if (N <= 0) return true;
     const dim3 grid((N + 63) / 64), block(64 * 8), gb((N + 255) / 256), bb(256);
     switch (s.J) {
 #define ZB_CASE(JV)                                                                                            \
     case JV:                                                                                                   \
-        hipLaunchKernelGGL((k_fit_z_lds<JV, 8>), grid, block, 0, st, s.cg, Uhr, Uhi, zt, row_order, N, bnorm);    \
-        hipLaunchKernelGGL((k_fit_b<JV>), gb, bb, 0, st, Uhr, Uhi, zt, blist, bzero, cfg_of, cfg_a0, N, bzeroflag); \
+        hipLaunchKernelGGL((k_fit_z_lds<JV, 8>), grid, block, 0, st, s.cg, Uhr, Uhi, zt, bpart, row_order, N,     \
+                           bnorm);                                                                             \
+        hipLaunchKernelGGL((k_fit_bsum<JV>), gb, bb, 0, st, bpart, blist, bzero, cfg_of, cfg_a0, N, bzeroflag);   \
         return true;
         ZB_CASE(4)
         ZB_CASE(6)
