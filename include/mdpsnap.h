@@ -193,7 +193,8 @@ int mdp_md_neighbors(mdp_md *md, int *off, int *nbj, int *nbshift, long *nentrie
 int mdp_md_stats(mdp_md *md, long *out6);
 
 /* Per-kernel timing over the steps run since it was enabled: with on = 1 the engine records a
- * hipEvent pair around each kernel class on its stream.  names are fixed: see mdp_md_timer_name. */
+ * hipEvent pair around each kernel class on its stream, with on = 2 around compute_ui, compute_yi and
+ * compute_dedr only (0 switches it off).  names are fixed: see mdp_md_timer_name. */
 int mdp_md_set_timing(mdp_md *md, int on);
 int mdp_md_timers(mdp_md *md, double *ms_total, long *launches, int n);
 const char *mdp_md_timer_name(int k);

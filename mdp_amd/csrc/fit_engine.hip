@@ -43,6 +43,7 @@ struct mdp_fit {
         t_hw, t_mz, t_mw, t_mcomb;
     int dbdr_variant = 0;
     bool zb_lds = false;     // Z + B from the LDS-staged half U (twojmax 4, 6, 8)
+    int zb_waves = 8;        // waves per Z workgroup (autotuned: 8 or 4)
     long cap = 0;            // pair capacity of the pair and dB/dr buffers
     long npairs = 0;         // pairs of the last compute
     FitTerms terms;
@@ -53,6 +54,7 @@ struct mdp_fit {
     int dbdr_groups = 1024;
     bool tuned = false;
     std::string tuning_json = "{}";
+    std::string zb_json;
     double stage_ms[4] = {0, 0, 0, 0};
 
     ~mdp_fit() {
@@ -285,7 +287,7 @@ struct mdp_fit {
                 if (zb_lds)
                     launch_fit_zb_lds(s, Ur.as<double>(), Ui.as<double>(), zt.as<double>(), blist.as<double>(),
                                       bpart.as<double>(), bzero.as<double>(), cfg_of.as<int>(), cfg_a0.as<int>(),
-                                      zorder.as<int>(), N, bnorm, bzeroflag, st);
+                                      zorder.as<int>(), N, bnorm, bzeroflag, st, zb_waves);
                 else
                     launch_fit_zb(s, Ur.as<double>(), Ui.as<double>(), zt.as<double>(), blist.as<double>(),
                                   bzero.as<double>(), cfg_of.as<int>(), cfg_a0.as<int>(), N, bnorm, bzeroflag, st);
@@ -351,7 +353,20 @@ struct mdp_fit {
             if (v == "5" && j468) dbdr_variant = 5;
             if (v == "6" && j468) dbdr_variant = 6;
         }
-        if ((env && std::string(env) == "0") || fv) return;
+        if (!(env && std::string(env) == "0") && zb_lds) {   // Z kernel: two waves per SIMD or one
+            double tz[2];
+            for (int k = 0; k < 2; ++k) {
+                zb_waves = k ? 4 : 8;
+                stage(2);   // warm-up
+                tz[k] = time_stage(2, 2);
+            }
+            zb_waves = tz[1] < tz[0] ? 4 : 8;
+            zb_json = "\"fit_zb\": {\"8\": " + std::to_string(tz[0]) + ", \"4\": " + std::to_string(tz[1]) + "}, ";
+        }
+        if ((env && std::string(env) == "0") || fv) {
+            tuning_json = "{" + zb_json + "\"chosen\": {\"fit_zb_waves\": " + std::to_string(zb_waves) + "}}";
+            return;
+        }
         int props_cu = 256;
         hipDeviceProp_t prop;
         int dev = 0;
@@ -366,7 +381,7 @@ struct mdp_fit {
         if (J == 6 || J == 8) vars = {0, 1, 2, 3, 4, 5, 6};
         double best = 1e30;
         int pick = dbdr_groups, pickv = 0;
-        std::string js = "{\"fit_dbdr\": {";
+        std::string js = "{" + zb_json + "\"fit_dbdr\": {";
         for (int v : vars)
             for (int k = 0; k < 4; ++k) {
                 dbdr_variant = v;
@@ -384,7 +399,7 @@ struct mdp_fit {
         dbdr_groups = pick;
         dbdr_variant = pickv;
         js += "}, \"chosen\": {\"fit_dbdr_variant\": " + std::to_string(pickv) + ", \"fit_dbdr_groups\": " +
-              std::to_string(pick) + "}}";
+              std::to_string(pick) + ", \"fit_zb_waves\": " + std::to_string(zb_waves) + "}}";
         tuning_json = js;
     }
 

@@ -33,10 +33,10 @@ void launch_fit_zb(const SnapDev& s, const double* Ur, const double* Ui, double*
 // Z + B with the half U (rows 2mb <= j, [e][N] layout) of 64 atoms staged in LDS per workgroup; row_order
 // = the half rows (j, mb) by decreasing work (yi_row_order).  Compiled for twojmax 4, 6, 8 (returns
 // false otherwise); same outputs as launch_fit_zb.  bpart: scratch of fit_zb_partials(J) * N doubles (the
-// per-row shares of B).
+// per-row shares of B).  waves: 8 (two per SIMD, 256 registers each) or 4 (one per SIMD, 512 registers).
 bool launch_fit_zb_lds(const SnapDev& s, const double* Uhr, const double* Uhi, double* zt, double* blist,
                        double* bpart, const double* bzero, const int* cfg_of, const int* cfg_a0,
-                       const int* row_order, int N, int bnorm, int bzeroflag, hipStream_t st);
+                       const int* row_order, int N, int bnorm, int bzeroflag, hipStream_t st, int waves = 8);
 int fit_zb_partials(int J);
 // Device copy of the dB/dr term table (see fit_term_table)
 struct FitTerms {

@@ -320,7 +320,9 @@ __global__ __launch_bounds__(64, 1) void k_fit_dbdr_mfma(const double* __restric
                             }
                             const int cp = colc / P;
                             const size_t lp = static_cast<size_t>(q0 + colc - cp * P - p0);
-                            db[lp + Pc * cp + 3 * Pc * tb] = v;
+                            // streamed past the caches: dB/dr is written once and not read here again,
+                            // and keeping it out of L2 leaves room for the Z blocks the A operands read
+                            __builtin_nontemporal_store(v, &db[lp + Pc * cp + 3 * Pc * tb]);
                         }
                     }
                 }

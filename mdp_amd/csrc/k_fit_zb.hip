@@ -194,14 +194,18 @@ int fit_zb_partials(int J) { return (J == 4 || J == 6 || J == 8) ? zb_n_bp(J) : 
 
 bool launch_fit_zb_lds(const SnapDev& s, const double* Uhr, const double* Uhi, double* zt, double* blist,
                        double* bpart, const double* bzero, const int* cfg_of, const int* cfg_a0,
-                       const int* row_order, int N, int bnorm, int bzeroflag, hipStream_t st) {
+                       const int* row_order, int N, int bnorm, int bzeroflag, hipStream_t st, int waves) {
     if (N <= 0) return true;
-    const dim3 grid((N + 63) / 64), block(64 * 8), gb((N + 255) / 256), bb(256);
+    const dim3 grid((N + 63) / 64), gb((N + 255) / 256), bb(256);
     switch (s.J) {
 #define ZB_CASE(JV)                                                                                            \
     case JV:                                                                                                   \
-        hipLaunchKernelGGL((k_fit_z_lds<JV, 8>), grid, block, 0, st, s.cg, Uhr, Uhi, zt, bpart, row_order, N,     \
-                           bnorm);                                                                             \
+        if (waves == 4)                                                                                        \
+            hipLaunchKernelGGL((k_fit_z_lds<JV, 4>), grid, dim3(256), 0, st, s.cg, Uhr, Uhi, zt, bpart, row_order, \
+                               N, bnorm);                                                                      \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_fit_z_lds<JV, 8>), grid, dim3(512), 0, st, s.cg, Uhr, Uhi, zt, bpart, row_order, \
+                               N, bnorm);                                                                      \
         hipLaunchKernelGGL((k_fit_bsum<JV>), gb, bb, 0, st, bpart, blist, bzero, cfg_of, cfg_a0, N, bzeroflag);   \
         return true;
         ZB_CASE(4)

@@ -706,6 +706,7 @@ struct mdp_md {
     std::string tuning_json = "{}";
     // timing
     bool timing = false;
+    unsigned timing_mask = ~0u;   // timer classes recorded while timing is on
     std::vector<hipEvent_t> ev_pool;
     struct Rec { int t; hipEvent_t a, b; };
     std::vector<Rec> recs;
@@ -723,7 +724,7 @@ struct mdp_md {
     }
     template <class F>
     void timed(int t, F&& f) {
-        if (!timing) {
+        if (!timing || !(timing_mask & (1u << t))) {
             f();
             return;
         }
@@ -1484,7 +1485,10 @@ int mdp_md_set_timing(mdp_md* md, int on) {
         md->flush_timers();
         MDP_HIP(hipMemsetAsync(md->pair_sum.p, 0, sizeof(unsigned long long), md->st));
         md->pair_steps0 = md->force_evals;
+        if (on < 0 || on > 2) throw Error("mdp_md_set_timing: on must be 0, 1 or 2");
         md->timing = on != 0;
+        // 2: only the SNAP compute kernels (fewer events inside a timed run)
+        md->timing_mask = on == 2 ? (1u << T_UI) | (1u << T_YI) | (1u << T_DEDR) : ~0u;
         for (int k = 0; k < T_COUNT; ++k) {
             md->t_ms[k] = 0;
             md->t_n[k] = 0;

@@ -161,8 +161,11 @@ class SnapMD:
         keys = ("pairs", "neighbor_entries", "rebuilds", "dangerous_rebuilds", "steps", "overflows")
         return dict(zip(keys, (int(v) for v in out)))
 
-    def set_timing(self, on=True):
-        self._check(self._c.mdp_md_set_timing(self._h, 1 if on else 0), "mdp_md_set_timing")
+    def set_timing(self, on=True, compute_only=False):
+        """Per-kernel HIP-event timers from now on (every kernel class, or with compute_only the three SNAP
+        compute kernels alone, which puts fewer events into a timed run); on=False switches them off."""
+        mode = 0 if not on else (2 if compute_only else 1)
+        self._check(self._c.mdp_md_set_timing(self._h, mode), "mdp_md_set_timing")
 
     def timers(self):
         k = self._c.mdp_md_num_timers()
