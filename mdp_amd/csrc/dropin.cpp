@@ -1000,8 +1000,9 @@ void mdp_hipAddUarraytotRij(double* ulisttot_r, double* ulisttot_i, double* ulis
         for (int t : h_type)
             if (t < 0) throw Error("AddUarraytot: negative atom type");
         hipStream_t st = library_stream();
+        double* sfw = dev<double>(0, std::max<size_t>(h_tj.size(), 1));
         launch_add_utot_rij(ulisttot_r, ulisttot_i, ulist_r, ulist_i, rij, wjelem, radelem, rmin0, rcutfac, ilist, type,
-                            pairnum, pairnumsum, map, tj, idxu_max, nelements, inum, switch_flag, chemflag, st);
+                            pairnum, pairnumsum, map, tj, idxu_max, nelements, inum, switch_flag, chemflag, sfw, st);
         finish(st);
     });
 }

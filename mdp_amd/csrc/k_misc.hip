@@ -959,32 +959,46 @@ void launch_dbidrj_atom_major(const SnapDev& s, double* db, const double* zr, co
                        zi, dur, dui, ai, tj, ne, bnorm, chem, ijnum);
 }
 
-// Utot += sfac(r) w_j U of every pair of each listed atom (the per-pair route, cpuSnap.cpp:358-395): U pair-major
-// ulist[k idxu_max + u], Utot atom-major ulisttot[(ii ne + jelem) idxu_max + u].  One thread per (atom, entry)
-// adds the atom's pairs in list order.
-__global__ __launch_bounds__(256) void k_add_utot_rij(double* __restrict__ utr, double* __restrict__ uti,
-                                                      const double* __restrict__ ur, const double* __restrict__ ui,
-                                                      const double* __restrict__ rij, const double* __restrict__ wj,
-                                                      const double* __restrict__ rad, double rmin0, double rcutfac,
-                                                      const int* __restrict__ ilist, const int* __restrict__ type,
-                                                      const int* __restrict__ pairnum,
-                                                      const int* __restrict__ pairnumsum, const int* __restrict__ map,
-                                                      const int* __restrict__ tj, int idxu_max, int ne, int inum,
-                                                      int sw, int chem) {
+// sfw[k] = sfac(r_k) w_j of every pair k of each listed atom, one thread per atom (cpuSnap.cpp:358-395)
+__global__ __launch_bounds__(256) void k_pair_sfw(double* __restrict__ sfw, const double* __restrict__ rij,
+                                                  const double* __restrict__ wj, const double* __restrict__ rad,
+                                                  double rmin0, double rcutfac, const int* __restrict__ ilist,
+                                                  const int* __restrict__ type, const int* __restrict__ pairnum,
+                                                  const int* __restrict__ pairnumsum, const int* __restrict__ tj,
+                                                  int inum, int sw) {
 #pragma clang fp contract(off)
-    const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= static_cast<long>(inum) * idxu_max) return;
-    const int ii = static_cast<int>(t / idxu_max), u = static_cast<int>(t % idxu_max);
+    const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ii >= inum) return;
     const int itype = type[ilist[ii]], m = pairnum[ii], start = pairnumsum[ii];
     for (int l = 0; l < m; ++l) {
-        const int k = start + l, jt = tj[k], je = chem ? map[jt] : 0;
+        const int k = start + l, jt = tj[k];
         const double x = rij[3 * k], y = rij[3 * k + 1], z = rij[3 * k + 2];
         const double r = sqrt(x * x + y * y + z * z);
         const double rcut = (rad[itype] + rad[jt]) * rcutfac;
         double sf = 0.0;
         if (sw == 0) sf = 1.0;
         else if (sw == 1) sf = r <= rmin0 ? 1.0 : (r > rcut ? 0.0 : 0.5 * (cos((r - rmin0) * (M_PI / (rcut - rmin0))) + 1.0));
-        sf *= wj[jt];
+        sfw[k] = sf * wj[jt];
+    }
+}
+
+// Utot += sfw_k U_k over the pairs of each listed atom (the per-pair route, cpuSnap.cpp:358-395): U pair-major
+// ulist[k idxu_max + u], Utot atom-major ulisttot[(ii ne + jelem) idxu_max + u].  One thread per (atom, entry)
+// adds the atom's pairs in list order; the weights come from k_pair_sfw.
+__global__ __launch_bounds__(256) void k_add_utot_rij(double* __restrict__ utr, double* __restrict__ uti,
+                                                      const double* __restrict__ ur, const double* __restrict__ ui,
+                                                      const double* __restrict__ sfw, const int* __restrict__ pairnum,
+                                                      const int* __restrict__ pairnumsum, const int* __restrict__ map,
+                                                      const int* __restrict__ tj, int idxu_max, int ne, int inum,
+                                                      int chem) {
+#pragma clang fp contract(off)
+    const long t = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<long>(inum) * idxu_max) return;
+    const int ii = static_cast<int>(t / idxu_max), u = static_cast<int>(t % idxu_max);
+    const int m = pairnum[ii], start = pairnumsum[ii];
+    for (int l = 0; l < m; ++l) {
+        const int k = start + l, je = chem ? map[tj[k]] : 0;
+        const double sf = sfw[k];
         const size_t kl = (static_cast<size_t>(ii) * ne + je) * idxu_max + u, kr = static_cast<size_t>(k) * idxu_max + u;
         utr[kl] += sf * ur[kr];
         uti[kl] += sf * ui[kr];
@@ -994,12 +1008,13 @@ __global__ __launch_bounds__(256) void k_add_utot_rij(double* __restrict__ utr, 
 void launch_add_utot_rij(double* utr, double* uti, const double* ur, const double* ui, const double* rij,
                          const double* wj, const double* rad, double rmin0, double rcutfac, const int* ilist,
                          const int* type, const int* pairnum, const int* pairnumsum, const int* map, const int* tj,
-                         int idxu_max, int ne, int inum, int sw, int chem, hipStream_t st) {
+                         int idxu_max, int ne, int inum, int sw, int chem, double* sfw, hipStream_t st) {
     const long n = static_cast<long>(inum) * idxu_max;
     if (n <= 0) return;
+    hipLaunchKernelGGL(k_pair_sfw, dim3((inum + 255) / 256), dim3(256), 0, st, sfw, rij, wj, rad, rmin0, rcutfac, ilist,
+                       type, pairnum, pairnumsum, tj, inum, sw);
     hipLaunchKernelGGL(k_add_utot_rij, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, utr, uti, ur, ui,
-                       rij, wj, rad, rmin0, rcutfac, ilist, type, pairnum, pairnumsum, map, tj, idxu_max, ne, inum, sw,
-                       chem);
+                       sfw, pairnum, pairnumsum, map, tj, idxu_max, ne, inum, chem);
 }
 
 // ---- cell-list neighbour search of the upstream core (cpuNeighborList3D.cpp:334-443) -------------------

@@ -113,11 +113,12 @@ int dedr_variants(int* ids);
 size_t scan_tmp_ints(int n);
 void exclusive_scan(const int* in, int* out, int n, int* tmp, hipStream_t st);
 
-// Utot += sfac w_j U over each listed atom's pairs, U pair-major, Utot atom-major (cpuSnap.cpp:358-395)
+// Utot += sfac w_j U over each listed atom's pairs, U pair-major, Utot atom-major (cpuSnap.cpp:358-395);
+// sfw: scratch of one double per pair (the pairs' sfac w_j, formed once per pair)
 void launch_add_utot_rij(double* utr, double* uti, const double* ur, const double* ui, const double* rij,
                          const double* wj, const double* rad, double rmin0, double rcutfac, const int* ilist,
                          const int* type, const int* pairnum, const int* pairnumsum, const int* map, const int* tj,
-                         int idxu_max, int ne, int inum, int sw, int chem, hipStream_t st);
+                         int idxu_max, int ne, int inum, int sw, int chem, double* sfw, hipStream_t st);
 
 // cell-list neighbour search of the upstream core (cpuNeighborList3D.cpp:334-443, cpuNeighborList2D.cpp:298-308)
 void launch_cell_list3d(int* clist, const double* x, const double* eta1, const double* eta2, const double* eta3,

@@ -79,3 +79,5 @@ def test_tungsten_coefficients_load():
     pot = S.tungsten(os.path.join(os.path.dirname(__file__), "golden", "snapW.coeff"))
     assert pot.coeff.shape == (56,) and np.all(np.isfinite(pot.coeff))
     assert (pot.a0, pot.rcutfac, pot.mass) == (3.1803, 4.8123, 183.84)
+    assert pot.zbl == (4.0, 4.8, 74.0) and S.tungsten(os.path.join(os.path.dirname(__file__), "golden",
+                                                                  "snapW.coeff"), with_zbl=False).zbl is None

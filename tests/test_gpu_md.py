@@ -206,3 +206,22 @@ def test_tungsten_potential_first_step(oracle):
     assert relerr(st["f"], f) < TOL
     assert relerr(st["vatom"], v) < TOL
     assert -17.0 < np.mean(e) < -14.0
+
+
+def test_timing_modes(oracle):
+    """set_timing: every stage, or only the three SNAP compute kernels (the bench's timed region)."""
+    x, box = lattice.crystal("bcc", 4, 3.316, 0.05, 13)
+    md = _engine(x, box, 6, _coeff(oracle, 6, 13))
+    md.compute()
+    md.set_timing(True)
+    md.run(3)
+    t = md.timers()
+    assert t["compute_yi"][1] == 3 and t["integrate"][1] == 3 and t["tally_integrate"][1] == 3
+    md.set_timing(True, compute_only=True)
+    md.run(4)
+    t = md.timers()
+    assert all(t[k][1] == 4 and t[k][0] > 0 for k in ("compute_ui", "compute_yi", "compute_dedr"))
+    assert t["integrate"][1] == 0 and t["pair_list"][1] == 0 and t["tally_integrate"][1] == 0
+    md.set_timing(False)
+    md.run(2)
+    assert md.timers()["compute_yi"][1] == 0
