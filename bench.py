@@ -366,8 +366,11 @@ def bench_surrogate(a):
     v = lattice.maxwell_boltzmann(n, pot.mass, 600.0, seed=SEED + 12)
 
     def engine():
-        return SnapMD(x, box, np.ones(n, np.int32), J, pot.coeff, [0.5], [1.0], [pot.mass], rcutfac=pot.rcutfac,
-                      rfac0=pot.rfac0, v=v, dt=0.001, skin=SKIN, rebuild_every=EVERY)
+        e = SnapMD(x, box, np.ones(n, np.int32), J, pot.coeff, [0.5], [1.0], [pot.mass], rcutfac=pot.rcutfac,
+                   rfac0=pot.rfac0, v=v, dt=0.001, skin=SKIN, rebuild_every=EVERY)
+        if pot.zbl:   # the exact reference pair term, with plain SNAP and with the surrogate alike
+            e.set_zbl(*pot.zbl)
+        return e
 
     def run_chunks(md, steps, chunk=50):
         """Steps in chunks; a run that fails (e.g. the neighbour build reports a collapsed system) stops

@@ -71,9 +71,10 @@ class Potential:
     """A single-element plain twojmax=8 SNAP potential and the BCC lattice the surrogate samples around:
     coeff = [beta0, beta_1..beta_55], lattice constant a0 (A), rcutfac, rfac0, mass (amu)."""
 
-    def __init__(self, name, coeff, a0, rcutfac, rfac0, mass):
+    def __init__(self, name, coeff, a0, rcutfac, rfac0, mass, zbl=None):
         self.name, self.coeff, self.a0 = name, np.ascontiguousarray(coeff, np.float64), float(a0)
         self.rcutfac, self.rfac0, self.mass = float(rcutfac), float(rfac0), float(mass)
+        self.zbl = zbl   # (cut_inner, cut_outer, z) of the ZBL reference pair term that goes with SNAP, or None
 
 
 def seeded_tantalum():
@@ -87,13 +88,15 @@ def load_coefficients(path):
         return np.array([float(v) for v in fh.read().split()])
 
 
-def tungsten(path):
-    """The upstream tungsten potential (simulating/snap/snapW.m:34-53: BCC a = 3.1803 A, rcutfac 4.8123,
-    rfac0 0.99363, twojmax 8, mass 183.84) with the coefficients of snapW.coeff at ``path``."""
+def tungsten(path, with_zbl=True):
+    """The upstream tungsten potential (simulating/snap/snapW.m:34-60: BCC a = 3.1803 A, rcutfac 4.8123,
+    rfac0 0.99363, twojmax 8, mass 183.84, and the ZBL reference term with inner/outer cutoffs 4 and 4.8 A,
+    z = 74) with the coefficients of snapW.coeff at ``path``."""
     c = load_coefficients(path)
     if len(c) != 56:
         raise ValueError(f"{path}: expected 56 coefficients, found {len(c)}")
-    return Potential("snapW.coeff, BCC W", c, 3.1803, 4.8123, 0.99363, 183.84)
+    return Potential("snapW.coeff + ZBL, BCC W" if with_zbl else "snapW.coeff, BCC W", c, 3.1803, 4.8123, 0.99363,
+                     183.84, (4.0, 4.8, 74.0) if with_zbl else None)
 
 
 class Model:
@@ -131,7 +134,9 @@ class Model:
 def make_training_set(nconfig=4000, ncell=5, tmin=300.0, tmax=3000.0, every=10, equil=10, per_run=10, seed=777,
                       fmax=200.0, rmin=2.0, log=None, pot=None):
     """Configurations of 250 atoms (BCC 5^3 cells at the potential's lattice constant -- the seeded Ta
-    workload by default, a = 3.316 A -- and random packings at the BCC density)
+    workload by default, a = 3.316 A -- and random packings at the BCC density).  With a ZBL reference
+    term the dynamics include it, and the labels are the SNAP part alone (the surrogate stands in for SNAP;
+    the engine adds the exact ZBL term to either)
     sampled from short plain twojmax=8 SNAP NVE runs started at temperatures spread over [tmin, tmax];
     labels are the plain SNAP total energy and forces.  Short runs keep the samples near their
     starting structures: with untrained (random) coefficients long hot runs drift into collapsed
@@ -159,6 +164,11 @@ def make_training_set(nconfig=4000, ncell=5, tmin=300.0, tmax=3000.0, every=10, 
             v = lattice.maxwell_boltzmann(len(x), mass, T, seed=seed + 100000 + run)
             md = SnapMD(x, box, np.ones(len(x), np.int32), J_PLAIN, coeff, [0.5], [1.0], [mass], rcutfac=pot.rcutfac,
                         rfac0=pot.rfac0, v=v, dt=0.0005, skin=0.3, rebuild_every=10)
+            lab = None
+            if pot.zbl:
+                md.set_zbl(*pot.zbl)
+                lab = SnapMD(x, box, np.ones(len(x), np.int32), J_PLAIN, coeff, [0.5], [1.0], [mass],
+                             rcutfac=pot.rcutfac, rfac0=pot.rfac0, dt=0.0005, skin=0.3, rebuild_every=10)
             md.compute()
             md.run(equil)
             for k in range(per_run):
@@ -167,12 +177,18 @@ def make_training_set(nconfig=4000, ncell=5, tmin=300.0, tmax=3000.0, every=10, 
                 md.run(every)
                 st = md.state()
                 xs = np.mod(st["x"], box)
+                if lab is not None:   # the SNAP part alone on the same configuration
+                    lab.set_state(xs)
+                    lab.compute()
+                    st = lab.state()
                 d, _ = cKDTree(xs, boxsize=box).query(xs, k=2)
                 if d[:, 1].min() < rmin or np.abs(st["f"]).max() > fmax:
                     dropped += 1
                     continue
                 out.append(dict(x=xs, box=box.copy(), E=float(st["eatom"].sum()), F=st["f"].copy(), kind=kind, T=T))
             md.close()
+            if lab is not None:
+                lab.close()
             run += 1
             if log and run % 50 == 0:
                 log(f"  {run} runs -> {len(out)} configurations ({dropped} dropped)")
