@@ -181,6 +181,92 @@ void launch_ui(const SnapDev& s, UiMode mode, double* Ur, double* Ui, const doub
 #undef UI_CASE
 }
 
+// compute_ui with the per-atom sums in LDS (half layout plus wself, from the pair geometry pgeo).  Lanes
+// are pairs: workgroup block b holds the atoms whose first pair lies in [b*UI_PB, (b+1)*UI_PB), so with at
+// most UI_MAXP pairs per atom a block has fewer than UI_PB + UI_MAXP <= 512 pairs, one per lane.  Each
+// lane runs the row-chain recursion of its pair; every finished half row, scaled by sfac w_j, goes to LDS;
+// then thread (ma, atom) adds the row entry of the atom's pairs in pair order and writes U (plus wself on
+// the diagonal) -- the same sum, in the same order, as upstream's per-atom loop (cpuSnap.cpp:312-542).  The
+// workgroups are persistent over the blocks; all lanes share the control flow of the recursion.
+constexpr int UI_PB = 416, UI_MAXP = 96, UI_NL = 512;
+
+template <int J>
+__global__ __launch_bounds__(UI_NL) void k_ui_lds(const double* __restrict__ rootpq, double* __restrict__ Ur,
+                                                 double* __restrict__ Ui, const double* __restrict__ pgeo,
+                                                 const int* __restrict__ poff, int N, double wself) {
+    constexpr int RW = J + 1;
+    __shared__ double sr[UI_NL * RW], si[UI_NL * RW];
+    __shared__ int s_off[UI_NL + 1];   // pair offsets (relative to the block) of the block's atoms
+    const int lane = threadIdx.x;
+    const int ptot = poff[N];
+    const size_t n = static_cast<size_t>(N);
+    for (int b = blockIdx.x; static_cast<long>(b) * UI_PB <= ptot; b += gridDim.x) {
+        // atoms a0..a1-1: a0 (a1) is the first atom with poff[a] >= b*UI_PB (>= (b+1)*UI_PB), by binary search
+        auto lower = [&](long v) {
+            int lo = 0, hi = N;
+            while (lo < hi) {
+                const int mid = lo + (hi - lo) / 2;
+                if (poff[mid] < v) lo = mid + 1;
+                else hi = mid;
+            }
+            return lo;
+        };
+        const int a0 = lower(static_cast<long>(b) * UI_PB), a1 = lower(static_cast<long>(b + 1) * UI_PB);
+        const int na = a1 - a0;
+        if (na <= 0) continue;
+        const int q0 = poff[a0], np = poff[a1] - q0;   // np < UI_PB + UI_MAXP <= UI_NL
+        __syncthreads();   // the previous block's readers are done with s_off and the row stage
+        for (int k = lane; k <= na; k += UI_NL) s_off[k] = poff[a0 + k] - q0;
+        PairGeom g;   // lanes past the block's pairs run the recursion on a zero pair; nothing reads them
+        g.ar = g.ai = g.br = g.bi = 0.0;
+        double sf = 0.0;
+        if (lane < np) {
+            const size_t q = static_cast<size_t>(q0 + lane);
+            g.ar = pgeo[5 * q];
+            g.ai = pgeo[5 * q + 1];
+            g.br = pgeo[5 * q + 2];
+            g.bi = pgeo[5 * q + 3];
+            sf = pgeo[5 * q + 4];
+        }
+        row_chain<J, 0, 0, J / 2>(g, rootpq, [&](auto Jc, auto MBc, auto& pr, auto& pim, auto&, auto&) {
+            constexpr int L = decltype(Jc)::value, MB = decltype(MBc)::value;
+            __syncthreads();   // the previous row's sums have read the stage
+#pragma unroll
+            for (int ma = 0; ma <= L; ++ma) {
+                sr[lane * RW + ma] = sf * pr[ma];
+                si[lane * RW + ma] = sf * pim[ma];
+            }
+            __syncthreads();
+            constexpr int row = h_block(L) + (L + 1) * MB;
+            for (int t = lane; t < na * (L + 1); t += UI_NL) {
+                const int ma = t / na, al = t - ma * na;   // consecutive threads: consecutive atoms
+                const int p1 = min(s_off[al + 1], UI_NL);   // the bound holds with <= UI_MAXP pairs per atom
+                double ar = 0.0, ai = 0.0;
+                for (int p = s_off[al]; p < p1; ++p) {
+                    ar += sr[p * RW + ma];
+                    ai += si[p * RW + ma];
+                }
+                const size_t o = static_cast<size_t>(row + ma) * n + a0 + al;
+                Ur[o] = ar + (ma == MB ? wself : 0.0);
+                Ui[o] = ai;
+            }
+        });
+    }
+}
+
+void launch_ui_geo_lds(const SnapDev& s, double* Ur, double* Ui, const double* pgeo, const int* pair_off, int N,
+                       double wself, int groups, hipStream_t st) {
+    if (N <= 0) return;
+    const int G = groups > 0 ? groups : 512;
+    switch (s.J) {
+        case 2: hipLaunchKernelGGL((k_ui_lds<2>), dim3(G), dim3(UI_NL), 0, st, s.rootpq, Ur, Ui, pgeo, pair_off, N, wself); break;
+        case 4: hipLaunchKernelGGL((k_ui_lds<4>), dim3(G), dim3(UI_NL), 0, st, s.rootpq, Ur, Ui, pgeo, pair_off, N, wself); break;
+        case 6: hipLaunchKernelGGL((k_ui_lds<6>), dim3(G), dim3(UI_NL), 0, st, s.rootpq, Ur, Ui, pgeo, pair_off, N, wself); break;
+        case 8: hipLaunchKernelGGL((k_ui_lds<8>), dim3(G), dim3(UI_NL), 0, st, s.rootpq, Ur, Ui, pgeo, pair_off, N, wself); break;
+        default: break;
+    }
+}
+
 void launch_ui_geo(const SnapDev& s, double* Ur, double* Ui, const double* pgeo, const int* pair_off, int N,
                    double wself, hipStream_t st) {
     if (N <= 0) return;

@@ -702,6 +702,7 @@ struct mdp_md {
     double rmse_last = 0, rmse_max = 0, rmse_sum = 0;
     // autotuned launch variants (see autotune())
     int dedr_variant = 3, yi_waves = 8;
+    int ui_variant = 0;   // compute_ui: 0 = one thread per atom and row group, G > 0 = LDS sums on G workgroups
     bool tuned = false;
     std::string tuning_json = "{}";
     // timing
@@ -998,6 +999,14 @@ struct mdp_md {
         forces_current = false;
     }
 
+    void ui_launch(int v) {
+        if (v == 0)
+            launch_ui_geo(tab.view, Ur.as<double>(), Ui.as<double>(), pgeo.as<double>(), poff.as<int>(), N, 1.0, st);
+        else
+            launch_ui_geo_lds(tab.view, Ur.as<double>(), Ui.as<double>(), pgeo.as<double>(), poff.as<int>(), N, 1.0,
+                              v, st);
+    }
+
     // plain SNAP forces; outputs to (fo, vo, eo), by default the engine's f, vir and eatom
     void snap_and_tally(bool integrate, double* fo = nullptr, double* vo = nullptr, double* eo = nullptr) {
         const SnapDev& s = tab.view;
@@ -1006,9 +1015,7 @@ struct mdp_md {
         if (!fo) fo = f.as<double>();
         if (!vo) vo = vir.as<double>();
         if (!eo) eo = eatom.as<double>();
-        timed(T_UI, [&] {
-            launch_ui_geo(s, Ur.as<double>(), Ui.as<double>(), pgeo.as<double>(), poff.as<int>(), N, 1.0, st);
-        });
+        timed(T_UI, [&] { ui_launch(ui_variant); });
         timed(T_YI, [&] {
             launch_yi_half(s, Ur.as<double>(), Ui.as<double>(), beta_t.as<double>(), Yr.as<double>(), Yi.as<double>(),
                            epart.as<double>(), row_order.as<int>(), N, st, yi_waves);
@@ -1201,8 +1208,26 @@ struct mdp_md {
             js += (k ? ", " : "") + std::string("\"") + std::to_string(ids[k]) + "\": " + std::to_string(t);
             if (t < best) { best = t; yi_waves = ids[k]; }
         }
+        // compute_ui: the per-atom kernel, or the LDS-sum kernel on 256, 512 or 1024 workgroups;
+        // MDP_UI_VARIANT pins one of 0, 256, 512, 1024
+        js += "}, \"compute_ui\": {";
+        int uids[4] = {0, 256, 512, 1024}, nu = 4;
+        if (const char* up = std::getenv("MDP_UI_VARIANT")) {
+            const int v = std::atoi(up);
+            if (v != 0 && v != 256 && v != 512 && v != 1024)
+                throw Error("MDP_UI_VARIANT is not a compute_ui variant (0, 256, 512 or 1024)");
+            uids[0] = v;
+            nu = 1;
+        }
+        best = 1e30;
+        for (int k = 0; k < nu; ++k) {
+            const double t = time_launch([&] { ui_launch(uids[k]); }, 3);
+            js += (k ? ", " : "") + std::string("\"") + std::to_string(uids[k]) + "\": " + std::to_string(t);
+            if (t < best) { best = t; ui_variant = uids[k]; }
+        }
         js += "}, \"chosen\": {\"compute_dedr\": " + std::to_string(dedr_variant) +
-              ", \"compute_yi_waves\": " + std::to_string(yi_waves) + "}}";
+              ", \"compute_yi_waves\": " + std::to_string(yi_waves) + ", \"compute_ui\": " +
+              std::to_string(ui_variant) + "}}";
         tuning_json = js;
         tuned = true;
     }

@@ -44,6 +44,10 @@ void launch_ui(const SnapDev& s, UiMode mode, double* Ur, double* Ui, const doub
 // [pair_off[i], pair_off[i+1]).
 void launch_ui_geo(const SnapDev& s, double* Ur, double* Ui, const double* pgeo, const int* pair_off, int natoms,
                    double wself, hipStream_t st);
+// The same result with the per-atom sums formed in LDS, one lane per pair (at most 96 pairs per atom);
+// persistent over `groups` workgroups of 512 lanes (0: 512).
+void launch_ui_geo_lds(const SnapDev& s, double* Ur, double* Ui, const double* pgeo, const int* pair_off,
+                       int natoms, double wself, int groups, hipStream_t st);
 
 // Y[half rows] of every atom (Y zeroed by the caller where untouched).  beta_t[triple] holds the
 // effective weight of each coupling triple (multiplicity, (j1+1)/(j+1) and 1/(j+1) factors folded

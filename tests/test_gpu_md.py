@@ -96,6 +96,55 @@ def test_every_dedr_variant_vs_oracle(oracle, monkeypatch, variant):
     assert relerr(st["vatom"], v) < TOL
 
 
+def _first_step_vs_oracle(oracle, x, box, J, seed):
+    coeff = _coeff(oracle, J, seed)
+    md = _engine(x, box, J, coeff)
+    md.compute()
+    st = md.state()
+    s = snapcpp.initsna(oracle, [1, J, RCUTFAC, RFAC0, 0, 1, 1, 0, 0, 0, 0], [0.5], [1.0], coeff)
+    B = np.diag(box)
+    p = snapcpp.pairs_from_box(x, np.ones(len(x), np.int32), B[0], B[1], B[2], [1, 1, 1], s)
+    e, f, v = snapcpp.snappotential_pairs(oracle, s, p)
+    assert md.stats()["pairs"] == p["ijnum"]
+    assert relerr(st["eatom"], e) < TOL
+    assert relerr(st["f"], f) < TOL
+    assert relerr(st["vatom"], v) < TOL
+    return md
+
+
+@pytest.mark.parametrize("variant", ["0", "256", "512", "1024"])
+@pytest.mark.parametrize("J,ncell,a,seed", [(8, 12, 3.316, 41), (4, 2, 3.316, 42), (6, 5, 3.0, 43)])
+def test_every_ui_variant_vs_oracle(oracle, monkeypatch, variant, J, ncell, a, seed):
+    """compute_ui pinned to the per-atom kernel (0) or to the LDS-sum kernel on 256/512/1024 workgroups:
+    3,456 atoms (blocks of ~21 atoms), 16 atoms (one block, most workgroups idle), a compressed lattice
+    (more pairs per atom)."""
+    monkeypatch.setenv("MDP_UI_VARIANT", variant)
+    x, box = lattice.crystal("bcc", ncell, a, 0.1, seed)
+    md = _first_step_vs_oracle(oracle, x, box, J, seed)
+    assert md.tuning()["chosen"]["compute_ui"] == int(variant)
+
+
+@pytest.mark.parametrize("variant", ["0", "512"])
+def test_ui_variants_sparse_gas(oracle, monkeypatch, variant):
+    """A dilute random gas: ragged pair counts, many atoms with no pair at all (U is wself on the diagonal)."""
+    monkeypatch.setenv("MDP_UI_VARIANT", variant)
+    rng = np.random.default_rng(77)
+    L = 40.0
+    box = np.full(3, L)
+    x = rng.uniform(0.0, L, size=(300, 3))
+    # keep pairs at least 1.6 A apart, so the energies stay in a sane range
+    keep = [0]
+    for i in range(1, len(x)):
+        d = x[keep] - x[i]
+        d -= L * np.round(d / L)
+        if np.min(np.sum(d * d, axis=1)) > 1.6 ** 2:
+            keep.append(i)
+    x = np.ascontiguousarray(x[keep])
+    md = _first_step_vs_oracle(oracle, x, box, 8, 78)
+    off, _, _ = md.neighbors()
+    assert np.any(np.diff(off) == 0)
+
+
 def test_first_step_128k_twojmax8_sampled(oracle):
     """The headline system: BCC Ta 40^3 cells (128,000 atoms), sigma = 0.1 A, twojmax = 8."""
     import sampled_oracle
