@@ -1199,6 +1199,15 @@ struct mdp_md {
         if (pick_xcd >= 0 && best_xcd <= 1.03 * best) dedr_variant = pick_xcd;
         js += "}, \"compute_yi\": {";
         nv = yi_half_variants(J, ids);
+        // MDP_YI_VARIANT pins compute_yi to one of yi_half_variants
+        if (const char* yp = std::getenv("MDP_YI_VARIANT")) {
+            const int v = std::atoi(yp);
+            bool known = false;
+            for (int k = 0; k < nv; ++k) known |= (ids[k] == v);
+            if (!known) throw Error("MDP_YI_VARIANT is not a compute_yi variant");
+            ids[0] = v;
+            nv = 1;
+        }
         best = 1e30;
         for (int k = 0; k < nv; ++k) {
             const double t = time_launch([&] {

@@ -112,6 +112,15 @@ def _first_step_vs_oracle(oracle, x, box, J, seed):
     return md
 
 
+@pytest.mark.parametrize("variant", ["8", "12", "4"])
+def test_every_yi_variant_vs_oracle(oracle, monkeypatch, variant):
+    """compute_yi pinned to each twojmax-8 variant: 8, 12 or 4 waves per workgroup."""
+    monkeypatch.setenv("MDP_YI_VARIANT", variant)
+    x, box = lattice.crystal("bcc", 12, 3.316, 0.1, 51)
+    md = _first_step_vs_oracle(oracle, x, box, 8, 51)
+    assert md.tuning()["chosen"]["compute_yi_waves"] == int(variant)
+
+
 @pytest.mark.parametrize("variant", ["0", "256", "512", "1024"])
 @pytest.mark.parametrize("J,ncell,a,seed", [(8, 12, 3.316, 41), (4, 2, 3.316, 42), (6, 5, 3.0, 43)])
 def test_every_ui_variant_vs_oracle(oracle, monkeypatch, variant, J, ncell, a, seed):
