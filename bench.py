@@ -335,7 +335,13 @@ def bench_surrogate(a):
     pot = S.tungsten(os.path.join(ROOT, "tests", "golden", "snapW.coeff")) if a.potential == "W" else S.seeded_tantalum()
     log(f"potential: {pot.name} (a0 {pot.a0} A, rcutfac {pot.rcutfac}, mass {pot.mass})")
     t0 = time.perf_counter()
-    data, dropped = S.make_training_set(nconfig=a.train_configs, log=log, pot=pot)
+    kinds = ("bcc", "liquid") if a.train_kinds == "both" else ("bcc",)
+    log(f"training runs: {'/'.join(kinds)} at {a.train_tmin:.0f}-{a.train_tmax:.0f} K, "
+        f"{a.train_vacancies} vacancies per BCC cell, {a.train_equil} steps before the first sample, "
+        f"{a.train_every} between samples, BCC displacements {a.train_sigma} A")
+    data, dropped = S.make_training_set(nconfig=a.train_configs, log=log, pot=pot, tmin=a.train_tmin,
+                                        tmax=a.train_tmax, kinds=kinds, vacancies=a.train_vacancies,
+                                        equil=a.train_equil, every=a.train_every, sigma=a.train_sigma)
     log(f"plain-SNAP training data: {len(data)} configurations in {time.perf_counter() - t0:.1f} s "
         f"({dropped} samples dropped: a pair under 2 A or a force above 200 eV/A)")
     ntest = max(1, len(data) // 10)
@@ -393,6 +399,8 @@ def bench_surrogate(a):
     t0 = time.perf_counter()
     md.run(a.plain_steps)
     t_plain = (time.perf_counter() - t0) / a.plain_steps
+    temp_plain = md.thermo()["temp"]
+    log(f"plain MD after {a.warmup + a.plain_steps} steps: {temp_plain:.0f} K")
     md.close()
     md = engine()
     md.set_surrogate(model, audit_every=0)
@@ -415,6 +423,8 @@ def bench_surrogate(a):
     timers = md.timers()
     aud = md.audits()
     st = md.stats()
+    temp_sur = md.thermo()["temp"]
+    log(f"surrogate MD with audits after {au_done} steps: {temp_sur:.0f} K")
     md.close()
     steps_timed = max(au_done, 1)
     elapsed = elapsed * a.steps / steps_timed if au_done else elapsed
@@ -433,7 +443,10 @@ def bench_surrogate(a):
         "config": {"workload": f"surrogate NVE MD, BCC {a.sur_ncell}^3 cells with 0.5% vacancies ({n} atoms), "
                                "600 K, dt=1 fs, audit every 50 steps", "potential": pot.name, "atoms": n,
                    "parallelism": "1 GPU",
-                   "train_configs": len(train), "train_steps": a.train_steps, "samples_dropped": dropped},
+                   "train_configs": len(train), "train_steps": a.train_steps, "samples_dropped": dropped,
+                   "train_runs": {"kinds": list(kinds), "tmin": a.train_tmin, "tmax": a.train_tmax,
+                                  "vacancies": a.train_vacancies, "equil": a.train_equil,
+                                  "every": a.train_every, "sigma": a.train_sigma}},
         "surrogate": {"atom_steps_per_s_without_audits": n / t_noaudit if na_done else None,
                       "plain_gpu_atom_steps_per_s": n / t_plain,
                       "steps_without_audits": w_done + na_done, "error_without_audits": na_err,
@@ -443,6 +456,7 @@ def bench_surrogate(a):
                       "heldout_force_rmse": f_te, "heldout_energy_rmse_per_atom": e_te,
                       "linear_j4_force_rmse": f_lin, "linear_j4_energy_rmse_per_atom": e_lin,
                       "audit_force_rmse_mean": aud["mean_rmse"], "audit_force_rmse_max": aud["max_rmse"],
+                      "md_temperature_plain_K": temp_plain, "md_temperature_surrogate_K": temp_sur,
                       "audits": aud["audits"], "fallbacks": aud["fallbacks"], "target_force_rmse": 0.05,
                       "pairs_per_atom": st["pairs"] / n},
         "roofline": None, "cpu_baseline": None,
@@ -549,6 +563,15 @@ def main():
     ap.add_argument("--train-configs", type=int, default=4000)
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--sur-ncell", type=int, default=80)
+    ap.add_argument("--train-tmin", type=float, default=300.0, help="surrogate: lowest training-run temperature (K)")
+    ap.add_argument("--train-tmax", type=float, default=3000.0, help="surrogate: highest training-run temperature (K)")
+    ap.add_argument("--train-kinds", choices=("both", "bcc"), default="both",
+                    help="surrogate: training runs from BCC cells and random packings, or BCC cells only")
+    ap.add_argument("--train-vacancies", type=int, default=0, help="surrogate: atoms taken out of each BCC training cell")
+    ap.add_argument("--train-equil", type=int, default=10, help="surrogate: MD steps of a training run before its first sample")
+    ap.add_argument("--train-every", type=int, default=10, help="surrogate: MD steps between a training run's samples")
+    ap.add_argument("--train-sigma", type=float, default=0.05,
+                    help="surrogate: RMS displacement (A) of the BCC training cells' starting positions")
     ap.add_argument("--potential", choices=("seeded", "W"), default="seeded",
                     help="surrogate workload: the seeded Ta coefficients or the upstream tungsten potential")
     ap.add_argument("--plain-steps", type=int, default=20)

@@ -132,7 +132,7 @@ class Model:
 
 
 def make_training_set(nconfig=4000, ncell=5, tmin=300.0, tmax=3000.0, every=10, equil=10, per_run=10, seed=777,
-                      fmax=200.0, rmin=2.0, log=None, pot=None):
+                      fmax=200.0, rmin=2.0, log=None, pot=None, kinds=("bcc", "liquid"), vacancies=0, sigma=0.05):
     """Configurations of 250 atoms (BCC 5^3 cells at the potential's lattice constant -- the seeded Ta
     workload by default, a = 3.316 A -- and random packings at the BCC density).  With a ZBL reference
     term the dynamics include it, and the labels are the SNAP part alone (the surrogate stands in for SNAP;
@@ -141,7 +141,9 @@ def make_training_set(nconfig=4000, ncell=5, tmin=300.0, tmax=3000.0, every=10, 
     labels are the plain SNAP total energy and forces.  Short runs keep the samples near their
     starting structures: with untrained (random) coefficients long hot runs drift into collapsed
     states.  Samples with a pair closer than ``rmin`` A or a force above ``fmax`` eV/A are dropped and
-    counted.  Returns (list of dict(x, box, E, F, kind, T), number dropped)."""
+    counted.  ``kinds`` alternates the starting structures; BCC cells start with Gaussian displacements (per component) of
+    RMS ``sigma`` A, and ``vacancies`` atoms are taken out of each at random.
+    Returns (list of dict(x, box, E, F, kind, T), number dropped)."""
     import os
     from scipy.spatial import cKDTree
     from .md import SnapMD
@@ -155,10 +157,14 @@ def make_training_set(nconfig=4000, ncell=5, tmin=300.0, tmax=3000.0, every=10, 
         nruns_est = max(2, (nconfig + per_run - 1) // per_run)
         run = 0
         while len(out) < nconfig:
-            kind = "bcc" if run % 2 == 0 else "liquid"
-            T = tmin + (tmax - tmin) * ((run // 2) % max(1, nruns_est // 2)) / max(1, nruns_est // 2 - 1)
+            nk = len(kinds)
+            kind = kinds[run % nk]
+            T = tmin + (tmax - tmin) * ((run // nk) % max(1, nruns_est // nk)) / max(1, nruns_est // nk - 1)
             if kind == "bcc":
-                x, box = lattice.crystal("bcc", ncell, a0, 0.05, seed + run)
+                x, box = lattice.crystal("bcc", ncell, a0, sigma, seed + run)
+                if vacancies:
+                    gone = np.random.default_rng(seed + 200000 + run).choice(len(x), vacancies, replace=False)
+                    x = np.ascontiguousarray(np.delete(x, gone, axis=0))
             else:
                 x, box = lattice.random_packing(2 * ncell ** 3, 2 / a0 ** 3, 2.2, seed=seed + run)
             v = lattice.maxwell_boltzmann(len(x), mass, T, seed=seed + 100000 + run)

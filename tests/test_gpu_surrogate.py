@@ -184,3 +184,38 @@ def test_md_engine_surrogate_vs_chain_rule(oracle, kind):
     st = md.state()
     assert relerr(st["f"], Fc) < 5e-2
     assert abs(st["eatom"].sum() - E0) < 2e-2 * max(1.0, abs(E0))
+
+
+@pytest.mark.parametrize("pot", ["Ta", "W"])
+def test_md_engine_surrogate_vs_training_path(oracle, pot):
+    """The MD engine's surrogate forces (adjoint route: B, MLP, per-atom beta, Y, dE/dr) against the training
+    path's forces for the same model and configuration (stored twojmax=4 dB/dr contracted with the same
+    MLP's dE/dB).  Both run the same BF16 network on FP64 descriptors and differ only in the FP64 order of
+    the chain rule (measured: 2e-15 of the max force)."""
+    import os
+    from mdp_amd import surrogate as S
+    from mdp_amd.md import SnapMD
+    if pot == "W":
+        P = S.tungsten(os.path.join(os.path.dirname(__file__), "golden", "snapW.coeff"), with_zbl=False)
+    else:
+        P = S.seeded_tantalum()
+    x, box = lattice.crystal("bcc", 5, P.a0, 0.12, 31)
+    x = np.ascontiguousarray(np.delete(x, [7], axis=0))
+    p, aff, _ = _rand_model(5)
+    aff[:14] = 0.0
+    aff[16:30] = 0.2
+    model = S.Model(p, aff, 14)
+    ds = S.DeviceSet([dict(x=x, box=box, E=0.0, F=np.zeros_like(x))], rcutfac=P.rcutfac, rfac0=P.rfac0)
+    ev = S.Evaluator(ds, model)
+    ev.forward()
+    Ft = ev.F.numpy().reshape(-1, 3)
+    md = SnapMD(x, box, np.ones(len(x), np.int32), 8, P.coeff, [0.5], [1.0], [P.mass], rcutfac=P.rcutfac,
+                rfac0=P.rfac0)
+    md.set_surrogate(model, audit_every=0)
+    print(f"engine vs training path, {pot}: {len(x)} atoms", flush=True)
+    md.compute()
+    Fe = md.state()["f"]
+    err = relerr(Fe, Ft)
+    print(f"relative difference {err:.3e}; RMS {np.sqrt(np.mean((Fe - Ft) ** 2)):.3e} of RMS "
+          f"{np.sqrt(np.mean(Ft ** 2)):.3e}", flush=True)
+    assert err < 1e-10
