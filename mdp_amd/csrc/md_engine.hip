@@ -339,6 +339,108 @@ __global__ __launch_bounds__(256) void k_neighbors_cell(const double* __restrict
     }
 }
 
+// The same lists from a per-wavefront neighbour tile in LDS.  One wave owns one centre cell: it lists the
+// 27 surrounding cells (periodic images applied) in the fixed order above, stages their atoms -- index,
+// image shift and the imaged position x_j + s L -- in LDS, NTILE candidates at a time, and its centre atoms
+// (eight lanes each, eight atoms per pass, ascending) scan the tile from LDS.  Candidates, their order and
+// the distance arithmetic are those of k_neighbors_cell, so the lists are identical.
+constexpr int NTILE = 256;
+
+__global__ __launch_bounds__(64) void k_neighbors_tile(const double* __restrict__ x,
+                                                       const int* __restrict__ cell_start,
+                                                       const int* __restrict__ cell_atoms, int* __restrict__ nb_cnt,
+                                                       int* __restrict__ nb_j, int* __restrict__ nb_s, double Lx,
+                                                       double Ly, double Lz, int ncx, int ncy, int ncz, double rcs2,
+                                                       int N, int* __restrict__ overflow) {
+    __shared__ double tx[NTILE], ty[NTILE], tz[NTILE];
+    __shared__ int tj[NTILE], ts[NTILE];
+    __shared__ int k_off[28], k_base[27], k_code[27];
+    __shared__ double k_dx[27], k_dy[27], k_dz[27];
+    const int c = blockIdx.x, lane = threadIdx.x;
+    const int cb = cell_start[c], cn = cell_start[c + 1] - cb;
+    if (cn == 0) return;   // uniform over the workgroup
+    const int cx = c % ncx, cy = (c / ncx) % ncy, cz = c / (ncx * ncy);
+    if (lane < 27) {
+        const int k = lane;
+        const int ux = cx + k % 3 - 1, uy = cy + (k / 3) % 3 - 1, uz = cz + k / 9 - 1;
+        const int sx = (ux < 0) ? -1 : (ux >= ncx ? 1 : 0);
+        const int sy = (uy < 0) ? -1 : (uy >= ncy ? 1 : 0);
+        const int sz = (uz < 0) ? -1 : (uz >= ncz ? 1 : 0);
+        const int nc = (ux - sx * ncx) + ncx * ((uy - sy * ncy) + ncy * (uz - sz * ncz));
+        k_base[k] = cell_start[nc];
+        k_off[k + 1] = cell_start[nc + 1] - cell_start[nc];   // count, summed below
+        k_code[k] = shift_code(sx, sy, sz);
+        k_dx[k] = sx * Lx;
+        k_dy[k] = sy * Ly;
+        k_dz[k] = sz * Lz;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        k_off[0] = 0;
+        for (int k = 0; k < 27; ++k) k_off[k + 1] += k_off[k];
+    }
+    __syncthreads();
+    const int total = k_off[27];
+    const int g = lane / NSUB, sub = lane % NSUB;
+    const unsigned long long group = 0xFFull << (lane & ~(NSUB - 1));
+    const unsigned long long below = (1ull << lane) - 1;
+    const int self_code = shift_code(0, 0, 0);
+    for (int a0 = 0; a0 < cn; a0 += 64 / NSUB) {
+        const bool live = a0 + g < cn;
+        const int i = live ? cell_atoms[cb + a0 + g] : 0;
+        const double xi = live ? x[3 * i] : 0.0, yi = live ? x[3 * i + 1] : 0.0, zi = live ? x[3 * i + 2] : 0.0;
+        int cnt = 0;
+        for (int t0 = 0; t0 < total; t0 += NTILE) {
+            const int nt = min(NTILE, total - t0);
+            __syncthreads();   // the previous pass has finished reading the tile
+            for (int t = lane; t < nt; t += 64) {
+                const int tt = t0 + t;
+                int k = 0;   // the cell holding candidate tt: k_off[k] <= tt < k_off[k + 1]
+#pragma unroll
+                for (int step = 16; step > 0; step >>= 1)
+                    if (k + step < 27 && k_off[k + step] <= tt) k += step;
+                const int j = cell_atoms[k_base[k] + tt - k_off[k]];
+                {
+#pragma clang fp contract(off)
+                    tx[t] = x[3 * j] + k_dx[k];
+                    ty[t] = x[3 * j + 1] + k_dy[k];
+                    tz[t] = x[3 * j + 2] + k_dz[k];
+                }
+                tj[t] = j;
+                ts[t] = k_code[k];
+            }
+            __syncthreads();
+            for (int m0 = 0; m0 < nt; m0 += NSUB) {   // uniform trip count
+                const int m = m0 + sub;
+                bool acc = false;
+                int j = 0;
+                if (live && m < nt) {
+                    j = tj[m];
+                    double d2;
+                    {
+#pragma clang fp contract(off)
+                        d2 = dist2_nc(tx[m] - xi, ty[m] - yi, tz[m] - zi);
+                    }
+                    acc = d2 <= rcs2 && !(j == i && ts[m] == self_code);
+                }
+                const unsigned long long mk = __ballot(acc) & group;
+                if (acc) {
+                    const int q = cnt + __popcll(mk & below);
+                    if (q < NBMAX) {
+                        nb_j[static_cast<size_t>(q) * N + i] = j;
+                        nb_s[static_cast<size_t>(q) * N + i] = ts[m];
+                    }
+                }
+                cnt += __popcll(mk);
+            }
+        }
+        if (live && sub == 0) {
+            if (cnt > NBMAX) atomicOr(overflow, 1);
+            nb_cnt[i] = min(cnt, NBMAX);
+        }
+    }
+}
+
 // Small boxes (fewer than three cells along an axis): every atom against every periodic image.
 __global__ void k_neighbors_brute(const double* __restrict__ x, int* __restrict__ nb_cnt, int* __restrict__ nb_j,
                                   int* __restrict__ nb_s, double Lx, double Ly, double Lz, double rcs, double rcs2,
@@ -366,6 +468,19 @@ __global__ void k_neighbors_brute(const double* __restrict__ x, int* __restrict_
                 }
     if (cnt > NBMAX) atomicOr(overflow, 1);
     nb_cnt[i] = min(cnt, NBMAX);
+}
+
+// total of the per-atom entry counts: one atomic per workgroup
+__global__ __launch_bounds__(256) void k_sum_counts(const int* __restrict__ cnt, int N,
+                                                    unsigned long long* __restrict__ total) {
+    __shared__ unsigned long long part[4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long v = i < N ? static_cast<unsigned long long>(cnt[i]) : 0ull;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(total, part[0] + part[1] + part[2] + part[3]);
 }
 
 // incoming entries: for every entry e = (i -> j) append e to j's list, then sort each list
@@ -703,6 +818,7 @@ struct mdp_md {
     // autotuned launch variants (see autotune())
     int dedr_variant = 3, yi_waves = 8;
     int ui_variant = 0;   // compute_ui: 0 = one thread per atom and row group, G > 0 = LDS sums on G workgroups
+    int nb_variant = 0;   // cell-list neighbour kernel (neighbors_launch)
     bool tuned = false;
     std::string tuning_json = "{}";
     // timing
@@ -865,7 +981,7 @@ struct mdp_md {
         eatom.ensure(sizeof(double) * n);
         vir.ensure(sizeof(double) * 6 * n);
         flags.ensure(sizeof(int) * 4);
-        maxd2.ensure(sizeof(unsigned long long));
+        maxd2.ensure(2 * sizeof(unsigned long long));   // [0] max displacement^2 bits, [1] neighbour entries
         pair_sum.ensure(sizeof(unsigned long long));
         MDP_HIP(hipMemsetAsync(pair_sum.p, 0, sizeof(unsigned long long), st));
         MDP_HIP(hipMemsetAsync(Yr.p, 0, sizeof(double) * n * iu, st));
@@ -885,13 +1001,28 @@ struct mdp_md {
         dedr.ensure(sizeof(double) * 3 * c);
     }
 
+    // the cell-list neighbour kernel: 0 = eight lanes per atom through L1, 1 = the per-wavefront LDS tile
+    void neighbors_launch(int v) {
+        const double rcs = rcnb;
+        if (v == 1)
+            hipLaunchKernelGGL(k_neighbors_tile, dim3(static_cast<unsigned>(ncell[0] * ncell[1] * ncell[2])), dim3(64), 0,
+                               st, x.as<double>(), cell_start.as<int>(), cell_atoms.as<int>(), nb_cnt.as<int>(),
+                               nb_j.as<int>(), nb_s.as<int>(), L[0], L[1], L[2], ncell[0], ncell[1], ncell[2],
+                               rcs * rcs, N, flags.as<int>());
+        else
+            hipLaunchKernelGGL(k_neighbors_cell, dim3(nb(static_cast<long>(N) * NSUB, 256)), dim3(256), 0, st,
+                               x.as<double>(), cell_of.as<int>(), cell_start.as<int>(), cell_atoms.as<int>(),
+                               nb_cnt.as<int>(), nb_j.as<int>(), nb_s.as<int>(), L[0], L[1], L[2], ncell[0], ncell[1],
+                               ncell[2], rcs * rcs, N, flags.as<int>());
+    }
+
     void rebuild(bool check_disp) {
         const double rcs = rcnb;
         const size_t ncells = static_cast<size_t>(ncell[0]) * ncell[1] * ncell[2];
         MDP_HIP(hipMemsetAsync(cell_cnt.p, 0, sizeof(int) * (ncells + 1), st));
         MDP_HIP(hipMemsetAsync(cell_fill.p, 0, sizeof(int) * (ncells + 1), st));
         MDP_HIP(hipMemsetAsync(flags.p, 0, sizeof(int) * 4, st));
-        MDP_HIP(hipMemsetAsync(maxd2.p, 0, sizeof(unsigned long long), st));
+        MDP_HIP(hipMemsetAsync(maxd2.p, 0, 2 * sizeof(unsigned long long), st));
         MDP_HIP(hipMemsetAsync(in_cnt.p, 0, sizeof(int) * N, st));
         hipLaunchKernelGGL(k_wrap_bin, dim3(nb(N, 256)), dim3(256), 0, st, x.as<double>(), xref.as<double>(),
                            cell_of.as<int>(), cell_cnt.as<int>(), maxd2.as<unsigned long long>(), L[0], L[1], L[2],
@@ -902,11 +1033,7 @@ struct mdp_md {
                                cell_fill.as<int>(), cell_atoms.as<int>(), N);
             hipLaunchKernelGGL(k_cell_sort, dim3(nb(static_cast<long>(ncells), 256)), dim3(256), 0, st,
                                cell_start.as<int>(), cell_atoms.as<int>(), static_cast<int>(ncells), flags.as<int>());
-            hipLaunchKernelGGL(k_neighbors_cell, dim3(nb(static_cast<long>(N) * NSUB, 256)), dim3(256), 0, st,
-                               x.as<double>(), cell_of.as<int>(), cell_start.as<int>(), cell_atoms.as<int>(),
-                               nb_cnt.as<int>(), nb_j.as<int>(),
-                               nb_s.as<int>(), L[0], L[1], L[2], ncell[0], ncell[1], ncell[2], rcs * rcs, N,
-                               flags.as<int>());
+            neighbors_launch(nb_variant);
         } else {
             hipLaunchKernelGGL(k_neighbors_brute, dim3(nb(N, 64)), dim3(64), 0, st, x.as<double>(), nb_cnt.as<int>(),
                                nb_j.as<int>(), nb_s.as<int>(), L[0], L[1], L[2], rcs, rcs * rcs, N, flags.as<int>());
@@ -914,11 +1041,13 @@ struct mdp_md {
         hipLaunchKernelGGL(k_incoming, dim3(nb(N, 256)), dim3(256), 0, st, nb_cnt.as<int>(), nb_j.as<int>(),
                            in_cnt.as<int>(), in_e.as<int>(), N, flags.as<int>());
         hipLaunchKernelGGL(k_incoming_sort, dim3(nb(N, 256)), dim3(256), 0, st, in_cnt.as<int>(), in_e.as<int>(), N);
+        hipLaunchKernelGGL(k_sum_counts, dim3(nb(N, 256)), dim3(256), 0, st, nb_cnt.as<int>(), N,
+                           maxd2.as<unsigned long long>() + 1);
         // a single small readback per rebuild: overflow flags, displacement and entry count
         int fl[4];
-        unsigned long long md2 = 0;
+        unsigned long long md2[2] = {0, 0};
         MDP_HIP(hipMemcpyAsync(fl, flags.p, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
-        MDP_HIP(hipMemcpyAsync(&md2, maxd2.p, sizeof(md2), hipMemcpyDeviceToHost, st));
+        MDP_HIP(hipMemcpyAsync(md2, maxd2.p, sizeof(md2), hipMemcpyDeviceToHost, st));
         MDP_HIP(hipStreamSynchronize(st));
         if (fl[0]) {
             stats[5] += 1;
@@ -927,14 +1056,11 @@ struct mdp_md {
                         ((fl[0] & 8) ? " non-finite or runaway positions" : ""));
         }
         double d2;
-        std::memcpy(&d2, &md2, sizeof(d2));
+        std::memcpy(&d2, &md2[0], sizeof(d2));
         stats[2] += 1;
         if (check_disp && d2 > 0.25 * skin * skin) stats[3] += 1;
         // entry count bounds the pair count until the next rebuild
-        std::vector<int> cnt(N);
-        MDP_HIP(hipMemcpy(cnt.data(), nb_cnt.p, sizeof(int) * N, hipMemcpyDeviceToHost));
-        long tot = 0;
-        for (int c : cnt) tot += c;
+        const long tot = static_cast<long>(md2[1]);
         stats[1] = tot;
         ensure_pairs(static_cast<size_t>(tot));
     }
@@ -1234,9 +1360,28 @@ struct mdp_md {
             js += (k ? ", " : "") + std::string("\"") + std::to_string(uids[k]) + "\": " + std::to_string(t);
             if (t < best) { best = t; ui_variant = uids[k]; }
         }
+        // cell-list neighbour kernel (when the box has cell lists): both write the same lists from the
+        // cell arrays of the last rebuild; MDP_NB_VARIANT pins 0 or 1
+        if (use_cells) {
+            js += "}, \"neighbor_build\": {";
+            int nids[2] = {0, 1}, nn = 2;
+            if (const char* np_ = std::getenv("MDP_NB_VARIANT")) {
+                const int v = std::atoi(np_);
+                if (v != 0 && v != 1) throw Error("MDP_NB_VARIANT is not a neighbour kernel variant (0 or 1)");
+                nids[0] = v;
+                nn = 1;
+            }
+            best = 1e30;
+            for (int k = 0; k < nn; ++k) {
+                const double t = time_launch([&] { neighbors_launch(nids[k]); }, 3);
+                js += (k ? ", " : "") + std::string("\"") + std::to_string(nids[k]) + "\": " + std::to_string(t);
+                if (t < best) { best = t; nb_variant = nids[k]; }
+            }
+            neighbors_launch(nb_variant);
+        }
         js += "}, \"chosen\": {\"compute_dedr\": " + std::to_string(dedr_variant) +
               ", \"compute_yi_waves\": " + std::to_string(yi_waves) + ", \"compute_ui\": " +
-              std::to_string(ui_variant) + "}}";
+              std::to_string(ui_variant) + ", \"neighbor_build\": " + std::to_string(nb_variant) + "}}";
         tuning_json = js;
         tuned = true;
     }

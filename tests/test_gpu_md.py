@@ -46,8 +46,12 @@ def _oracle_bruteforce(oracle, x, box, rc):
     return off, nj, ns.reshape(-1, 3)
 
 
+@pytest.mark.parametrize("variant", ["0", "1"])
 @pytest.mark.parametrize("ncell,a", [(6, 3.316), (2, 3.316), (5, 3.0)])
-def test_neighbor_sets_bit_exact(oracle, ncell, a):
+def test_neighbor_sets_bit_exact(oracle, monkeypatch, ncell, a, variant):
+    """Neighbour lists of the cell-list kernels (0: eight lanes per atom through L1, 1: the per-wavefront
+    LDS tile) as sorted sets against brute force (the 2-cell box uses the brute-force kernel)."""
+    monkeypatch.setenv("MDP_NB_VARIANT", variant)
     x, box = lattice.crystal("bcc", ncell, a, 0.15, 100 + ncell)
     md = _engine(x, box, 4, _coeff(oracle, 4, 1))
     md.compute()
@@ -59,6 +63,25 @@ def test_neighbor_sets_bit_exact(oracle, ncell, a):
         a_ = sorted(zip(nj[off[i]:off[i + 1]].tolist(), map(tuple, ns[off[i]:off[i + 1]].tolist())))
         b_ = sorted(zip(j2[o2[i]:o2[i + 1]].tolist(), map(tuple, s2[o2[i]:o2[i + 1]].tolist())))
         assert a_ == b_, i
+
+
+def test_neighbor_variants_identical(oracle, monkeypatch):
+    """Both cell-list kernels write the same lists, entry for entry (same candidate order), on a hot
+    disordered system with ragged counts, and on a dilute gas with empty cells and isolated atoms."""
+    rng = np.random.default_rng(5)
+    x1, box1 = lattice.crystal("bcc", 9, 3.316, 0.35, 61)
+    x2 = rng.uniform(0.0, 40.0, size=(400, 3))
+    for x, box in ((x1, box1), (x2, np.full(3, 40.0))):
+        lists = []
+        for v in ("0", "1"):
+            monkeypatch.setenv("MDP_NB_VARIANT", v)
+            md = _engine(x, box, 4, _coeff(oracle, 4, 1))
+            md.compute()
+            assert md.tuning()["chosen"]["neighbor_build"] == int(v)
+            lists.append(md.neighbors())
+            md.close()
+        for a_, b_ in zip(*lists):
+            assert np.array_equal(a_, b_)
 
 
 @pytest.mark.parametrize("J,ncell,seed", [(4, 6, 1), (6, 6, 2), (8, 6, 3), (8, 4, 4)])
