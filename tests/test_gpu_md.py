@@ -177,6 +177,33 @@ def test_ui_variants_sparse_gas(oracle, monkeypatch, variant):
     assert np.any(np.diff(off) == 0)
 
 
+@pytest.mark.parametrize("J", [4, 8])
+def test_first_step_two_types_vs_oracle(oracle, J):
+    """Two atom types with their own radii and weights (pair cutoff (R_i + R_j) rcutfac, neighbour weight
+    w_j): energies, forces and virials against the oracle.  With chemflag = 0 upstream applies the first
+    coefficient block to every atom (cpuSnap.cpp:573, :924); the oracle gets a different second block to show
+    that it is not read."""
+    from mdp_amd.md import SnapMD
+    x, box = lattice.crystal("bcc", 6, 3.316, 0.1, 71)
+    types = np.where(np.random.default_rng(72).random(len(x)) < 0.35, 2, 1).astype(np.int32)
+    rad, wt = [0.5, 0.44], [1.0, 0.73]
+    param = [2, J, RCUTFAC, RFAC0, 0, 1, 1, 0, 0, 0, 0]
+    s0 = snapcpp.initsna(oracle, [1, J] + param[2:], [0.5], [1.0])
+    c1 = lattice.snap_coefficients(s0.ncoeff, seed=73)
+    c2 = lattice.snap_coefficients(s0.ncoeff, seed=74)
+    md = SnapMD(x, box, types, J, c1, rad, wt, [MASS, 92.9])
+    md.compute()
+    st = md.state()
+    s = snapcpp.initsna(oracle, param, rad, wt, np.concatenate([c1, c2]))
+    B = np.diag(box)
+    p = snapcpp.pairs_from_box(x, types, B[0], B[1], B[2], [1, 1, 1], s)
+    e, f, v = snapcpp.snappotential_pairs(oracle, s, p)
+    assert md.stats()["pairs"] == p["ijnum"]
+    assert relerr(st["eatom"], e) < TOL
+    assert relerr(st["f"], f) < TOL
+    assert relerr(st["vatom"], v) < TOL
+
+
 def test_first_step_128k_twojmax8_sampled(oracle):
     """The headline system: BCC Ta 40^3 cells (128,000 atoms), sigma = 0.1 A, twojmax = 8."""
     import sampled_oracle
