@@ -43,7 +43,7 @@ struct mdp_fit {
         t_hw, t_mz, t_mw, t_mcomb;
     int dbdr_variant = 0;
     bool zb_lds = false;     // Z + B from the LDS-staged half U (twojmax 4, 6, 8)
-    int zb_waves = 8;        // waves per Z workgroup (autotuned: 8 or 4)
+    int zb_waves = 8;        // waves per Z workgroup (autotuned: 8, 4 or 12)
     long cap = 0;            // pair capacity of the pair and dB/dr buffers
     long npairs = 0;         // pairs of the last compute
     FitTerms terms;
@@ -353,15 +353,26 @@ struct mdp_fit {
             if (v == "5" && j468) dbdr_variant = 5;
             if (v == "6" && j468) dbdr_variant = 6;
         }
-        if (!(env && std::string(env) == "0") && zb_lds) {   // Z kernel: two waves per SIMD or one
-            double tz[2];
-            for (int k = 0; k < 2; ++k) {
-                zb_waves = k ? 4 : 8;
+        const char* zp = std::getenv("MDP_FIT_ZB_WAVES");   // pin the Z kernel's waves per workgroup
+        if (zp) {
+            const int w = std::atoi(zp);
+            if (w != 4 && w != 8 && w != 12) throw Error("MDP_FIT_ZB_WAVES must be 4, 8 or 12");
+            zb_waves = w;
+        }
+        if (!zp && !(env && std::string(env) == "0") && zb_lds) {   // Z kernel: 8, 4 or 12 waves per workgroup
+            const int zw[3] = {8, 4, 12};
+            double tz[3];
+            zb_json = "\"fit_zb\": {";
+            int best = 0;
+            for (int k = 0; k < 3; ++k) {
+                zb_waves = zw[k];
                 stage(2);   // warm-up
                 tz[k] = time_stage(2, 2);
+                if (tz[k] < tz[best]) best = k;
+                zb_json += (k ? ", \"" : "\"") + std::to_string(zw[k]) + "\": " + std::to_string(tz[k]);
             }
-            zb_waves = tz[1] < tz[0] ? 4 : 8;
-            zb_json = "\"fit_zb\": {\"8\": " + std::to_string(tz[0]) + ", \"4\": " + std::to_string(tz[1]) + "}, ";
+            zb_waves = zw[best];
+            zb_json += "}, ";
         }
         if ((env && std::string(env) == "0") || fv) {
             tuning_json = "{" + zb_json + "\"chosen\": {\"fit_zb_waves\": " + std::to_string(zb_waves) + "}}";

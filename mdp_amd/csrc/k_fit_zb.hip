@@ -98,7 +98,11 @@ __device__ __forceinline__ void z_rows_lds(int mb, int atom, bool live, const do
                 for (int m = 0; m <= JO; ++m) { zr[m] = 0.0; zi[m] = 0.0; }
                 z_row_lds<J, J1, J2, JO>(mb, malim, lr, li, lane, cg, scale, zr, zi);
                 if (live) {
-                    double* __restrict__ zo = zt + (static_cast<size_t>(atom) * IZ + ZB + (JO + 1) * mb) * 2;
+                    // the empty asm keeps each coupling's store address here: hoisted out of the row loop, the
+                    // addresses of all couplings took a VGPR pair each (256 VGPRs and 126 spilled at twojmax 8)
+                    size_t aoff = static_cast<size_t>(atom) * IZ;
+                    asm volatile("" : "+v"(aoff));
+                    double* __restrict__ zo = zt + (aoff + ZB + (JO + 1) * mb) * 2;
 #pragma unroll
                     for (int ma = 0; ma <= JO; ++ma)
                         if (ma <= malim) {
@@ -116,7 +120,11 @@ __device__ __forceinline__ void z_rows_lds(int mb, int atom, bool live, const do
                             const int e = HB + (JO + 1) * mb + ma;
                             sb += w * (lr[e * 64 + lane] * zr[ma] + li[e * 64 + lane] * zi[ma]);
                         }
-                    if (live) bpart[static_cast<size_t>(BP + mb) * n + atom] = sb;
+                    if (live) {
+                        size_t bo = static_cast<size_t>(BP + mb) * n + atom;
+                        asm volatile("" : "+v"(bo));
+                        bpart[bo] = sb;
+                    }
                 }
                 // couplings are independent: keep the scheduler from interleaving them (register pressure)
                 __builtin_amdgcn_sched_barrier(0);
@@ -203,6 +211,9 @@ bool launch_fit_zb_lds(const SnapDev& s, const double* Uhr, const double* Uhi, d
         if (waves == 4)                                                                                        \
             hipLaunchKernelGGL((k_fit_z_lds<JV, 4>), grid, dim3(256), 0, st, s.cg, Uhr, Uhi, zt, bpart, row_order, \
                                N, bnorm);                                                                      \
+        else if (waves == 12)                                                                                  \
+            hipLaunchKernelGGL((k_fit_z_lds<JV, 12>), grid, dim3(768), 0, st, s.cg, Uhr, Uhi, zt, bpart,        \
+                               row_order, N, bnorm);                                                           \
         else                                                                                                   \
             hipLaunchKernelGGL((k_fit_z_lds<JV, 8>), grid, dim3(512), 0, st, s.cg, Uhr, Uhi, zt, bpart, row_order, \
                                N, bnorm);                                                                      \

@@ -113,6 +113,24 @@ def test_small_batch_vs_oracle(oracle, monkeypatch, J, variant):
         _check_config(oracle, fit, c, cfg, b, db, ai, aj, rij, J)
 
 
+@pytest.mark.parametrize("waves", ["4", "8", "12"])
+@pytest.mark.parametrize("J", [6, 8])
+def test_z_kernel_waves_vs_oracle(oracle, monkeypatch, J, waves):
+    """The Z + B kernel pinned to 4, 8 or 12 waves per workgroup (B and, through Z, dB/dr against the oracle)."""
+    from mdp_amd.fit import SnapFit
+    monkeypatch.setenv("MDP_FIT_ZB_WAVES", waves)
+    monkeypatch.setenv("MDP_FIT_DBDR_VARIANT", "6")
+    cfg = _small_batch()
+    print(f"small batch J={J}, Z kernel {waves} waves", flush=True)
+    fit = SnapFit(cfg, J)
+    fit.compute()
+    assert fit.tuning()["chosen"]["fit_zb_waves"] == int(waves)
+    b, db = fit.raw()
+    ai, aj, rij = fit.pairs()
+    for c in range(len(cfg)):
+        _check_config(oracle, fit, c, cfg, b, db, ai, aj, rij, J)
+
+
 @pytest.mark.parametrize("J", [4, 8])
 def test_dbdr_variants_agree(monkeypatch, J):
     """The autotuner may pick any dB/dr variant: B is the same kernel in all of them (bit for bit), and
