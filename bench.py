@@ -312,6 +312,10 @@ def bench_fit(a):
                    "atoms": natoms, "pairs": npairs, "twojmax": J, "parallelism": "1 GPU",
                    "autotuned": fit.tuning().get("chosen", {})},
         "roofline": {"bound": "mfma", "kernel": "fit_" + dom, "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "pipe": ("FP64 matrix (v_mfma_f64_16x16x4f64)"
+                              if dom == "dbdr" and str(fit.tuning().get("chosen", {}).get(
+                                  "fit_dbdr_variant", os.environ.get("MDP_FIT_DBDR_VARIANT"))) == "6"
+                              else "FP64 vector"),
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "flops_per_launch": work, "ms_per_launch": prof[dom]},
         "cpu_baseline": None,
@@ -670,6 +674,7 @@ def main():
                    "atoms": n, "twojmax": J, "pairs_per_atom": round(ppa, 3), "parallelism": "1 GPU",
                    "autotuned": md.tuning().get("chosen", {})},
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "pipe": "FP64 vector (FMA; the FP64 vector and matrix peaks of MI355X are the same)",
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "flops_per_launch": per_launch[dom], "ms_per_launch": ms_avg},
         "cpu_baseline": None,
