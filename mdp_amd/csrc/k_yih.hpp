@@ -16,16 +16,25 @@
 
 namespace mdp {
 
-// one U row of layer J1 (row mb1, any mb1 in 0..J1) of this lane's atom from the LDS half layout
-template <int J1>
+// one U row of layer J1 (row mb1, any mb1 in 0..J1) of this lane's atom from the LDS half layout.
+// CX: the real and imaginary parts of an element are adjacent (lr holds (re, im) pairs, li is unused), so
+// one 16-byte LDS read fetches an element.
+template <int J1, bool CX = false>
 __device__ __forceinline__ void lds_row(int mb1, const double* __restrict__ lr, const double* __restrict__ li,
                                         int lane, double (&ar)[J1 + 1], double (&ai)[J1 + 1]) {
+    const double2* __restrict__ lc = reinterpret_cast<const double2*>(lr);
     if (2 * mb1 <= J1) {
         const int b = (h_block(J1) + (J1 + 1) * mb1) * 64 + lane;
 #pragma unroll
         for (int m = 0; m <= J1; ++m) {
-            ar[m] = lr[b + m * 64];
-            ai[m] = li[b + m * 64];
+            if constexpr (CX) {
+                const double2 v = lc[b + m * 64];
+                ar[m] = v.x;
+                ai[m] = v.y;
+            } else {
+                ar[m] = lr[b + m * 64];
+                ai[m] = li[b + m * 64];
+            }
         }
     } else {
         const int b = (h_block(J1) + (J1 + 1) * (J1 - mb1)) * 64 + lane;
@@ -33,8 +42,14 @@ __device__ __forceinline__ void lds_row(int mb1, const double* __restrict__ lr, 
 #pragma unroll
         for (int m = 0; m <= J1; ++m) {
             const double s = (m & 1) ? -sm : sm;
-            ar[m] = s * lr[b + (J1 - m) * 64];
-            ai[m] = -s * li[b + (J1 - m) * 64];
+            if constexpr (CX) {
+                const double2 v = lc[b + (J1 - m) * 64];
+                ar[m] = s * v.x;
+                ai[m] = -s * v.y;
+            } else {
+                ar[m] = s * lr[b + (J1 - m) * 64];
+                ai[m] = -s * li[b + (J1 - m) * 64];
+            }
         }
     }
 }
@@ -102,7 +117,7 @@ __device__ __forceinline__ void z_terms(int malim, const double (&ar)[J1 + 1], c
 // of layer J1 are read from LDS once and used by every J2 whose partner row mb2 = mb + (J1+J2-JO)/2 - mb1
 // lies in layer J2, instead of once per coupling (30% fewer LDS reads at twojmax 8).  The same terms as
 // z_row_lds, summed in the order (J1, mb1, J2) instead of (J1, J2, mb1).
-template <int J, int JO>
+template <int J, int JO, bool CX = false>
 __device__ __forceinline__ void y_couplings_shared_a(int mb, int malim, const double* __restrict__ lr,
                                                      const double* __restrict__ li, int lane,
                                                      const double* __restrict__ cg,
@@ -120,7 +135,7 @@ __device__ __forceinline__ void y_couplings_shared_a(int mb, int malim, const do
         });
         for (int mb1 = lo; mb1 <= hi; ++mb1) {
             double ar[J1 + 1], ai[J1 + 1];
-            lds_row<J1>(mb1, lr, li, lane, ar, ai);
+            lds_row<J1, CX>(mb1, lr, li, lane, ar, ai);
             static_for<0, J1 + 1>([&](auto J2c) {
                 constexpr int J2 = decltype(J2c)::value;
                 if constexpr (coupled(J1, J2, JO)) {
@@ -129,7 +144,7 @@ __device__ __forceinline__ void y_couplings_shared_a(int mb, int malim, const do
                         constexpr int T = triple_id(J, J1, J2, JO);
                         const double cb = cg[cg_block(J, J1, J2, JO) + mb1 * (J2 + 1) + mb2] * beta_t[T];
                         double br[J2 + 1], bi[J2 + 1];
-                        lds_row<J2>(mb2, lr, li, lane, br, bi);
+                        lds_row<J2, CX>(mb2, lr, li, lane, br, bi);
                         z_terms<J1, J2, JO>(malim, ar, ai, br, bi, cb, yr, yi);
                     }
                 }
@@ -139,8 +154,9 @@ __device__ __forceinline__ void y_couplings_shared_a(int mb, int malim, const do
 }
 
 // PA: per-atom weights beta_t[triple * N + atom] (the learned surrogate's dE/dB) and no energy partials;
-// otherwise wave-uniform weights beta_t[triple].  SA: couplings through y_couplings_shared_a.
-template <int J, int JO, bool PA = false, bool SA = false>
+// otherwise wave-uniform weights beta_t[triple].  SA: couplings through y_couplings_shared_a; CX: the
+// complex-interleaved LDS layout (with SA only).
+template <int J, int JO, bool PA = false, bool SA = false, bool CX = false>
 __device__ __forceinline__ void y_row_lds(const double* __restrict__ cg, int row, int mb, int atom, bool live, int N,
                                           const double* __restrict__ lr, const double* __restrict__ li, int lane,
                                           const double* __restrict__ beta_t, double* __restrict__ Yr,
@@ -151,7 +167,7 @@ __device__ __forceinline__ void y_row_lds(const double* __restrict__ cg, int row
 #pragma unroll
     for (int m = 0; m <= JO; ++m) { yr[m] = 0.0; yi[m] = 0.0; }
     if constexpr (SA && !PA) {
-        y_couplings_shared_a<J, JO>(mb, malim, lr, li, lane, cg, beta_t, yr, yi);
+        y_couplings_shared_a<J, JO, CX>(mb, malim, lr, li, lane, cg, beta_t, yr, yi);
     } else {
         static_for<0, J + 1>([&](auto J1c) {
             constexpr int J1 = decltype(J1c)::value;
@@ -176,7 +192,12 @@ __device__ __forceinline__ void y_row_lds(const double* __restrict__ cg, int row
             Yr[o + ma * n] = yr[ma];
             Yi[o + ma * n] = yi[ma];
             const double w = (2 * mb < JO || ma < mb) ? 1.0 : 0.5;
-            e += w * (lr[(hrow + ma) * 64 + lane] * yr[ma] + li[(hrow + ma) * 64 + lane] * yi[ma]);
+            if constexpr (CX) {
+                const double2 u = reinterpret_cast<const double2*>(lr)[(hrow + ma) * 64 + lane];
+                e += w * (u.x * yr[ma] + u.y * yi[ma]);
+            } else {
+                e += w * (lr[(hrow + ma) * 64 + lane] * yr[ma] + li[(hrow + ma) * 64 + lane] * yi[ma]);
+            }
         }
     }
     if constexpr (!PA) epart[static_cast<size_t>(row) * n + atom] = e;
@@ -184,14 +205,16 @@ __device__ __forceinline__ void y_row_lds(const double* __restrict__ cg, int row
 
 // The coefficient tables (cg, beta_t, order) are separate read-only, non-aliased kernel arguments so
 // that their wave-uniform loads go through the scalar cache into SGPRs.
-template <int J, int W, bool PA = false, bool SA = false>
+template <int J, int W, bool PA = false, bool SA = false, bool CX = false>
 __global__ __launch_bounds__(64 * W) void k_yi_half(const double* __restrict__ cg, const double* __restrict__ Ur,
                                                     const double* __restrict__ Ui, const double* __restrict__ beta_t,
                                                     double* __restrict__ Yr, double* __restrict__ Yi,
                                                     double* __restrict__ epart, const int* __restrict__ order, int N) {
     constexpr int HE = n_half_elems(J);
     constexpr int NR = n_half_rows(J);
-    __shared__ double lr[HE * 64], li[HE * 64];
+    __shared__ double lbuf[2 * HE * 64];   // lr | li, or (re, im) pairs under CX
+    double* const lr = lbuf;
+    double* const li = CX ? nullptr : lbuf + HE * 64;
     __shared__ int next_row;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int atom = blockIdx.x * 64 + lane;
@@ -199,8 +222,14 @@ __global__ __launch_bounds__(64 * W) void k_yi_half(const double* __restrict__ c
     const size_t n = static_cast<size_t>(N);
     if (threadIdx.x == 0) next_row = 0;
     for (int e = wave; e < HE; e += W) {
-        lr[e * 64 + lane] = live ? Ur[static_cast<size_t>(e) * n + atom] : 0.0;
-        li[e * 64 + lane] = live ? Ui[static_cast<size_t>(e) * n + atom] : 0.0;
+        const double ur = live ? Ur[static_cast<size_t>(e) * n + atom] : 0.0;
+        const double ui = live ? Ui[static_cast<size_t>(e) * n + atom] : 0.0;
+        if constexpr (CX) {
+            reinterpret_cast<double2*>(lbuf)[e * 64 + lane] = make_double2(ur, ui);
+        } else {
+            lr[e * 64 + lane] = ur;
+            li[e * 64 + lane] = ui;
+        }
     }
     __syncthreads();
     for (;;) {
@@ -213,7 +242,8 @@ __global__ __launch_bounds__(64 * W) void k_yi_half(const double* __restrict__ c
         const int jo = half_row_j(row), mb = half_row_mb(row);
         static_for<0, J + 1>([&](auto JOc) {
             constexpr int JO = decltype(JOc)::value;
-            if (jo == JO) y_row_lds<J, JO, PA, SA>(cg, row, mb, atom, live, N, lr, li, lane, beta_t, Yr, Yi, epart);
+            if (jo == JO)
+                y_row_lds<J, JO, PA, SA, CX>(cg, row, mb, atom, live, N, lr, li, lane, beta_t, Yr, Yi, epart);
         });
     }
 }
