@@ -80,7 +80,7 @@ __host__ __device__ constexpr int zb_n_bp(int J) {
 // entry beyond it.  For bispectrum triples (JO >= J1) the row's share of B, sum' U*(JO).Z over its
 // entries (the middle entry of an even layer halved; cpuSnap.cpp:1971-2072), goes to its own slot of
 // bpart ([slot][N]), summed per atom in fixed order by k_fit_bsum.
-template <int J, int JO>
+template <int J, int JO, bool CX>
 __device__ __forceinline__ void z_rows_lds(int mb, int atom, bool live, const double* __restrict__ lr,
                                            const double* __restrict__ li, int lane, const double* __restrict__ cg,
                                            double scale, double* __restrict__ zt, double* __restrict__ bpart,
@@ -96,7 +96,7 @@ __device__ __forceinline__ void z_rows_lds(int mb, int atom, bool live, const do
                 double zr[JO + 1], zi[JO + 1];
 #pragma unroll
                 for (int m = 0; m <= JO; ++m) { zr[m] = 0.0; zi[m] = 0.0; }
-                z_row_lds<J, J1, J2, JO>(mb, malim, lr, li, lane, cg, scale, zr, zi);
+                z_row_lds<J, J1, J2, JO, CX>(mb, malim, lr, li, lane, cg, scale, zr, zi);
                 if (live) {
                     // the empty asm keeps each coupling's store address here: hoisted out of the row loop, the
                     // addresses of all couplings took a VGPR pair each (256 VGPRs and 126 spilled at twojmax 8)
@@ -118,7 +118,12 @@ __device__ __forceinline__ void z_rows_lds(int mb, int atom, bool live, const do
                         if (ma <= malim) {
                             const double w = (2 * mb < JO || ma < mb) ? 1.0 : 0.5;
                             const int e = HB + (JO + 1) * mb + ma;
-                            sb += w * (lr[e * 64 + lane] * zr[ma] + li[e * 64 + lane] * zi[ma]);
+                            if constexpr (CX) {
+                                const double2 u = reinterpret_cast<const double2*>(lr)[e * 64 + lane];
+                                sb += w * (u.x * zr[ma] + u.y * zi[ma]);
+                            } else {
+                                sb += w * (lr[e * 64 + lane] * zr[ma] + li[e * 64 + lane] * zi[ma]);
+                            }
                         }
                     if (live) {
                         size_t bo = static_cast<size_t>(BP + mb) * n + atom;
@@ -133,13 +138,16 @@ __device__ __forceinline__ void z_rows_lds(int mb, int atom, bool live, const do
     });
 }
 
-template <int J, int W>
+// (re, im) of each U element adjacent in LDS: one 16-byte read per element
+template <int J, int W, bool CX = true>
 __global__ __launch_bounds__(64 * W) void k_fit_z_lds(const double* __restrict__ cg, const double* __restrict__ Ur,
                                                       const double* __restrict__ Ui, double* __restrict__ zt,
                                                       double* __restrict__ bpart, const int* __restrict__ order, int N,
                                                       int bnorm) {
     constexpr int HE = n_half_elems(J), NR = n_half_rows(J);
-    __shared__ double lr[HE * 64], li[HE * 64];
+    __shared__ double lbuf[2 * HE * 64];   // lr | li, or (re, im) pairs under CX
+    double* const lr = lbuf;
+    double* const li = CX ? nullptr : lbuf + HE * 64;
     __shared__ int next_row;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int atom = blockIdx.x * 64 + lane;
@@ -147,8 +155,14 @@ __global__ __launch_bounds__(64 * W) void k_fit_z_lds(const double* __restrict__
     const size_t n = static_cast<size_t>(N);
     if (threadIdx.x == 0) next_row = 0;
     for (int e = wave; e < HE; e += W) {
-        lr[e * 64 + lane] = live ? Ur[static_cast<size_t>(e) * n + atom] : 0.0;
-        li[e * 64 + lane] = live ? Ui[static_cast<size_t>(e) * n + atom] : 0.0;
+        const double ur = live ? Ur[static_cast<size_t>(e) * n + atom] : 0.0;
+        const double ui = live ? Ui[static_cast<size_t>(e) * n + atom] : 0.0;
+        if constexpr (CX) {
+            reinterpret_cast<double2*>(lbuf)[e * 64 + lane] = make_double2(ur, ui);
+        } else {
+            lr[e * 64 + lane] = ur;
+            li[e * 64 + lane] = ui;
+        }
     }
     __syncthreads();
     for (;;) {
@@ -161,7 +175,8 @@ __global__ __launch_bounds__(64 * W) void k_fit_z_lds(const double* __restrict__
         static_for<0, J + 1>([&](auto JOc) {
             constexpr int JO = decltype(JOc)::value;
             if (jo == JO)
-                z_rows_lds<J, JO>(mb, atom, live, lr, li, lane, cg, bnorm ? 1.0 / (JO + 1) : 1.0, zt, bpart, n);
+                z_rows_lds<J, JO, CX>(mb, atom, live, lr, li, lane, cg, bnorm ? 1.0 / (JO + 1) : 1.0, zt, bpart,
+                                      n);
         });
     }
 }

@@ -54,7 +54,7 @@ __device__ __forceinline__ void lds_row(int mb1, const double* __restrict__ lr, 
     }
 }
 
-template <int J, int J1, int J2, int JO>
+template <int J, int J1, int J2, int JO, bool CX = false>
 __device__ __forceinline__ void z_row_lds(int mb, int malim, const double* __restrict__ lr,
                                           const double* __restrict__ li, int lane, const double* __restrict__ cg,
                                           double scale, double (&zr)[JO + 1], double (&zi)[JO + 1]) {
@@ -67,8 +67,8 @@ __device__ __forceinline__ void z_row_lds(int mb, int malim, const double* __res
         const int mb1 = mb1min + ib, mb2 = mb2max - ib;
         const double cb = c[mb1 * (J2 + 1) + mb2] * scale;
         double ar[J1 + 1], ai[J1 + 1], br[J2 + 1], bi[J2 + 1];
-        lds_row<J1>(mb1, lr, li, lane, ar, ai);
-        lds_row<J2>(mb2, lr, li, lane, br, bi);
+        lds_row<J1, CX>(mb1, lr, li, lane, ar, ai);
+        lds_row<J2, CX>(mb2, lr, li, lane, br, bi);
         static_for<0, JO + 1>([&](auto MAc) {
             constexpr int MA = decltype(MAc)::value;
             constexpr int LO = cg_lo(MA, J1, J2, JO), HI2 = cg_hi2(MA, J1, J2, JO), NA = cg_n(MA, J1, J2, JO);
@@ -155,7 +155,7 @@ __device__ __forceinline__ void y_couplings_shared_a(int mb, int malim, const do
 
 // PA: per-atom weights beta_t[triple * N + atom] (the learned surrogate's dE/dB) and no energy partials;
 // otherwise wave-uniform weights beta_t[triple].  SA: couplings through y_couplings_shared_a; CX: the
-// complex-interleaved LDS layout (with SA only).
+// complex-interleaved LDS layout.
 template <int J, int JO, bool PA = false, bool SA = false, bool CX = false>
 __device__ __forceinline__ void y_row_lds(const double* __restrict__ cg, int row, int mb, int atom, bool live, int N,
                                           const double* __restrict__ lr, const double* __restrict__ li, int lane,
@@ -177,7 +177,7 @@ __device__ __forceinline__ void y_row_lds(const double* __restrict__ cg, int row
                     constexpr int T = triple_id(J, J1, J2, JO);
                     const double beta =
                         PA ? (live ? beta_t[static_cast<size_t>(T) * n + atom] : 0.0) : beta_t[T];
-                    z_row_lds<J, J1, J2, JO>(mb, malim, lr, li, lane, cg, beta, yr, yi);
+                    z_row_lds<J, J1, J2, JO, CX>(mb, malim, lr, li, lane, cg, beta, yr, yi);
                 }
             });
         });
@@ -252,12 +252,14 @@ __global__ __launch_bounds__(64 * W) void k_yi_half(const double* __restrict__ c
 // it: B = 2 sum' U*(j).Z(j1,j2,j) (cpuSnap.cpp:1899-2072 semantics, no bzero subtraction), Z rows from
 // z_row_lds, the middle column of even layers weighted 1/2.  Waves take whole triples from a shared
 // counter; blist[jjb * N + atom].
-template <int J, int W>
+template <int J, int W, bool CX = true>
 __global__ __launch_bounds__(64 * W) void k_b_half(const double* __restrict__ cg, const double* __restrict__ Ur,
                                                    const double* __restrict__ Ui, double* __restrict__ blist, int N,
                                                    int bnorm) {
     constexpr int HE = n_half_elems(J), NB = n_bisp(J);
-    __shared__ double lr[HE * 64], li[HE * 64];
+    __shared__ double lbuf[2 * HE * 64];   // lr | li, or (re, im) pairs under CX
+    double* const lr = lbuf;
+    double* const li = CX ? nullptr : lbuf + HE * 64;
     __shared__ int next_task;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int atom = blockIdx.x * 64 + lane;
@@ -265,8 +267,14 @@ __global__ __launch_bounds__(64 * W) void k_b_half(const double* __restrict__ cg
     const size_t n = static_cast<size_t>(N);
     if (threadIdx.x == 0) next_task = 0;
     for (int e = wave; e < HE; e += W) {
-        lr[e * 64 + lane] = live ? Ur[static_cast<size_t>(e) * n + atom] : 0.0;
-        li[e * 64 + lane] = live ? Ui[static_cast<size_t>(e) * n + atom] : 0.0;
+        const double ur = live ? Ur[static_cast<size_t>(e) * n + atom] : 0.0;
+        const double ui = live ? Ui[static_cast<size_t>(e) * n + atom] : 0.0;
+        if constexpr (CX) {
+            reinterpret_cast<double2*>(lbuf)[e * 64 + lane] = make_double2(ur, ui);
+        } else {
+            lr[e * 64 + lane] = ur;
+            li[e * 64 + lane] = ui;
+        }
     }
     __syncthreads();
     for (;;) {
@@ -289,15 +297,21 @@ __global__ __launch_bounds__(64 * W) void k_b_half(const double* __restrict__ cg
                                 double zr[JO + 1], zi[JO + 1];
 #pragma unroll
                                 for (int m = 0; m <= JO; ++m) { zr[m] = 0.0; zi[m] = 0.0; }
-                                z_row_lds<J, J1, J2, JO>(mb, malim, lr, li, lane, cg, bnorm ? 1.0 / (JO + 1) : 1.0, zr,
-                                                         zi);
+                                z_row_lds<J, J1, J2, JO, CX>(mb, malim, lr, li, lane, cg, bnorm ? 1.0 / (JO + 1) : 1.0,
+                                                             zr, zi);
                                 const int hrow = h_block(JO) + (JO + 1) * mb;
 #pragma unroll
                                 for (int ma = 0; ma <= JO; ++ma)
                                     if (ma <= malim) {
                                         const double w = (2 * mb < JO || ma < mb) ? 1.0 : 0.5;
-                                        sum += w * (lr[(hrow + ma) * 64 + lane] * zr[ma] +
-                                                    li[(hrow + ma) * 64 + lane] * zi[ma]);
+                                        if constexpr (CX) {
+                                            const double2 u =
+                                                reinterpret_cast<const double2*>(lr)[(hrow + ma) * 64 + lane];
+                                            sum += w * (u.x * zr[ma] + u.y * zi[ma]);
+                                        } else {
+                                            sum += w * (lr[(hrow + ma) * 64 + lane] * zr[ma] +
+                                                        li[(hrow + ma) * 64 + lane] * zi[ma]);
+                                        }
                                     }
                             }
                             if (live) blist[static_cast<size_t>(JJB) * n + atom] = 2.0 * sum;
