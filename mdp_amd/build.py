@@ -32,8 +32,8 @@ FLAGS = ["-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC", "-Wall", "-Wno-
 # constants; machine-level loop-invariant code motion would hoist thousands of them out of the row loop
 # into SGPRs and spill those to VGPR lanes, so it is disabled for these translation units.
 FILE_FLAGS = {f: ["-mllvm", "-disable-machine-licm"]
-              for f in ("k_yih_8_8.hip", "k_yih_8_12.hip", "k_yih_8_4.hip", "k_yih_8_12s.hip", "k_yih_8_12c.hip", "k_yih_small.hip",
-                        "k_fit_zb.hip")}
+              for f in ("k_yih_8_8.hip", "k_yih_8_12.hip", "k_yih_8_4.hip", "k_yih_8_12s.hip", "k_yih_8_12c.hip",
+                        "k_yih_small.hip", "k_fit_zb.hip")}
 
 
 def hipcc():
