@@ -653,8 +653,8 @@ def main():
     dom = max(per_launch, key=lambda k: timers[k][0])
     ms_avg = timers[dom][0] / max(timers[dom][1], 1)
     achieved = per_launch[dom] / (ms_avg * 1e-3) / 1e12
-    # HBM bytes per launch of the dominant kernel, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
-    # workload and calibrated with a copy kernel of known size (profiles/traffic_<kernel>_bcc40_j8.*)
+    # HBM bytes per launch of the dominant kernel, measured on this workload with rocprofv3's L2 read and
+    # write request counters by size, which count exact bytes (profiles/traffic_<kernel>_bcc40_j8.*)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_bcc40_j8.json")
     if os.path.exists(tfile) and a.ncell == NCELL:
