@@ -185,9 +185,10 @@ void launch_ui(const SnapDev& s, UiMode mode, double* Ur, double* Ui, const doub
 // are pairs: workgroup block b holds the atoms whose first pair lies in [b*UI_PB, (b+1)*UI_PB), so with at
 // most UI_MAXP pairs per atom a block has fewer than UI_PB + UI_MAXP <= 512 pairs, one per lane.  Each
 // lane runs the row-chain recursion of its pair; every finished half row, scaled by sfac w_j, goes to LDS;
-// then thread (ma, atom) adds the row entry of the atom's pairs in pair order and writes U (plus wself on
-// the diagonal) -- the same sum, in the same order, as upstream's per-atom loop (cpuSnap.cpp:312-542).  The
-// workgroups are persistent over the blocks; all lanes share the control flow of the recursion.
+// then thread (ma, atom) adds the row entry of the atom's pairs in pair order and writes U, with wself added
+// to the diagonal last (upstream adds it first, cpuSnap.cpp:312-542; the sums agree to rounding, as those of
+// the per-atom kernel do).  The workgroups are persistent over the blocks; all lanes share the control flow
+// of the recursion.
 constexpr int UI_PB = 416, UI_MAXP = 96, UI_NL = 512;
 
 template <int J>
