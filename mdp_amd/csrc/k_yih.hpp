@@ -113,11 +113,40 @@ __device__ __forceinline__ void z_terms(int malim, const double (&ar)[J1 + 1], c
     });
 }
 
+// The terms of a coupling (J1, J1) -> JO whose two U rows are the same row (mb1 = mb2): the terms (m1, m2)
+// and (m2, m1) are equal (the two CG factors change sign together under the swap), so each unordered pair
+// is taken once, the off-diagonal ones twice.
+template <int J1, int JO>
+__device__ __forceinline__ void z_terms_same_row(int malim, const double (&ar)[J1 + 1], const double (&ai)[J1 + 1],
+                                                 double cb, double (&zr)[JO + 1], double (&zi)[JO + 1]) {
+    static_for<0, JO + 1>([&](auto MAc) {
+        constexpr int MA = decltype(MAc)::value;
+        constexpr int LO = cg_lo(MA, J1, J1, JO), HI2 = cg_hi2(MA, J1, J1, JO), NA = cg_n(MA, J1, J1, JO);
+        if (MA <= malim) {
+            double sr = 0.0, si = 0.0;
+            static_for<0, NA>([&](auto IAc) {
+                constexpr int IA = decltype(IAc)::value;
+                constexpr int M1 = LO + IA, M2 = HI2 - IA;
+                if constexpr (M1 <= M2) {
+                    constexpr double cga = (M1 < M2 ? 2.0 : 1.0) * cg_const(J1, J1, JO, M1, M2);
+                    sr += cga * (ar[M1] * ar[M2] - ai[M1] * ai[M2]);
+                    si += cga * (ar[M1] * ai[M2] + ai[M1] * ar[M2]);
+                }
+            });
+            zr[MA] += cb * sr;
+            zi[MA] += cb * si;
+        }
+    });
+}
+
 // All couplings of output row (JO, mb) with the U row of the larger layer shared: for each J1 the rows mb1
 // of layer J1 are read from LDS once and used by every J2 whose partner row mb2 = mb + (J1+J2-JO)/2 - mb1
 // lies in layer J2, instead of once per coupling (30% fewer LDS reads at twojmax 8).  The same terms as
 // z_row_lds, summed in the order (J1, mb1, J2) instead of (J1, J2, mb1).
-template <int J, int JO, bool CX = false>
+// SYM: a coupling (J1, J1) takes each unordered pair of rows {mb1, mb2} once (the sum is symmetric under
+// the swap of its two U factors): mb1 > mb2 is skipped, mb1 < mb2 counts twice, mb1 = mb2 through
+// z_terms_same_row -- 17% fewer terms at twojmax 8.
+template <int J, int JO, bool CX = false, bool SYM = false>
 __device__ __forceinline__ void y_couplings_shared_a(int mb, int malim, const double* __restrict__ lr,
                                                      const double* __restrict__ li, int lane,
                                                      const double* __restrict__ cg,
@@ -140,12 +169,18 @@ __device__ __forceinline__ void y_couplings_shared_a(int mb, int malim, const do
                 constexpr int J2 = decltype(J2c)::value;
                 if constexpr (coupled(J1, J2, JO)) {
                     const int mb2 = mb + (J1 + J2 - JO) / 2 - mb1;
-                    if (mb2 >= 0 && mb2 <= J2) {
+                    constexpr bool SAME = SYM && (J1 == J2);
+                    if (mb2 >= 0 && mb2 <= J2 && (!SAME || mb2 >= mb1)) {
                         constexpr int T = triple_id(J, J1, J2, JO);
-                        const double cb = cg[cg_block(J, J1, J2, JO) + mb1 * (J2 + 1) + mb2] * beta_t[T];
-                        double br[J2 + 1], bi[J2 + 1];
-                        lds_row<J2, CX>(mb2, lr, li, lane, br, bi);
-                        z_terms<J1, J2, JO>(malim, ar, ai, br, bi, cb, yr, yi);
+                        double cb = cg[cg_block(J, J1, J2, JO) + mb1 * (J2 + 1) + mb2] * beta_t[T];
+                        if (SAME && mb2 == mb1) {
+                            z_terms_same_row<J1, JO>(malim, ar, ai, cb, yr, yi);
+                        } else {
+                            if (SAME) cb *= 2.0;
+                            double br[J2 + 1], bi[J2 + 1];
+                            lds_row<J2, CX>(mb2, lr, li, lane, br, bi);
+                            z_terms<J1, J2, JO>(malim, ar, ai, br, bi, cb, yr, yi);
+                        }
                     }
                 }
             });
@@ -155,8 +190,8 @@ __device__ __forceinline__ void y_couplings_shared_a(int mb, int malim, const do
 
 // PA: per-atom weights beta_t[triple * N + atom] (the learned surrogate's dE/dB) and no energy partials;
 // otherwise wave-uniform weights beta_t[triple].  SA: couplings through y_couplings_shared_a; CX: the
-// complex-interleaved LDS layout.
-template <int J, int JO, bool PA = false, bool SA = false, bool CX = false>
+// complex-interleaved LDS layout; SYM (with SA): the swap symmetry of the (J1, J1) couplings.
+template <int J, int JO, bool PA = false, bool SA = false, bool CX = false, bool SYM = false>
 __device__ __forceinline__ void y_row_lds(const double* __restrict__ cg, int row, int mb, int atom, bool live, int N,
                                           const double* __restrict__ lr, const double* __restrict__ li, int lane,
                                           const double* __restrict__ beta_t, double* __restrict__ Yr,
@@ -167,7 +202,7 @@ __device__ __forceinline__ void y_row_lds(const double* __restrict__ cg, int row
 #pragma unroll
     for (int m = 0; m <= JO; ++m) { yr[m] = 0.0; yi[m] = 0.0; }
     if constexpr (SA && !PA) {
-        y_couplings_shared_a<J, JO, CX>(mb, malim, lr, li, lane, cg, beta_t, yr, yi);
+        y_couplings_shared_a<J, JO, CX, SYM>(mb, malim, lr, li, lane, cg, beta_t, yr, yi);
     } else {
         static_for<0, J + 1>([&](auto J1c) {
             constexpr int J1 = decltype(J1c)::value;
@@ -205,7 +240,7 @@ __device__ __forceinline__ void y_row_lds(const double* __restrict__ cg, int row
 
 // The coefficient tables (cg, beta_t, order) are separate read-only, non-aliased kernel arguments so
 // that their wave-uniform loads go through the scalar cache into SGPRs.
-template <int J, int W, bool PA = false, bool SA = false, bool CX = false>
+template <int J, int W, bool PA = false, bool SA = false, bool CX = false, bool SYM = false>
 __global__ __launch_bounds__(64 * W) void k_yi_half(const double* __restrict__ cg, const double* __restrict__ Ur,
                                                     const double* __restrict__ Ui, const double* __restrict__ beta_t,
                                                     double* __restrict__ Yr, double* __restrict__ Yi,
@@ -243,7 +278,7 @@ __global__ __launch_bounds__(64 * W) void k_yi_half(const double* __restrict__ c
         static_for<0, J + 1>([&](auto JOc) {
             constexpr int JO = decltype(JOc)::value;
             if (jo == JO)
-                y_row_lds<J, JO, PA, SA, CX>(cg, row, mb, atom, live, N, lr, li, lane, beta_t, Yr, Yi, epart);
+                y_row_lds<J, JO, PA, SA, CX, SYM>(cg, row, mb, atom, live, N, lr, li, lane, beta_t, Yr, Yi, epart);
         });
     }
 }
