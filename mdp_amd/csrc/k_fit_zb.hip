@@ -96,7 +96,7 @@ __device__ __forceinline__ void z_rows_lds(int mb, int atom, bool live, const do
                 double zr[JO + 1], zi[JO + 1];
 #pragma unroll
                 for (int m = 0; m <= JO; ++m) { zr[m] = 0.0; zi[m] = 0.0; }
-                z_row_lds<J, J1, J2, JO, CX>(mb, malim, lr, li, lane, cg, scale, zr, zi);
+                z_row_lds<J, J1, J2, JO, CX, true>(mb, malim, lr, li, lane, cg, scale, zr, zi);
                 if (live) {
                     // the empty asm keeps each coupling's store address here: hoisted out of the row loop, the
                     // addresses of all couplings took a VGPR pair each (256 VGPRs and 126 spilled at twojmax 8)

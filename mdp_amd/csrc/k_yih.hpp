@@ -54,41 +54,6 @@ __device__ __forceinline__ void lds_row(int mb1, const double* __restrict__ lr, 
     }
 }
 
-template <int J, int J1, int J2, int JO, bool CX = false>
-__device__ __forceinline__ void z_row_lds(int mb, int malim, const double* __restrict__ lr,
-                                          const double* __restrict__ li, int lane, const double* __restrict__ cg,
-                                          double scale, double (&zr)[JO + 1], double (&zi)[JO + 1]) {
-    constexpr int CGB = cg_block(J, J1, J2, JO);
-    const double* __restrict__ c = cg + CGB;
-    const int mb1min = max(0, (2 * mb - JO - J2 + J1) / 2);
-    const int mb2max = (2 * mb - JO - (2 * mb1min - J1) + J2) / 2;
-    const int nb = min(J1, (2 * mb - JO + J2 + J1) / 2) - mb1min + 1;
-    for (int ib = 0; ib < nb; ++ib) {
-        const int mb1 = mb1min + ib, mb2 = mb2max - ib;
-        const double cb = c[mb1 * (J2 + 1) + mb2] * scale;
-        double ar[J1 + 1], ai[J1 + 1], br[J2 + 1], bi[J2 + 1];
-        lds_row<J1, CX>(mb1, lr, li, lane, ar, ai);
-        lds_row<J2, CX>(mb2, lr, li, lane, br, bi);
-        static_for<0, JO + 1>([&](auto MAc) {
-            constexpr int MA = decltype(MAc)::value;
-            constexpr int LO = cg_lo(MA, J1, J2, JO), HI2 = cg_hi2(MA, J1, J2, JO), NA = cg_n(MA, J1, J2, JO);
-            if (MA <= malim) {
-                double sr = 0.0, si = 0.0;
-                static_for<0, NA>([&](auto IAc) {
-                    constexpr int IA = decltype(IAc)::value;
-                    constexpr int M1 = LO + IA, M2 = HI2 - IA;
-                    // compile-time coefficient: no scalar load, whose wait would also drain the LDS reads
-                    constexpr double cga = cg_const(J1, J2, JO, M1, M2);
-                    sr += cga * (ar[M1] * br[M2] - ai[M1] * bi[M2]);
-                    si += cga * (ar[M1] * bi[M2] + ai[M1] * br[M2]);
-                });
-                zr[MA] += cb * sr;
-                zi[MA] += cb * si;
-            }
-        });
-    }
-}
-
 // The Z contributions of one coupling (J1, J2) -> JO to the output row for one pair of U rows a (layer
 // J1) and b (layer J2), scaled by the mb-direction coefficient cb (the terms of z_row_lds).
 template <int J1, int J2, int JO>
@@ -138,6 +103,35 @@ __device__ __forceinline__ void z_terms_same_row(int malim, const double (&ar)[J
         }
     });
 }
+
+// SYM: a coupling (J1, J1) takes each unordered pair of rows {mb1, mb2} once, as y_couplings_shared_a does.
+template <int J, int J1, int J2, int JO, bool CX = false, bool SYM = false>
+__device__ __forceinline__ void z_row_lds(int mb, int malim, const double* __restrict__ lr,
+                                          const double* __restrict__ li, int lane, const double* __restrict__ cg,
+                                          double scale, double (&zr)[JO + 1], double (&zi)[JO + 1]) {
+    constexpr int CGB = cg_block(J, J1, J2, JO);
+    constexpr bool SAME = SYM && (J1 == J2);
+    const double* __restrict__ c = cg + CGB;
+    const int mb1min = max(0, (2 * mb - JO - J2 + J1) / 2);
+    const int mb2max = (2 * mb - JO - (2 * mb1min - J1) + J2) / 2;
+    const int nb = min(J1, (2 * mb - JO + J2 + J1) / 2) - mb1min + 1;
+    for (int ib = 0; ib < nb; ++ib) {
+        const int mb1 = mb1min + ib, mb2 = mb2max - ib;
+        if (SAME && mb1 > mb2) break;   // mb1 rises and mb2 falls with ib
+        double cb = c[mb1 * (J2 + 1) + mb2] * scale;
+        double ar[J1 + 1], ai[J1 + 1];
+        lds_row<J1, CX>(mb1, lr, li, lane, ar, ai);
+        if (SAME && mb1 == mb2) {
+            z_terms_same_row<J1, JO>(malim, ar, ai, cb, zr, zi);
+        } else {
+            if (SAME) cb *= 2.0;
+            double br[J2 + 1], bi[J2 + 1];
+            lds_row<J2, CX>(mb2, lr, li, lane, br, bi);
+            z_terms<J1, J2, JO>(malim, ar, ai, br, bi, cb, zr, zi);
+        }
+    }
+}
+
 
 // All couplings of output row (JO, mb) with the U row of the larger layer shared: for each J1 the rows mb1
 // of layer J1 are read from LDS once and used by every J2 whose partner row mb2 = mb + (J1+J2-JO)/2 - mb1
@@ -190,7 +184,7 @@ __device__ __forceinline__ void y_couplings_shared_a(int mb, int malim, const do
 
 // PA: per-atom weights beta_t[triple * N + atom] (the learned surrogate's dE/dB) and no energy partials;
 // otherwise wave-uniform weights beta_t[triple].  SA: couplings through y_couplings_shared_a; CX: the
-// complex-interleaved LDS layout; SYM (with SA): the swap symmetry of the (J1, J1) couplings.
+// complex-interleaved LDS layout; SYM: the swap symmetry of the (J1, J1) couplings.
 template <int J, int JO, bool PA = false, bool SA = false, bool CX = false, bool SYM = false>
 __device__ __forceinline__ void y_row_lds(const double* __restrict__ cg, int row, int mb, int atom, bool live, int N,
                                           const double* __restrict__ lr, const double* __restrict__ li, int lane,
@@ -212,7 +206,7 @@ __device__ __forceinline__ void y_row_lds(const double* __restrict__ cg, int row
                     constexpr int T = triple_id(J, J1, J2, JO);
                     const double beta =
                         PA ? (live ? beta_t[static_cast<size_t>(T) * n + atom] : 0.0) : beta_t[T];
-                    z_row_lds<J, J1, J2, JO, CX>(mb, malim, lr, li, lane, cg, beta, yr, yi);
+                    z_row_lds<J, J1, J2, JO, CX, SYM>(mb, malim, lr, li, lane, cg, beta, yr, yi);
                 }
             });
         });
@@ -332,8 +326,8 @@ __global__ __launch_bounds__(64 * W) void k_b_half(const double* __restrict__ cg
                                 double zr[JO + 1], zi[JO + 1];
 #pragma unroll
                                 for (int m = 0; m <= JO; ++m) { zr[m] = 0.0; zi[m] = 0.0; }
-                                z_row_lds<J, J1, J2, JO, CX>(mb, malim, lr, li, lane, cg, bnorm ? 1.0 / (JO + 1) : 1.0,
-                                                             zr, zi);
+                                z_row_lds<J, J1, J2, JO, CX, true>(mb, malim, lr, li, lane, cg,
+                                                                   bnorm ? 1.0 / (JO + 1) : 1.0, zr, zi);
                                 const int hrow = h_block(JO) + (JO + 1) * mb;
 #pragma unroll
                                 for (int ma = 0; ma <= JO; ++ma)

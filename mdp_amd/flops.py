@@ -14,6 +14,9 @@ enumerated from the loop bounds of the index spaces -- the denominators of the r
                       10*na*nb (complex product and CG weight) + 4*nb (row weight) + 4 (beta * Z).
 * ``yi_md`` per atom: the same restricted to the Y entries the force needs (middle row of even layers
                       only up to ma = mb) plus 4 per entry for the energy partial U.Y.
+* ``yi_md_sym`` per atom: what compute_yi executes when the (j1, j1) couplings take each unordered pair
+                      of U rows once (the swap symmetry of their two factors): for those couplings only the
+                      row pairs mb1 < mb2 (whole) and mb1 = mb2 (the terms m1 <= m2) are computed.
 * ``ei``   per atom : the upstream fused energy (cpuSnap.cpp:544-719): Z over the half domain of each
                       bispectrum triple (10*na*nb + 4*nb) + 4 for the U.Z product + 2 for beta*B.
 * ``zb``   per atom : every Z entry (cpuSnap.cpp:1899-1969: 10*na*nb + 4*nb per idxz entry) plus B of
@@ -57,7 +60,7 @@ def snap_flops(J):
         for mb in range(j // 2 + 1):
             for ma in range((mb if j == 2 * mb else j) + 1):
                 dedr_gen += 4 + 5 + 10 * (ma < j) + 10 * (ma > 0)
-    yi = yi_md = ei = zb = 0
+    yi = yi_md = yi_md_sym = ei = zb = 0
     terms = elems = 0
     half = lambda l: sum((mb + 1 if l == 2 * mb else l + 1) for mb in range(l // 2 + 1))
     for j1, j2, j in _triples(J):
@@ -78,12 +81,22 @@ def snap_flops(J):
                 yi += w + 4
                 if 2 * mb < j or ma <= mb:
                     yi_md += w + 4 + 4
+                    if j1 == j2:   # rows mb1 < mb2 once, mb1 = mb2 with the terms m1 <= m2
+                        c = mb + (j1 + j2 - j) // 2
+                        lo = max(0, c - j2)
+                        hi = min(j1, c)
+                        n_lt = sum(1 for mb1 in range(lo, hi + 1) if mb1 < c - mb1)
+                        n_eq = sum(1 for mb1 in range(lo, hi + 1) if mb1 == c - mb1)
+                        yi_md_sym += 10 * na * n_lt + 4 * n_lt + n_eq * (10 * ((na + 1) // 2) + 4) + 4 + 4
+                    else:
+                        yi_md_sym += w + 4 + 4
                     if j >= j1:
                         ei += w + 4
         if j >= j1:
             ei += 2
     dbdr = _row_updates(J, True) + 3 * (4 * elems + 2 * terms)
-    return dict(ui=ui, dedr=dedr, dedr_gen=dedr_gen, yi=yi, yi_md=yi_md, ei=ei, zb=zb, dbdr=dbdr)
+    return dict(ui=ui, dedr=dedr, dedr_gen=dedr_gen, yi=yi, yi_md=yi_md, yi_md_sym=yi_md_sym, ei=ei, zb=zb,
+                dbdr=dbdr)
 
 
 def dedr_key(variant):
