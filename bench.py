@@ -663,10 +663,10 @@ def main():
     log(f"dominant kernel {dom}: {ms_avg:.3f} ms/launch, {achieved:.2f} TFLOP/s "
         f"({100 * achieved / FP64_PEAK_TFLOPS:.1f}% of FP64 peak); whole step "
         f"{step_flops / (elapsed / a.steps) / 1e12:.2f} TFLOP/s")
-    # The algorithmic count is the upstream compute_yi's.  compute_yi variant 115 skips the terms that the
-    # swap symmetry of the (j1, j1) couplings makes redundant, so the FP64 work it executes is smaller.
+    # The algorithmic count is the upstream compute_yi's.  compute_yi variants 115 and 116 skip the terms that
+    # the swap symmetry of the (j1, j1) couplings makes redundant, so the FP64 work they execute is smaller.
     executed = None
-    if dom == "compute_yi" and int(md.tuning().get("chosen", {}).get("compute_yi_waves", 0)) == 115:
+    if dom == "compute_yi" and int(md.tuning().get("chosen", {}).get("compute_yi_waves", 0)) in (115, 116):
         executed = fl["yi_md_sym"] * n
         log(f"  compute_yi executes {executed / per_launch[dom]:.3f} of the algorithmic flops: "
             f"{executed / (ms_avg * 1e-3) / 1e12:.2f} TFLOP/s executed")

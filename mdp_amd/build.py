@@ -19,7 +19,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmdpsnap.so")
 HEADER = os.path.join(ROOT, "include", "mdpsnap.h")
 
-SOURCES = ["k_fit.hip", "k_fit_mfma.hip", "k_fitlsq.hip", "k_fit_zb.hip", "k_yih_8_8.hip", "k_yih_8_12.hip", "k_yih_8_4.hip", "k_yih_8_12s.hip", "k_yih_8_12c.hip", "k_yih_8_12y.hip", "k_yih_small.hip", "k_yi.hip", "k_yia.hip", "k_yih.hip", "k_dedr.hip", "k_bz.hip", "k_ui.hip", "k_misc.hip", "md_engine.hip", "fit_engine.hip", "k_mlp.hip", "k_sur.hip", "surrogate_api.cpp", "dropin.cpp", "velocity.cpp",
+SOURCES = ["k_fit.hip", "k_fit_mfma.hip", "k_fitlsq.hip", "k_fit_zb.hip", "k_yih_8_8.hip", "k_yih_8_12.hip", "k_yih_8_4.hip", "k_yih_8_12s.hip", "k_yih_8_12c.hip", "k_yih_8_12y.hip", "k_yih_8_12u1.hip", "k_yih_small.hip", "k_yi.hip", "k_yia.hip", "k_yih.hip", "k_dedr.hip", "k_bz.hip", "k_ui.hip", "k_misc.hip", "md_engine.hip", "fit_engine.hip", "k_mlp.hip", "k_sur.hip", "surrogate_api.cpp", "dropin.cpp", "velocity.cpp",
            "runtime.cpp", "snap_tables.cpp", "pairpot.hip", "cellsort.hip", "app_input.cpp"]
 
 ARCH = os.environ.get("MDP_OFFLOAD_ARCH", "gfx950")
@@ -33,7 +33,8 @@ FLAGS = ["-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC", "-Wall", "-Wno-
 # into SGPRs and spill those to VGPR lanes, so it is disabled for these translation units.
 FILE_FLAGS = {f: ["-mllvm", "-disable-machine-licm"]
               for f in ("k_yih_8_8.hip", "k_yih_8_12.hip", "k_yih_8_4.hip", "k_yih_8_12s.hip", "k_yih_8_12c.hip",
-                        "k_yih_8_12y.hip", "k_yih_small.hip", "k_fit_zb.hip")}
+                        "k_yih_8_12y.hip", "k_yih_8_12u1.hip", "k_yih_small.hip",
+                        "k_fit_zb.hip")}
 
 
 def hipcc():

@@ -21,6 +21,7 @@ YH_DECL(8, 4)
 YH_DECL(8, 12s)
 YH_DECL(8, 12c)
 YH_DECL(8, 12y)
+YH_DECL(8, 12u1)
 #undef YH_DECL
 
 void yi_row_order(int J, int* out) {
@@ -50,9 +51,9 @@ void yi_row_order(int J, int* out) {
 int yi_half_variants(int J, int* ids) {
     if (J == 8) {   // waves per workgroup; 112: 12 waves, the larger layer's U rows shared across couplings;
                     // 113: the same with (re, im) adjacent in LDS; 115: 113 with the swap symmetry of the
-                    // (j1, j1) couplings
-        ids[0] = 8; ids[1] = 12; ids[2] = 4; ids[3] = 112; ids[4] = 113; ids[5] = 115;
-        return 6;
+                    // (j1, j1) couplings; 116: 115 with the row index made wave-uniform explicitly
+        ids[0] = 8; ids[1] = 12; ids[2] = 4; ids[3] = 112; ids[4] = 113; ids[5] = 115; ids[6] = 116;
+        return 7;
     }
     ids[0] = 8;
     return 1;
@@ -74,6 +75,8 @@ void launch_yi_half(const SnapDev& s, const double* Ur, const double* Ui, const 
                 launch_yi_half_8_12c(s, Ur, Ui, beta_t, Yr, Yi, epart, order, N, st);
             else if (waves == 115)
                 launch_yi_half_8_12y(s, Ur, Ui, beta_t, Yr, Yi, epart, order, N, st);
+            else if (waves == 116)
+                launch_yi_half_8_12u1(s, Ur, Ui, beta_t, Yr, Yi, epart, order, N, st);
             else if (waves == 4)
                 launch_yi_half_8_4(s, Ur, Ui, beta_t, Yr, Yi, epart, order, N, st);
             else

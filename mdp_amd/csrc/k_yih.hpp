@@ -147,7 +147,7 @@ __device__ __forceinline__ void z_row_lds(int mb, int malim, const double* __res
 // SYM: a coupling (J1, J1) takes each unordered pair of rows {mb1, mb2} once (the sum is symmetric under
 // the swap of its two U factors): mb1 > mb2 is skipped, mb1 < mb2 counts twice, mb1 = mb2 through
 // z_terms_same_row -- 17% fewer terms at twojmax 8.
-template <int J, int JO, bool CX = false, bool SYM = false>
+template <int J, int JO, bool CX = false, bool SYM = false, int UNI = 0>
 __device__ __forceinline__ void y_couplings_shared_a(int mb, int malim, const double* __restrict__ lr,
                                                      const double* __restrict__ li, int lane,
                                                      const double* __restrict__ cg,
@@ -163,7 +163,9 @@ __device__ __forceinline__ void y_couplings_shared_a(int mb, int malim, const do
                 hi = max(hi, min(J1, (2 * mb - JO + J2 + J1) / 2));
             }
         });
-        for (int mb1 = lo; mb1 <= hi; ++mb1) {
+        for (int mb1v = lo; mb1v <= hi; ++mb1v) {
+            // UNI: the row index made wave-uniform explicitly (scalar branches for the row tests below)
+            const int mb1 = UNI ? __builtin_amdgcn_readfirstlane(mb1v) : mb1v;
             double ar[J1 + 1], ai[J1 + 1];
             lds_row<J1, CX>(mb1, lr, li, lane, ar, ai);
             static_for<0, J1 + 1>([&](auto J2c) {
@@ -191,8 +193,10 @@ __device__ __forceinline__ void y_couplings_shared_a(int mb, int malim, const do
 
 // PA: per-atom weights beta_t[triple * N + atom] (the learned surrogate's dE/dB) and no energy partials;
 // otherwise wave-uniform weights beta_t[triple].  SA: couplings through y_couplings_shared_a; CX: the
-// complex-interleaved LDS layout; SYM: the swap symmetry of the (J1, J1) couplings.
-template <int J, int JO, bool PA = false, bool SA = false, bool CX = false, bool SYM = false>
+// complex-interleaved LDS layout; SYM: the swap symmetry of the (J1, J1) couplings; UNI: 1 makes the
+// shared-row loop's row index wave-uniform explicitly (140 instead of 166 VGPRs at twojmax 8, the row tests
+// become scalar branches), 2 also the output row's (j, mb), which costs spills.
+template <int J, int JO, bool PA = false, bool SA = false, bool CX = false, bool SYM = false, int UNI = 0>
 __device__ __forceinline__ void y_row_lds(const double* __restrict__ cg, int row, int mb, int atom, bool live, int N,
                                           const double* __restrict__ lr, const double* __restrict__ li, int lane,
                                           const double* __restrict__ beta_t, double* __restrict__ Yr,
@@ -203,7 +207,7 @@ __device__ __forceinline__ void y_row_lds(const double* __restrict__ cg, int row
 #pragma unroll
     for (int m = 0; m <= JO; ++m) { yr[m] = 0.0; yi[m] = 0.0; }
     if constexpr (SA && !PA) {
-        y_couplings_shared_a<J, JO, CX, SYM>(mb, malim, lr, li, lane, cg, beta_t, yr, yi);
+        y_couplings_shared_a<J, JO, CX, SYM, UNI>(mb, malim, lr, li, lane, cg, beta_t, yr, yi);
     } else {
         static_for<0, J + 1>([&](auto J1c) {
             constexpr int J1 = decltype(J1c)::value;
@@ -241,7 +245,7 @@ __device__ __forceinline__ void y_row_lds(const double* __restrict__ cg, int row
 
 // The coefficient tables (cg, beta_t, order) are separate read-only, non-aliased kernel arguments so
 // that their wave-uniform loads go through the scalar cache into SGPRs.
-template <int J, int W, bool PA = false, bool SA = false, bool CX = false, bool SYM = false>
+template <int J, int W, bool PA = false, bool SA = false, bool CX = false, bool SYM = false, int UNI = 0>
 __global__ __launch_bounds__(64 * W) void k_yi_half(const double* __restrict__ cg, const double* __restrict__ Ur,
                                                     const double* __restrict__ Ui, const double* __restrict__ beta_t,
                                                     double* __restrict__ Yr, double* __restrict__ Yi,
@@ -275,11 +279,13 @@ __global__ __launch_bounds__(64 * W) void k_yi_half(const double* __restrict__ c
         r = __builtin_amdgcn_readfirstlane(__shfl(r, 0));
         if (r >= NR) break;
         const int row = __builtin_amdgcn_readfirstlane(order[r]);
-        const int jo = half_row_j(row), mb = half_row_mb(row);
+        const int jo = UNI > 1 ? __builtin_amdgcn_readfirstlane(half_row_j(row)) : half_row_j(row);
+        const int mb = UNI > 1 ? __builtin_amdgcn_readfirstlane(half_row_mb(row)) : half_row_mb(row);
         static_for<0, J + 1>([&](auto JOc) {
             constexpr int JO = decltype(JOc)::value;
             if (jo == JO)
-                y_row_lds<J, JO, PA, SA, CX, SYM>(cg, row, mb, atom, live, N, lr, li, lane, beta_t, Yr, Yi, epart);
+                y_row_lds<J, JO, PA, SA, CX, SYM, UNI>(cg, row, mb, atom, live, N, lr, li, lane, beta_t, Yr, Yi,
+                                                      epart);
         });
     }
 }

@@ -135,11 +135,11 @@ def _first_step_vs_oracle(oracle, x, box, J, seed):
     return md
 
 
-@pytest.mark.parametrize("variant", ["8", "12", "4", "112", "113", "115"])
+@pytest.mark.parametrize("variant", ["8", "12", "4", "112", "113", "115", "116"])
 def test_every_yi_variant_vs_oracle(oracle, monkeypatch, variant):
     """compute_yi pinned to each twojmax-8 variant: 8, 12 or 4 waves per workgroup, or 12 waves with the
-    larger layer's U rows shared across couplings (112), also with (re, im) adjacent in LDS (113), and with the
-    swap symmetry of the (j1, j1) couplings (115)."""
+    larger layer's U rows shared across couplings (112), also with (re, im) adjacent in LDS (113), with the
+    swap symmetry of the (j1, j1) couplings (115), and with the row index made wave-uniform (116)."""
     monkeypatch.setenv("MDP_YI_VARIANT", variant)
     x, box = lattice.crystal("bcc", 12, 3.316, 0.1, 51)
     md = _first_step_vs_oracle(oracle, x, box, 8, 51)
