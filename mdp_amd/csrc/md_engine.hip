@@ -483,7 +483,8 @@ __global__ __launch_bounds__(256) void k_sum_counts(const int* __restrict__ cnt,
     if (threadIdx.x == 0) atomicAdd(total, part[0] + part[1] + part[2] + part[3]);
 }
 
-// incoming entries: for every entry e = (i -> j) append e to j's list, then sort each list
+// incoming entries: for every entry e = (i -> j) append e to j's list, then sort each list.  The lists are
+// atom-major (slot k of atom j at j*NBMAX + k): the tally's 16 lanes per atom then read one contiguous run.
 __global__ void k_incoming(const int* __restrict__ nb_cnt, const int* __restrict__ nb_j, int* __restrict__ in_cnt,
                            int* __restrict__ in_e, int N, int* __restrict__ overflow) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -493,7 +494,7 @@ __global__ void k_incoming(const int* __restrict__ nb_cnt, const int* __restrict
         const int j = nb_j[e];
         const int slot = atomicAdd(&in_cnt[j], 1);
         if (slot < NBMAX)
-            in_e[static_cast<size_t>(slot) * N + j] = e;
+            in_e[static_cast<size_t>(j) * NBMAX + slot] = e;
         else
             atomicOr(overflow, 4);
     }
@@ -504,8 +505,8 @@ __global__ void k_incoming_sort(int* __restrict__ in_cnt, int* __restrict__ in_e
     if (i >= N) return;
     const int n = min(in_cnt[i], NBMAX);
     in_cnt[i] = n;
-    int* a = in_e + i;   // slot k at a[k*N]
-    const size_t st = static_cast<size_t>(N);
+    int* a = in_e + static_cast<size_t>(i) * NBMAX;   // slot k at a[k]
+    const size_t st = 1;
     for (int k = 1; k < n; ++k) {
         const int v = a[k * st];
         int m = k - 1;
@@ -657,7 +658,7 @@ __global__ __launch_bounds__(256) void k_tally(const double* __restrict__ dedr, 
         }
         const int nin = in_cnt[i];
         for (int k = sub; k < nin; k += PSUB) {
-            const int p = entry_pair[in_e[static_cast<size_t>(k) * N + i]];
+            const int p = entry_pair[in_e[static_cast<size_t>(i) * NBMAX + k]];
             if (p >= 0) {
                 a[0] -= dedr[3 * static_cast<size_t>(p)];
                 a[1] -= dedr[3 * static_cast<size_t>(p) + 1];
