@@ -206,7 +206,8 @@ __global__ __launch_bounds__(64, 1) void k_fit_dbdr_mfma(const double* __restric
             for (int k = 0; k < (LINES + 63) / 64; ++k)
                 if (k * 64 + lane < LINES) touch ^= zw[(k * 64 + lane) * 32];
         }
-        for (int c0 = 0; c0 < ncol; c0 += 64) {
+        for (int c0v = 0; c0v < ncol; c0v += 64) {
+            const int c0 = __builtin_amdgcn_readfirstlane(c0v);   // wave-uniform explicitly: fewer VGPRs
             const int nt = min(4, (ncol - c0 + 15) >> 4);   // column tiles in use
             const int col = c0 + lane;
             const int comp = col < ncol ? col / P : 0;      // columns: component-major within the atom
