@@ -40,7 +40,7 @@ struct mdp_fit {
     std::vector<double> boxes_h, tilts_h;   // tilts_h empty: every cell orthogonal
     DevBuf x, type, cfg_of, cfg_a0, cfg_L, cfg_T, cfg_p0, rc2, pcnt, poff, scan_tmp, flags, counter;
     DevBuf pi, pj, pti, ptj, rij, Ur, Ui, zt, blist, bpart, dblist, bzero, t_lo, t_b, t_z, t_w, zorder, t_wtab, t_hlo, t_hb, t_hz,
-        t_hw, t_mz, t_mw, t_mcomb;
+        t_hw, t_mz, t_mw, t_mcomb, pgeo;
     int dbdr_variant = 0;
     bool zb_lds = false;     // Z + B from the LDS-staged half U (twojmax 4, 6, 8)
     int zb_waves = 8;        // waves per Z workgroup (autotuned: 8, 4 or 12)
@@ -253,6 +253,7 @@ struct mdp_fit {
         ptj.ensure(sizeof(int) * c);
         rij.ensure(sizeof(double) * 3 * c);
         dblist.ensure(sizeof(double) * 3 * c * tab.idx.idxb_max);
+        pgeo.ensure(sizeof(double) * 5 * c);
         cap = c;
         if (gexec) {   // buffers moved: the captured graph refers to the old ones
             MDP_HIP(hipGraphExecDestroy(gexec));
@@ -280,8 +281,14 @@ struct mdp_fit {
                 launch_fit_guard(flags.as<int>(), poff.as<int>(), N, st);
                 break;
             case 1:
-                launch_ui(s, zb_lds ? UI_HALF_WSELF : UI_FULL_WSELF, Ur.as<double>(), Ui.as<double>(), rij.as<double>(),
-                          poff.as<int>(), nullptr, pti.as<int>(), ptj.as<int>(), N, 1.0, -1, st);
+                if (zb_lds) {   // half U + wself from the per-pair geometry (one geometry per pair)
+                    launch_pair_geo(s, rij.as<double>(), pti.as<int>(), ptj.as<int>(), poff.as<int>() + N,
+                                    static_cast<int>(cap), pgeo.as<double>(), st);
+                    launch_ui_geo(s, Ur.as<double>(), Ui.as<double>(), pgeo.as<double>(), poff.as<int>(), N, 1.0, st);
+                } else {
+                    launch_ui(s, UI_FULL_WSELF, Ur.as<double>(), Ui.as<double>(), rij.as<double>(), poff.as<int>(),
+                              nullptr, pti.as<int>(), ptj.as<int>(), N, 1.0, -1, st);
+                }
                 break;
             case 2:
                 if (zb_lds)

@@ -181,6 +181,32 @@ void launch_ui(const SnapDev& s, UiMode mode, double* Ur, double* Ui, const doub
 #undef UI_CASE
 }
 
+// The per-pair geometry that launch_ui_geo reads, pgeo[5q..5q+4] = (a_r, a_i, b_r, b_i, sfac w_j), for the
+// pairs of a list whose length is on the device (np_dev): one lane per pair.
+__global__ __launch_bounds__(256) void k_pair_geo(SnapDev s, const double* __restrict__ rij,
+                                                  const int* __restrict__ ti, const int* __restrict__ tj,
+                                                  const int* __restrict__ np_dev, double* __restrict__ pgeo) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= *np_dev) return;
+    const int tjq = tj[q];
+    PairGeom g;
+    pair_geometry<false>(rij[3 * static_cast<size_t>(q)], rij[3 * static_cast<size_t>(q) + 1],
+                         rij[3 * static_cast<size_t>(q) + 2], (s.radelem[ti[q]] + s.radelem[tjq]) * s.rcutfac, s.rmin0,
+                         s.rfac0, s.switchflag, s.wjelem[tjq], g);
+    double* __restrict__ o = pgeo + 5 * static_cast<size_t>(q);
+    o[0] = g.ar;
+    o[1] = g.ai;
+    o[2] = g.br;
+    o[3] = g.bi;
+    o[4] = g.sfac;
+}
+
+void launch_pair_geo(const SnapDev& s, const double* rij, const int* ti, const int* tj, const int* np_dev, int cap,
+                     double* pgeo, hipStream_t st) {
+    if (cap <= 0) return;
+    hipLaunchKernelGGL(k_pair_geo, dim3((cap + 255) / 256), dim3(256), 0, st, s, rij, ti, tj, np_dev, pgeo);
+}
+
 // compute_ui with the per-atom sums in LDS (half layout plus wself, from the pair geometry pgeo).  Lanes
 // are pairs: workgroup block b holds the atoms whose first pair lies in [b*UI_PB, (b+1)*UI_PB), so with at
 // most UI_MAXP pairs per atom a block has fewer than UI_PB + UI_MAXP <= 512 pairs, one per lane.  Each

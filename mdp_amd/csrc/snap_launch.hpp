@@ -44,6 +44,9 @@ void launch_ui(const SnapDev& s, UiMode mode, double* Ur, double* Ui, const doub
 // [pair_off[i], pair_off[i+1]).
 void launch_ui_geo(const SnapDev& s, double* Ur, double* Ui, const double* pgeo, const int* pair_off, int natoms,
                    double wself, hipStream_t st);
+// pgeo[5q..5q+4] = (a_r, a_i, b_r, b_i, sfac w_j) of every pair q < *np_dev (cap: the launch's pair capacity)
+void launch_pair_geo(const SnapDev& s, const double* rij, const int* ti, const int* tj, const int* np_dev, int cap,
+                     double* pgeo, hipStream_t st);
 // The same result with the per-atom sums formed in LDS, one lane per pair (at most 96 pairs per atom);
 // persistent over `groups` workgroups of 512 lanes (0: 512).
 void launch_ui_geo_lds(const SnapDev& s, double* Ur, double* Ui, const double* pgeo, const int* pair_off,
