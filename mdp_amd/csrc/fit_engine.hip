@@ -39,6 +39,7 @@ struct mdp_fit {
     std::vector<int> a0_h;   // [C+1]
     std::vector<double> boxes_h, tilts_h;   // tilts_h empty: every cell orthogonal
     DevBuf x, type, cfg_of, cfg_a0, cfg_L, cfg_T, cfg_p0, rc2, pcnt, poff, scan_tmp, flags, counter;
+    DevBuf chunk_ord;   // 64-atom chunks of the pair build, most candidate tests first
     DevBuf pi, pj, pti, ptj, rij, Ur, Ui, zt, blist, bpart, dblist, bzero, t_lo, t_b, t_z, t_w, zorder, t_wtab, t_hlo, t_hb, t_hz,
         t_hw, t_mz, t_mw, t_mcomb, pgeo;
     int dbdr_variant = 0;
@@ -157,6 +158,25 @@ struct mdp_fit {
         put(cfg_L, boxes, sizeof(double) * 3 * C);
         if (!tilts_h.empty()) put(cfg_T, tilts_h.data(), sizeof(double) * 3 * C);
         put(rc2, rc2h.data(), sizeof(double) * rc2h.size());
+        {   // the pair build runs one wave per 64 consecutive atoms, each lane testing every atom of its
+            // configuration (every image of it in a box narrower than two cutoffs or tilted); launching the
+            // chunks with the most tests first keeps the largest configurations from ending last
+            double rcmax = 0.0;
+            for (double r : rc2h) rcmax = std::max(rcmax, std::sqrt(r));
+            std::vector<double> cost_c(C);
+            for (int c = 0; c < C; ++c) {
+                bool narrow = !tilts_h.empty() && (tilts_h[3 * c] != 0.0 || tilts_h[3 * c + 1] != 0.0 || tilts_h[3 * c + 2] != 0.0);
+                for (int d = 0; d < 3; ++d) narrow = narrow || boxes[3 * c + d] <= 2.0 * rcmax;
+                cost_c[c] = static_cast<double>(a0_h[c + 1] - a0_h[c]) * (narrow ? 27.0 : 1.0);
+            }
+            const int nch = (N + 63) / 64;
+            std::vector<double> cost(nch, 0.0);
+            for (int g = 0; g < N; ++g) cost[g / 64] += cost_c[cof[g]];
+            std::vector<int> ord(nch);
+            for (int k = 0; k < nch; ++k) ord[k] = k;
+            std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+            put(chunk_ord, ord.data(), sizeof(int) * std::max(nch, 1));
+        }
         put(bzero, bz.data(), sizeof(double) * bz.size());
         put(t_lo, lo.data(), sizeof(int) * lo.size());
         put(t_b, tb.data(), sizeof(int) * tb.size());
@@ -210,7 +230,7 @@ struct mdp_fit {
         terms.z = t_z.as<int>();
         terms.w = t_w.as<double>();
         cfg_p0.ensure(sizeof(int) * (C + 1));
-        pcnt.ensure(sizeof(int) * n);
+        pcnt.ensure(sizeof(int) * n * (FIT_PAIR_SPLIT + 1));
         poff.ensure(sizeof(int) * (n + 1));
         scan_tmp.ensure(sizeof(int) * (scan_tmp_ints(N) + 8));
         flags.ensure(sizeof(int) * 4);
@@ -270,12 +290,12 @@ struct mdp_fit {
             case 0:
                 launch_fill_zero_int(flags.as<int>(), 4, st);
                 launch_fit_pairs(false, x.as<double>(), type.as<int>(), cfg_of.as<int>(), cfg_a0.as<int>(),
-                                 cfg_L.as<double>(), tilts_h.empty() ? nullptr : cfg_T.as<double>(), rc2.as<double>(), ntypes, N, pcnt.as<int>(), nullptr, 0, nullptr,
+                                 cfg_L.as<double>(), tilts_h.empty() ? nullptr : cfg_T.as<double>(), rc2.as<double>(), ntypes, N, chunk_ord.as<int>(), pcnt.as<int>(), nullptr, 0, nullptr,
                                  nullptr, nullptr, nullptr, nullptr, flags.as<int>(), st);
                 exclusive_scan(pcnt.as<int>(), poff.as<int>(), N, scan_tmp.as<int>(), st);
                 launch_fit_cfg_p0(poff.as<int>(), cfg_a0.as<int>(), cfg_p0.as<int>(), C, st);
                 launch_fit_pairs(true, x.as<double>(), type.as<int>(), cfg_of.as<int>(), cfg_a0.as<int>(),
-                                 cfg_L.as<double>(), tilts_h.empty() ? nullptr : cfg_T.as<double>(), rc2.as<double>(), ntypes, N, pcnt.as<int>(), poff.as<int>(), cap,
+                                 cfg_L.as<double>(), tilts_h.empty() ? nullptr : cfg_T.as<double>(), rc2.as<double>(), ntypes, N, chunk_ord.as<int>(), pcnt.as<int>(), poff.as<int>(), cap,
                                  rij.as<double>(), pi.as<int>(), pj.as<int>(), pti.as<int>(), ptj.as<int>(),
                                  flags.as<int>(), st);
                 launch_fit_guard(flags.as<int>(), poff.as<int>(), N, st);
@@ -315,7 +335,7 @@ struct mdp_fit {
 
     long count_pairs() {
         launch_fit_pairs(false, x.as<double>(), type.as<int>(), cfg_of.as<int>(), cfg_a0.as<int>(), cfg_L.as<double>(),
-                         tilts_h.empty() ? nullptr : cfg_T.as<double>(), rc2.as<double>(), ntypes, N, pcnt.as<int>(), nullptr, 0, nullptr, nullptr, nullptr, nullptr,
+                         tilts_h.empty() ? nullptr : cfg_T.as<double>(), rc2.as<double>(), ntypes, N, chunk_ord.as<int>(), pcnt.as<int>(), nullptr, 0, nullptr, nullptr, nullptr, nullptr,
                          nullptr, flags.as<int>(), st);
         exclusive_scan(pcnt.as<int>(), poff.as<int>(), N, scan_tmp.as<int>(), st);
         int tot = 0;

@@ -14,9 +14,14 @@ namespace mdp {
 // Pair build by per-configuration brute force over periodic images.  fill = false writes the pair
 // count of every atom to pcnt; fill = true writes rij/pi/pj/pti/ptj at the CSR offsets poff (N+1
 // entries); when poff[N] exceeds cap it leaves the outputs unwritten, sets bit 0 of flags[0] and
-// stores poff[N] in flags[1].
+// stores poff[N] in flags[1].  Each atom's candidates are taken in FIT_PAIR_SPLIT slices by as many
+// workgroups; pcnt holds (FIT_PAIR_SPLIT + 1) * N ints (the totals, then the slice counts that the fill
+// pass reads).  chunk_order ((N+63)/64 entries, a permutation) gives the launch order of the 64-atom
+// chunks.
+constexpr int FIT_PAIR_SPLIT = 8;
 void launch_fit_pairs(bool fill, const double* x, const int* type, const int* cfg_of, const int* cfg_a0,
-                      const double* cfg_L, const double* cfg_T, const double* rc2, int ntypes, int N, int* pcnt,
+                      const double* cfg_L, const double* cfg_T, const double* rc2, int ntypes, int N,
+                      const int* chunk_order, int* pcnt,
                       const int* poff, long cap, double* rij, int* pi, int* pj, int* pti, int* ptj, int* flags,
                       hipStream_t st);
 // after an overflow flagged by the fill pass: poff[0..N] = 0 (no pairs for the remaining stages)

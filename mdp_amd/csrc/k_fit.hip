@@ -24,6 +24,14 @@ __device__ __forceinline__ double image_delta(double xj, double s, double L, dou
     const double g = xj + s * L;
     return g - xi;
 }
+// rint(t) for |t| < 2^51 (far beyond the 2^31 image indices the enumerating paths can hold): adding and
+// subtracting 1.5 * 2^52 leaves t rounded to the nearest integer, ties to even, exactly as rint does, in
+// two full-rate adds instead of the slower v_rndne_f64
+__device__ __forceinline__ double rint_add(double t) {
+#pragma clang fp contract(off)
+    const double m = 6755399441055744.0;
+    return (t + m) - m;
+}
 __device__ __forceinline__ double dist2(double dx, double dy, double dz) {
 #pragma clang fp contract(off)
     return dx * dx + dy * dy + dz * dz;
@@ -34,20 +42,28 @@ __device__ __forceinline__ double dist2(double dx, double dy, double dz) {
 // ghost construction of fullneighborlist.jl).  Order: j ascending, then (sz, sy, sx) ascending.
 // FILL = false counts, FILL = true writes the CSR slot poff[g].. when the total fits cap; otherwise
 // it writes nothing, sets bit 0 of flags[0] and stores the needed total in flags[1].
+// FIT_PAIR_SPLIT 64-lane workgroups per chunk of 64 consecutive atoms, the chunks taken in the order
+// chunk_order gives (most candidate tests first); workgroup part p of a chunk tests, for each of its atoms,
+// the p-th of FIT_PAIR_SPLIT consecutive slices of the configuration's atoms, so that an atom's pairs are
+// the concatenation of its slices' pairs in slice order.  The count pass writes the slice counts to
+// pcnt[N + p*N + g]; the fill pass starts slice p of atom g at poff[g] plus the counts of slices < p.
 template <bool FILL>
-__global__ __launch_bounds__(256) void k_fit_pairs(const double* __restrict__ x, const int* __restrict__ type,
-                                                   const int* __restrict__ cfg_of, const int* __restrict__ cfg_a0,
-                                                   const double* __restrict__ cfg_L, const double* __restrict__ cfg_T,
-                                                   const double* __restrict__ rc2, int ntypes, int N,
-                                                   int* __restrict__ pcnt,
+__global__ __launch_bounds__(64) void k_fit_pairs(const double* __restrict__ x, const int* __restrict__ type,
+                                                  const int* __restrict__ cfg_of, const int* __restrict__ cfg_a0,
+                                                  const double* __restrict__ cfg_L, const double* __restrict__ cfg_T,
+                                                  const double* __restrict__ rc2, int ntypes, int N,
+                                                  const int* __restrict__ chunk_order, int* __restrict__ pcnt,
                                                    const int* __restrict__ poff, long cap, double* __restrict__ rij,
                                                    int* __restrict__ pi, int* __restrict__ pj,
                                                    int* __restrict__ pti, int* __restrict__ ptj,
                                                    int* __restrict__ flags) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int part = static_cast<int>(blockIdx.x % FIT_PAIR_SPLIT);
+    const int g = chunk_order[blockIdx.x / FIT_PAIR_SPLIT] * 64 + static_cast<int>(threadIdx.x);
     if (g >= N) return;
     const int c = cfg_of[g];
-    const int j0 = cfg_a0[c], j1 = cfg_a0[c + 1];
+    const int ja = cfg_a0[c], nc = cfg_a0[c + 1] - ja;
+    const int j0 = ja + static_cast<int>(static_cast<long>(nc) * part / FIT_PAIR_SPLIT);
+    const int j1 = ja + static_cast<int>(static_cast<long>(nc) * (part + 1) / FIT_PAIR_SPLIT);
     const double Lx = cfg_L[3 * c], Ly = cfg_L[3 * c + 1], Lz = cfg_L[3 * c + 2];
     // tilt of a triclinic cell a = (Lx, 0, 0), b = (bx, Ly, 0), c = (cx, cy, Lz); all zero when orthogonal
     const double bx = cfg_T ? cfg_T[3 * c] : 0.0, cx = cfg_T ? cfg_T[3 * c + 1] : 0.0,
@@ -58,8 +74,9 @@ __global__ __launch_bounds__(256) void k_fit_pairs(const double* __restrict__ x,
     long q = 0;
     if constexpr (FILL) {
         q = poff[g];
+        for (int p = 0; p < part; ++p) q += pcnt[N + p * N + g];
         if (static_cast<long>(poff[N]) > cap) {   // buffers too small: write nothing, report the need
-            if (g == 0) {
+            if (g == 0 && part == 0) {
                 atomicOr(flags, 1);
                 flags[1] = poff[N];
             }
@@ -96,35 +113,78 @@ __global__ __launch_bounds__(256) void k_fit_pairs(const double* __restrict__ x,
     };
     // RC: the cutoff is rcmax2 for every partner (a compile-time choice, so no per-candidate table read)
     auto nearest = [&](int j, double xj, double yj, double zj, int tj, auto rc) __attribute__((always_inline)) {
-        const double dx = image_delta(xj, rint((xi - xj) * iLx), Lx, xi);
-        const double dy = image_delta(yj, rint((yi - yj) * iLy), Ly, yi);
-        const double dz = image_delta(zj, rint((zi - zj) * iLz), Lz, zi);
+        const double dx = image_delta(xj, rint_add((xi - xj) * iLx), Lx, xi);
+        const double dy = image_delta(yj, rint_add((yi - yj) * iLy), Ly, yi);
+        const double dz = image_delta(zj, rint_add((zi - zj) * iLz), Lz, zi);
         double r2;
         if constexpr (decltype(rc)::value) r2 = rcmax2;
         else r2 = rc2[ti * (ntypes + 1) + tj];
-        if (j != g && dist2(dx, dy, dz) <= r2) emit(j, tj, dx, dy, dz);
+        // the test is evaluated by every lane without a branch (a branch on j != g would let the compiler
+        // sink each candidate's load into it and wait for it there); only the rare hit branches
+        const bool hit = (j != g) & (dist2(dx, dy, dz) <= r2);
+        if constexpr (FILL) {
+            if (hit) emit(j, tj, dx, dy, dz);
+        } else {
+            cnt += hit;
+        }
     };
-    // candidates four at a time: their loads are issued together, the tests and emits run in j order
-    auto scan = [&](int ja, int jb, auto rc) __attribute__((always_inline)) {
+    // candidates in batches of four, software-pipelined: the next batch's loads are issued before the
+    // current batch is tested, and the scheduling barriers keep the compiler from moving them back down
+    // to their first use.  UNI: j is wave-uniform, so the candidates sit in scalar registers.
+    auto scan = [&](int ja, int jb, auto rc, auto uni) __attribute__((always_inline)) {
+        constexpr int B = 4;
+        constexpr bool U = decltype(uni)::value;
         int j = ja;
-        for (; j + 4 <= jb; j += 4) {
-            double px[4], py[4], pz[4];
-            int pt[4];
+        if (j + B <= jb) {
+            double px[B], py[B], pz[B];
+            int pt[B];
+            auto load = [&](int jj, double* ax, double* ay, double* az, int* at) __attribute__((always_inline)) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                px[u] = x[3 * (j + u)];
-                py[u] = x[3 * (j + u) + 1];
-                pz[u] = x[3 * (j + u) + 2];
-                pt[u] = type[j + u];
+                for (int u = 0; u < B; ++u) {
+                    ax[u] = x[3 * (jj + u)];
+                    ay[u] = x[3 * (jj + u) + 1];
+                    az[u] = x[3 * (jj + u) + 2];
+                    at[u] = type[jj + u];
+                }
+            };
+            auto pin = [&](double* ax, double* ay, double* az, int* at) __attribute__((always_inline)) {
+#pragma unroll
+                for (int u = 0; u < B; ++u) {
+                    if constexpr (U) {
+                        asm volatile("" : "+s"(ax[u]), "+s"(ay[u]), "+s"(az[u]), "+s"(at[u]));
+                    } else {
+                        asm volatile("" : "+v"(ax[u]), "+v"(ay[u]), "+v"(az[u]), "+v"(at[u]));
+                    }
+                }
+            };
+            load(j, px, py, pz, pt);
+            pin(px, py, pz, pt);
+            for (; j + 2 * B <= jb; j += B) {
+                double nx[B], ny[B], nz[B];
+                int nt[B];
+                load(j + B, nx, ny, nz, nt);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < B; ++u) nearest(j + u, px[u], py[u], pz[u], pt[u], rc);
+                __builtin_amdgcn_sched_barrier(0);
+                pin(nx, ny, nz, nt);
+#pragma unroll
+                for (int u = 0; u < B; ++u) {
+                    px[u] = nx[u];
+                    py[u] = ny[u];
+                    pz[u] = nz[u];
+                    pt[u] = nt[u];
+                }
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) nearest(j + u, px[u], py[u], pz[u], pt[u], rc);
+            for (int u = 0; u < B; ++u) nearest(j + u, px[u], py[u], pz[u], pt[u], rc);
+            j += B;
         }
         for (; j < jb; ++j) nearest(j, x[3 * j], x[3 * j + 1], x[3 * j + 2], type[j], rc);
     };
-    auto scan_rc = [&](int ja, int jb) __attribute__((always_inline)) {
-        if (rconst) scan(ja, jb, std::true_type{});
-        else scan(ja, jb, std::false_type{});
+    auto scan_rc = [&](int ja, int jb, auto uni) __attribute__((always_inline)) {
+        if (rconst) scan(ja, jb, std::true_type{}, uni);
+        else scan(ja, jb, std::false_type{}, uni);
     };
     if (tri) {
         // every image x_j + s1 a + s2 b + s3 c within the cutoff, ordered by j then (s3, s2, s1): the z
@@ -155,9 +215,9 @@ __global__ __launch_bounds__(256) void k_fit_pairs(const double* __restrict__ x,
     } else if (single) {
         const int cu = __builtin_amdgcn_readfirstlane(c);
         if (__all(c == cu))   // the wave's atoms share one configuration: j, x_j and t_j are wave-uniform
-            scan_rc(__builtin_amdgcn_readfirstlane(j0), __builtin_amdgcn_readfirstlane(j1));
+            scan_rc(__builtin_amdgcn_readfirstlane(j0), __builtin_amdgcn_readfirstlane(j1), std::true_type{});
         else
-            scan_rc(j0, j1);
+            scan_rc(j0, j1, std::false_type{});
     } else {
         // every image x_j + s L within the cutoff, s from the real-number bound rounded outwards (the
         // exact test decides, so the reciprocal, an ulp off at worst, only ever adds a candidate)
@@ -182,7 +242,16 @@ __global__ __launch_bounds__(256) void k_fit_pairs(const double* __restrict__ x,
             }
         }
     }
-    if constexpr (!FILL) pcnt[g] = cnt;
+    if constexpr (!FILL) pcnt[N + part * N + g] = cnt;
+}
+
+// pcnt[g] = the sum of atom g's slice counts
+__global__ void k_fit_pair_total(int* __restrict__ pcnt, int N) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N) return;
+    int t = 0;
+    for (int p = 0; p < FIT_PAIR_SPLIT; ++p) t += pcnt[N + p * N + g];
+    pcnt[g] = t;
 }
 
 // first pair of every configuration: cfg_p0[c] = poff[cfg_a0[c]]
@@ -811,17 +880,19 @@ int fit_term_table_halves(int J, int bnorm, std::vector<int>& lo, std::vector<in
 // ---- launchers ----------------------------------------------------------------------------------
 
 void launch_fit_pairs(bool fill, const double* x, const int* type, const int* cfg_of, const int* cfg_a0,
-                      const double* cfg_L, const double* cfg_T, const double* rc2, int ntypes, int N, int* pcnt,
-                      const int* poff, long cap, double* rij, int* pi, int* pj, int* pti, int* ptj, int* flags,
-                      hipStream_t st) {
+                      const double* cfg_L, const double* cfg_T, const double* rc2, int ntypes, int N,
+                      const int* chunk_order, int* pcnt, const int* poff, long cap, double* rij, int* pi, int* pj,
+                      int* pti, int* ptj, int* flags, hipStream_t st) {
     if (N <= 0) return;
-    const dim3 grid((N + 255) / 256), block(256);
-    if (fill)
+    const dim3 grid(((N + 63) / 64) * FIT_PAIR_SPLIT), block(64);
+    if (fill) {
         hipLaunchKernelGGL((k_fit_pairs<true>), grid, block, 0, st, x, type, cfg_of, cfg_a0, cfg_L, cfg_T, rc2, ntypes, N,
-                           pcnt, poff, cap, rij, pi, pj, pti, ptj, flags);
-    else
+                           chunk_order, pcnt, poff, cap, rij, pi, pj, pti, ptj, flags);
+    } else {
         hipLaunchKernelGGL((k_fit_pairs<false>), grid, block, 0, st, x, type, cfg_of, cfg_a0, cfg_L, cfg_T, rc2, ntypes,
-                           N, pcnt, poff, cap, rij, pi, pj, pti, ptj, flags);
+                           N, chunk_order, pcnt, poff, cap, rij, pi, pj, pti, ptj, flags);
+        hipLaunchKernelGGL(k_fit_pair_total, dim3((N + 255) / 256), dim3(256), 0, st, pcnt, N);
+    }
 }
 
 void launch_fit_guard(const int* flags, int* poff, int N, hipStream_t st) {
